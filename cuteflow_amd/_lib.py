@@ -1,0 +1,140 @@
+"""ctypes binding of libcuteflow_hip.so (the C ABI of include/cuteflow_hip.h).
+
+The library is the only compute path: if it is missing, or no GPU is
+visible when a handle is created, calls raise instead of falling back to
+anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .build import LIB_PATH
+
+CF_MAX_IO = 20
+CF_MAX_HYDRO = 64
+
+CF_OK = 0
+CF_ERR_ARG = -1
+CF_ERR_HIP = -2
+CF_ERR_DT = -3
+CF_ERR_MESH = -4
+CF_ERR_NOMEM = -5
+
+
+class CfParams(C.Structure):
+    """Mirror of struct cf_params."""
+
+    _fields_ = [
+        ("gp", C.c_double),
+        ("tolisec", C.c_double),
+        ("tol", C.c_double),
+        ("cfl", C.c_double),
+        ("ts", C.c_double),
+        ("constant_dt", C.c_double),
+        ("override_manning", C.c_double),
+        ("tol_reg_perm", C.c_double),
+        ("iflux", C.c_int),
+        ("ilimiteur", C.c_int),
+        ("friction", C.c_int),
+        ("fricimplic", C.c_int),
+        ("inlet", C.c_int),
+        ("outlet", C.c_int),
+        ("is_dt_constant", C.c_int),
+        ("is_override_manning", C.c_int),
+        ("nombre_entrees", C.c_int),
+        ("reserved0", C.c_int),
+        ("debitglob", C.c_double * CF_MAX_IO),
+        ("h_sortie", C.c_double * CF_MAX_IO),
+        ("long_entree", C.c_double * CF_MAX_IO),
+        ("hydro_n", C.c_int * CF_MAX_IO),
+        ("hydro_t", C.c_double * (CF_MAX_IO * CF_MAX_HYDRO)),
+        ("hydro_q", C.c_double * (CF_MAX_IO * CF_MAX_HYDRO)),
+    ]
+
+
+class CfMesh(C.Structure):
+    """Mirror of struct cf_mesh."""
+
+    _fields_ = [
+        ("nelt", C.c_int),
+        ("nnt", C.c_int),
+        ("conn", C.POINTER(C.c_int)),
+        ("coords", C.POINTER(C.c_double)),
+        ("boundary", C.POINTER(C.c_int)),
+        ("manning", C.POINTER(C.c_double)),
+    ]
+
+
+# exported symbol -> (restype, argtypes)
+_DP = C.POINTER(C.c_double)
+_IP = C.POINTER(C.c_int)
+_H = C.c_void_p
+SIGNATURES = {
+    "cf_build_boundary": (C.c_int, [C.c_int, C.c_int, _IP, C.c_int, _IP, _IP, C.c_int, _IP, _IP,
+                                    C.c_int, _IP, _IP]),
+    "cf_inlet_lengths": (C.c_int, [C.c_int, C.c_int, _IP, _DP, _IP, C.c_int, _DP]),
+    "cf_create": (C.c_int, [C.POINTER(CfMesh), C.POINTER(CfParams), C.POINTER(_H)]),
+    "cf_destroy": (None, [_H]),
+    "cf_error_string": (C.c_char_p, [C.c_int]),
+    "cf_set_state": (C.c_int, [_H, _DP]),
+    "cf_get_state": (C.c_int, [_H, _DP]),
+    "cf_set_state_device": (C.c_int, [_H, C.c_void_p]),
+    "cf_get_state_device": (C.c_int, [_H, C.c_void_p]),
+    "cf_initial_dt": (C.c_int, [_H, _DP]),
+    "cf_get_time": (C.c_int, [_H, _DP, _DP]),
+    "cf_step": (C.c_int, [_H, C.c_int, _DP, _IP]),
+    "cf_step_async": (C.c_int, [_H, C.c_int]),
+    "cf_step_finish": (C.c_int, [_H, _DP, _IP]),
+    "cf_set_inflow": (C.c_int, [_H, _DP, C.c_int]),
+    "cf_set_hydrograph": (C.c_int, [_H, C.c_int, C.c_int, _DP, _DP]),
+    "cf_set_manning": (C.c_int, [_H, _DP]),
+    "cf_get_geometry": (C.c_int, [_H, _DP, _DP, _DP, _DP]),
+    "cf_get_stream": (C.c_void_p, [_H]),
+    "cf_set_stream": (C.c_int, [_H, C.c_void_p]),
+    "cf_debug_pow": (C.c_int, [C.c_int, _DP, C.c_double, _DP]),
+}
+
+_lib = None
+
+
+class CuteflowError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        msg = load().cf_error_string(status).decode() if _lib is not None else str(status)
+        super().__init__(f"{what}: {msg} (status {status})")
+
+
+def load(path: str | None = None):
+    """Load libcuteflow_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status: int, what: str) -> None:
+    if status != CF_OK:
+        raise CuteflowError(status, what)
+
+
+def dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_DP)
+
+
+def iptr(a: np.ndarray):
+    assert a.dtype == np.int32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_IP)

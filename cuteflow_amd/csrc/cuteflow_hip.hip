@@ -1,0 +1,1047 @@
+// MI355X-native finite-volume St-Venant time step (FP64, unstructured
+// triangles) behind the C ABI of include/cuteflow_hip.h.
+//
+// One fused kernel per time step replaces the reference's compute_flux,
+// compute_source_terms, update_sol_with_flux_and_source, cflcond and
+// reduction kernels (flux_riemann.cuf:4-429, source_terms.cuf:4-213,
+// finite_volumes_method.cuf:260-305, cfl_condition.cuf:4-104):
+//   - one thread per cell gathers its three faces (no atomics, no
+//     intermediate residual arrays in HBM);
+//   - the new state goes to the other half of a ping-pong buffer (no
+//     vdlg0 <- vdlg copy, finite_volumes_method.cuf:230);
+//   - the CFL dt of the new state is reduced with wavefront shuffles, a
+//     workgroup LDS pass and one 64-bit atomic min per workgroup (dt > 0, so
+//     the IEEE bit pattern orders like an unsigned integer); the last
+//     workgroup to finish publishes dt, tc and the stop/error status for the
+//     next step, so a whole run of steps needs no host synchronisation.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "../../include/cuteflow_hip.h"
+#include "cf_device.h"
+
+#pragma clang fp contract(off)
+
+using namespace cf;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+
+// geometry arrays (structure of arrays, each nelt doubles)
+enum { G_N1 = 0, G_N2 = 3, G_LN = 6, G_SURF = 9, G_COT = 10, G_XC = 11, G_YC = 12, G_COUNT = 13 };
+
+struct KParams {
+    double gp, tolisec, tol, cfl, ts, constant_dt, override_manning;
+    double pad_dt;  // min(1e6, 1e5 if nelt % 1024 != 0): reduction padding
+    int ilimiteur, inlet, outlet, is_dt_constant, is_override_manning;
+    int reserved;
+    double debitglob[CF_MAX_IO];
+    double h_sortie[CF_MAX_IO];
+    double long_entree[CF_MAX_IO];
+    int hydro_n[CF_MAX_IO];
+};
+
+struct Ctrl {
+    int *hdr;                  // [0] status, [1] nan flag, [2] steps done
+    unsigned *counter;         // per step: finished workgroups
+    unsigned long long *dtbits;  // per step: atomic-min target of the next dt
+    double *dt;                // dt[k] used by step k
+    double *tc;                // tc[k] at the start of step k
+};
+
+struct StepArgs {
+    const double4 *__restrict__ st_in;
+    double4 *__restrict__ st_out;
+    const int4 *__restrict__ nbr;
+    const double *__restrict__ geo;
+    const double4 *__restrict__ dfc;
+    const double *__restrict__ manning;
+    const double *__restrict__ hydro;  // t[20*64] then q[20*64]
+    Ctrl c;
+    int nelt;
+    int step;
+    KParams p;
+};
+
+__device__ double inflow_q(const KParams &p, const double *hydro, int id, double tc) {
+    int n = p.hydro_n[id];
+    if (n <= 0) return p.debitglob[id];
+    const double *t = hydro + id * CF_MAX_HYDRO;
+    const double *q = hydro + CF_MAX_IO * CF_MAX_HYDRO + id * CF_MAX_HYDRO;
+    if (tc <= t[0]) return q[0];
+    for (int i = 0; i + 1 < n; i++)
+        if (tc < t[i + 1]) return q[i] + (q[i + 1] - q[i]) * ((tc - t[i]) / (t[i + 1] - t[i]));
+    return q[n - 1];
+}
+
+__device__ __forceinline__ int nb_at(const int4 &v, int j) { return j == 0 ? v.x : (j == 1 ? v.y : v.z); }
+__device__ __forceinline__ double d4_at(const double4 &v, int j) { return j == 0 ? v.x : (j == 1 ? v.y : v.z); }
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v = dmin(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// FRIC: 0 no friction, 1 explicit (fricimplic 0), 2 fricimplic 1, 3 fricimplic 2
+template <int IFLUX, int FRIC>
+__global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
+    __shared__ double s_min[kWaves];
+    if (a.c.hdr[0] != 0) return;  // stopped (ts reached) or failed earlier
+    const int nelt = a.nelt;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const double dt = a.c.dt[a.step];
+    const double tc = a.c.tc[a.step];
+    const KParams &p = a.p;
+    const double g = p.gp, tls = p.tolisec;
+    double dti = 1000000.0;
+    int isnan_local = 0;
+
+    if (i < nelt) {
+        const double4 u0 = a.st_in[i];
+        const int4 nb = a.nbr[i];
+        const double zml = u0.w;
+        double4 dfi = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (IFLUX == 3) dfi = a.dfc[i];
+        double fl[3][3], fn1[3], fn2[3], fln[3], fzr[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const double n1 = a.geo[(G_N1 + j) * nelt + i];
+            const double n2 = a.geo[(G_N2 + j) * nelt + i];
+            const double ln = a.geo[(G_LN + j) * nelt + i];
+            fn1[j] = n1; fn2[j] = n2; fln[j] = ln; fzr[j] = 0.0;
+            const int code = nb_at(nb, j);
+            double hautl, vitul, vitvl;
+            rot(tls, n1, n2, u0.x, u0.y, u0.z, hautl, vitul, vitvl);
+            double flux[3] = {0.0, 0.0, 0.0};
+            int kv = -1;
+            if (code >= 0) {
+                kv = code;
+            } else {
+                const int cls = (-code) & 3, id = (-code) >> 2;
+                if (cls == 1) {  // inlet (flux_riemann.cuf:115-139)
+                    if (p.inlet == 1) {
+                        flux[0] = hautl * vitul;
+                        flux[1] = n1 * (hautl * (vitul * vitul) + (g * (hautl * hautl)) / 2.0) - ((n2 * hautl) * vitul) * vitvl;
+                        flux[2] = n2 * (hautl * (vitul * vitul) + (g * (hautl * hautl)) / 2.0) + ((n1 * hautl) * vitul) * vitvl;
+                    } else if (p.inlet == 2) {
+                        const double dn = -(inflow_q(p, a.hydro, id - 1, tc) / p.long_entree[id - 1]);
+                        flux[0] = dn;
+                        flux[1] = n1 * ((dn * dn) / (hautl + tls) + (g * (hautl * hautl)) / 2.0);
+                        flux[2] = n2 * ((dn * dn) / (hautl + tls) + (g * (hautl * hautl)) / 2.0);
+                    }
+                } else if (cls == 3) {  // wall (:141-145)
+                    flux[1] = ((n1 * g) * (hautl * hautl)) / 2.0;
+                    flux[2] = ((n2 * g) * (hautl * hautl)) / 2.0;
+                } else if (cls == 2) {
+                    if (p.outlet == 1) {  // fixed level (:147-163)
+                        if (vitul >= 0.0 && vitul / sqrt(g * hautl) < 1.0) {
+                            hautl = dmax(0.0, p.h_sortie[id - 1] - zml);
+                            flux[0] = hautl * vitul;
+                            flux[1] = n1 * (hautl * (vitul * vitul) + (g * (hautl * hautl)) / 2.0) - ((n2 * hautl) * vitul) * vitvl;
+                            flux[2] = n2 * (hautl * (vitul * vitul) + (g * (hautl * hautl)) / 2.0) + ((n1 * hautl) * vitul) * vitvl;
+                        } else {
+                            flux[1] = ((n1 * g) * (hautl * hautl)) / 2.0;
+                            flux[2] = ((n2 * g) * (hautl * hautl)) / 2.0;
+                        }
+                    } else {  // transmissive: Riemann problem with itself (:166-171)
+                        kv = i;
+                    }
+                }
+            }
+            if (kv >= 0) {  // interior face (:166-409)
+                const double4 ur4 = a.st_in[kv];
+                const double zmr = ur4.w;
+                fzr[j] = zmr;
+                double zij = dmax(zml, zmr);
+                double hijet = dmax(0.0, (hautl + zml) - zij);
+                double hjiet = dmax(0.0, (ur4.x + zmr) - zij);
+                double hautr, vitur, vitvr;
+                rot(tls, n1, n2, ur4.x, ur4.y, ur4.z, hautr, vitur, vitvr);
+                if (IFLUX == 1) {
+                    if (hijet < tls) { hijet = 0.0; vitul = 0.0; vitvl = 0.0; }
+                    if (hjiet < tls) { hjiet = 0.0; vitur = 0.0; vitvr = 0.0; }
+                    hllc_flux(g, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr, flux);
+                } else {
+                    const double d_mrl = d4_at(dfi, j);
+                    const int ja = (j + 1) % 3, jb = (j + 2) % 3;
+                    const int c1 = nb_at(nb, ja), c2 = nb_at(nb, jb);
+                    const int kv1 = c1 >= 0 ? c1 : i, kv2 = c2 >= 0 ? c2 : i;
+                    const double4 s1 = a.st_in[kv1], s2 = a.st_in[kv2];
+                    double h1, u1, w1, h2, u2, w2;
+                    rot(tls, n1, n2, s1.x, s1.y, s1.z, h1, u1, w1);
+                    rot(tls, n1, n2, s2.x, s2.y, s2.z, h2, u2, w2);
+                    const int tjv = (nb.w >> (2 * j)) & 3;
+                    const int4 nbk = (kv == i) ? nb : a.nbr[kv];
+                    const int ka = (tjv + 1) % 3, kb = (tjv + 2) % 3;
+                    const int e1 = nb_at(nbk, ka), e2 = nb_at(nbk, kb);
+                    const int kw1 = e1 >= 0 ? e1 : kv, kw2 = e2 >= 0 ? e2 : kv;
+                    const double4 g1 = a.st_in[kw1], g2 = a.st_in[kw2];
+                    double g1h, g1u, g1w, g2h, g2u, g2w;
+                    rot(tls, n1, n2, g1.x, g1.y, g1.z, g1h, g1u, g1w);
+                    rot(tls, n1, n2, g2.x, g2.y, g2.z, g2h, g2u, g2w);
+                    double hl_upw, vl_upw, hr_upw, vr_upw, zml_upw, zmr_upw, d_mkvol, d_mkvoisin;
+                    if ((kv1 == i && kv2 == i) || (kw1 == kv && kw2 == kv)) {
+                        hl_upw = hautl; vl_upw = vitvl;
+                        hr_upw = hautr; vr_upw = vitvr;
+                        d_mkvol = d_mrl; d_mkvoisin = d_mrl;
+                        zml_upw = zml; zmr_upw = zmr;
+                    } else {
+                        if (kv1 == i) {
+                            hl_upw = h2; vl_upw = w2;
+                            d_mkvol = d4_at(dfi, jb);
+                            zml_upw = s2.w;
+                        } else if (kv2 == i) {
+                            hl_upw = h1; vl_upw = w1;
+                            d_mkvol = d4_at(dfi, ja);
+                            zml_upw = s1.w;
+                        } else {
+                            const double d1 = d4_at(dfi, ja), d2 = d4_at(dfi, jb);
+                            hl_upw = (d1 * h1 + d2 * h2) / (d1 + d2);
+                            vl_upw = (d1 * w1 + d2 * w2) / (d1 + d2);
+                            zml_upw = (d1 * s1.w + d2 * s2.w) / (d1 + d2);
+                            d_mkvol = (d1 + d2) / 2.0;
+                        }
+                        const double4 dfk = (kv == i) ? dfi : a.dfc[kv];
+                        if (kw1 == kv) {
+                            hr_upw = g2h; vr_upw = g2w;
+                            d_mkvoisin = d4_at(dfk, kb);
+                            zmr_upw = g2.w;
+                        } else if (kw2 == kv) {
+                            hr_upw = g1h; vr_upw = g1w;
+                            d_mkvoisin = d4_at(dfk, ka);
+                            zmr_upw = g1.w;
+                        } else {
+                            const double d1 = d4_at(dfk, ka), d2 = d4_at(dfk, kb);
+                            hr_upw = (d1 * g1h + d2 * g2h) / (d1 + d2);
+                            vr_upw = (d1 * g1w + d2 * g2w) / (d1 + d2);
+                            zmr_upw = (d1 * g1.w + d2 * g2.w) / (d1 + d2);
+                            d_mkvoisin = (d1 + d2) / 2.0;
+                        }
+                    }
+                    // hydrostatic reconstruction (flux_riemann.cuf:370-400)
+                    zij = dmax(zml, zmr);
+                    hijet = dmax(0.0, (hautl + zml) - zij);
+                    hl_upw = dmax(0.0, (hl_upw + zml_upw) - zij);
+                    hjiet = dmax(0.0, (hautr + zmr) - zij);
+                    hr_upw = dmax(0.0, (hr_upw + zmr_upw) - zij);
+                    if (hijet < tls) { hijet = 0.0; vitul = 0.0; vitvl = 0.0; }
+                    if (hl_upw < tls) { hl_upw = 0.0; vl_upw = 0.0; }
+                    if (hjiet < tls) { hjiet = 0.0; vitur = 0.0; vitvr = 0.0; }
+                    if (hr_upw < tls) { hr_upw = 0.0; vr_upw = 0.0; }
+                    waf_flux(g, p.ilimiteur, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr,
+                             hl_upw, vl_upw, hr_upw, vr_upw, d_mrl, d_mkvol, d_mkvoisin, dt, flux);
+                }
+            }
+            fl[j][0] = flux[0] * ln;
+            fl[j][1] = flux[1] * ln;
+            fl[j][2] = flux[2] * ln;
+        }
+        // face sum (flux_riemann.cuf:411-426)
+        const double resc0 = (fl[0][0] + fl[1][0]) + fl[2][0];
+        const double resc1 = (fl[0][1] + fl[1][1]) + fl[2][1];
+        const double resc2 = (fl[0][2] + fl[1][2]) + fl[2][2];
+
+        // bed slope source over interior faces (source_terms.cuf:50-100)
+        const double q1 = u0.x, q2 = u0.y, q3 = u0.z;
+        double sb1 = 0.0, sb2 = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            if (nb_at(nb, j) >= 0) {
+                const double zij = dmax(zml, fzr[j]);
+                const double hijet = dmax(0.0, (q1 + zml) - zij);
+                sb1 = sb1 + (((g / 2.0) * (hijet * hijet - q1 * q1)) * fln[j]) * fn1[j];
+                sb2 = sb2 + (((g / 2.0) * (hijet * hijet - q1 * q1)) * fln[j]) * fn2[j];
+            }
+        }
+        const double surf = a.geo[G_SURF * nelt + i];
+        double r0, r1, r2;
+        if (FRIC == 0) {
+            // update without friction (finite_volumes_method.cuf:294)
+            r0 = (0.0 - resc0) / surf;
+            r1 = (sb1 - resc1) / surf;
+            r2 = (sb2 - resc2) / surf;
+        } else {
+            // friction (source_terms.cuf:102-209)
+            const double tol = p.tol;
+            const double mann = p.is_override_manning ? p.override_manning : a.manning[i];
+            double sf1 = 0.0, sf2 = 0.0, vitu = 0.0, vitv = 0.0, normvit = 0.0;
+            if (q1 > tls) {
+                vitu = q2 / q1;
+                vitv = q3 / q1;
+                normvit = sqrt(vitu * vitu + vitv * vitv);
+                if (normvit < tol) normvit = tol;
+                const double p13 = cf_pow(q1, -1.0 / 3.0);
+                sf1 = -(((((g * mann) * mann) * vitu) * normvit) * surf * p13);
+                sf2 = -(((((g * mann) * mann) * vitv) * normvit) * surf * p13);
+            }
+            const double x0 = ((0.0 + 0.0) - resc0) / surf;
+            const double x1 = ((sb1 + sf1) - resc1) / surf;
+            const double x2 = ((sb2 + sf2) - resc2) / surf;
+            double a0 = 1.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 1.0, a5 = 0.0, a6 = 0.0, a7 = 0.0, a8 = 1.0;
+            if (FRIC == 2 && normvit > tol && q1 > tls) {
+                const double dtgn2h = ((((dt / 2.0) * g) * mann) * mann) * cf_pow(q1, -4.0 / 3.0);
+                const double f21 = -((((7.0 / 3.0) * dtgn2h) * vitu) * normvit);
+                const double f22 = 1.0 + (dtgn2h * (2.0 * (vitu * vitu) + vitv * vitv)) / normvit;
+                const double f23 = ((dtgn2h * vitu) * vitv) / normvit;
+                const double f31 = -((((7.0 / 3.0) * dtgn2h) * vitv) * normvit);
+                const double f32 = f23;
+                const double f33 = 1.0 + (dtgn2h * (vitu * vitu + 2.0 * (vitv * vitv))) / normvit;
+                const double det = f22 * f33 - f23 * f32;
+                a3 = -(f21 * f33 - f23 * f31) / det;
+                a4 = f33 / det;
+                a5 = -f23 / det;
+                a6 = (f21 * f32 - f22 * f31) / det;
+                a7 = -f32 / det;
+                a8 = f22 / det;
+            } else if (FRIC == 3 && normvit > tol && q1 > tls) {
+                const double beta = -(((((dt * g) * mann) * mann) * normvit) * cf_pow(q1, -4.0 / 3.0));
+                const double det = (1.0 - beta) * (1.0 - beta);
+                a4 = (1.0 - beta) / det;
+                a8 = (1.0 - beta) / det;
+            }
+            if (FRIC == 1) {
+                r0 = x0; r1 = x1; r2 = x2;
+            } else {
+                // res = afm1 * resini (finite_volumes_method.cuf:286-289)
+                r0 = (a0 * x0 + a1 * x1) + a2 * x2;
+                r1 = (a3 * x0 + a4 * x1) + a5 * x2;
+                r2 = (a6 * x0 + a7 * x1) + a8 * x2;
+            }
+        }
+        // update and wet/dry clamp (finite_volumes_method.cuf:297-303)
+        double h = u0.x + dt * r0;
+        double hu = u0.y + dt * r1;
+        double hv = u0.z + dt * r2;
+        if (h <= tls) { h = 0.0; hu = 0.0; hv = 0.0; }
+        a.st_out[i] = make_double4(h, hu, hv, zml);
+        isnan_local = (h != h) || (hu != hu) || (hv != hv);
+        // cflcond on the new state (cfl_condition.cuf:20-36)
+        if (h > tls) {
+            const double uu = hu / h, vv = hv / h;
+            const double nv = sqrt(uu * uu + vv * vv);
+            const double vmax = dmax(0.01, nv + sqrt(g * fabs(h)));
+            dti = dmin((p.cfl * a.geo[G_COT * nelt + i]) / vmax, 1000000.0);
+        }
+        if (p.is_dt_constant == 1) dti = p.constant_dt;
+    }
+
+    // dt reduction: wavefront shuffles, LDS across waves, one atomic per group
+    double m = wave_min(dti);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) s_min[wid] = m;
+    const int anynan = __syncthreads_or(isnan_local);
+    if (threadIdx.x == 0) {
+        double bm = s_min[0];
+#pragma unroll
+        for (int w = 1; w < kWaves; w++) bm = dmin(bm, s_min[w]);
+        atomicMin(&a.c.dtbits[a.step + 1], (unsigned long long)__double_as_longlong(bm));
+        if (anynan) atomicOr(&a.c.hdr[1], 1);
+        __threadfence();
+        const unsigned ticket = atomicAdd(&a.c.counter[a.step], 1u);
+        if (ticket == gridDim.x - 1) {
+            // last workgroup: publish dt, tc and status for the next step
+            __threadfence();
+            const unsigned long long bits = atomicAdd(&a.c.dtbits[a.step + 1], 0ull);
+            const double dtn = __longlong_as_double((long long)bits);
+            const int nanf = atomicAdd(&a.c.hdr[1], 0);
+            const double tcn = tc + dt;
+            a.c.dt[a.step + 1] = dtn;
+            a.c.tc[a.step + 1] = tcn;
+            a.c.hdr[2] = a.step + 1;
+            if (nanf || !(dtn <= 9999.0 && dtn >= kDtLow)) a.c.hdr[0] = -1;
+            else if (!(((tcn - p.ts) + 2.0 * p.tol) < p.tol)) a.c.hdr[0] = 1;
+            __threadfence();
+        }
+    }
+}
+
+// Prepares the control block for a run of nsteps steps.
+__global__ void ctrl_init_kernel(Ctrl c, int nsteps, double dt0, double tc0, double pad,
+                                 double ts, double tol) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long padbits = (unsigned long long)__double_as_longlong(pad);
+    for (int k = t; k < nsteps; k += gridDim.x * blockDim.x) {
+        c.counter[k] = 0u;
+        c.dtbits[k + 1] = padbits;
+    }
+    if (t == 0) {
+        c.dt[0] = dt0;
+        c.tc[0] = tc0;
+        c.hdr[1] = 0;
+        c.hdr[2] = 0;
+        c.hdr[0] = (((tc0 - ts) + 2.0 * tol) < tol) ? 0 : 1;
+    }
+}
+
+// cflcond + reduction for the current state (finite_volumes_method.cuf:113-116)
+__global__ __launch_bounds__(kBlock) void cfl_kernel(const double4 *__restrict__ st,
+                                                      const double *__restrict__ cot, int nelt,
+                                                      KParams p, unsigned long long *out) {
+    __shared__ double s_min[kWaves];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    double dti = 1000000.0;
+    if (i < nelt) {
+        const double4 u = st[i];
+        const double h = u.x;
+        if (h > p.tolisec) {
+            const double uu = u.y / h, vv = u.z / h;
+            const double nv = sqrt(uu * uu + vv * vv);
+            const double vmax = dmax(0.01, nv + sqrt(p.gp * fabs(h)));
+            dti = dmin((p.cfl * cot[i]) / vmax, 1000000.0);
+        }
+        if (p.is_dt_constant == 1) dti = p.constant_dt;
+    }
+    double m = wave_min(dti);
+    if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double bm = s_min[0];
+        for (int w = 1; w < kWaves; w++) bm = dmin(bm, s_min[w]);
+        atomicMin(out, (unsigned long long)__double_as_longlong(bm));
+    }
+}
+
+__global__ void fill_u64_kernel(unsigned long long *p, unsigned long long v) { *p = v; }
+
+// Geometry of each cell (maillage.cuf:132-159, :1254-1331, flux_riemann.cuf:50-74).
+__global__ void geometry_kernel(const int *__restrict__ conn, const double *__restrict__ coords,
+                                int nelt, double *__restrict__ geo, double *__restrict__ zm_out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nelt) return;
+    const int s1 = conn[3 * i] - 1, s2 = conn[3 * i + 1] - 1, s3 = conn[3 * i + 2] - 1;
+    const double vx[3] = {coords[3 * s1], coords[3 * s2], coords[3 * s3]};
+    const double vy[3] = {coords[3 * s1 + 1], coords[3 * s2 + 1], coords[3 * s3 + 1]};
+    const double vz[3] = {coords[3 * s1 + 2], coords[3 * s2 + 2], coords[3 * s3 + 2]};
+    const double x1 = vx[0], y1 = vy[0], x2 = vx[1], y2 = vy[1], x3 = vx[2], y3 = vy[2];
+    geo[G_XC * nelt + i] = ((x1 + x2) + x3) / 3.0;
+    geo[G_YC * nelt + i] = ((y1 + y2) + y3) / 3.0;
+    {
+        const double ca = sqrt((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2));
+        const double cb = sqrt((x2 - x3) * (x2 - x3) + (y2 - y3) * (y2 - y3));
+        const double cc = sqrt((x3 - x1) * (x3 - x1) + (y3 - y1) * (y3 - y1));
+        const double pp = ((ca + cb) + cc) / 2.0;
+        const double aire = sqrt(((pp * (pp - ca)) * (pp - cb)) * (pp - cc));
+        double cot = (0.5 * aire) / pp;
+        geo[G_COT * nelt + i] = kCote * cot;
+    }
+    {
+        const double aa = sqrt((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
+        const double bb = sqrt((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2));
+        const double cc = sqrt((x1 - x3) * (x1 - x3) + (y1 - y3) * (y1 - y3));
+        const double s = ((aa + bb) + cc) / 2.0;
+        geo[G_SURF * nelt + i] = sqrt(((s * (s - aa)) * (s - bb)) * (s - cc));
+        const double xg = ((x1 + x2) + x3) / 3.0, yg = ((y1 + y2) + y3) / 3.0;
+        const double d1 = sqrt((x1 - xg) * (x1 - xg) + (y1 - yg) * (y1 - yg));
+        const double d2 = sqrt((x2 - xg) * (x2 - xg) + (y2 - yg) * (y2 - yg));
+        const double d3 = sqrt((x3 - xg) * (x3 - xg) + (y3 - yg) * (y3 - yg));
+        zm_out[i] = ((d1 * vz[0] + d2 * vz[1]) + d3 * vz[2]) / ((d1 + d2) + d3);
+    }
+    for (int j = 0; j < 3; j++) {
+        const int c1 = j, c2 = (j + 1) % 3, c3 = (j + 2) % 3;
+        const double ax = vx[c1], ay = vy[c1], bx = vx[c2], by = vy[c2], cx = vx[c3], cy = vy[c3];
+        const double l = sqrt((bx - ax) * (bx - ax) + (by - ay) * (by - ay));
+        double n1 = (by - ay) / l, n2 = (ax - bx) / l;
+        const double orient = n1 * ((ax + bx) / 2.0 - ((ax + bx) + cx) / 3.0) +
+                              n2 * ((ay + by) / 2.0 - ((ay + by) + cy) / 3.0);
+        if (orient < 0.0) {
+            n1 = (ay - by) / l;
+            n2 = (bx - ax) / l;
+        }
+        geo[(G_N1 + j) * nelt + i] = n1;
+        geo[(G_N2 + j) * nelt + i] = n2;
+        geo[(G_LN + j) * nelt + i] = l;
+    }
+}
+
+// Centroid distance across each face (0 across a boundary).
+__global__ void face_dist_kernel(const int4 *__restrict__ nbr, const double *__restrict__ geo,
+                                 int nelt, double4 *__restrict__ dfc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nelt) return;
+    const int4 nb = nbr[i];
+    const double xi = geo[G_XC * nelt + i], yi = geo[G_YC * nelt + i];
+    double d[3];
+    for (int j = 0; j < 3; j++) {
+        const int k = nb_at(nb, j);
+        const int kk = k >= 0 ? k : i;
+        const double xk = geo[G_XC * nelt + kk], yk = geo[G_YC * nelt + kk];
+        d[j] = sqrt((xi - xk) * (xi - xk) + (yi - yk) * (yi - yk));
+    }
+    dfc[i] = make_double4(d[0], d[1], d[2], 0.0);
+}
+
+__global__ void pack_state_kernel(const double *__restrict__ U, const double *__restrict__ zm,
+                                  int nelt, double4 *__restrict__ st) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nelt) st[i] = make_double4(U[3 * i], U[3 * i + 1], U[3 * i + 2], zm[i]);
+}
+
+__global__ void unpack_state_kernel(const double4 *__restrict__ st, int nelt, double *__restrict__ U) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nelt) {
+        const double4 s = st[i];
+        U[3 * i] = s.x;
+        U[3 * i + 1] = s.y;
+        U[3 * i + 2] = s.z;
+    }
+}
+
+__global__ void pow_kernel(const double *__restrict__ x, double y, int n, double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = cf_pow(x[i], y);
+}
+
+}  // namespace
+
+struct cf_handle {
+    int nelt = 0, nnt = 0;
+    cf_params prm{};
+    KParams kp{};
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    double4 *st[2] = {nullptr, nullptr};
+    int cur = 0;
+    int4 *nbr = nullptr;
+    double *geo = nullptr;
+    double4 *dfc = nullptr;
+    double *manning = nullptr;
+    double *hydro = nullptr;
+    double *zm = nullptr;
+    double *stage = nullptr;
+    Ctrl c{};
+    int cap = 0;
+    unsigned long long *dt0bits = nullptr;
+    double dt = 0.0, tc = 0.0;
+    int status = 0;
+    int fric = 0;
+    int pending = -1;  // steps enqueued by cf_step_async, not yet finished
+};
+
+#define HIPCHK(x)                                   \
+    do {                                            \
+        hipError_t e_ = (x);                        \
+        if (e_ != hipSuccess) {                     \
+            fprintf(stderr, "cuteflow_hip: %s failed: %s\n", #x, hipGetErrorString(e_)); \
+            return CF_ERR_HIP;                      \
+        }                                           \
+    } while (0)
+
+static int grid_of(int n, int b) { return (n + b - 1) / b; }
+
+template <typename T>
+static void dfree(T *&ptr) {
+    if (ptr) (void)hipFree((void *)ptr);
+    ptr = nullptr;
+}
+
+static void fill_kparams(cf_handle *h) {
+    const cf_params &p = h->prm;
+    KParams &k = h->kp;
+    k.gp = p.gp; k.tolisec = p.tolisec; k.tol = p.tol; k.cfl = p.cfl; k.ts = p.ts;
+    k.constant_dt = p.constant_dt; k.override_manning = p.override_manning;
+    k.pad_dt = (h->nelt % 1024 != 0) ? 100000.0 : 1000000.0;
+    k.ilimiteur = p.ilimiteur; k.inlet = p.inlet; k.outlet = p.outlet;
+    k.is_dt_constant = p.is_dt_constant; k.is_override_manning = p.is_override_manning;
+    for (int i = 0; i < CF_MAX_IO; i++) {
+        k.debitglob[i] = p.debitglob[i];
+        k.h_sortie[i] = p.h_sortie[i];
+        k.long_entree[i] = p.long_entree[i];
+        k.hydro_n[i] = p.hydro_n[i];
+    }
+}
+
+static int upload_hydro(cf_handle *h) {
+    std::vector<double> buf(2 * CF_MAX_IO * CF_MAX_HYDRO);
+    memcpy(buf.data(), h->prm.hydro_t, sizeof(double) * CF_MAX_IO * CF_MAX_HYDRO);
+    memcpy(buf.data() + CF_MAX_IO * CF_MAX_HYDRO, h->prm.hydro_q, sizeof(double) * CF_MAX_IO * CF_MAX_HYDRO);
+    HIPCHK(hipMemcpyAsync(h->hydro, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CF_OK;
+}
+
+static int compute_dt(cf_handle *h) {
+    const unsigned long long init = (unsigned long long)__builtin_bit_cast(long long, h->kp.pad_dt);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, h->stream, h->dt0bits, init);
+    hipLaunchKernelGGL(cfl_kernel, dim3(grid_of(h->nelt, kBlock)), dim3(kBlock), 0, h->stream,
+                       h->st[h->cur], h->geo + G_COT * h->nelt, h->nelt, h->kp, h->dt0bits);
+    unsigned long long bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, h->dt0bits, sizeof(bits), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->dt = __builtin_bit_cast(double, bits);
+    return CF_OK;
+}
+
+static int ensure_ctrl(cf_handle *h, int nsteps) {
+    if (nsteps <= h->cap) return CF_OK;
+    int cap = std::max(nsteps, 2 * h->cap);
+    if (h->c.counter) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        dfree(h->c.counter); dfree(h->c.dtbits); dfree(h->c.dt); dfree(h->c.tc);
+    }
+    HIPCHK(hipMalloc(&h->c.counter, sizeof(unsigned) * (size_t)cap));
+    HIPCHK(hipMalloc(&h->c.dtbits, sizeof(unsigned long long) * ((size_t)cap + 1)));
+    HIPCHK(hipMalloc(&h->c.dt, sizeof(double) * ((size_t)cap + 1)));
+    HIPCHK(hipMalloc(&h->c.tc, sizeof(double) * ((size_t)cap + 1)));
+    h->cap = cap;
+    return CF_OK;
+}
+
+// ------------------------------------------------------------------------
+// Host preprocessing: directed-edge matching (maillage.cuf:881-914) and
+// boundary classification (maillage.cuf:981-1110).
+
+namespace {
+struct Edge {
+    int64_t key;
+    int cell;
+};
+inline int64_t ekey(int a, int b) { return ((int64_t)a << 32) | (uint32_t)b; }
+}  // namespace
+
+extern "C" int cf_build_boundary(int nelt, int nnt, const int *conn, int ndi, const int *ndinput,
+                                 const int *num_in, int ndo, const int *ndoutput, const int *num_out,
+                                 int ndw, const int *ndwall, int *boundary) {
+    if (nelt <= 0 || nnt <= 0 || !conn || !boundary) return CF_ERR_ARG;
+    const size_t N = (size_t)nelt;
+    for (size_t i = 0; i < 3 * N; i++)
+        if (conn[i] < 1 || conn[i] > nnt) return CF_ERR_MESH;
+    std::vector<Edge> ed(3 * N);
+    for (size_t i = 0; i < N; i++)
+        for (int k = 0; k < 3; k++) {
+            ed[3 * i + k].key = ekey(conn[3 * i + k], conn[3 * i + (k + 1) % 3]);
+            ed[3 * i + k].cell = (int)i;
+        }
+    std::sort(ed.begin(), ed.end(), [](const Edge &x, const Edge &y) {
+        return x.key != y.key ? x.key < y.key : x.cell < y.cell;
+    });
+    // boundary(i,k,1) at [k*N + i], boundary(i,k,2) at [3N + k*N + i]
+    for (size_t i = 0; i < N; i++)
+        for (int k = 0; k < 3; k++) {
+            const int a = conn[3 * i + k], b = conn[3 * i + (k + 1) % 3];
+            const int64_t want = ekey(b, a);
+            auto it = std::lower_bound(ed.begin(), ed.end(), want,
+                                       [](const Edge &x, int64_t v) { return x.key < v; });
+            int nb = (int)i + 1;
+            for (; it != ed.end() && it->key == want; ++it)
+                if (it->cell != (int)i) nb = it->cell + 1;  // highest matching id
+            boundary[k * N + i] = nb;
+            boundary[3 * N + k * N + i] = (int)i + 1;
+        }
+    // first position of each node in each list
+    auto positions = [&](int nd, const int *list) {
+        std::vector<int> pos((size_t)nnt + 1, -1);
+        for (int q = nd - 1; q >= 0; q--)
+            if (list[q] >= 1 && list[q] <= nnt) pos[list[q]] = q;
+        return pos;
+    };
+    std::vector<int> pi = positions(ndi, ndinput), po = positions(ndo, ndoutput), pw = positions(ndw, ndwall);
+    for (size_t i = 0; i < N; i++)
+        for (int k = 0; k < 3; k++) {
+            if (boundary[k * N + i] != (int)i + 1) continue;
+            const int n1 = conn[3 * i + k], n2 = conn[3 * i + (k + 1) % 3];
+            if (ndi > 0 && n1 != n2 && pi[n1] >= 0 && pi[n2] >= 0) {
+                const int q = std::min(pi[n1], pi[n2]);
+                boundary[k * N + i] = -1;
+                boundary[3 * N + k * N + i] = num_in ? num_in[q] : 1;
+                continue;
+            }
+            if (ndo > 0 && n1 != n2 && po[n1] >= 0 && po[n2] >= 0) {
+                const int q = std::min(po[n1], po[n2]);
+                boundary[k * N + i] = -2;
+                boundary[3 * N + k * N + i] = num_out ? num_out[q] : 1;
+                continue;
+            }
+            if (ndw > 0 && n1 != n2 && pw[n1] >= 0 && pw[n2] >= 0) {
+                boundary[k * N + i] = -3;
+                continue;
+            }
+            if (ndi > 0 && ndw > 0 && ((pw[n1] >= 0 && pi[n2] >= 0) || (pi[n1] >= 0 && pw[n2] >= 0)))
+                boundary[k * N + i] = -3;
+            if (ndo > 0 && ndw > 0 && ((pw[n1] >= 0 && po[n2] >= 0) || (po[n1] >= 0 && pw[n2] >= 0)))
+                boundary[k * N + i] = -3;
+        }
+    return CF_OK;
+}
+
+extern "C" int cf_inlet_lengths(int nelt, int nnt, const int *conn, const double *coords,
+                                const int *boundary, int nentrees, double *long_entree) {
+    if (nelt <= 0 || !conn || !coords || !boundary || !long_entree || nentrees < 0 || nentrees > CF_MAX_IO)
+        return CF_ERR_ARG;
+    const size_t N = (size_t)nelt;
+    for (int k = 0; k < nentrees; k++) long_entree[k] = 0.0;
+    for (size_t i = 0; i < N; i++)
+        for (int k = 0; k < 3; k++) {
+            if (boundary[k * N + i] != -1) continue;
+            const int id = boundary[3 * N + k * N + i];
+            const int n1 = conn[3 * i + k] - 1, n2 = conn[3 * i + (k + 1) % 3] - 1;
+            if (id < 1 || id > nentrees || n1 < 0 || n2 < 0 || n1 >= nnt || n2 >= nnt) continue;
+            const double dx = coords[3 * n1] - coords[3 * n2], dy = coords[3 * n1 + 1] - coords[3 * n2 + 1];
+            long_entree[id - 1] = long_entree[id - 1] + sqrt(dx * dx + dy * dy);
+        }
+    return CF_OK;
+}
+
+extern "C" const char *cf_error_string(int s) {
+    switch (s) {
+    case CF_OK: return "ok";
+    case CF_ERR_ARG: return "invalid argument or unsupported option";
+    case CF_ERR_HIP: return "HIP runtime error";
+    case CF_ERR_DT: return "time step out of range (dt > 9999, dt < 1e-12) or NaN state";
+    case CF_ERR_MESH: return "inconsistent mesh";
+    case CF_ERR_NOMEM: return "out of memory";
+    default: return "unknown status";
+    }
+}
+
+static int validate_params(const cf_params *p) {
+    if (!(p->gp > 0.0) || !(p->cfl > 0.0) || !(p->tolisec >= 0.0)) return CF_ERR_ARG;
+    if (p->iflux != 1 && p->iflux != 3) return CF_ERR_ARG;
+    if (p->inlet != 1 && p->inlet != 2) return CF_ERR_ARG;  // 'phys_inflow' not provided
+    if (p->outlet != 1 && p->outlet != 2) return CF_ERR_ARG;
+    if (p->friction != 0 && p->friction != 1) return CF_ERR_ARG;
+    if (p->fricimplic < 0 || p->fricimplic > 2) return CF_ERR_ARG;
+    if (p->is_dt_constant == 1 && !(p->constant_dt > 0.0)) return CF_ERR_ARG;
+    if (p->nombre_entrees < 0 || p->nombre_entrees > CF_MAX_IO) return CF_ERR_ARG;
+    for (int i = 0; i < CF_MAX_IO; i++)
+        if (p->hydro_n[i] < 0 || p->hydro_n[i] > CF_MAX_HYDRO) return CF_ERR_ARG;
+    return CF_OK;
+}
+
+static void destroy_buffers(cf_handle *h) {
+    dfree(h->st[0]); dfree(h->st[1]); dfree(h->nbr); dfree(h->geo); dfree(h->dfc);
+    dfree(h->manning); dfree(h->hydro); dfree(h->zm); dfree(h->stage);
+    dfree(h->c.hdr); dfree(h->c.counter); dfree(h->c.dtbits); dfree(h->c.dt); dfree(h->c.tc);
+    dfree(h->dt0bits);
+}
+
+extern "C" int cf_create(const cf_mesh *mesh, const cf_params *params, cf_handle **out) {
+    if (!mesh || !params || !out || mesh->nelt <= 0 || mesh->nnt <= 0 || !mesh->conn ||
+        !mesh->coords || !mesh->boundary)
+        return CF_ERR_ARG;
+    int st = validate_params(params);
+    if (st) return st;
+    *out = nullptr;
+    const int nelt = mesh->nelt;
+    const size_t N = (size_t)nelt;
+    if (N > (size_t)INT32_MAX / 4) return CF_ERR_ARG;
+    for (size_t i = 0; i < 3 * N; i++)
+        if (mesh->conn[i] < 1 || mesh->conn[i] > mesh->nnt) return CF_ERR_MESH;
+
+    // neighbour codes: >= 0 neighbour (0-based); < 0 -(class | id << 2);
+    // w packs, per face, the face of the neighbour that shares the edge.
+    std::vector<int4> nbr(N);
+    const int *b = mesh->boundary;
+    for (size_t i = 0; i < N; i++) {
+        int c[3];
+        for (int k = 0; k < 3; k++) {
+            int v = b[k * N + i];
+            const int id = b[3 * N + k * N + i];
+            if (v == (int)i + 1) v = -3;  // set_undefined_boudaries_as_walls
+            if (v > 0) {
+                if (v > nelt) return CF_ERR_MESH;
+                c[k] = v - 1;
+            } else if (v == -1 || v == -2) {
+                if (id < 1 || id > CF_MAX_IO) return CF_ERR_MESH;
+                if (v == -1 && params->inlet == 2 && !(params->long_entree[id - 1] > 0.0)) return CF_ERR_ARG;
+                c[k] = -((v == -1 ? 1 : 2) | (id << 2));
+            } else if (v == -3) {
+                c[k] = -3;
+            } else {
+                return CF_ERR_MESH;
+            }
+        }
+        int w = 0;
+        for (int j = 0; j < 3; j++) {
+            int tjv = j;  // boundary faces: the cell itself, same face
+            if (c[j] >= 0) {
+                const int A = mesh->conn[3 * i + j], B = mesh->conn[3 * i + (j + 1) % 3];
+                const int *q = mesh->conn + 3 * (size_t)c[j];
+                if ((q[0] == A && q[1] == B) || (q[1] == A && q[0] == B)) tjv = 0;
+                else if ((q[1] == A && q[2] == B) || (q[2] == A && q[1] == B)) tjv = 1;
+                else if ((q[2] == A && q[0] == B) || (q[0] == A && q[2] == B)) tjv = 2;
+                else tjv = 0;  // no matching face: tjv = 1 (1-based), see the oracle header
+            }
+            w |= tjv << (2 * j);
+        }
+        nbr[i] = make_int4(c[0], c[1], c[2], w);
+    }
+
+    cf_handle *h = new (std::nothrow) cf_handle();
+    if (!h) return CF_ERR_NOMEM;
+    h->nelt = nelt;
+    h->nnt = mesh->nnt;
+    h->prm = *params;
+    h->fric = params->friction == 0 ? 0 : 1 + params->fricimplic;
+    fill_kparams(h);
+    int rc = CF_OK;
+    int *dconn = nullptr;
+    double *dcoords = nullptr;
+    do {
+        if (hipStreamCreate(&h->stream) != hipSuccess) { rc = CF_ERR_HIP; break; }
+        h->own_stream = true;
+        if (hipMalloc(&h->st[0], N * sizeof(double4)) != hipSuccess ||
+            hipMalloc(&h->st[1], N * sizeof(double4)) != hipSuccess ||
+            hipMalloc(&h->nbr, N * sizeof(int4)) != hipSuccess ||
+            hipMalloc(&h->geo, G_COUNT * N * sizeof(double)) != hipSuccess ||
+            hipMalloc(&h->dfc, N * sizeof(double4)) != hipSuccess ||
+            hipMalloc(&h->manning, N * sizeof(double)) != hipSuccess ||
+            hipMalloc(&h->hydro, 2 * CF_MAX_IO * CF_MAX_HYDRO * sizeof(double)) != hipSuccess ||
+            hipMalloc(&h->zm, N * sizeof(double)) != hipSuccess ||
+            hipMalloc(&h->stage, 3 * N * sizeof(double)) != hipSuccess ||
+            hipMalloc(&h->c.hdr, 4 * sizeof(int)) != hipSuccess ||
+            hipMalloc(&h->dt0bits, sizeof(unsigned long long)) != hipSuccess ||
+            hipMalloc(&dconn, 3 * N * sizeof(int)) != hipSuccess ||
+            hipMalloc(&dcoords, 3 * (size_t)mesh->nnt * sizeof(double)) != hipSuccess) {
+            rc = CF_ERR_NOMEM;
+            break;
+        }
+        if (hipMemcpyAsync(dconn, mesh->conn, 3 * N * sizeof(int), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+            hipMemcpyAsync(dcoords, mesh->coords, 3 * (size_t)mesh->nnt * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+            hipMemcpyAsync(h->nbr, nbr.data(), N * sizeof(int4), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
+            hipMemsetAsync(h->c.hdr, 0, 4 * sizeof(int), h->stream) != hipSuccess) {
+            rc = CF_ERR_HIP;
+            break;
+        }
+        if (mesh->manning) {
+            if (hipMemcpyAsync(h->manning, mesh->manning, N * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess) { rc = CF_ERR_HIP; break; }
+        } else if (hipMemsetAsync(h->manning, 0, N * sizeof(double), h->stream) != hipSuccess) {
+            rc = CF_ERR_HIP;
+            break;
+        }
+        hipLaunchKernelGGL(geometry_kernel, dim3(grid_of(nelt, 256)), dim3(256), 0, h->stream, dconn, dcoords, nelt, h->geo, h->zm);
+        hipLaunchKernelGGL(face_dist_kernel, dim3(grid_of(nelt, 256)), dim3(256), 0, h->stream, h->nbr, h->geo, nelt, h->dfc);
+        // zero state with the bed level in the 4th component
+        if (hipMemsetAsync(h->stage, 0, 3 * N * sizeof(double), h->stream) != hipSuccess) { rc = CF_ERR_HIP; break; }
+        hipLaunchKernelGGL(pack_state_kernel, dim3(grid_of(nelt, 256)), dim3(256), 0, h->stream, h->stage, h->zm, nelt, h->st[0]);
+        hipLaunchKernelGGL(pack_state_kernel, dim3(grid_of(nelt, 256)), dim3(256), 0, h->stream, h->stage, h->zm, nelt, h->st[1]);
+        if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) { rc = CF_ERR_HIP; break; }
+        if ((rc = upload_hydro(h)) != CF_OK) break;
+        if ((rc = ensure_ctrl(h, 1024)) != CF_OK) break;
+        if ((rc = compute_dt(h)) != CF_OK) break;
+    } while (0);
+    dfree(dconn);
+    dfree(dcoords);
+    if (rc != CF_OK) {
+        destroy_buffers(h);
+        if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return CF_OK;
+}
+
+extern "C" void cf_destroy(cf_handle *h) {
+    if (!h) return;
+    (void)hipStreamSynchronize(h->stream);
+    destroy_buffers(h);
+    if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+extern "C" int cf_set_state_device(cf_handle *h, const double *dU) {
+    if (!h || !dU || h->pending >= 0) return CF_ERR_ARG;
+    hipLaunchKernelGGL(pack_state_kernel, dim3(grid_of(h->nelt, 256)), dim3(256), 0, h->stream, dU, h->zm, h->nelt, h->st[h->cur]);
+    HIPCHK(hipGetLastError());
+    h->tc = 0.0;
+    h->status = 0;
+    return compute_dt(h);
+}
+
+extern "C" int cf_set_state(cf_handle *h, const double *U) {
+    if (!h || !U) return CF_ERR_ARG;
+    HIPCHK(hipMemcpyAsync(h->stage, U, 3 * (size_t)h->nelt * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    return cf_set_state_device(h, h->stage);
+}
+
+extern "C" int cf_get_state_device(cf_handle *h, double *dU) {
+    if (!h || !dU || h->pending >= 0) return CF_ERR_ARG;
+    hipLaunchKernelGGL(unpack_state_kernel, dim3(grid_of(h->nelt, 256)), dim3(256), 0, h->stream, h->st[h->cur], h->nelt, dU);
+    HIPCHK(hipGetLastError());
+    return CF_OK;
+}
+
+extern "C" int cf_get_state(cf_handle *h, double *U) {
+    if (!h || !U) return CF_ERR_ARG;
+    int rc = cf_get_state_device(h, h->stage);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(U, h->stage, 3 * (size_t)h->nelt * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CF_OK;
+}
+
+extern "C" int cf_initial_dt(cf_handle *h, double *dt) {
+    if (!h || !dt) return CF_ERR_ARG;
+    *dt = h->dt;
+    return CF_OK;
+}
+
+extern "C" int cf_get_time(cf_handle *h, double *tc, double *dt) {
+    if (!h) return CF_ERR_ARG;
+    if (tc) *tc = h->tc;
+    if (dt) *dt = h->dt;
+    return CF_OK;
+}
+
+template <int IFLUX, int FRIC>
+static void launch_step(cf_handle *h, StepArgs &a, int grid) {
+    hipLaunchKernelGGL((step_kernel<IFLUX, FRIC>), dim3(grid), dim3(kBlock), 0, h->stream, a);
+}
+
+typedef void (*step_launcher)(cf_handle *, StepArgs &, int);
+
+static step_launcher pick_launcher(int iflux, int fric) {
+    static const step_launcher tab[2][4] = {
+        {launch_step<1, 0>, launch_step<1, 1>, launch_step<1, 2>, launch_step<1, 3>},
+        {launch_step<3, 0>, launch_step<3, 1>, launch_step<3, 2>, launch_step<3, 3>}};
+    return tab[iflux == 3 ? 1 : 0][fric];
+}
+
+extern "C" int cf_step_async(cf_handle *h, int nsteps) {
+    if (!h || nsteps < 0) return CF_ERR_ARG;
+    if (h->pending >= 0) return CF_ERR_ARG;  // previous run not finished
+    if (h->status < 0) return CF_ERR_DT;
+    int rc = ensure_ctrl(h, std::max(nsteps, 1));
+    if (rc) return rc;
+    h->pending = nsteps;
+    if (nsteps == 0) return CF_OK;
+    const int grid = grid_of(h->nelt, kBlock);
+    hipLaunchKernelGGL(ctrl_init_kernel, dim3(std::min(grid_of(nsteps, 256), 1024)), dim3(256), 0, h->stream,
+                       h->c, nsteps, h->dt, h->tc, h->kp.pad_dt, h->kp.ts, h->kp.tol);
+    StepArgs a;
+    a.geo = h->geo;
+    a.nbr = h->nbr;
+    a.dfc = h->dfc;
+    a.manning = h->manning;
+    a.hydro = h->hydro;
+    a.c = h->c;
+    a.nelt = h->nelt;
+    a.p = h->kp;
+    step_launcher L = pick_launcher(h->prm.iflux, h->fric);
+    for (int k = 0; k < nsteps; k++) {
+        a.step = k;
+        a.st_in = h->st[(h->cur + k) & 1];
+        a.st_out = h->st[(h->cur + k + 1) & 1];
+        L(h, a, grid);
+    }
+    HIPCHK(hipGetLastError());
+    return CF_OK;
+}
+
+extern "C" int cf_step_finish(cf_handle *h, double *dt_hist, int *steps_done) {
+    if (!h) return CF_ERR_ARG;
+    if (steps_done) *steps_done = 0;
+    if (h->pending < 0) return CF_ERR_ARG;
+    const int nsteps = h->pending;
+    h->pending = -1;
+    if (nsteps == 0) {
+        if (dt_hist) dt_hist[0] = h->dt;
+        return CF_OK;
+    }
+    int hdr[4];
+    HIPCHK(hipMemcpyAsync(hdr, h->c.hdr, sizeof(hdr), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const int done = hdr[2];
+    std::vector<double> dts((size_t)done + 1), tcs(1);
+    HIPCHK(hipMemcpyAsync(dts.data(), h->c.dt, sizeof(double) * ((size_t)done + 1), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(tcs.data(), h->c.tc + done, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (dt_hist) memcpy(dt_hist, dts.data(), sizeof(double) * ((size_t)done + 1));
+    h->cur = (h->cur + done) & 1;
+    h->dt = dts[done];
+    h->tc = tcs[0];
+    if (steps_done) *steps_done = done;
+    if (hdr[0] < 0) {
+        h->status = -1;
+        return CF_ERR_DT;
+    }
+    return CF_OK;
+}
+
+extern "C" int cf_step(cf_handle *h, int nsteps, double *dt_hist, int *steps_done) {
+    if (steps_done) *steps_done = 0;
+    int rc = cf_step_async(h, nsteps);
+    if (rc) return rc;
+    return cf_step_finish(h, dt_hist, steps_done);
+}
+
+extern "C" int cf_set_inflow(cf_handle *h, const double *debitglob, int n) {
+    if (!h || !debitglob || n < 0 || n > CF_MAX_IO) return CF_ERR_ARG;
+    for (int i = 0; i < n; i++) h->prm.debitglob[i] = debitglob[i];
+    fill_kparams(h);
+    return CF_OK;
+}
+
+extern "C" int cf_set_hydrograph(cf_handle *h, int inlet, int npts, const double *t, const double *q) {
+    if (!h || inlet < 1 || inlet > CF_MAX_IO || npts < 0 || npts > CF_MAX_HYDRO || (npts > 0 && (!t || !q)))
+        return CF_ERR_ARG;
+    for (int i = 0; i + 1 < npts; i++)
+        if (!(t[i + 1] > t[i])) return CF_ERR_ARG;
+    h->prm.hydro_n[inlet - 1] = npts;
+    for (int i = 0; i < npts; i++) {
+        h->prm.hydro_t[(inlet - 1) * CF_MAX_HYDRO + i] = t[i];
+        h->prm.hydro_q[(inlet - 1) * CF_MAX_HYDRO + i] = q[i];
+    }
+    fill_kparams(h);
+    return upload_hydro(h);
+}
+
+extern "C" int cf_set_manning(cf_handle *h, const double *n) {
+    if (!h || !n) return CF_ERR_ARG;
+    HIPCHK(hipMemcpyAsync(h->manning, n, (size_t)h->nelt * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CF_OK;
+}
+
+extern "C" int cf_get_geometry(cf_handle *h, double *surf, double *zm, double *cotemin_arr, double *cotemin) {
+    if (!h) return CF_ERR_ARG;
+    const size_t N = (size_t)h->nelt;
+    std::vector<double> cot(N);
+    if (surf) HIPCHK(hipMemcpyAsync(surf, h->geo + G_SURF * N, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (zm) HIPCHK(hipMemcpyAsync(zm, h->zm, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(cot.data(), h->geo + G_COT * N, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (cotemin_arr) memcpy(cotemin_arr, cot.data(), N * sizeof(double));
+    if (cotemin) {
+        double m = 1000000.0;
+        for (size_t i = 0; i < N; i++) m = std::min(m, cot[i]);
+        *cotemin = m;
+    }
+    return CF_OK;
+}
+
+extern "C" void *cf_get_stream(cf_handle *h) { return h ? (void *)h->stream : nullptr; }
+
+extern "C" int cf_set_stream(cf_handle *h, void *stream) {
+    if (!h) return CF_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->own_stream) (void)hipStreamDestroy(h->stream);
+    h->stream = (hipStream_t)stream;
+    h->own_stream = false;
+    return CF_OK;
+}
+
+extern "C" int cf_debug_pow(int n, const double *x, double y, double *out) {
+    if (n <= 0 || !x || !out) return CF_ERR_ARG;
+    double *dx = nullptr, *dout = nullptr;
+    HIPCHK(hipMalloc(&dx, sizeof(double) * (size_t)n));
+    HIPCHK(hipMalloc(&dout, sizeof(double) * (size_t)n));
+    HIPCHK(hipMemcpy(dx, x, sizeof(double) * (size_t)n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(pow_kernel, dim3(grid_of(n, 256)), dim3(256), 0, 0, dx, y, n, dout);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpy(out, dout, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost));
+    dfree(dx);
+    dfree(dout);
+    return CF_OK;
+}

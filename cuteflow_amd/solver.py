@@ -1,0 +1,171 @@
+"""Host-side mirror of the reference time loop around the C ABI.
+
+`Solver` owns one cf_handle (one subdomain).  `solve_finite_volumes` is the
+reference driver loop (finite_volumes_method.cuf:4-258): initial dt, then
+steps until tc >= ts - tol, with the per-step work done on the device in
+batches of steps (no host synchronisation inside a batch).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .mesh import Mesh
+from .params import Params
+
+
+class Solver:
+    def __init__(self, mesh: Mesh, params: Params, boundary: np.ndarray | None = None,
+                 ts: float | None = None):
+        lib = _lib.load()
+        self.lib = lib
+        self.mesh = mesh
+        self.params = params
+        self.boundary = mesh.boundary_table() if boundary is None else np.ascontiguousarray(boundary, np.int32)
+        self.long_entree = mesh.inlet_lengths(self.boundary, params.nombre_entrees)
+        self.cfp = params.to_cf(self.long_entree, ts=ts)
+        self._conn = np.ascontiguousarray(mesh.conn.reshape(-1), np.int32)
+        self._coords = np.ascontiguousarray(mesh.coords.reshape(-1), np.float64)
+        self._mann = np.ascontiguousarray(mesh.manning, np.float64)
+        cm = _lib.CfMesh()
+        cm.nelt = mesh.nelt
+        cm.nnt = mesh.nnt
+        cm.conn = _lib.iptr(self._conn)
+        cm.coords = _lib.dptr(self._coords)
+        cm.boundary = _lib.iptr(self.boundary)
+        cm.manning = _lib.dptr(self._mann)
+        h = C.c_void_p()
+        _lib.check(lib.cf_create(C.byref(cm), C.byref(self.cfp), C.byref(h)), "cf_create")
+        self.h = h
+        self.nelt = mesh.nelt
+
+    # -- state ---------------------------------------------------------------
+    def set_state(self, U: np.ndarray) -> None:
+        U = np.ascontiguousarray(U, np.float64)
+        assert U.shape == (self.nelt, 3)
+        _lib.check(self.lib.cf_set_state(self.h, _lib.dptr(U)), "cf_set_state")
+
+    def get_state(self) -> np.ndarray:
+        U = np.zeros((self.nelt, 3))
+        _lib.check(self.lib.cf_get_state(self.h, _lib.dptr(U)), "cf_get_state")
+        return U
+
+    def set_state_device(self, ptr: int) -> None:
+        _lib.check(self.lib.cf_set_state_device(self.h, C.c_void_p(ptr)), "cf_set_state_device")
+
+    def get_state_device(self, ptr: int) -> None:
+        _lib.check(self.lib.cf_get_state_device(self.h, C.c_void_p(ptr)), "cf_get_state_device")
+
+    @property
+    def dt(self) -> float:
+        v = C.c_double()
+        _lib.check(self.lib.cf_initial_dt(self.h, C.byref(v)), "cf_initial_dt")
+        return v.value
+
+    @property
+    def tc(self) -> float:
+        t = C.c_double()
+        _lib.check(self.lib.cf_get_time(self.h, C.byref(t), None), "cf_get_time")
+        return t.value
+
+    def geometry(self):
+        n = self.nelt
+        surf, zm, cot = np.zeros(n), np.zeros(n), np.zeros(n)
+        cmin = C.c_double()
+        _lib.check(self.lib.cf_get_geometry(self.h, _lib.dptr(surf), _lib.dptr(zm), _lib.dptr(cot),
+                                            C.byref(cmin)), "cf_get_geometry")
+        return {"surf": surf, "zm": zm, "cotemin_arr": cot, "cotemin": cmin.value}
+
+    # -- stepping --------------------------------------------------------------
+    def step(self, nsteps: int, raise_on_error: bool = True):
+        """Run up to nsteps steps; returns (dt_hist[:done+1], done, status)."""
+        hist = np.zeros(nsteps + 1)
+        done = C.c_int()
+        st = self.lib.cf_step(self.h, int(nsteps), _lib.dptr(hist), C.byref(done))
+        if st != _lib.CF_OK and (raise_on_error or st != _lib.CF_ERR_DT):
+            raise _lib.CuteflowError(st, "cf_step")
+        return hist[: done.value + 1], done.value, st
+
+    def set_manning(self, n: np.ndarray) -> None:
+        n = np.ascontiguousarray(n, np.float64)
+        assert n.shape == (self.nelt,)
+        _lib.check(self.lib.cf_set_manning(self.h, _lib.dptr(n)), "cf_set_manning")
+
+    def set_inflow(self, q) -> None:
+        q = np.ascontiguousarray(q, np.float64)
+        _lib.check(self.lib.cf_set_inflow(self.h, _lib.dptr(q), len(q)), "cf_set_inflow")
+
+    def set_hydrograph(self, inlet: int, t, q) -> None:
+        t = np.ascontiguousarray(t, np.float64)
+        q = np.ascontiguousarray(q, np.float64)
+        _lib.check(self.lib.cf_set_hydrograph(self.h, inlet, len(t), _lib.dptr(t), _lib.dptr(q)),
+                   "cf_set_hydrograph")
+
+    @property
+    def stream(self) -> int:
+        return self.lib.cf_get_stream(self.h) or 0
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.cf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def initial_state(params: Params, mesh: Mesh, zm: np.ndarray) -> np.ndarray:
+    """set_initial_condition (initial_condition.cuf:4-80): dam line or plane
+    initial condition from the centroid and the cell bed level zm."""
+    xc, yc = mesh.centroids()
+    U = np.zeros((mesh.nelt, 3))
+    if params.eqlin_barrage == 1:
+        alpha = (params.y1eqbar - params.y2eqbar) / (params.x1eqbar - params.x2eqbar)
+        beta = params.y1eqbar - alpha * params.x1eqbar
+        up = (alpha * xc - yc) + beta >= 0.0
+        h = np.where(up, params.h_amont - zm, params.h_aval - zm)
+        u = np.where(up, params.u_amont, params.u_aval)
+        v = np.where(up, params.v_amont, params.v_aval)
+        wet = h > params.tolisec
+        U[:, 0] = np.where(wet, h, 0.0)
+        U[:, 1] = np.where(wet, h * u, 0.0)
+        U[:, 2] = np.where(wet, h * v, 0.0)
+    if params.plan == 1:
+        h = ((((-params.xplan) * (params.xpoint - xc)) - params.yplan * (params.ypoint - yc)) / params.zplan
+             + params.zpoint) - zm
+        wet = h > params.tolisec
+        U[:, 0] = np.where(wet, h, 0.0)
+        U[:, 1] = np.where(wet, h * params.u_aval, 0.0)
+        U[:, 2] = np.where(wet, h * params.v_aval, 0.0)
+    return U
+
+
+def solve_finite_volumes(solver: Solver, batch: int = 1000, max_steps: int | None = None,
+                         on_batch=None):
+    """The reference loop (finite_volumes_method.cuf:113-242) until tc >= ts - tol
+    (or max_steps): returns (dt_hist, steps, tc)."""
+    hists, total = [], 0
+    while True:
+        n = batch if max_steps is None else min(batch, max_steps - total)
+        if n <= 0:
+            break
+        hist, done, _ = solver.step(n)
+        hists.append(hist[:done])
+        total += done
+        if on_batch is not None:
+            on_batch(solver, total)
+        if done < n:
+            break
+    dt_hist = np.concatenate(hists) if hists else np.zeros(0)
+    return dt_hist, total, solver.tc

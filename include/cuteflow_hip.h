@@ -1,0 +1,153 @@
+/*
+ * cuteflow_hip.h - C ABI of the MI355X-native St-Venant finite-volume step.
+ *
+ * The reference (CuteFlow, CUDA Fortran) has no operator API: its hot path is
+ * a set of argument-less kernels that talk through module globals and are
+ * launched by the Fortran host loop.  This header is the drop-in boundary
+ * that replaces that loop body for one subdomain.  A Fortran host binds it
+ * with iso_c_binding, Python with ctypes.  All pointers are plain host
+ * pointers unless the name says _device; no ownership is taken (copy
+ * semantics).  Indices in tables are 1-based, as in the reference.
+ *
+ * Status convention: 0 = ok; negative = error (see cf_error_string).
+ * Threading: one host thread per handle; all device work of a handle is
+ * ordered on the handle's HIP stream.
+ */
+#ifndef CUTEFLOW_HIP_H
+#define CUTEFLOW_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CF_MAX_IO 20      /* inlets/outlets (global_data.cuf:54, dimension(20)) */
+#define CF_MAX_HYDRO 64   /* points per inflow hydrograph */
+
+enum {
+    CF_OK = 0,
+    CF_ERR_ARG = -1,        /* invalid argument or unsupported option */
+    CF_ERR_HIP = -2,        /* HIP runtime error */
+    CF_ERR_DT = -3,         /* dt sanity failure (cfl_condition.cuf:129-132) or NaN state */
+    CF_ERR_MESH = -4,       /* inconsistent mesh */
+    CF_ERR_NOMEM = -5
+};
+
+/* Every constant the reference kernels read from `constant` memory
+ * (global_data_device.cuf, set by data_transfer.cuf:12-96).  String options
+ * are passed as the integer codes of data_transfer.cuf:43-66. */
+typedef struct cf_params {
+    double gp;               /* GP */
+    double tolisec;          /* wet/dry threshold */
+    double tol;              /* friction velocity floor and loop tolerance (default 1e-6f) */
+    double cfl;
+    double ts;               /* end time; use 1e300 for a fixed step count */
+    double constant_dt;      /* used when is_dt_constant == 1 */
+    double override_manning; /* used when is_override_manning == 1 */
+    double tol_reg_perm;     /* steady-state tolerance (not used by the step) */
+    int iflux;               /* 1 = HLLC, 3 = WAF (Loukili & Soulaimani) */
+    int ilimiteur;           /* WAF limiter 0..4 */
+    int friction;            /* 0/1 */
+    int fricimplic;          /* 0 explicit, 1 (I-dt/2 J)^-1, 2 (I-dt B)^-1 */
+    int inlet;               /* 1 'transm', 2 'inflow' */
+    int outlet;              /* 1 'fixedheight', 2 'transm' */
+    int is_dt_constant;
+    int is_override_manning;
+    int nombre_entrees;
+    int reserved0;
+    double debitglob[CF_MAX_IO];
+    double h_sortie[CF_MAX_IO];
+    double long_entree[CF_MAX_IO];
+    /* inflow hydrograph (no reference counterpart; its debit_var/loi_debitglob
+       are parsed but never used, global_data.cuf:48-49): per inlet hydro_n
+       points (t, Q), linear in between, constant outside; 0 = debitglob. */
+    int hydro_n[CF_MAX_IO];
+    double hydro_t[CF_MAX_IO * CF_MAX_HYDRO];
+    double hydro_q[CF_MAX_IO * CF_MAX_HYDRO];
+} cf_params;
+
+/* Mesh as the reference holds it after lecture_maillage (maillage.cuf:4-166). */
+typedef struct cf_mesh {
+    int nelt;                /* cells (triangles) */
+    int nnt;                 /* nodes */
+    const int *conn;         /* 3*nelt, connectivite(i,1:3), 1-based, cell-major */
+    const double *coords;    /* 3*nnt, coordonnees(n,1:3) = x,y,z, node-major */
+    const int *boundary;     /* 6*nelt, boundary(nelt,3,2) in Fortran column-major
+                                order; self-references are turned into walls
+                                (set_undefined_boudaries_as_walls) */
+    const double *manning;   /* nelt per-cell n (manning_nelt), or NULL */
+} cf_mesh;
+
+typedef struct cf_handle cf_handle;
+
+/* Builds boundary(nelt,3,2) (Fortran order) from the connectivity and the
+ * inlet/outlet/wall node lists: replaces tab_bound_tri + boundary_kind
+ * (maillage.cuf:781-1110) with an O(N log N) directed-edge match.  Self
+ * references (undefined faces) are kept, as in the reference's
+ * boundary_table file (maillage.cuf:55-68).  num_in/num_out may be NULL
+ * (all 1). */
+int cf_build_boundary(int nelt, int nnt, const int *conn,
+                      int ndi, const int *ndinput, const int *num_in,
+                      int ndo, const int *ndoutput, const int *num_out,
+                      int ndw, const int *ndwall, int *boundary);
+
+/* Inlet lengths long_entree (maillage.cuf:1423-1452) from a boundary table. */
+int cf_inlet_lengths(int nelt, int nnt, const int *conn, const double *coords,
+                     const int *boundary, int nentrees, double *long_entree);
+
+/* Creates a solver for one subdomain: copies the mesh in, computes geometry
+ * on the device (compute_surface_and_bathy, maillage.cuf:1284-1331, the
+ * cotemin loop :132-159, construct_coo_table_elemwise :1254-1282) and the
+ * neighbour stencil (find_upwind_cells :954-978). */
+int cf_create(const cf_mesh *mesh, const cf_params *params, cf_handle **out);
+void cf_destroy(cf_handle *h);
+const char *cf_error_string(int status);
+
+/* State vdlg(3,nelt) (h,hu,hv per cell); copy in / copy out.  Setting the
+ * state resets tc to 0 and recomputes dt (finite_volumes_method.cuf:113-116). */
+int cf_set_state(cf_handle *h, const double *U);
+int cf_get_state(cf_handle *h, double *U);
+/* Same on device memory (3*nelt doubles) on the handle's stream. */
+int cf_set_state_device(cf_handle *h, const double *dU);
+int cf_get_state_device(cf_handle *h, double *dU);
+
+/* Current dt (from the current state) and time. */
+int cf_initial_dt(cf_handle *h, double *dt);
+int cf_get_time(cf_handle *h, double *tc, double *dt);
+
+/* Runs up to nsteps steps of the loop body of finite_volumes_method.cuf:123-242
+ * (flux, source, update, cfl/dt, state swap, tc += dt); stops early when
+ * tc >= ts - tol or on a dt/NaN failure.  dt_hist (nsteps+1, may be NULL)
+ * receives the dt used by each step and then the next dt; *steps_done the
+ * number of steps run.  Returns CF_ERR_DT on a dt failure. */
+int cf_step(cf_handle *h, int nsteps, double *dt_hist, int *steps_done);
+/* cf_step split in two: cf_step_async only enqueues the steps on the
+ * handle's stream (no host synchronisation); cf_step_finish waits for them
+ * and returns what cf_step returns.  Every cf_step_async must be followed by
+ * one cf_step_finish before any other call on the handle. */
+int cf_step_async(cf_handle *h, int nsteps);
+int cf_step_finish(cf_handle *h, double *dt_hist, int *steps_done);
+
+/* Inflow discharges debitglob(1:n) and hydrographs. */
+int cf_set_inflow(cf_handle *h, const double *debitglob, int n);
+int cf_set_hydrograph(cf_handle *h, int inlet, int npts, const double *t, const double *q);
+/* Per-cell Manning n (manning_nelt) */
+int cf_set_manning(cf_handle *h, const double *n);
+
+/* Geometry computed on the device: surf, zm, cotemin_arr (each nelt, may be
+ * NULL) and cotemin = min(cotemin_arr). */
+int cf_get_geometry(cf_handle *h, double *surf, double *zm, double *cotemin_arr,
+                    double *cotemin);
+
+/* Stream of the handle (hipStream_t as void*) and its replacement, so that a
+ * caller (e.g. PyTorch) can order the solver on its own stream. */
+void *cf_get_stream(cf_handle *h);
+int cf_set_stream(cf_handle *h, void *stream);
+
+/* Elementwise pow used by the friction kernel, evaluated on the device for
+ * n values (x > 0): lets tests compare the device pow bit-for-bit. */
+int cf_debug_pow(int n, const double *x, double y, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

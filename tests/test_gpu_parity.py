@@ -1,0 +1,203 @@
+"""Parity of the HIP time step (through the C ABI) with the CPU oracle.
+
+The bar: bit-identical states, dt sequences and wet/dry masks on the same
+seeded inputs (the kernels and the oracle evaluate the same FP64 operations
+in the same order, without FMA contraction).  The north-star tolerances
+(1e-10 relative L-inf on h, hu, hv; 1e-12 on dt) are asserted as well, so a
+failure message shows which bar broke.
+"""
+import numpy as np
+import pytest
+
+from cuteflow_amd import mesh as M
+from cuteflow_amd.solver import Solver
+from oracle import oracle as O
+
+from cases import dam_case, lake_case, river_case
+
+pytestmark = pytest.mark.gpu
+
+RTOL_STATE = 1e-10
+RTOL_DT = 1e-12
+
+
+def _compare(U_gpu, U_cpu, dt_gpu, dt_cpu, tolisec):
+    assert dt_gpu.shape == dt_cpu.shape, (dt_gpu.shape, dt_cpu.shape)
+    rel_dt = np.max(np.abs(dt_gpu - dt_cpu) / np.abs(dt_cpu))
+    assert rel_dt <= RTOL_DT, f"dt sequence differs: {rel_dt}"
+    for k in range(3):
+        scale = max(np.max(np.abs(U_cpu[:, k])), 1e-300)
+        err = np.max(np.abs(U_gpu[:, k] - U_cpu[:, k])) / scale
+        assert err <= RTOL_STATE, f"component {k}: rel Linf {err}"
+    wet_g = U_gpu[:, 0] > tolisec
+    wet_c = U_cpu[:, 0] > tolisec
+    assert np.array_equal(wet_g, wet_c), f"{np.sum(wet_g != wet_c)} wet/dry mismatches"
+    # stronger than the tolerances: the same bits
+    assert np.array_equal(dt_gpu.view(np.int64), dt_cpu.view(np.int64))
+    assert np.array_equal(U_gpu.view(np.int64), U_cpu.view(np.int64)), \
+        f"{np.sum(U_gpu != U_cpu)} values differ in the last bits"
+
+
+def _run_both(m, p, U0, nsteps, perm=None):
+    prep = O.Prepared(m)
+    ref = O.run(prep, p, U0, nsteps)
+    if perm is not None:
+        inv = np.empty_like(perm)
+        inv[perm] = np.arange(len(perm))
+        mg, U0g = m.permuted(perm), U0[perm]
+    else:
+        mg, U0g = m, U0
+    with Solver(mg, p) as s:
+        s.set_state(U0g)
+        dt0 = s.dt
+        hist, done, st = s.step(nsteps, raise_on_error=False)
+        U = s.get_state()
+        tc = s.tc
+    if perm is not None:
+        U = U[inv]
+    return ref, U, np.asarray(hist), done, st, dt0, tc
+
+
+def test_geometry_bitwise():
+    m, p, _ = river_case()
+    prep = O.Prepared(m)
+    with Solver(m, p) as s:
+        g = s.geometry()
+    assert np.array_equal(g["surf"], prep.surf)
+    assert np.array_equal(g["zm"], prep.zm)
+    assert np.array_equal(g["cotemin_arr"], prep.cot)
+    assert g["cotemin"] == prep.cotemin
+
+
+def test_pow_bitwise():
+    from cuteflow_amd import _lib
+    lib = _lib.load()
+    rng = np.random.default_rng(0)
+    x = np.concatenate([10.0 ** rng.uniform(-6, 3, 200000), rng.uniform(1e-3, 20, 100000)])
+    for y in (-1.0 / 3.0, -4.0 / 3.0, 1.0 / 3.0, 0.5, 2.5):
+        out = np.zeros_like(x)
+        _lib.check(lib.cf_debug_pow(len(x), _lib.dptr(x), y, _lib.dptr(out)), "cf_debug_pow")
+        ref = np.array([O.pow_(v, y) for v in x[:20000]])
+        assert np.array_equal(out[:20000], ref)
+        ulp = np.abs(out - np.power(x, y)) / np.spacing(np.power(x, y))
+        assert ulp.max() <= 2.0
+
+
+def test_initial_dt():
+    m, p, U0 = dam_case(30)
+    prep = O.Prepared(m)
+    with Solver(m, p) as s:
+        s.set_state(U0)
+        assert s.dt == O.cfl_dt(prep, p, U0)
+
+
+@pytest.mark.parametrize("n,steps", [(20, 200), (100, 1000)])
+def test_dam_break_hllc(n, steps):
+    """Circular dam break, HLLC, walls; (100, 1000) is the full 20k-cell case."""
+    m, p, U0 = dam_case(n)
+    ref, U, hist, done, st, _, tc = _run_both(m, p, U0, steps)
+    assert st == 0 and ref["status"] == 0 and done == steps == ref["steps"]
+    _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
+    assert tc == ref["tc"]
+
+
+@pytest.mark.parametrize("fricimplic", [0, 1, 2])
+def test_river_waf_friction(fricimplic):
+    m, p, U0 = river_case(fricimplic=fricimplic)
+    ref, U, hist, done, st, _, _ = _run_both(m, p, U0, 300)
+    assert st == 0 and ref["status"] == 0 and done == 300
+    _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
+    assert np.sum(U[:, 0] > p.tolisec) > 0
+
+
+@pytest.mark.parametrize("ilim", [0, 1, 2, 3, 4])
+def test_waf_limiters(ilim):
+    m, p, U0 = river_case(ilimiteur=ilim, nx=30, ny=9)
+    ref, U, hist, done, st, _, _ = _run_both(m, p, U0, 150)
+    assert st == 0 and done == 150
+    _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
+
+
+def test_river_hllc_transmissive_outlet():
+    m, p, U0 = river_case(iflux=1, outlet="transm")
+    ref, U, hist, done, st, _, _ = _run_both(m, p, U0, 300)
+    assert st == 0 and done == 300
+    _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
+
+
+def test_reordering_invariance():
+    """RCM and random cell orders give the same per-cell results."""
+    m, p, U0 = river_case(nx=40, ny=10)
+    for perm in (M.rcm_order(m), M.random_order(m, 7)):
+        ref, U, hist, done, st, _, _ = _run_both(m, p, U0, 200, perm=perm)
+        _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
+
+
+def test_lake_at_rest():
+    m, p = lake_case()
+    prep = O.Prepared(m)
+    U0 = np.zeros((m.nelt, 3))
+    U0[:, 0] = np.where(-prep.zm > p.tolisec, -prep.zm, 0.0)
+    ref, U, hist, done, st, _, _ = _run_both(m, p, U0, 200)
+    _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
+    wet = U[:, 0] > 0
+    assert np.max(np.abs(U[wet, 0] + prep.zm[wet])) < 1e-6
+    assert np.max(np.abs(U[:, 1:])) < 1e-6
+
+
+def test_stop_at_ts():
+    m, p, U0 = dam_case(16)
+    p.ts = 0.5
+    prep = O.Prepared(m)
+    ref = O.run(prep, p, U0, 10000)
+    with Solver(m, p) as s:
+        s.set_state(U0)
+        hist, done, st = s.step(10000)
+        U = s.get_state()
+        assert done == ref["steps"] < 10000
+        assert s.tc == ref["tc"]
+    _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
+
+
+def test_dt_error_all_dry():
+    """An all-dry domain gives dt = 1e5 > 9999: the run stops with an error
+    after the first step (cfl_condition.cuf:129-132)."""
+    m, p, U0 = dam_case(10)
+    U0[:] = 0.0
+    with Solver(m, p) as s:
+        s.set_state(U0)
+        hist, done, st = s.step(10, raise_on_error=False)
+        assert st == -3 and done == 1
+        with pytest.raises(RuntimeError):
+            s.step(1)
+
+
+def test_nan_detected():
+    m, p, U0 = dam_case(10)
+    U0[5, 1] = np.nan
+    with Solver(m, p) as s:
+        s.set_state(U0)
+        hist, done, st = s.step(10, raise_on_error=False)
+        assert st == -3 and done == 1
+
+
+def test_constant_dt():
+    m, p, U0 = dam_case(12)
+    p.is_dt_constant, p.constant_dt = 1, 0.01
+    ref, U, hist, done, st, _, _ = _run_both(m, p, U0, 50)
+    assert np.all(hist == 0.01)
+    _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
+
+
+def test_chunked_steps_equal_single_run():
+    m, p, U0 = river_case(nx=30, ny=9)
+    with Solver(m, p) as s:
+        s.set_state(U0)
+        h1, _, _ = s.step(120)
+        U1 = s.get_state()
+    with Solver(m, p) as s:
+        s.set_state(U0)
+        parts = [s.step(k)[0][:-1] for k in (1, 7, 50, 62)]
+        U2 = s.get_state()
+    assert np.array_equal(np.concatenate(parts), h1[:-1])
+    assert np.array_equal(U1, U2)
