@@ -52,11 +52,10 @@ struct KParams {
 };
 
 struct Ctrl {
-    int *hdr;                  // [0] status, [1] nan flag, [2] steps done
-    unsigned *counter;         // per step: finished workgroups
-    unsigned long long *dtbits;  // per step: atomic-min target of the next dt
-    double *dt;                // dt[k] used by step k
-    double *tc;                // tc[k] at the start of step k
+    int *hdr;          // [0] status (0 running, 1 ts reached, -1 dt/NaN failure), [1] steps done
+    double *partial;   // per workgroup: min CFL dt of the new state, NaN if a state is NaN
+    double *dt;        // dt[k] used by step k
+    double *tc;        // tc[k] at the start of step k
 };
 
 struct StepArgs {
@@ -70,6 +69,8 @@ struct StepArgs {
     Ctrl c;
     int nelt;
     int step;
+    int xcd_map;  // 1: consecutive tiles on the same XCD
+    int reserved;
     KParams p;
 };
 
@@ -93,13 +94,44 @@ __device__ __forceinline__ double wave_min(double v) {
     return v;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs; remap the workgroup
+// id so that each XCD works on one contiguous range of tiles, whose
+// neighbour gathers then hit that XCD's own L2.
+__device__ __forceinline__ int tile_of(int b, int nblocks, int xcd_map) {
+    if (!xcd_map) return b;
+    const int q = nblocks >> 3, r = nblocks & 7, x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
+// Flux of a face whose two reconstructed depths are both below eps: the
+// Riemann solvers return a zero flux there; same operations as their
+// rotation back to the global frame.
+__device__ __forceinline__ void zero_flux(double n1, double n2, double flux[3]) {
+    const double z = 0.0;
+    flux[0] = z;
+    flux[1] = n1 * z - n2 * z;
+    flux[2] = n2 * z + n1 * z;
+}
+
+// Occupancy target of the step kernel (waves per SIMD); 0 lets the compiler
+// choose the register budget.
+#ifndef CF_STEP_WAVES_PER_EU
+#define CF_STEP_WAVES_PER_EU 0
+#endif
+#if CF_STEP_WAVES_PER_EU > 0
+#define CF_STEP_OCCUPANCY __attribute__((amdgpu_waves_per_eu(CF_STEP_WAVES_PER_EU, 8)))
+#else
+#define CF_STEP_OCCUPANCY
+#endif
+
 // FRIC: 0 no friction, 1 explicit (fricimplic 0), 2 fricimplic 1, 3 fricimplic 2
 template <int IFLUX, int FRIC>
-__global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
+__global__ __launch_bounds__(kBlock) CF_STEP_OCCUPANCY void step_kernel(StepArgs a) {
     __shared__ double s_min[kWaves];
     if (a.c.hdr[0] != 0) return;  // stopped (ts reached) or failed earlier
     const int nelt = a.nelt;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int tile = tile_of(blockIdx.x, gridDim.x, a.xcd_map);
+    const int i = tile * kBlock + threadIdx.x;
     const double dt = a.c.dt[a.step];
     const double tc = a.c.tc[a.step];
     const KParams &p = a.p;
@@ -111,15 +143,22 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
         const double4 u0 = a.st_in[i];
         const int4 nb = a.nbr[i];
         const double zml = u0.w;
+        // first ring: the three neighbour states, gathered once
+        double4 unb[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int c = nb_at(nb, j);
+            unb[j] = c >= 0 ? a.st_in[c] : u0;
+        }
         double4 dfi = make_double4(0.0, 0.0, 0.0, 0.0);
         if (IFLUX == 3) dfi = a.dfc[i];
-        double fl[3][3], fn1[3], fn2[3], fln[3], fzr[3];
+        double fl[3][3], fn1[3], fn2[3], fln[3];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             const double n1 = a.geo[(G_N1 + j) * nelt + i];
             const double n2 = a.geo[(G_N2 + j) * nelt + i];
             const double ln = a.geo[(G_LN + j) * nelt + i];
-            fn1[j] = n1; fn2[j] = n2; fln[j] = ln; fzr[j] = 0.0;
+            fn1[j] = n1; fn2[j] = n2; fln[j] = ln;
             const int code = nb_at(nb, j);
             double hautl, vitul, vitvl;
             rot(tls, n1, n2, u0.x, u0.y, u0.z, hautl, vitul, vitvl);
@@ -160,87 +199,90 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
                 }
             }
             if (kv >= 0) {  // interior face (:166-409)
-                const double4 ur4 = a.st_in[kv];
+                const double4 ur4 = (kv == i) ? u0 : unb[j];
                 const double zmr = ur4.w;
-                fzr[j] = zmr;
-                double zij = dmax(zml, zmr);
+                const double zij = dmax(zml, zmr);
                 double hijet = dmax(0.0, (hautl + zml) - zij);
-                double hjiet = dmax(0.0, (ur4.x + zmr) - zij);
                 double hautr, vitur, vitvr;
                 rot(tls, n1, n2, ur4.x, ur4.y, ur4.z, hautr, vitur, vitvr);
                 if (IFLUX == 1) {
+                    double hjiet = dmax(0.0, (ur4.x + zmr) - zij);
                     if (hijet < tls) { hijet = 0.0; vitul = 0.0; vitvl = 0.0; }
                     if (hjiet < tls) { hjiet = 0.0; vitur = 0.0; vitvr = 0.0; }
                     hllc_flux(g, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr, flux);
                 } else {
-                    const double d_mrl = d4_at(dfi, j);
-                    const int ja = (j + 1) % 3, jb = (j + 2) % 3;
-                    const int c1 = nb_at(nb, ja), c2 = nb_at(nb, jb);
-                    const int kv1 = c1 >= 0 ? c1 : i, kv2 = c2 >= 0 ? c2 : i;
-                    const double4 s1 = a.st_in[kv1], s2 = a.st_in[kv2];
-                    double h1, u1, w1, h2, u2, w2;
-                    rot(tls, n1, n2, s1.x, s1.y, s1.z, h1, u1, w1);
-                    rot(tls, n1, n2, s2.x, s2.y, s2.z, h2, u2, w2);
-                    const int tjv = (nb.w >> (2 * j)) & 3;
-                    const int4 nbk = (kv == i) ? nb : a.nbr[kv];
-                    const int ka = (tjv + 1) % 3, kb = (tjv + 2) % 3;
-                    const int e1 = nb_at(nbk, ka), e2 = nb_at(nbk, kb);
-                    const int kw1 = e1 >= 0 ? e1 : kv, kw2 = e2 >= 0 ? e2 : kv;
-                    const double4 g1 = a.st_in[kw1], g2 = a.st_in[kw2];
-                    double g1h, g1u, g1w, g2h, g2u, g2w;
-                    rot(tls, n1, n2, g1.x, g1.y, g1.z, g1h, g1u, g1w);
-                    rot(tls, n1, n2, g2.x, g2.y, g2.z, g2h, g2u, g2w);
-                    double hl_upw, vl_upw, hr_upw, vr_upw, zml_upw, zmr_upw, d_mkvol, d_mkvoisin;
-                    if ((kv1 == i && kv2 == i) || (kw1 == kv && kw2 == kv)) {
-                        hl_upw = hautl; vl_upw = vitvl;
-                        hr_upw = hautr; vr_upw = vitvr;
-                        d_mkvol = d_mrl; d_mkvoisin = d_mrl;
-                        zml_upw = zml; zmr_upw = zmr;
-                    } else {
-                        if (kv1 == i) {
-                            hl_upw = h2; vl_upw = w2;
-                            d_mkvol = d4_at(dfi, jb);
-                            zml_upw = s2.w;
-                        } else if (kv2 == i) {
-                            hl_upw = h1; vl_upw = w1;
-                            d_mkvol = d4_at(dfi, ja);
-                            zml_upw = s1.w;
-                        } else {
-                            const double d1 = d4_at(dfi, ja), d2 = d4_at(dfi, jb);
-                            hl_upw = (d1 * h1 + d2 * h2) / (d1 + d2);
-                            vl_upw = (d1 * w1 + d2 * w2) / (d1 + d2);
-                            zml_upw = (d1 * s1.w + d2 * s2.w) / (d1 + d2);
-                            d_mkvol = (d1 + d2) / 2.0;
-                        }
-                        const double4 dfk = (kv == i) ? dfi : a.dfc[kv];
-                        if (kw1 == kv) {
-                            hr_upw = g2h; vr_upw = g2w;
-                            d_mkvoisin = d4_at(dfk, kb);
-                            zmr_upw = g2.w;
-                        } else if (kw2 == kv) {
-                            hr_upw = g1h; vr_upw = g1w;
-                            d_mkvoisin = d4_at(dfk, ka);
-                            zmr_upw = g1.w;
-                        } else {
-                            const double d1 = d4_at(dfk, ka), d2 = d4_at(dfk, kb);
-                            hr_upw = (d1 * g1h + d2 * g2h) / (d1 + d2);
-                            vr_upw = (d1 * g1w + d2 * g2w) / (d1 + d2);
-                            zmr_upw = (d1 * g1.w + d2 * g2.w) / (d1 + d2);
-                            d_mkvoisin = (d1 + d2) / 2.0;
-                        }
-                    }
-                    // hydrostatic reconstruction (flux_riemann.cuf:370-400)
-                    zij = dmax(zml, zmr);
-                    hijet = dmax(0.0, (hautl + zml) - zij);
-                    hl_upw = dmax(0.0, (hl_upw + zml_upw) - zij);
-                    hjiet = dmax(0.0, (hautr + zmr) - zij);
-                    hr_upw = dmax(0.0, (hr_upw + zmr_upw) - zij);
+                    // reconstructed depths of flux_riemann.cuf:371-394 (the
+                    // values solver_Riemann_waf receives)
+                    double hjiet = dmax(0.0, (hautr + zmr) - zij);
                     if (hijet < tls) { hijet = 0.0; vitul = 0.0; vitvl = 0.0; }
-                    if (hl_upw < tls) { hl_upw = 0.0; vl_upw = 0.0; }
                     if (hjiet < tls) { hjiet = 0.0; vitur = 0.0; vitvr = 0.0; }
-                    if (hr_upw < tls) { hr_upw = 0.0; vr_upw = 0.0; }
-                    waf_flux(g, p.ilimiteur, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr,
-                             hl_upw, vl_upw, hr_upw, vr_upw, d_mrl, d_mkvol, d_mkvoisin, dt, flux);
+                    if (hijet < kEpsWaf && hjiet < kEpsWaf) {
+                        zero_flux(n1, n2, flux);  // dry/dry: no upwind data needed
+                    } else {
+                        const double d_mrl = d4_at(dfi, j);
+                        const int ja = (j + 1) % 3, jb = (j + 2) % 3;
+                        const bool own1 = nb_at(nb, ja) < 0, own2 = nb_at(nb, jb) < 0;
+                        const double4 s1 = own1 ? u0 : unb[ja];
+                        const double4 s2 = own2 ? u0 : unb[jb];
+                        double h1, u1, w1, h2, u2, w2;
+                        rot(tls, n1, n2, s1.x, s1.y, s1.z, h1, u1, w1);
+                        rot(tls, n1, n2, s2.x, s2.y, s2.z, h2, u2, w2);
+                        const int tjv = (nb.w >> (2 * j)) & 3;
+                        const int4 nbk = (kv == i) ? nb : a.nbr[kv];
+                        const int ka = (tjv + 1) % 3, kb = (tjv + 2) % 3;
+                        const int e1 = nb_at(nbk, ka), e2 = nb_at(nbk, kb);
+                        const int kw1 = e1 >= 0 ? e1 : kv, kw2 = e2 >= 0 ? e2 : kv;
+                        const double4 g1 = a.st_in[kw1], g2 = a.st_in[kw2];
+                        double g1h, g1u, g1w, g2h, g2u, g2w;
+                        rot(tls, n1, n2, g1.x, g1.y, g1.z, g1h, g1u, g1w);
+                        rot(tls, n1, n2, g2.x, g2.y, g2.z, g2h, g2u, g2w);
+                        double hl_upw, vl_upw, hr_upw, vr_upw, zml_upw, zmr_upw, d_mkvol, d_mkvoisin;
+                        if ((own1 && own2) || (kw1 == kv && kw2 == kv)) {
+                            hl_upw = hautl; vl_upw = vitvl;
+                            hr_upw = hautr; vr_upw = vitvr;
+                            d_mkvol = d_mrl; d_mkvoisin = d_mrl;
+                            zml_upw = zml; zmr_upw = zmr;
+                        } else {
+                            if (own1) {
+                                hl_upw = h2; vl_upw = w2;
+                                d_mkvol = d4_at(dfi, jb);
+                                zml_upw = s2.w;
+                            } else if (own2) {
+                                hl_upw = h1; vl_upw = w1;
+                                d_mkvol = d4_at(dfi, ja);
+                                zml_upw = s1.w;
+                            } else {
+                                const double d1 = d4_at(dfi, ja), d2 = d4_at(dfi, jb);
+                                hl_upw = (d1 * h1 + d2 * h2) / (d1 + d2);
+                                vl_upw = (d1 * w1 + d2 * w2) / (d1 + d2);
+                                zml_upw = (d1 * s1.w + d2 * s2.w) / (d1 + d2);
+                                d_mkvol = (d1 + d2) / 2.0;
+                            }
+                            const double4 dfk = (kv == i) ? dfi : a.dfc[kv];
+                            if (kw1 == kv) {
+                                hr_upw = g2h; vr_upw = g2w;
+                                d_mkvoisin = d4_at(dfk, kb);
+                                zmr_upw = g2.w;
+                            } else if (kw2 == kv) {
+                                hr_upw = g1h; vr_upw = g1w;
+                                d_mkvoisin = d4_at(dfk, ka);
+                                zmr_upw = g1.w;
+                            } else {
+                                const double d1 = d4_at(dfk, ka), d2 = d4_at(dfk, kb);
+                                hr_upw = (d1 * g1h + d2 * g2h) / (d1 + d2);
+                                vr_upw = (d1 * g1w + d2 * g2w) / (d1 + d2);
+                                zmr_upw = (d1 * g1.w + d2 * g2.w) / (d1 + d2);
+                                d_mkvoisin = (d1 + d2) / 2.0;
+                            }
+                        }
+                        // upwind hydrostatic reconstruction (flux_riemann.cuf:373,376,384-400)
+                        hl_upw = dmax(0.0, (hl_upw + zml_upw) - zij);
+                        hr_upw = dmax(0.0, (hr_upw + zmr_upw) - zij);
+                        if (hl_upw < tls) { hl_upw = 0.0; vl_upw = 0.0; }
+                        if (hr_upw < tls) { hr_upw = 0.0; vr_upw = 0.0; }
+                        waf_flux(g, p.ilimiteur, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr,
+                                 hl_upw, vl_upw, hr_upw, vr_upw, d_mrl, d_mkvol, d_mkvoisin, dt, flux);
+                    }
                 }
             }
             fl[j][0] = flux[0] * ln;
@@ -258,7 +300,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             if (nb_at(nb, j) >= 0) {
-                const double zij = dmax(zml, fzr[j]);
+                const double zij = dmax(zml, unb[j].w);
                 const double hijet = dmax(0.0, (q1 + zml) - zij);
                 sb1 = sb1 + (((g / 2.0) * (hijet * hijet - q1 * q1)) * fln[j]) * fn1[j];
                 sb2 = sb2 + (((g / 2.0) * (hijet * hijet - q1 * q1)) * fln[j]) * fn2[j];
@@ -336,58 +378,75 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepArgs a) {
         if (p.is_dt_constant == 1) dti = p.constant_dt;
     }
 
-    // dt reduction: wavefront shuffles, LDS across waves, one atomic per group
-    double m = wave_min(dti);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) s_min[wid] = m;
+    // per-workgroup dt minimum: wavefront shuffles, then LDS across waves;
+    // one plain store per workgroup (reduced by finalize_kernel)
+    const double m = wave_min(dti);
+    if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = m;
     const int anynan = __syncthreads_or(isnan_local);
     if (threadIdx.x == 0) {
         double bm = s_min[0];
 #pragma unroll
         for (int w = 1; w < kWaves; w++) bm = dmin(bm, s_min[w]);
-        atomicMin(&a.c.dtbits[a.step + 1], (unsigned long long)__double_as_longlong(bm));
-        if (anynan) atomicOr(&a.c.hdr[1], 1);
-        __threadfence();
-        const unsigned ticket = atomicAdd(&a.c.counter[a.step], 1u);
-        if (ticket == gridDim.x - 1) {
-            // last workgroup: publish dt, tc and status for the next step
-            __threadfence();
-            const unsigned long long bits = atomicAdd(&a.c.dtbits[a.step + 1], 0ull);
-            const double dtn = __longlong_as_double((long long)bits);
-            const int nanf = atomicAdd(&a.c.hdr[1], 0);
-            const double tcn = tc + dt;
-            a.c.dt[a.step + 1] = dtn;
-            a.c.tc[a.step + 1] = tcn;
-            a.c.hdr[2] = a.step + 1;
-            if (nanf || !(dtn <= 9999.0 && dtn >= kDtLow)) a.c.hdr[0] = -1;
-            else if (!(((tcn - p.ts) + 2.0 * p.tol) < p.tol)) a.c.hdr[0] = 1;
-            __threadfence();
-        }
+        a.c.partial[blockIdx.x] = anynan ? __builtin_nan("") : bm;
     }
 }
 
-// Prepares the control block for a run of nsteps steps.
-__global__ void ctrl_init_kernel(Ctrl c, int nsteps, double dt0, double tc0, double pad,
-                                 double ts, double tol) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned long long padbits = (unsigned long long)__double_as_longlong(pad);
-    for (int k = t; k < nsteps; k += gridDim.x * blockDim.x) {
-        c.counter[k] = 0u;
-        c.dtbits[k + 1] = padbits;
+// Reduces a set of per-workgroup minima (NaN marks a NaN state) starting
+// from `pad`; thread 0 gets the result and the NaN flag.
+__device__ __forceinline__ double block_min_1024(const double *__restrict__ part, int n, double pad, int &nan) {
+    __shared__ double s[1024 / 64];
+    double m = pad;
+    int bad = 0;
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const double v = part[k];
+        if (v != v) bad = 1;
+        else m = dmin(m, v);
     }
-    if (t == 0) {
-        c.dt[0] = dt0;
-        c.tc[0] = tc0;
-        c.hdr[1] = 0;
-        c.hdr[2] = 0;
-        c.hdr[0] = (((tc0 - ts) + 2.0 * tol) < tol) ? 0 : 1;
+    m = wave_min(m);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
+    nan = __syncthreads_or(bad);
+    double r = s[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = dmin(r, s[w]);
+    return r;
+}
+
+// End of a step: dt of the new state (the reference's reduction, cfl_condition.cuf:68-104),
+// tc += dt, dt sanity check (:129-132) and the loop condition (finite_volumes_method.cuf:123).
+__global__ __launch_bounds__(1024) void finalize_kernel(Ctrl c, int nparts, int step, double pad,
+                                                         double ts, double tol) {
+    if (c.hdr[0] != 0) return;
+    int nan;
+    const double dtn = block_min_1024(c.partial, nparts, pad, nan);
+    if (threadIdx.x == 0) {
+        const double tcn = c.tc[step] + c.dt[step];
+        c.dt[step + 1] = dtn;
+        c.tc[step + 1] = tcn;
+        c.hdr[1] = step + 1;
+        if (nan || !(dtn <= 9999.0 && dtn >= kDtLow)) c.hdr[0] = -1;
+        else if (!(((tcn - ts) + 2.0 * tol) < tol)) c.hdr[0] = 1;
     }
 }
 
-// cflcond + reduction for the current state (finite_volumes_method.cuf:113-116)
+// dt of the current state (finite_volumes_method.cuf:113-116), into *out.
+__global__ __launch_bounds__(1024) void reduce_dt_kernel(const double *part, int nparts, double pad, double *out) {
+    int nan;
+    const double m = block_min_1024(part, nparts, pad, nan);
+    if (threadIdx.x == 0) *out = nan ? __builtin_nan("") : m;
+}
+
+// Prepares the control block for a run of steps.
+__global__ void ctrl_init_kernel(Ctrl c, double dt0, double tc0, double ts, double tol) {
+    c.dt[0] = dt0;
+    c.tc[0] = tc0;
+    c.hdr[1] = 0;
+    c.hdr[0] = (((tc0 - ts) + 2.0 * tol) < tol) ? 0 : 1;
+}
+
+// cflcond for the current state, one minimum per workgroup (cfl_condition.cuf:4-39);
+// like the reference, the initial dt does not look for NaN states
 __global__ __launch_bounds__(kBlock) void cfl_kernel(const double4 *__restrict__ st,
                                                       const double *__restrict__ cot, int nelt,
-                                                      KParams p, unsigned long long *out) {
+                                                      KParams p, double *__restrict__ part) {
     __shared__ double s_min[kWaves];
     const int i = blockIdx.x * kBlock + threadIdx.x;
     double dti = 1000000.0;
@@ -402,17 +461,16 @@ __global__ __launch_bounds__(kBlock) void cfl_kernel(const double4 *__restrict__
         }
         if (p.is_dt_constant == 1) dti = p.constant_dt;
     }
-    double m = wave_min(dti);
+    const double m = wave_min(dti);
     if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         double bm = s_min[0];
         for (int w = 1; w < kWaves; w++) bm = dmin(bm, s_min[w]);
-        atomicMin(out, (unsigned long long)__double_as_longlong(bm));
+        part[blockIdx.x] = bm;
     }
 }
 
-__global__ void fill_u64_kernel(unsigned long long *p, unsigned long long v) { *p = v; }
 
 // Geometry of each cell (maillage.cuf:132-159, :1254-1331, flux_riemann.cuf:50-74).
 __global__ void geometry_kernel(const int *__restrict__ conn, const double *__restrict__ coords,
@@ -521,7 +579,9 @@ struct cf_handle {
     double *stage = nullptr;
     Ctrl c{};
     int cap = 0;
-    unsigned long long *dt0bits = nullptr;
+    double *dt0 = nullptr;     // initial dt of the current state
+    int nblocks = 0;           // workgroups of a step launch
+    int xcd_map = 1;
     double dt = 0.0, tc = 0.0;
     int status = 0;
     int fric = 0;
@@ -571,26 +631,25 @@ static int upload_hydro(cf_handle *h) {
 }
 
 static int compute_dt(cf_handle *h) {
-    const unsigned long long init = (unsigned long long)__builtin_bit_cast(long long, h->kp.pad_dt);
-    hipLaunchKernelGGL(fill_u64_kernel, dim3(1), dim3(1), 0, h->stream, h->dt0bits, init);
-    hipLaunchKernelGGL(cfl_kernel, dim3(grid_of(h->nelt, kBlock)), dim3(kBlock), 0, h->stream,
-                       h->st[h->cur], h->geo + G_COT * h->nelt, h->nelt, h->kp, h->dt0bits);
-    unsigned long long bits = 0;
-    HIPCHK(hipMemcpyAsync(&bits, h->dt0bits, sizeof(bits), hipMemcpyDeviceToHost, h->stream));
+    hipLaunchKernelGGL(cfl_kernel, dim3(h->nblocks), dim3(kBlock), 0, h->stream,
+                       h->st[h->cur], h->geo + G_COT * h->nelt, h->nelt, h->kp, h->c.partial);
+    hipLaunchKernelGGL(reduce_dt_kernel, dim3(1), dim3(1024), 0, h->stream, h->c.partial, h->nblocks,
+                       h->kp.pad_dt, h->dt0);
+    HIPCHK(hipGetLastError());
+    double dt = 0.0;
+    HIPCHK(hipMemcpyAsync(&dt, h->dt0, sizeof(dt), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    h->dt = __builtin_bit_cast(double, bits);
+    h->dt = dt;
     return CF_OK;
 }
 
 static int ensure_ctrl(cf_handle *h, int nsteps) {
     if (nsteps <= h->cap) return CF_OK;
     int cap = std::max(nsteps, 2 * h->cap);
-    if (h->c.counter) {
+    if (h->c.dt) {
         HIPCHK(hipStreamSynchronize(h->stream));
-        dfree(h->c.counter); dfree(h->c.dtbits); dfree(h->c.dt); dfree(h->c.tc);
+        dfree(h->c.dt); dfree(h->c.tc);
     }
-    HIPCHK(hipMalloc(&h->c.counter, sizeof(unsigned) * (size_t)cap));
-    HIPCHK(hipMalloc(&h->c.dtbits, sizeof(unsigned long long) * ((size_t)cap + 1)));
     HIPCHK(hipMalloc(&h->c.dt, sizeof(double) * ((size_t)cap + 1)));
     HIPCHK(hipMalloc(&h->c.tc, sizeof(double) * ((size_t)cap + 1)));
     h->cap = cap;
@@ -721,8 +780,8 @@ static int validate_params(const cf_params *p) {
 static void destroy_buffers(cf_handle *h) {
     dfree(h->st[0]); dfree(h->st[1]); dfree(h->nbr); dfree(h->geo); dfree(h->dfc);
     dfree(h->manning); dfree(h->hydro); dfree(h->zm); dfree(h->stage);
-    dfree(h->c.hdr); dfree(h->c.counter); dfree(h->c.dtbits); dfree(h->c.dt); dfree(h->c.tc);
-    dfree(h->dt0bits);
+    dfree(h->c.hdr); dfree(h->c.partial); dfree(h->c.dt); dfree(h->c.tc);
+    dfree(h->dt0);
 }
 
 extern "C" int cf_create(const cf_mesh *mesh, const cf_params *params, cf_handle **out) {
@@ -783,6 +842,11 @@ extern "C" int cf_create(const cf_mesh *mesh, const cf_params *params, cf_handle
     h->nnt = mesh->nnt;
     h->prm = *params;
     h->fric = params->friction == 0 ? 0 : 1 + params->fricimplic;
+    h->nblocks = grid_of(nelt, kBlock);
+    {
+        const char *env = getenv("CUTEFLOW_XCD_MAP");
+        h->xcd_map = env ? atoi(env) != 0 : 1;
+    }
     fill_kparams(h);
     int rc = CF_OK;
     int *dconn = nullptr;
@@ -800,7 +864,8 @@ extern "C" int cf_create(const cf_mesh *mesh, const cf_params *params, cf_handle
             hipMalloc(&h->zm, N * sizeof(double)) != hipSuccess ||
             hipMalloc(&h->stage, 3 * N * sizeof(double)) != hipSuccess ||
             hipMalloc(&h->c.hdr, 4 * sizeof(int)) != hipSuccess ||
-            hipMalloc(&h->dt0bits, sizeof(unsigned long long)) != hipSuccess ||
+            hipMalloc(&h->dt0, sizeof(double)) != hipSuccess ||
+            hipMalloc(&h->c.partial, (size_t)grid_of(nelt, kBlock) * sizeof(double)) != hipSuccess ||
             hipMalloc(&dconn, 3 * N * sizeof(int)) != hipSuccess ||
             hipMalloc(&dcoords, 3 * (size_t)mesh->nnt * sizeof(double)) != hipSuccess) {
             rc = CF_ERR_NOMEM;
@@ -916,9 +981,8 @@ extern "C" int cf_step_async(cf_handle *h, int nsteps) {
     if (rc) return rc;
     h->pending = nsteps;
     if (nsteps == 0) return CF_OK;
-    const int grid = grid_of(h->nelt, kBlock);
-    hipLaunchKernelGGL(ctrl_init_kernel, dim3(std::min(grid_of(nsteps, 256), 1024)), dim3(256), 0, h->stream,
-                       h->c, nsteps, h->dt, h->tc, h->kp.pad_dt, h->kp.ts, h->kp.tol);
+    hipLaunchKernelGGL(ctrl_init_kernel, dim3(1), dim3(1), 0, h->stream, h->c, h->dt, h->tc, h->kp.ts,
+                       h->kp.tol);
     StepArgs a;
     a.geo = h->geo;
     a.nbr = h->nbr;
@@ -927,13 +991,17 @@ extern "C" int cf_step_async(cf_handle *h, int nsteps) {
     a.hydro = h->hydro;
     a.c = h->c;
     a.nelt = h->nelt;
+    a.xcd_map = h->xcd_map;
+    a.reserved = 0;
     a.p = h->kp;
     step_launcher L = pick_launcher(h->prm.iflux, h->fric);
     for (int k = 0; k < nsteps; k++) {
         a.step = k;
         a.st_in = h->st[(h->cur + k) & 1];
         a.st_out = h->st[(h->cur + k + 1) & 1];
-        L(h, a, grid);
+        L(h, a, h->nblocks);
+        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, h->stream, h->c, h->nblocks, k,
+                           h->kp.pad_dt, h->kp.ts, h->kp.tol);
     }
     HIPCHK(hipGetLastError());
     return CF_OK;
@@ -952,7 +1020,7 @@ extern "C" int cf_step_finish(cf_handle *h, double *dt_hist, int *steps_done) {
     int hdr[4];
     HIPCHK(hipMemcpyAsync(hdr, h->c.hdr, sizeof(hdr), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    const int done = hdr[2];
+    const int done = hdr[1];
     std::vector<double> dts((size_t)done + 1), tcs(1);
     HIPCHK(hipMemcpyAsync(dts.data(), h->c.dt, sizeof(double) * ((size_t)done + 1), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipMemcpyAsync(tcs.data(), h->c.tc + done, sizeof(double), hipMemcpyDeviceToHost, h->stream));

@@ -186,7 +186,7 @@ def write_boundary_table(path: str, bnd: np.ndarray) -> None:
     rows = np.concatenate([bnd[0].T, bnd[1].T], axis=1)
     with open(path, "w") as fh:
         for r in rows:
-            fh.write(" ".join(f"{int(v):12d}" for v in r) + "\n")
+            fh.write(" " + " ".join(f"{int(v):12d}" for v in r) + "\n")
     assert rows.shape == (n, 6)
 
 

@@ -895,11 +895,41 @@ int or_run(const or_params *p, int nelt, const int *conn, const double *vx,
         tc = tc + dt;
         n++;
         dt = dtn;
+        /* dt range check of cfl_condition.cuf:129-132 applied to the new dt;
+           NaN states are an error too (the reference does not catch them) */
         if (!(dt <= 9999.0 && dt >= LIT_1EM12)) status = -1;
+        for (size_t i = 0; i < n3 && status == 0; i++)
+            if (U[i] != U[i]) status = -1;
     }
     dt_hist[n] = dt;
     if (tc_out) *tc_out = tc;
     if (steps_done) *steps_done = n;
     free(U1); free(resc); free(sb); free(sf); free(afm1); free(qc);
     return status;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Direct access to the face solvers and the limiter for the pin tests.     */
+
+void or_riemann_face(int iflux, int ilimiteur, double gp, double n1, double n2,
+                     double hl, double ul, double vl, double hr, double ur, double vr,
+                     double hl_upw, double vl_upw, double hr_upw, double vr_upw,
+                     double d_mrl, double d_mkvol, double d_mkvoisin, double dt,
+                     double flux[3])
+{
+    if (iflux == 1) {
+        or_hllc(gp, n1, n2, hl, ul, vl, hr, ur, vr, flux);
+    } else {
+        or_params p;
+        memset(&p, 0, sizeof(p));
+        p.gp = gp;
+        p.ilimiteur = ilimiteur;
+        or_waf(&p, n1, n2, hl, ul, vl, hr, ur, vr, hl_upw, vl_upw, hr_upw, vr_upw,
+               d_mrl, d_mkvol, d_mkvoisin, dt, flux);
+    }
+}
+
+double or_limiter_value(int ilimiteur, double r, double c)
+{
+    return or_limiter(ilimiteur, r, c);
 }

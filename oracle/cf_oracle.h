@@ -109,6 +109,14 @@ int or_run(const or_params *p, int nelt, const int *conn, const double *vx,
            double *U, int nsteps, double *dt_hist, double *tc_out,
            int *steps_done, double *qout_hist, int nthreads);
 
+/* face solvers (flux in the global frame) and limiter, for the pin tests */
+void or_riemann_face(int iflux, int ilimiteur, double gp, double n1, double n2,
+                     double hl, double ul, double vl, double hr, double ur, double vr,
+                     double hl_upw, double vl_upw, double hr_upw, double vr_upw,
+                     double d_mrl, double d_mkvol, double d_mkvoisin, double dt,
+                     double flux[3]);
+double or_limiter_value(int ilimiteur, double r, double c);
+
 #ifdef __cplusplus
 }
 #endif

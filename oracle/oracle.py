@@ -60,6 +60,11 @@ def load():
         lib.or_cfl_dt.restype = C.c_double
         lib.or_run.restype = C.c_int
         lib.or_inflow_q.restype = C.c_double
+        lib.or_inflow_q.argtypes = [C.POINTER(OrParams), C.c_int, C.c_double]
+        lib.or_riemann_face.restype = None
+        lib.or_riemann_face.argtypes = [C.c_int, C.c_int] + [C.c_double] * 17 + [_D]
+        lib.or_limiter_value.restype = C.c_double
+        lib.or_limiter_value.argtypes = [C.c_int, C.c_double, C.c_double]
         _lib = lib
     return _lib
 
@@ -164,6 +169,23 @@ def run(prep: Prepared, params, U0: np.ndarray, nsteps: int, ts: float | None = 
     if qout:
         out["qout"] = qo[: done.value]
     return out
+
+
+def riemann(iflux, left, right, n=(1.0, 0.0), gp=9.81, ilimiteur=1, upw=(0.0, 0.0, 0.0, 0.0),
+            d=(1.0, 1.0, 1.0), dt=0.0):
+    """Face flux (global frame) for face-frame states left=(h,un,ut), right=(h,un,ut)."""
+    out = np.zeros(3)
+    load().or_riemann_face(int(iflux), int(ilimiteur), gp, n[0], n[1], *left, *right, *upw, *d, dt, _d(out))
+    return out
+
+
+def limiter(ilimiteur, r, c):
+    return load().or_limiter_value(int(ilimiteur), float(r), float(c))
+
+
+def inflow_q(params, inlet, tc):
+    p = or_params(params, np.zeros(MAX_IO))
+    return load().or_inflow_q(C.byref(p), int(inlet) - 1, float(tc))
 
 
 def pow_(x: float, y: float) -> float:

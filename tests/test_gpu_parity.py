@@ -101,13 +101,34 @@ def test_dam_break_hllc(n, steps):
     assert tc == ref["tc"]
 
 
-@pytest.mark.parametrize("fricimplic", [0, 1, 2])
+@pytest.mark.parametrize("fricimplic", [1, 2])
 def test_river_waf_friction(fricimplic):
     m, p, U0 = river_case(fricimplic=fricimplic)
     ref, U, hist, done, st, _, _ = _run_both(m, p, U0, 300)
     assert st == 0 and ref["status"] == 0 and done == 300
     _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
     assert np.sum(U[:, 0] > p.tolisec) > 0
+
+
+def test_river_explicit_friction_blowup_matches():
+    """Explicit friction (fricimplic 0) on the shallow wetting front is
+    unstable: dt collapses below 1e-12 after a few steps.  Both sides must
+    stop at the same step with the same state and dt sequence."""
+    m, p, U0 = river_case(fricimplic=0)
+    ref, U, hist, done, st, _, _ = _run_both(m, p, U0, 300)
+    assert ref["status"] == -1 and st == -3
+    assert done == ref["steps"] < 300
+    assert np.array_equal(hist, ref["dt_hist"])
+    assert np.array_equal(U, ref["U"])
+
+
+def test_dam_break_explicit_friction():
+    m, p, U0 = dam_case(24)
+    p.friction, p.fricimplic = 1, 0
+    m.manning[:] = 0.03
+    ref, U, hist, done, st, _, _ = _run_both(m, p, U0, 300)
+    assert st == 0 and ref["status"] == 0 and done == 300
+    _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
 
 
 @pytest.mark.parametrize("ilim", [0, 1, 2, 3, 4])

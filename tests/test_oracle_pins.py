@@ -1,0 +1,265 @@
+"""Pins of the CPU oracle (no GPU): reference-held fixtures, known answers of
+the reference's run logs, closed forms, invariants and special cases.
+
+Fixtures (tests/golden/, extracted by make_fixtures.py from the reference's
+calibration cases): the two HDF5 CGNS meshes with their inflow/outflow node
+lists, the reference's own boundary tables written by those runs, and the
+parameter block of the run logs.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from cuteflow_amd import mesh as M
+from cuteflow_amd.params import f32
+from oracle import oracle as O
+
+from cases import dam_case, lake_case, river_case
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = ["divergent_canal", "river_piers"]
+
+
+def _fixture(name):
+    m = M.read_text_mesh(os.path.join(GOLDEN, f"{name}.mesh"))
+    table = M.read_boundary_table(os.path.join(GOLDEN, f"{name}.boundary_table"))
+    exp = json.load(open(os.path.join(GOLDEN, f"{name}.expected.json")))
+    return m, table, exp
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_boundary_table_matches_reference(name):
+    """Neighbours (reversed-edge rule) and boundary classes/ids equal the
+    reference's own boundary_table file, entry for entry."""
+    m, table, exp = _fixture(name)
+    assert (m.nnt, m.nelt) == (exp["nnt"], exp["nelt"])
+    prep = O.Prepared(m, allpairs=True)
+    assert np.array_equal(prep.boundary_fortran(walls=False), table)
+    assert np.sum(table[0] == -1) > 0 and np.sum(table[0] == -2) > 0
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_product_boundary_builder_matches_reference(name):
+    """The library's O(N log N) builder (host code, no GPU) gives the same table."""
+    m, table, _ = _fixture(name)
+    assert np.array_equal(m.boundary_table(), table)
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_geometry_known_answers(name):
+    """cotemin and the inlet length printed by the reference's GPU runs."""
+    m, _, exp = _fixture(name)
+    prep = O.Prepared(m)
+    for log in exp["logs"].values():
+        cot = float(log["cotemin"])
+        if name == "divergent_canal":
+            assert prep.cotemin == cot                      # all 17 digits
+        else:
+            # list-directed output printed 16 significant digits
+            assert "%.16g" % prep.cotemin == "%.16g" % cot == log["cotemin"]
+        assert prep.long_entree[0] == float(log["long_entree_1"])
+        assert f32(1e-6) == float(log["tol"])               # default-real widening
+        assert tuple(log["nelt_nnt"]) == (m.nelt, m.nnt)
+    # the double literal 1.8d0 would not reproduce the canal value
+    assert name != "divergent_canal" or prep.cotemin != prep.cotemin / f32(1.8) * 1.8
+
+
+def test_neighbour_rules_agree():
+    for seed in range(3):
+        m = M.dam_break_mesh(14, jitter=0.25, seed=seed)
+        perm = M.random_order(m, seed)
+        m = m.permuted(perm)
+        a, b = O.Prepared(m, allpairs=True), O.Prepared(m, allpairs=False)
+        assert np.array_equal(a.b1, b.b1) and np.array_equal(a.b2, b.b2)
+        assert np.array_equal(a.boundary_fortran(), m.boundary_table())
+
+
+def test_geometry_closed_forms():
+    # right triangle 3-4-5 and an equilateral triangle
+    coords = np.array([[0, 0, 1.0], [3, 0, 2.0], [0, 4, 3.0], [10, 0, 0], [12, 0, 0], [11, math.sqrt(3), 0]])
+    m = M.Mesh(coords=coords, conn=np.array([[1, 2, 3], [4, 5, 6]]), manning=np.zeros(2))
+    prep = O.Prepared(m)
+    assert prep.surf[0] == 6.0
+    assert abs(prep.surf[1] - math.sqrt(3)) < 1e-15
+    # 1.8 * 0.5*area/half-perimeter (maillage.cuf:156-157) = 0.9 * inradius,
+    # inradius (3+4-5)/2 = 1 here
+    assert prep.cot[0] == f32(1.8) * 0.5
+    assert abs(prep.cot[1] - f32(1.8) * 0.5 * (math.sqrt(3) / 3)) < 1e-15
+    d = np.array([math.hypot(0 - 1, 0 - 4 / 3), math.hypot(3 - 1, 0 - 4 / 3), math.hypot(0 - 1, 4 - 4 / 3)])
+    assert abs(prep.zm[0] - (d @ [1, 2, 3]) / d.sum()) < 1e-15
+    m2 = M.dam_break_mesh(30, size=200.0)
+    assert abs(O.Prepared(m2).surf.sum() - 200.0 ** 2) < 1e-8
+
+
+def test_pow_accuracy():
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([10.0 ** rng.uniform(-8, 4, 20000), rng.uniform(0.5, 2.0, 5000), [1.0, 2.0, 1e-300, 5e-324]])
+    for y in (-1.0 / 3.0, -4.0 / 3.0, 1.0 / 3.0, 0.5, 3.7):
+        for x in xs:
+            r = O.pow_(x, y)
+            try:
+                e = math.pow(x, y)
+            except OverflowError:
+                assert math.isinf(r)
+                continue
+            if e == 0.0 or math.isinf(e) or e < 1e-300:
+                continue
+            assert abs(r - e) <= 2 * math.ulp(e), (x, y, r, e)
+    assert O.pow_(8.0, 1.0 / 3.0) == 2.0
+    assert math.isnan(O.pow_(0.0, 0.5)) and math.isnan(O.pow_(-1.0, 0.5))
+
+
+def test_hllc_consistency_and_symmetry():
+    g = 9.81
+    for (h, u, v) in [(1.0, 0.3, -0.2), (2.5, -1.0, 0.7), (0.01, 0.0, 0.0)]:
+        f = O.riemann(1, (h, u, v), (h, u, v), n=(1.0, 0.0))
+        phys = np.array([h * u, h * u * u + g * h * h / 2, h * u * v])
+        assert np.allclose(f, phys, rtol=1e-7, atol=1e-12)
+    # supersonic to the right: exactly the left physical flux
+    f = O.riemann(1, (1.0, 5.0, 0.1), (0.8, 4.0, 0.0), n=(1.0, 0.0))
+    assert np.array_equal(f, [1.0 * 5.0, 1.0 * 25.0 + (g * 1.0 * 1.0) / 2, (1.0 * 5.0) * 0.1])
+    # mirror symmetry: swapping sides and flipping the normal negates the flux
+    L, R = (1.2, 0.4, 0.1), (0.7, -0.3, 0.5)
+    f1 = O.riemann(1, L, R, n=(0.6, 0.8))
+    f2 = O.riemann(1, (R[0], -R[1], -R[2]), (L[0], -L[1], -L[2]), n=(-0.6, -0.8))
+    assert np.allclose(f1, -f2, rtol=1e-7, atol=1e-12)
+    # dry/dry: zero flux
+    assert np.all(O.riemann(1, (0.0, 0, 0), (0.0, 0, 0), n=(0.6, 0.8)) == 0)
+    # wall-like: symmetric states with opposite normal velocity give no mass flux
+    f = O.riemann(1, (1.0, 0.5, 0.0), (1.0, -0.5, 0.0))
+    assert abs(f[0]) < 1e-12 and f[1] > g / 2
+
+
+def test_waf_reduces_to_hllc():
+    """ilimiteur = 0 (A = 1) makes WAF the HLLC flux (flux_riemann.cuf:475-476)."""
+    rng = np.random.default_rng(2)
+    for _ in range(200):
+        L = (rng.uniform(0.1, 3), rng.uniform(-2, 2), rng.uniform(-1, 1))
+        R = (rng.uniform(0.1, 3), rng.uniform(-2, 2), rng.uniform(-1, 1))
+        n = rng.normal(size=2)
+        n /= np.linalg.norm(n)
+        fw = O.riemann(3, L, R, n=tuple(n), ilimiteur=0, upw=(1.0, 0.2, 1.1, 0.1), d=(2.0, 2.0, 2.0), dt=0.05)
+        fh = O.riemann(1, L, R, n=tuple(n))
+        assert np.allclose(fw, fh, rtol=1e-7, atol=1e-9)
+
+
+def test_limiter_closed_forms():
+    c = 0.3
+    assert O.limiter(1, -1.0, c) == 1.0
+    assert O.limiter(1, 0.25, c) == 1.0 - 2.0 * (1.0 - c) * 0.25
+    assert O.limiter(1, 0.75, c) == c
+    assert O.limiter(1, 1.5, c) == 1.0 - (1.0 - c) * 1.5
+    assert O.limiter(1, 3.0, c) == 2.0 * c - 1.0
+    # the reference's superbee sends r = 0.5 and r = 1 to the last branch
+    assert O.limiter(1, 0.5, c) == 2.0 * c - 1.0 and O.limiter(1, 1.0, c) == 2.0 * c - 1.0
+    assert O.limiter(2, 2.0, c) == 1.0 - 2.0 * (1.0 - c) * 2.0 / 3.0
+    assert O.limiter(3, 2.0, c) == 1.0 - (1.0 - c) * 2.0 * 3.0 / 5.0
+    assert O.limiter(4, 0.5, c) == 1.0 - (1.0 - c) * 0.5 and O.limiter(4, 2.0, c) == c
+    assert O.limiter(0, 0.5, c) == 1.0 and O.limiter(7, 0.5, c) == 1.0
+
+
+def test_dt_closed_form():
+    m, p, U0 = dam_case(16)
+    prep = O.Prepared(m)
+    U = np.zeros_like(U0)
+    U[:, 0] = 2.0
+    U[:, 1] = 2.0 * 0.5
+    dt = O.cfl_dt(prep, p, U)
+    expect = min(np.min(p.cfl * prep.cot / (0.5 + np.sqrt(9.81 * 2.0))), 1e5)
+    assert dt == expect
+    U[:] = 0.0
+    assert O.cfl_dt(prep, p, U) == 1e5        # all dry: the 1e5 padding wins
+
+
+def test_mass_conservation_walls():
+    m, p, U0 = dam_case(24)
+    prep = O.Prepared(m)
+    r = O.run(prep, p, U0, 300)
+    v0 = np.sum(U0[:, 0] * prep.surf)
+    v1 = np.sum(r["U"][:, 0] * prep.surf)
+    assert r["status"] == 0 and abs(v1 - v0) <= 1e-12 * v0
+
+
+@pytest.mark.parametrize("iflux", [1, 3])
+def test_lake_at_rest(iflux):
+    m, p = lake_case(iflux=iflux)
+    prep = O.Prepared(m)
+    U0 = np.zeros((m.nelt, 3))
+    U0[:, 0] = np.where(-prep.zm > p.tolisec, -prep.zm, 0.0)
+    r = O.run(prep, p, U0, 300)
+    wet = r["U"][:, 0] > 0
+    assert r["status"] == 0
+    assert np.max(np.abs(r["U"][wet, 0] + prep.zm[wet])) < 1e-6
+    assert np.max(np.abs(r["U"][:, 1:])) < 1e-6
+
+
+@pytest.mark.parametrize("fricimplic", [0, 1, 2])
+def test_friction_single_cell_recurrence(fricimplic):
+    """One closed triangle (walls), flat bed, uniform velocity along x: only
+    friction acts; compare with the closed-form scalar recurrence."""
+    coords = np.array([[0.0, 0.0, 0.0], [10.0, 0.0, 0.0], [0.0, 10.0, 0.0]])
+    m = M.Mesh(coords=coords, conn=np.array([[1, 2, 3]]), manning=np.array([0.03]))
+    from cuteflow_amd.params import Params
+    p = Params(iflux=1, tolisec=1e-5, cfl=0.9, ts=1e300, inlet="transm", outlet="transm",
+               friction=1, fricimplic=fricimplic, is_dt_constant=1, constant_dt=0.01)
+    prep = O.Prepared(m)
+    h, u, n, g, dt = 1.5, 2.0, 0.03, 9.81, 0.01
+    U0 = np.array([[h, h * u, 0.0]])
+    r = O.run(prep, p, U0, 50)
+    hu = h * u
+    for _ in range(50):
+        vel = hu / h
+        sf = -g * n * n * vel * abs(vel) * h ** (-1 / 3)
+        if fricimplic == 0:
+            hu = hu + dt * sf
+        elif fricimplic == 1:
+            dtgn2h = dt / 2 * g * n * n * h ** (-4 / 3)
+            hu = hu + dt * sf / (1 + 2 * dtgn2h * abs(vel))
+        else:
+            beta = -dt * g * n * n * abs(vel) * h ** (-4 / 3)
+            hu = hu + dt * sf / (1 - beta)
+    assert r["status"] == 0
+    assert abs(r["U"][0, 0] - h) < 1e-12
+    assert abs(r["U"][0, 1] - hu) < 1e-10 * abs(hu)
+    assert abs(r["U"][0, 2]) < 1e-12
+
+
+def test_hydrograph_interpolation():
+    from cuteflow_amd.params import Params
+    p = Params()
+    p.hydrographs = {1: ([0.0, 100.0, 300.0], [10.0, 50.0, 30.0])}
+    assert O.inflow_q(p, 1, -5.0) == 10.0
+    assert O.inflow_q(p, 1, 50.0) == 30.0
+    assert O.inflow_q(p, 1, 200.0) == 40.0
+    assert O.inflow_q(p, 1, 400.0) == 30.0
+    p.debitglob[1] = 7.0
+    assert O.inflow_q(p, 2, 123.0) == 7.0
+
+
+def test_inflow_discharge_enters():
+    """With walls and one inlet on a fully wet reach (no wet/dry clamping),
+    the volume grows by the integral of Q(t)."""
+    m, p, U0 = river_case(nx=30, ny=9, friction=0)
+    p.outlet = "fixedheight"
+    m.ndoutput = np.zeros(0, np.int32)
+    m.numero_ndoutput = np.zeros(0, np.int32)
+    prep = O.Prepared(m)
+    U0 = np.zeros((m.nelt, 3))
+    U0[:, 0] = (prep.zm.max() + 2.0) - prep.zm
+    r = O.run(prep, p, U0, 100)
+    dts = r["dt_hist"][:-1]
+    tcs = np.concatenate([[0.0], np.cumsum(dts)[:-1]])
+    q_in = np.array([O.inflow_q(p, 1, t) for t in tcs])
+    vol = np.sum(r["U"][:, 0] * prep.surf) - np.sum(U0[:, 0] * prep.surf)
+    assert r["status"] == 0
+    assert abs(vol - np.sum(q_in * dts)) <= 1e-9 * np.sum(q_in * dts)
+
+
+def test_nan_state_is_an_error():
+    m, p, U0 = dam_case(10)
+    U0[5, 1] = np.nan
+    r = O.run(O.Prepared(m), p, U0, 10)
+    assert r["status"] == -1 and r["steps"] == 1
