@@ -75,6 +75,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU-baseline sample (0 disables)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--options", type=int, default=0,
+                    help="cf_params.options bits: 1 keep the input cell order, 2 run every tile")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -92,7 +94,7 @@ def main():
     t0 = time.perf_counter()
     m, p, U0 = make_workload(args.nx, args.ny, args.order)
     log(f"bench: workload nelt={m.nelt} order={args.order} built in {time.perf_counter() - t0:.1f}s")
-    s = Solver(m, p)
+    s = Solver(m, p, options=args.options)
     s.set_state(U0)
     lib = s.lib
     if args.warmup:
@@ -119,11 +121,16 @@ def main():
     wall = w1 - w0
     kern_ms = ev0.elapsed_time(ev1)
     per_launch_s = kern_ms / 1000.0 / args.steps
+    stats = s.last_run_stats()
     value = m.nelt * args.steps / wall
-    achieved = BYTES_WAF * m.nelt / per_launch_s / 1e9
+    # cells the step kernel actually processed per launch: dry tiles with dry
+    # neighbours are skipped (their update is the identity)
+    active_cells = stats["active_fraction"] * m.nelt
+    achieved = BYTES_WAF * active_cells / per_launch_s / 1e9
     wet = int(np.sum(s.get_state()[:, 0] > p.tolisec))
     log(f"bench: {args.steps} steps in {wall:.3f}s wall, {kern_ms:.2f} ms on the stream, "
-        f"{value:.3e} cell-updates/s, {achieved:.0f} GB/s algorithmic, wet cells {wet}")
+        f"{value:.3e} cell-updates/s, active tiles {stats['active_fraction']:.3f}, "
+        f"{achieved:.0f} GB/s algorithmic over processed cells, wet cells {wet}")
 
     workload_key = f"river_waf_{m.nelt}_{args.order}"
     traffic = traffic_from_profile(workload_key)
@@ -146,6 +153,8 @@ def main():
                         "0->3000 m3/s over 1 h, fixed outlet level, dry threshold 1e-3 m",
             "nelt": m.nelt,
             "order": args.order,
+            "device_order": "caller's" if args.options & 1 else "Hilbert curve of centroids",
+            "dry_tile_skipping": not (args.options & 2),
             "iflux": 3,
             "fricimplic": 1,
             "tc_end": s.tc,
@@ -160,7 +169,11 @@ def main():
             "traffic": traffic,
             "kernel": "step_kernel<3,2>",
             "bytes_per_cell_update": BYTES_WAF,
+            "cells_per_launch": active_cells,
+            "active_tile_fraction": stats["active_fraction"],
             "avg_launch_us": per_launch_s * 1e6,
+            "timing": "HIP events on the solver stream over the timed steps (step kernel + "
+                      "one-workgroup finalize kernel per step)",
         },
     }
 

@@ -16,6 +16,9 @@ from .build import LIB_PATH
 CF_MAX_IO = 20
 CF_MAX_HYDRO = 64
 
+CF_OPT_KEEP_ORDER = 1
+CF_OPT_NO_TILE_SKIP = 2
+
 CF_OK = 0
 CF_ERR_ARG = -1
 CF_ERR_HIP = -2
@@ -45,7 +48,7 @@ class CfParams(C.Structure):
         ("is_dt_constant", C.c_int),
         ("is_override_manning", C.c_int),
         ("nombre_entrees", C.c_int),
-        ("reserved0", C.c_int),
+        ("options", C.c_int),
         ("debitglob", C.c_double * CF_MAX_IO),
         ("h_sortie", C.c_double * CF_MAX_IO),
         ("long_entree", C.c_double * CF_MAX_IO),
@@ -92,6 +95,7 @@ SIGNATURES = {
     "cf_set_hydrograph": (C.c_int, [_H, C.c_int, C.c_int, _DP, _DP]),
     "cf_set_manning": (C.c_int, [_H, _DP]),
     "cf_get_geometry": (C.c_int, [_H, _DP, _DP, _DP, _DP]),
+    "cf_last_run_stats": (C.c_int, [_H, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _IP]),
     "cf_get_stream": (C.c_void_p, [_H]),
     "cf_set_stream": (C.c_int, [_H, C.c_void_p]),
     "cf_debug_pow": (C.c_int, [C.c_int, _DP, C.c_double, _DP]),

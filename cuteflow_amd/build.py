@@ -32,7 +32,7 @@ HIP_FLAGS = [
     "-Wall",
 ]
 
-SOURCES = [os.path.join(CSRC, "cuteflow_hip.hip")]
+SOURCES = [os.path.join(CSRC, "cuteflow_hip.hip"), os.path.join(CSRC, "cf_prep.cpp")]
 
 
 def _run(cmd: list[str], cwd: str | None = None) -> None:

@@ -9,11 +9,14 @@
 //     intermediate residual arrays in HBM);
 //   - the new state goes to the other half of a ping-pong buffer (no
 //     vdlg0 <- vdlg copy, finite_volumes_method.cuf:230);
-//   - the CFL dt of the new state is reduced with wavefront shuffles, a
-//     workgroup LDS pass and one 64-bit atomic min per workgroup (dt > 0, so
-//     the IEEE bit pattern orders like an unsigned integer); the last
-//     workgroup to finish publishes dt, tc and the stop/error status for the
-//     next step, so a whole run of steps needs no host synchronisation.
+//   - the CFL dt of the new state is reduced with wavefront shuffles and a
+//     workgroup LDS pass to one plain store per workgroup; a one-workgroup
+//     finalize kernel reduces those, publishes dt, tc and the stop/error
+//     status for the next step, so a run of steps needs no host
+//     synchronisation and no device-wide fence or atomic (on gfx950 an
+//     agent-scope fence writes back and invalidates the XCD's L2);
+//   - cells are stored along a Hilbert curve (256-cell tiles are compact
+//     patches) and tiles that are dry with dry neighbours are skipped.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -27,6 +30,7 @@
 
 #include "../../include/cuteflow_hip.h"
 #include "cf_device.h"
+#include "cf_prep.h"
 
 #pragma clang fp contract(off)
 
@@ -56,6 +60,7 @@ struct Ctrl {
     double *partial;   // per workgroup: min CFL dt of the new state, NaN if a state is NaN
     double *dt;        // dt[k] used by step k
     double *tc;        // tc[k] at the start of step k
+    int *ran;          // ran[k]: tiles that ran step k (skipped tiles store +inf partials)
 };
 
 struct StepArgs {
@@ -67,6 +72,14 @@ struct StepArgs {
     const double *__restrict__ manning;
     const double *__restrict__ hydro;  // t[20*64] then q[20*64]
     Ctrl c;
+    // dry-tile skipping: a tile runs step s only if it or a neighbour tile
+    // held a wet cell after step s-1 or s-2 (stamp >= s-1), or always[tile]
+    unsigned *stamp;
+    const int *__restrict__ tile_off;
+    const int *__restrict__ tile_adj;
+    const unsigned char *__restrict__ always;
+    unsigned gstep;  // global number of step 0 of this launch
+    int skip;        // 1: dry-tile skipping enabled
     int nelt;
     int step;
     int xcd_map;  // 1: consecutive tiles on the same XCD
@@ -113,10 +126,11 @@ __device__ __forceinline__ void zero_flux(double n1, double n2, double flux[3]) 
     flux[2] = n2 * z + n1 * z;
 }
 
-// Occupancy target of the step kernel (waves per SIMD); 0 lets the compiler
-// choose the register budget.
+// Occupancy target of the step kernel (waves per SIMD): 4 waves caps the
+// kernel at 128 VGPRs without spilling (measured best on MI355X; 6 and 8
+// spill); 0 lets the compiler choose the register budget.
 #ifndef CF_STEP_WAVES_PER_EU
-#define CF_STEP_WAVES_PER_EU 0
+#define CF_STEP_WAVES_PER_EU 4
 #endif
 #if CF_STEP_WAVES_PER_EU > 0
 #define CF_STEP_OCCUPANCY __attribute__((amdgpu_waves_per_eu(CF_STEP_WAVES_PER_EU, 8)))
@@ -131,7 +145,18 @@ __global__ __launch_bounds__(kBlock) CF_STEP_OCCUPANCY void step_kernel(StepArgs
     if (a.c.hdr[0] != 0) return;  // stopped (ts reached) or failed earlier
     const int nelt = a.nelt;
     const int tile = tile_of(blockIdx.x, gridDim.x, a.xcd_map);
+    const unsigned s_glob = a.gstep + (unsigned)a.step;
+    // A tile whose cells and neighbour tiles' cells were all dry after the
+    // last two steps would recompute (0, 0, 0) for every cell (all faces
+    // dry/dry, no source, clamp), and both state buffers already hold that:
+    // it is skipped.  Reading a stamp raised concurrently only makes the
+    // tile run, which is always correct.
+    if (a.skip && !a.always[tile] && a.stamp[tile] + 1u < s_glob) {
+        if (threadIdx.x == 0) a.c.partial[blockIdx.x] = __builtin_inf();  // no dt contribution
+        return;
+    }
     const int i = tile * kBlock + threadIdx.x;
+    int wet_local = 0;
     const double dt = a.c.dt[a.step];
     const double tc = a.c.tc[a.step];
     const KParams &p = a.p;
@@ -368,6 +393,7 @@ __global__ __launch_bounds__(kBlock) CF_STEP_OCCUPANCY void step_kernel(StepArgs
         if (h <= tls) { h = 0.0; hu = 0.0; hv = 0.0; }
         a.st_out[i] = make_double4(h, hu, hv, zml);
         isnan_local = (h != h) || (hu != hu) || (hv != hv);
+        wet_local = (h != 0.0) || isnan_local;
         // cflcond on the new state (cfl_condition.cuf:20-36)
         if (h > tls) {
             const double uu = hu / h, vv = hv / h;
@@ -389,24 +415,42 @@ __global__ __launch_bounds__(kBlock) CF_STEP_OCCUPANCY void step_kernel(StepArgs
         for (int w = 1; w < kWaves; w++) bm = dmin(bm, s_min[w]);
         a.c.partial[blockIdx.x] = anynan ? __builtin_nan("") : bm;
     }
+    // a wet tile keeps itself and its neighbour tiles running for two steps
+    if (a.skip && __syncthreads_or(wet_local)) {
+        const int o = a.tile_off[tile], n = a.tile_off[tile + 1] - o;
+        for (int k = threadIdx.x; k <= n; k += kBlock)
+            a.stamp[k == 0 ? tile : a.tile_adj[o + k - 1]] = s_glob + 1u;
+    }
 }
 
 // Reduces a set of per-workgroup minima (NaN marks a NaN state) starting
 // from `pad`; thread 0 gets the result and the NaN flag.
-__device__ __forceinline__ double block_min_1024(const double *__restrict__ part, int n, double pad, int &nan) {
+__device__ __forceinline__ double block_min_1024(const double *__restrict__ part, int n, double pad, int &nan,
+                                                 int &nskip) {
     __shared__ double s[1024 / 64];
+    __shared__ int sk[1024 / 64];
     double m = pad;
-    int bad = 0;
+    int bad = 0, skipped = 0;
     for (int k = threadIdx.x; k < n; k += blockDim.x) {
         const double v = part[k];
         if (v != v) bad = 1;
         else m = dmin(m, v);
+        skipped += (v == __builtin_inf());
     }
     m = wave_min(m);
-    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = m;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) skipped += __shfl_xor(skipped, off, 64);
+    if ((threadIdx.x & 63) == 0) {
+        s[threadIdx.x >> 6] = m;
+        sk[threadIdx.x >> 6] = skipped;
+    }
     nan = __syncthreads_or(bad);
     double r = s[0];
-    for (int w = 1; w < (int)(blockDim.x >> 6); w++) r = dmin(r, s[w]);
+    nskip = sk[0];
+    for (int w = 1; w < (int)(blockDim.x >> 6); w++) {
+        r = dmin(r, s[w]);
+        nskip += sk[w];
+    }
     return r;
 }
 
@@ -415,12 +459,13 @@ __device__ __forceinline__ double block_min_1024(const double *__restrict__ part
 __global__ __launch_bounds__(1024) void finalize_kernel(Ctrl c, int nparts, int step, double pad,
                                                          double ts, double tol) {
     if (c.hdr[0] != 0) return;
-    int nan;
-    const double dtn = block_min_1024(c.partial, nparts, pad, nan);
+    int nan, nskip;
+    const double dtn = block_min_1024(c.partial, nparts, pad, nan, nskip);
     if (threadIdx.x == 0) {
         const double tcn = c.tc[step] + c.dt[step];
         c.dt[step + 1] = dtn;
         c.tc[step + 1] = tcn;
+        c.ran[step] = nparts - nskip;
         c.hdr[1] = step + 1;
         if (nan || !(dtn <= 9999.0 && dtn >= kDtLow)) c.hdr[0] = -1;
         else if (!(((tcn - ts) + 2.0 * tol) < tol)) c.hdr[0] = 1;
@@ -429,8 +474,8 @@ __global__ __launch_bounds__(1024) void finalize_kernel(Ctrl c, int nparts, int 
 
 // dt of the current state (finite_volumes_method.cuf:113-116), into *out.
 __global__ __launch_bounds__(1024) void reduce_dt_kernel(const double *part, int nparts, double pad, double *out) {
-    int nan;
-    const double m = block_min_1024(part, nparts, pad, nan);
+    int nan, nskip;
+    const double m = block_min_1024(part, nparts, pad, nan, nskip);
     if (threadIdx.x == 0) *out = nan ? __builtin_nan("") : m;
 }
 
@@ -539,20 +584,32 @@ __global__ void face_dist_kernel(const int4 *__restrict__ nbr, const double *__r
     dfc[i] = make_double4(d[0], d[1], d[2], 0.0);
 }
 
+// State in the caller's order and layout (3, nelt) <-> internal double4
+// (h, hu, hv, zm) in the internal cell order; perm: internal -> caller.
 __global__ void pack_state_kernel(const double *__restrict__ U, const double *__restrict__ zm,
-                                  int nelt, double4 *__restrict__ st) {
+                                  const int *__restrict__ perm, int nelt, double4 *__restrict__ st) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nelt) st[i] = make_double4(U[3 * i], U[3 * i + 1], U[3 * i + 2], zm[i]);
+    if (i < nelt) {
+        const size_t o = 3 * (size_t)perm[i];
+        st[i] = make_double4(U[o], U[o + 1], U[o + 2], zm[i]);
+    }
 }
 
-__global__ void unpack_state_kernel(const double4 *__restrict__ st, int nelt, double *__restrict__ U) {
+__global__ void unpack_state_kernel(const double4 *__restrict__ st, const int *__restrict__ perm,
+                                    int nelt, double *__restrict__ U) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < nelt) {
         const double4 s = st[i];
-        U[3 * i] = s.x;
-        U[3 * i + 1] = s.y;
-        U[3 * i + 2] = s.z;
+        const size_t o = 3 * (size_t)perm[i];
+        U[o] = s.x;
+        U[o + 1] = s.y;
+        U[o + 2] = s.z;
     }
+}
+
+__global__ void fill_u32_kernel(unsigned *p, int n, unsigned v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 __global__ void pow_kernel(const double *__restrict__ x, double y, int n, double *__restrict__ out) {
@@ -580,8 +637,19 @@ struct cf_handle {
     Ctrl c{};
     int cap = 0;
     double *dt0 = nullptr;     // initial dt of the current state
-    int nblocks = 0;           // workgroups of a step launch
-    int xcd_map = 1;
+    int nblocks = 0;           // workgroups of a step launch (one tile each)
+    int xcd_map = 0;
+    // internal cell order (internal -> caller) on the device and the host
+    int *perm = nullptr;
+    std::vector<int> hperm;
+    // dry-tile skipping
+    unsigned *stamp = nullptr;
+    int *tile_off = nullptr;
+    int *tile_adj = nullptr;
+    unsigned char *always = nullptr;
+    unsigned gstep = 0;        // steps run since creation
+    int skip = 1;
+    long long last_tiles_run = 0, last_tiles_total = 0;
     double dt = 0.0, tc = 0.0;
     int status = 0;
     int fric = 0;
@@ -648,106 +716,12 @@ static int ensure_ctrl(cf_handle *h, int nsteps) {
     int cap = std::max(nsteps, 2 * h->cap);
     if (h->c.dt) {
         HIPCHK(hipStreamSynchronize(h->stream));
-        dfree(h->c.dt); dfree(h->c.tc);
+        dfree(h->c.dt); dfree(h->c.tc); dfree(h->c.ran);
     }
     HIPCHK(hipMalloc(&h->c.dt, sizeof(double) * ((size_t)cap + 1)));
     HIPCHK(hipMalloc(&h->c.tc, sizeof(double) * ((size_t)cap + 1)));
+    HIPCHK(hipMalloc(&h->c.ran, sizeof(int) * ((size_t)cap + 1)));
     h->cap = cap;
-    return CF_OK;
-}
-
-// ------------------------------------------------------------------------
-// Host preprocessing: directed-edge matching (maillage.cuf:881-914) and
-// boundary classification (maillage.cuf:981-1110).
-
-namespace {
-struct Edge {
-    int64_t key;
-    int cell;
-};
-inline int64_t ekey(int a, int b) { return ((int64_t)a << 32) | (uint32_t)b; }
-}  // namespace
-
-extern "C" int cf_build_boundary(int nelt, int nnt, const int *conn, int ndi, const int *ndinput,
-                                 const int *num_in, int ndo, const int *ndoutput, const int *num_out,
-                                 int ndw, const int *ndwall, int *boundary) {
-    if (nelt <= 0 || nnt <= 0 || !conn || !boundary) return CF_ERR_ARG;
-    const size_t N = (size_t)nelt;
-    for (size_t i = 0; i < 3 * N; i++)
-        if (conn[i] < 1 || conn[i] > nnt) return CF_ERR_MESH;
-    std::vector<Edge> ed(3 * N);
-    for (size_t i = 0; i < N; i++)
-        for (int k = 0; k < 3; k++) {
-            ed[3 * i + k].key = ekey(conn[3 * i + k], conn[3 * i + (k + 1) % 3]);
-            ed[3 * i + k].cell = (int)i;
-        }
-    std::sort(ed.begin(), ed.end(), [](const Edge &x, const Edge &y) {
-        return x.key != y.key ? x.key < y.key : x.cell < y.cell;
-    });
-    // boundary(i,k,1) at [k*N + i], boundary(i,k,2) at [3N + k*N + i]
-    for (size_t i = 0; i < N; i++)
-        for (int k = 0; k < 3; k++) {
-            const int a = conn[3 * i + k], b = conn[3 * i + (k + 1) % 3];
-            const int64_t want = ekey(b, a);
-            auto it = std::lower_bound(ed.begin(), ed.end(), want,
-                                       [](const Edge &x, int64_t v) { return x.key < v; });
-            int nb = (int)i + 1;
-            for (; it != ed.end() && it->key == want; ++it)
-                if (it->cell != (int)i) nb = it->cell + 1;  // highest matching id
-            boundary[k * N + i] = nb;
-            boundary[3 * N + k * N + i] = (int)i + 1;
-        }
-    // first position of each node in each list
-    auto positions = [&](int nd, const int *list) {
-        std::vector<int> pos((size_t)nnt + 1, -1);
-        for (int q = nd - 1; q >= 0; q--)
-            if (list[q] >= 1 && list[q] <= nnt) pos[list[q]] = q;
-        return pos;
-    };
-    std::vector<int> pi = positions(ndi, ndinput), po = positions(ndo, ndoutput), pw = positions(ndw, ndwall);
-    for (size_t i = 0; i < N; i++)
-        for (int k = 0; k < 3; k++) {
-            if (boundary[k * N + i] != (int)i + 1) continue;
-            const int n1 = conn[3 * i + k], n2 = conn[3 * i + (k + 1) % 3];
-            if (ndi > 0 && n1 != n2 && pi[n1] >= 0 && pi[n2] >= 0) {
-                const int q = std::min(pi[n1], pi[n2]);
-                boundary[k * N + i] = -1;
-                boundary[3 * N + k * N + i] = num_in ? num_in[q] : 1;
-                continue;
-            }
-            if (ndo > 0 && n1 != n2 && po[n1] >= 0 && po[n2] >= 0) {
-                const int q = std::min(po[n1], po[n2]);
-                boundary[k * N + i] = -2;
-                boundary[3 * N + k * N + i] = num_out ? num_out[q] : 1;
-                continue;
-            }
-            if (ndw > 0 && n1 != n2 && pw[n1] >= 0 && pw[n2] >= 0) {
-                boundary[k * N + i] = -3;
-                continue;
-            }
-            if (ndi > 0 && ndw > 0 && ((pw[n1] >= 0 && pi[n2] >= 0) || (pi[n1] >= 0 && pw[n2] >= 0)))
-                boundary[k * N + i] = -3;
-            if (ndo > 0 && ndw > 0 && ((pw[n1] >= 0 && po[n2] >= 0) || (po[n1] >= 0 && pw[n2] >= 0)))
-                boundary[k * N + i] = -3;
-        }
-    return CF_OK;
-}
-
-extern "C" int cf_inlet_lengths(int nelt, int nnt, const int *conn, const double *coords,
-                                const int *boundary, int nentrees, double *long_entree) {
-    if (nelt <= 0 || !conn || !coords || !boundary || !long_entree || nentrees < 0 || nentrees > CF_MAX_IO)
-        return CF_ERR_ARG;
-    const size_t N = (size_t)nelt;
-    for (int k = 0; k < nentrees; k++) long_entree[k] = 0.0;
-    for (size_t i = 0; i < N; i++)
-        for (int k = 0; k < 3; k++) {
-            if (boundary[k * N + i] != -1) continue;
-            const int id = boundary[3 * N + k * N + i];
-            const int n1 = conn[3 * i + k] - 1, n2 = conn[3 * i + (k + 1) % 3] - 1;
-            if (id < 1 || id > nentrees || n1 < 0 || n2 < 0 || n1 >= nnt || n2 >= nnt) continue;
-            const double dx = coords[3 * n1] - coords[3 * n2], dy = coords[3 * n1 + 1] - coords[3 * n2 + 1];
-            long_entree[id - 1] = long_entree[id - 1] + sqrt(dx * dx + dy * dy);
-        }
     return CF_OK;
 }
 
@@ -780,8 +754,19 @@ static int validate_params(const cf_params *p) {
 static void destroy_buffers(cf_handle *h) {
     dfree(h->st[0]); dfree(h->st[1]); dfree(h->nbr); dfree(h->geo); dfree(h->dfc);
     dfree(h->manning); dfree(h->hydro); dfree(h->zm); dfree(h->stage);
-    dfree(h->c.hdr); dfree(h->c.partial); dfree(h->c.dt); dfree(h->c.tc);
-    dfree(h->dt0);
+    dfree(h->c.hdr); dfree(h->c.partial); dfree(h->c.dt); dfree(h->c.tc); dfree(h->c.ran);
+    dfree(h->dt0); dfree(h->perm); dfree(h->stamp); dfree(h->tile_off); dfree(h->tile_adj);
+    dfree(h->always);
+}
+
+template <typename T>
+static bool dalloc(T *&p, size_t count) {
+    return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T)) == hipSuccess;
+}
+
+template <typename T>
+static bool h2d(T *dst, const T *src, size_t count, hipStream_t s) {
+    return count == 0 || hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, s) == hipSuccess;
 }
 
 extern "C" int cf_create(const cf_mesh *mesh, const cf_params *params, cf_handle **out) {
@@ -797,44 +782,16 @@ extern "C" int cf_create(const cf_mesh *mesh, const cf_params *params, cf_handle
     for (size_t i = 0; i < 3 * N; i++)
         if (mesh->conn[i] < 1 || mesh->conn[i] > mesh->nnt) return CF_ERR_MESH;
 
-    // neighbour codes: >= 0 neighbour (0-based); < 0 -(class | id << 2);
-    // w packs, per face, the face of the neighbour that shares the edge.
+    cfprep::Prep prep;
+    st = cfprep::prepare(mesh, params, kBlock, (params->options & CF_OPT_KEEP_ORDER) != 0, prep);
+    if (st) return st;
     std::vector<int4> nbr(N);
-    const int *b = mesh->boundary;
-    for (size_t i = 0; i < N; i++) {
-        int c[3];
-        for (int k = 0; k < 3; k++) {
-            int v = b[k * N + i];
-            const int id = b[3 * N + k * N + i];
-            if (v == (int)i + 1) v = -3;  // set_undefined_boudaries_as_walls
-            if (v > 0) {
-                if (v > nelt) return CF_ERR_MESH;
-                c[k] = v - 1;
-            } else if (v == -1 || v == -2) {
-                if (id < 1 || id > CF_MAX_IO) return CF_ERR_MESH;
-                if (v == -1 && params->inlet == 2 && !(params->long_entree[id - 1] > 0.0)) return CF_ERR_ARG;
-                c[k] = -((v == -1 ? 1 : 2) | (id << 2));
-            } else if (v == -3) {
-                c[k] = -3;
-            } else {
-                return CF_ERR_MESH;
-            }
-        }
-        int w = 0;
-        for (int j = 0; j < 3; j++) {
-            int tjv = j;  // boundary faces: the cell itself, same face
-            if (c[j] >= 0) {
-                const int A = mesh->conn[3 * i + j], B = mesh->conn[3 * i + (j + 1) % 3];
-                const int *q = mesh->conn + 3 * (size_t)c[j];
-                if ((q[0] == A && q[1] == B) || (q[1] == A && q[0] == B)) tjv = 0;
-                else if ((q[1] == A && q[2] == B) || (q[2] == A && q[1] == B)) tjv = 1;
-                else if ((q[2] == A && q[0] == B) || (q[0] == A && q[2] == B)) tjv = 2;
-                else tjv = 0;  // no matching face: tjv = 1 (1-based), see the oracle header
-            }
-            w |= tjv << (2 * j);
-        }
-        nbr[i] = make_int4(c[0], c[1], c[2], w);
-    }
+    for (size_t i = 0; i < N; i++)
+        nbr[i] = make_int4(prep.nbr[i].c[0], prep.nbr[i].c[1], prep.nbr[i].c[2], prep.nbr[i].w);
+    std::vector<double> mann(N, 0.0);
+    if (mesh->manning)
+        for (size_t i = 0; i < N; i++) mann[i] = mesh->manning[prep.perm[i]];
+    const int ntiles = (int)prep.tile_always.size();
 
     cf_handle *h = new (std::nothrow) cf_handle();
     if (!h) return CF_ERR_NOMEM;
@@ -843,10 +800,10 @@ extern "C" int cf_create(const cf_mesh *mesh, const cf_params *params, cf_handle
     h->prm = *params;
     h->fric = params->friction == 0 ? 0 : 1 + params->fricimplic;
     h->nblocks = grid_of(nelt, kBlock);
-    {
-        const char *env = getenv("CUTEFLOW_XCD_MAP");
-        h->xcd_map = env ? atoi(env) != 0 : 1;
-    }
+    h->hperm = prep.perm;
+    // skipping dry tiles is exact except with a constant dt (every cell then
+    // contributes constant_dt to the minimum)
+    h->skip = (params->options & CF_OPT_NO_TILE_SKIP) || params->is_dt_constant == 1 ? 0 : 1;
     fill_kparams(h);
     int rc = CF_OK;
     int *dconn = nullptr;
@@ -854,42 +811,35 @@ extern "C" int cf_create(const cf_mesh *mesh, const cf_params *params, cf_handle
     do {
         if (hipStreamCreate(&h->stream) != hipSuccess) { rc = CF_ERR_HIP; break; }
         h->own_stream = true;
-        if (hipMalloc(&h->st[0], N * sizeof(double4)) != hipSuccess ||
-            hipMalloc(&h->st[1], N * sizeof(double4)) != hipSuccess ||
-            hipMalloc(&h->nbr, N * sizeof(int4)) != hipSuccess ||
-            hipMalloc(&h->geo, G_COUNT * N * sizeof(double)) != hipSuccess ||
-            hipMalloc(&h->dfc, N * sizeof(double4)) != hipSuccess ||
-            hipMalloc(&h->manning, N * sizeof(double)) != hipSuccess ||
-            hipMalloc(&h->hydro, 2 * CF_MAX_IO * CF_MAX_HYDRO * sizeof(double)) != hipSuccess ||
-            hipMalloc(&h->zm, N * sizeof(double)) != hipSuccess ||
-            hipMalloc(&h->stage, 3 * N * sizeof(double)) != hipSuccess ||
-            hipMalloc(&h->c.hdr, 4 * sizeof(int)) != hipSuccess ||
-            hipMalloc(&h->dt0, sizeof(double)) != hipSuccess ||
-            hipMalloc(&h->c.partial, (size_t)grid_of(nelt, kBlock) * sizeof(double)) != hipSuccess ||
-            hipMalloc(&dconn, 3 * N * sizeof(int)) != hipSuccess ||
-            hipMalloc(&dcoords, 3 * (size_t)mesh->nnt * sizeof(double)) != hipSuccess) {
+        if (!dalloc(h->st[0], N) || !dalloc(h->st[1], N) || !dalloc(h->nbr, N) ||
+            !dalloc(h->geo, G_COUNT * N) || !dalloc(h->dfc, N) || !dalloc(h->manning, N) ||
+            !dalloc(h->hydro, 2 * CF_MAX_IO * CF_MAX_HYDRO) || !dalloc(h->zm, N) ||
+            !dalloc(h->stage, 3 * N) || !dalloc(h->c.hdr, 4) || !dalloc(h->dt0, 1) ||
+            !dalloc(h->c.partial, (size_t)h->nblocks) || !dalloc(h->perm, N) ||
+            !dalloc(h->stamp, (size_t)ntiles) || !dalloc(h->tile_off, prep.tile_off.size()) ||
+            !dalloc(h->tile_adj, prep.tile_adj.size()) || !dalloc(h->always, (size_t)ntiles) ||
+            !dalloc(dconn, 3 * N) || !dalloc(dcoords, 3 * (size_t)mesh->nnt)) {
             rc = CF_ERR_NOMEM;
             break;
         }
-        if (hipMemcpyAsync(dconn, mesh->conn, 3 * N * sizeof(int), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-            hipMemcpyAsync(dcoords, mesh->coords, 3 * (size_t)mesh->nnt * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-            hipMemcpyAsync(h->nbr, nbr.data(), N * sizeof(int4), hipMemcpyHostToDevice, h->stream) != hipSuccess ||
-            hipMemsetAsync(h->c.hdr, 0, 4 * sizeof(int), h->stream) != hipSuccess) {
-            rc = CF_ERR_HIP;
-            break;
-        }
-        if (mesh->manning) {
-            if (hipMemcpyAsync(h->manning, mesh->manning, N * sizeof(double), hipMemcpyHostToDevice, h->stream) != hipSuccess) { rc = CF_ERR_HIP; break; }
-        } else if (hipMemsetAsync(h->manning, 0, N * sizeof(double), h->stream) != hipSuccess) {
+        if (!h2d(dconn, prep.conn.data(), 3 * N, h->stream) ||
+            !h2d(dcoords, mesh->coords, 3 * (size_t)mesh->nnt, h->stream) ||
+            !h2d(h->nbr, nbr.data(), N, h->stream) || !h2d(h->manning, mann.data(), N, h->stream) ||
+            !h2d(h->perm, prep.perm.data(), N, h->stream) ||
+            !h2d(h->tile_off, prep.tile_off.data(), prep.tile_off.size(), h->stream) ||
+            !h2d(h->tile_adj, prep.tile_adj.data(), prep.tile_adj.size(), h->stream) ||
+            !h2d(h->always, prep.tile_always.data(), (size_t)ntiles, h->stream) ||
+            hipMemsetAsync(h->c.hdr, 0, 4 * sizeof(int), h->stream) != hipSuccess ||
+            hipMemsetAsync(h->stamp, 0, (size_t)ntiles * sizeof(unsigned), h->stream) != hipSuccess ||
+            hipMemsetAsync(h->stage, 0, 3 * N * sizeof(double), h->stream) != hipSuccess) {
             rc = CF_ERR_HIP;
             break;
         }
         hipLaunchKernelGGL(geometry_kernel, dim3(grid_of(nelt, 256)), dim3(256), 0, h->stream, dconn, dcoords, nelt, h->geo, h->zm);
         hipLaunchKernelGGL(face_dist_kernel, dim3(grid_of(nelt, 256)), dim3(256), 0, h->stream, h->nbr, h->geo, nelt, h->dfc);
-        // zero state with the bed level in the 4th component
-        if (hipMemsetAsync(h->stage, 0, 3 * N * sizeof(double), h->stream) != hipSuccess) { rc = CF_ERR_HIP; break; }
-        hipLaunchKernelGGL(pack_state_kernel, dim3(grid_of(nelt, 256)), dim3(256), 0, h->stream, h->stage, h->zm, nelt, h->st[0]);
-        hipLaunchKernelGGL(pack_state_kernel, dim3(grid_of(nelt, 256)), dim3(256), 0, h->stream, h->stage, h->zm, nelt, h->st[1]);
+        // zero state with the bed level in the 4th component, in both buffers
+        hipLaunchKernelGGL(pack_state_kernel, dim3(grid_of(nelt, 256)), dim3(256), 0, h->stream, h->stage, h->zm, h->perm, nelt, h->st[0]);
+        hipLaunchKernelGGL(pack_state_kernel, dim3(grid_of(nelt, 256)), dim3(256), 0, h->stream, h->stage, h->zm, h->perm, nelt, h->st[1]);
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) { rc = CF_ERR_HIP; break; }
         if ((rc = upload_hydro(h)) != CF_OK) break;
         if ((rc = ensure_ctrl(h, 1024)) != CF_OK) break;
@@ -917,7 +867,11 @@ extern "C" void cf_destroy(cf_handle *h) {
 
 extern "C" int cf_set_state_device(cf_handle *h, const double *dU) {
     if (!h || !dU || h->pending >= 0) return CF_ERR_ARG;
-    hipLaunchKernelGGL(pack_state_kernel, dim3(grid_of(h->nelt, 256)), dim3(256), 0, h->stream, dU, h->zm, h->nelt, h->st[h->cur]);
+    hipLaunchKernelGGL(pack_state_kernel, dim3(grid_of(h->nelt, 256)), dim3(256), 0, h->stream, dU, h->zm,
+                       h->perm, h->nelt, h->st[h->cur]);
+    // every tile runs the next two steps (both state buffers become consistent)
+    const int ntiles = h->nblocks;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_of(ntiles, 256)), dim3(256), 0, h->stream, h->stamp, ntiles, h->gstep);
     HIPCHK(hipGetLastError());
     h->tc = 0.0;
     h->status = 0;
@@ -925,14 +879,15 @@ extern "C" int cf_set_state_device(cf_handle *h, const double *dU) {
 }
 
 extern "C" int cf_set_state(cf_handle *h, const double *U) {
-    if (!h || !U) return CF_ERR_ARG;
+    if (!h || !U || h->pending >= 0) return CF_ERR_ARG;
     HIPCHK(hipMemcpyAsync(h->stage, U, 3 * (size_t)h->nelt * sizeof(double), hipMemcpyHostToDevice, h->stream));
     return cf_set_state_device(h, h->stage);
 }
 
 extern "C" int cf_get_state_device(cf_handle *h, double *dU) {
     if (!h || !dU || h->pending >= 0) return CF_ERR_ARG;
-    hipLaunchKernelGGL(unpack_state_kernel, dim3(grid_of(h->nelt, 256)), dim3(256), 0, h->stream, h->st[h->cur], h->nelt, dU);
+    hipLaunchKernelGGL(unpack_state_kernel, dim3(grid_of(h->nelt, 256)), dim3(256), 0, h->stream, h->st[h->cur],
+                       h->perm, h->nelt, dU);
     HIPCHK(hipGetLastError());
     return CF_OK;
 }
@@ -993,6 +948,12 @@ extern "C" int cf_step_async(cf_handle *h, int nsteps) {
     a.nelt = h->nelt;
     a.xcd_map = h->xcd_map;
     a.reserved = 0;
+    a.stamp = h->stamp;
+    a.tile_off = h->tile_off;
+    a.tile_adj = h->tile_adj;
+    a.always = h->always;
+    a.gstep = h->gstep;
+    a.skip = h->skip;
     a.p = h->kp;
     step_launcher L = pick_launcher(h->prm.iflux, h->fric);
     for (int k = 0; k < nsteps; k++) {
@@ -1022,11 +983,17 @@ extern "C" int cf_step_finish(cf_handle *h, double *dt_hist, int *steps_done) {
     HIPCHK(hipStreamSynchronize(h->stream));
     const int done = hdr[1];
     std::vector<double> dts((size_t)done + 1), tcs(1);
+    std::vector<int> ran((size_t)done + 1, 0);
     HIPCHK(hipMemcpyAsync(dts.data(), h->c.dt, sizeof(double) * ((size_t)done + 1), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipMemcpyAsync(tcs.data(), h->c.tc + done, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    if (done > 0) HIPCHK(hipMemcpyAsync(ran.data(), h->c.ran, sizeof(int) * (size_t)done, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    h->last_tiles_run = 0;
+    for (int k = 0; k < done; k++) h->last_tiles_run += ran[k];
+    h->last_tiles_total = (long long)done * h->nblocks;
     if (dt_hist) memcpy(dt_hist, dts.data(), sizeof(double) * ((size_t)done + 1));
     h->cur = (h->cur + done) & 1;
+    h->gstep += (unsigned)done;
     h->dt = dts[done];
     h->tc = tcs[0];
     if (steps_done) *steps_done = done;
@@ -1066,26 +1033,40 @@ extern "C" int cf_set_hydrograph(cf_handle *h, int inlet, int npts, const double
 }
 
 extern "C" int cf_set_manning(cf_handle *h, const double *n) {
-    if (!h || !n) return CF_ERR_ARG;
-    HIPCHK(hipMemcpyAsync(h->manning, n, (size_t)h->nelt * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    if (!h || !n || h->pending >= 0) return CF_ERR_ARG;
+    const size_t N = (size_t)h->nelt;
+    std::vector<double> m(N);
+    for (size_t i = 0; i < N; i++) m[i] = n[h->hperm[i]];
+    HIPCHK(hipMemcpyAsync(h->manning, m.data(), N * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return CF_OK;
 }
 
 extern "C" int cf_get_geometry(cf_handle *h, double *surf, double *zm, double *cotemin_arr, double *cotemin) {
-    if (!h) return CF_ERR_ARG;
+    if (!h || h->pending >= 0) return CF_ERR_ARG;
     const size_t N = (size_t)h->nelt;
-    std::vector<double> cot(N);
-    if (surf) HIPCHK(hipMemcpyAsync(surf, h->geo + G_SURF * N, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    if (zm) HIPCHK(hipMemcpyAsync(zm, h->zm, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    std::vector<double> s(N), z(N), cot(N);
+    HIPCHK(hipMemcpyAsync(s.data(), h->geo + G_SURF * N, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(z.data(), h->zm, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipMemcpyAsync(cot.data(), h->geo + G_COT * N, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    if (cotemin_arr) memcpy(cotemin_arr, cot.data(), N * sizeof(double));
-    if (cotemin) {
-        double m = 1000000.0;
-        for (size_t i = 0; i < N; i++) m = std::min(m, cot[i]);
-        *cotemin = m;
+    double m = 1000000.0;
+    for (size_t i = 0; i < N; i++) {
+        const size_t o = (size_t)h->hperm[i];
+        if (surf) surf[o] = s[i];
+        if (zm) zm[o] = z[i];
+        if (cotemin_arr) cotemin_arr[o] = cot[i];
+        m = std::min(m, cot[i]);
     }
+    if (cotemin) *cotemin = m;
+    return CF_OK;
+}
+
+extern "C" int cf_last_run_stats(cf_handle *h, long long *tiles_run, long long *tiles_total, int *tile_cells) {
+    if (!h) return CF_ERR_ARG;
+    if (tiles_run) *tiles_run = h->last_tiles_run;
+    if (tiles_total) *tiles_total = h->last_tiles_total;
+    if (tile_cells) *tile_cells = kBlock;
     return CF_OK;
 }
 

@@ -18,7 +18,8 @@ from .params import Params
 
 class Solver:
     def __init__(self, mesh: Mesh, params: Params, boundary: np.ndarray | None = None,
-                 ts: float | None = None):
+                 ts: float | None = None, options: int = 0):
+        """options: CF_OPT_* bits (device layout only; results are identical)."""
         lib = _lib.load()
         self.lib = lib
         self.mesh = mesh
@@ -26,6 +27,7 @@ class Solver:
         self.boundary = mesh.boundary_table() if boundary is None else np.ascontiguousarray(boundary, np.int32)
         self.long_entree = mesh.inlet_lengths(self.boundary, params.nombre_entrees)
         self.cfp = params.to_cf(self.long_entree, ts=ts)
+        self.cfp.options = int(options)
         self._conn = np.ascontiguousarray(mesh.conn.reshape(-1), np.int32)
         self._coords = np.ascontiguousarray(mesh.coords.reshape(-1), np.float64)
         self._mann = np.ascontiguousarray(mesh.manning, np.float64)
@@ -87,6 +89,15 @@ class Solver:
         if st != _lib.CF_OK and (raise_on_error or st != _lib.CF_ERR_DT):
             raise _lib.CuteflowError(st, "cf_step")
         return hist[: done.value + 1], done.value, st
+
+    def last_run_stats(self) -> dict:
+        """Tiles that ran in the last step() (dry tiles with dry neighbours are
+        skipped) out of tiles x steps."""
+        run, tot, cells = C.c_longlong(), C.c_longlong(), C.c_int()
+        _lib.check(self.lib.cf_last_run_stats(self.h, C.byref(run), C.byref(tot), C.byref(cells)),
+                   "cf_last_run_stats")
+        return {"tiles_run": run.value, "tiles_total": tot.value, "tile_cells": cells.value,
+                "active_fraction": run.value / tot.value if tot.value else 0.0}
 
     def set_manning(self, n: np.ndarray) -> None:
         n = np.ascontiguousarray(n, np.float64)

@@ -23,6 +23,14 @@ extern "C" {
 #define CF_MAX_IO 20      /* inlets/outlets (global_data.cuf:54, dimension(20)) */
 #define CF_MAX_HYDRO 64   /* points per inflow hydrograph */
 
+/* cf_params.options bits.  By default the library stores cells in its own
+ * order (a Hilbert curve of the centroids, so that each 256-cell tile is a
+ * compact patch) and skips tiles that are dry with dry neighbours; both are
+ * exact: per-cell results are identical bits in any order, with or without
+ * skipping. */
+#define CF_OPT_KEEP_ORDER 1      /* keep the caller's cell order on the device */
+#define CF_OPT_NO_TILE_SKIP 2    /* run every tile every step */
+
 enum {
     CF_OK = 0,
     CF_ERR_ARG = -1,        /* invalid argument or unsupported option */
@@ -53,7 +61,7 @@ typedef struct cf_params {
     int is_dt_constant;
     int is_override_manning;
     int nombre_entrees;
-    int reserved0;
+    int options;             /* CF_OPT_* bits (layout choices; results do not depend on them) */
     double debitglob[CF_MAX_IO];
     double h_sortie[CF_MAX_IO];
     double long_entree[CF_MAX_IO];
@@ -126,6 +134,11 @@ int cf_step(cf_handle *h, int nsteps, double *dt_hist, int *steps_done);
  * one cf_step_finish before any other call on the handle. */
 int cf_step_async(cf_handle *h, int nsteps);
 int cf_step_finish(cf_handle *h, double *dt_hist, int *steps_done);
+
+/* Work done by the last cf_step/cf_step_finish: tiles that ran (the others
+ * were dry with dry neighbours and skipped), tiles x steps in total, and the
+ * cells per tile. */
+int cf_last_run_stats(cf_handle *h, long long *tiles_run, long long *tiles_total, int *tile_cells);
 
 /* Inflow discharges debitglob(1:n) and hydrographs. */
 int cf_set_inflow(cf_handle *h, const double *debitglob, int n);

@@ -146,6 +146,26 @@ def test_river_hllc_transmissive_outlet():
     _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
 
 
+@pytest.mark.parametrize("options", [0, 1, 2, 3])
+def test_layout_options_identical(options):
+    """Internal Hilbert order and dry-tile skipping (on/off) give the oracle's
+    bits on a wetting floodplain over many steps."""
+    m, p, U0 = river_case(nx=120, ny=24, q_peak=2000.0, t_ramp=30.0)
+    prep = O.Prepared(m)
+    ref = O.run(prep, p, U0, 800)
+    with Solver(m, p, options=options) as s:
+        s.set_state(U0)
+        hist, done, st = s.step(400)
+        hist2, done2, st2 = s.step(400)
+        U = s.get_state()
+        stats = s.last_run_stats()
+    assert st == st2 == 0 and done == done2 == 400
+    _compare(U, ref["U"], np.concatenate([hist[:-1], hist2]), ref["dt_hist"], p.tolisec)
+    if options & 2:
+        assert stats["active_fraction"] == 1.0
+    assert stats["tiles_total"] == 400 * ((m.nelt + 255) // 256)
+
+
 def test_reordering_invariance():
     """RCM and random cell orders give the same per-cell results."""
     m, p, U0 = river_case(nx=40, ny=10)
