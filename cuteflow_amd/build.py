@@ -55,5 +55,34 @@ def build_hip(verbose: bool = False) -> str:
     return LIB_PATH
 
 
+FORTRAN_DIR = os.path.join(PKG_DIR, "fortran")
+BIN_DIR = os.path.join(PKG_DIR, "bin")
+FORTRAN_EXE = os.path.join(BIN_DIR, "cuteflow_hip")
+AMDFLANG = os.environ.get("AMDFLANG", "/opt/rocm/bin/amdflang")
+
+
+def build_fortran(verbose: bool = False) -> str:
+    """Compile the Fortran driver (amdflang) against libcuteflow_hip.so."""
+    os.makedirs(BIN_DIR, exist_ok=True)
+    obj = os.path.join(PKG_DIR, "_build", "fortran")
+    os.makedirs(obj, exist_ok=True)
+    mod_o = os.path.join(obj, "cuteflow_hip_mod.o")
+    main_o = os.path.join(obj, "cuteflow_main.o")
+    steps = [
+        [AMDFLANG, "-O2", "-module-dir", obj, "-c", os.path.join(FORTRAN_DIR, "cuteflow_hip_mod.f90"), "-o", mod_o],
+        [AMDFLANG, "-O2", "-module-dir", obj, "-I", obj, "-c", os.path.join(FORTRAN_DIR, "cuteflow_main.f90"),
+         "-o", main_o],
+        [AMDFLANG, "-o", FORTRAN_EXE + ".tmp", mod_o, main_o, "-L" + LIB_DIR, "-lcuteflow_hip",
+         "-Wl,-rpath,$ORIGIN/../lib"],
+    ]
+    for cmd in steps:
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+    os.replace(FORTRAN_EXE + ".tmp", FORTRAN_EXE)
+    return FORTRAN_EXE
+
+
 if __name__ == "__main__":
     print(build_hip(verbose=True))
+    print(build_fortran(verbose=True))

@@ -1,0 +1,161 @@
+"""Case driver: the reference program (main.cuf + solve_finite_volumes,
+finite_volumes_method.cuf:4-258) for one subdomain, with the step on the GPU.
+
+    python -m cuteflow_amd <case_dir>
+
+reads <case_dir>/donnees.f, the text mesh, the boundary table (built and
+written when elt_bound = 0, read otherwise), the initial condition (file,
+dam line or plane), runs to ts, prints the status block every freqaffich
+steps and writes the snapshots (legacy VTK, element/node text) and the
+restart file like the reference.
+"""
+from __future__ import annotations
+
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import io as cio
+from .mesh import read_boundary_table, read_text_mesh, write_boundary_table
+from .params import read_donnees
+from .solver import Solver, initial_state
+
+
+class SnapshotWriter:
+    """global_save_sol_visu (pre_post_traitement.cuf:5-49): snapshot k is
+    written when tc is within +-tolaffiche of k*ts/visu_snapshots, with
+    tolaffiche = 1.5*dt after the first one."""
+
+    def __init__(self, prm, mesh, zm, surf, outdir):
+        self.prm, self.mesh, self.zm, self.surf, self.outdir = prm, mesh, zm, surf, outdir
+        self.times = cio.snapshot_times(prm.ts, prm.visu_snapshots) if prm.solvisu > 0 else np.zeros(0)
+        self.cpt = 0
+        self.tolaffiche = prm.tolaffiche
+        self.written = []
+
+    @property
+    def pending(self):
+        return self.cpt < len(self.times)
+
+    def next_time(self):
+        return self.times[self.cpt] if self.pending else np.inf
+
+    def check(self, tc, dt, get_state):
+        if not self.pending:
+            return
+        if self.cpt > 0:
+            self.tolaffiche = 1.5 * dt
+        t = self.times[self.cpt]
+        if t - self.tolaffiche <= tc <= t + self.tolaffiche:
+            self.write(self.cpt, tc, get_state())
+            self.cpt += 1
+        elif tc > t + self.tolaffiche:
+            print("Skipped save sol ! : The timestep is too large for the number of snapshots to save!",
+                  self.cpt)
+            self.cpt += 1
+
+    def write(self, k, tc, U):
+        p, m, d = self.prm, self.mesh, self.outdir
+        if p.solvisu in (1, 2):
+            cio.write_vtk(os.path.join(d, f"_solution-Paraview_{k}.vtk"), m.coords, m.conn, U, self.zm,
+                          p.tolisec, p.sol_z_offset)
+        if p.solvisu in (1, 4):
+            nodes = cio.solution_on_nodes(m.conn, m.nnt, U, self.surf)
+            cio.write_simple_nodes(os.path.join(d, f"solution_noeuds_simple_{k}.txt"), tc, nodes,
+                                   m.coords[:, 2], p.sol_z_offset)
+        if p.solvisu in (1, 5):
+            cio.write_simple_elements(os.path.join(d, f"solution_elements_simple_{k}.txt"), tc, U, self.zm,
+                                      p.sol_z_offset)
+        self.written.append((k, tc))
+
+
+def run_case(case_dir: str, verbose: bool = True, max_steps: int | None = None):
+    """Runs a case directory; returns a dict with the final state and times."""
+    prm = read_donnees(os.path.join(case_dir, "donnees.f"))
+    if prm.is_cgns:
+        raise NotImplementedError("CGNS meshes are not read by this build; use the text mesh format")
+    mesh = read_text_mesh(os.path.join(case_dir, prm.meshfile_path, prm.meshfile), prm.nombre_entrees,
+                          prm.nombre_sorties)
+    prm.nombre_entrees, prm.nombre_sorties = mesh.nombre_entrees, mesh.nombre_sorties
+    file_bc = os.path.join(case_dir, "boundary_table_" + prm.meshfile.strip())
+    if prm.elt_bound == 0:
+        bnd = mesh.boundary_table()
+        write_boundary_table(file_bc, bnd)
+    else:
+        bnd = read_boundary_table(file_bc, mesh.nelt)
+    if prm.is_override_manning == 0:
+        pass  # per-cell n from the mesh file
+    s = Solver(mesh, prm, boundary=bnd)
+    geo = s.geometry()
+    if verbose:
+        print(" cotemin             = ", geo["cotemin"])
+        for k in range(prm.nombre_entrees):
+            print(" debitglob       ", k + 1, " =", prm.debitglob[k])
+            print(" longueur entree ", k + 1, " =", s.long_entree[k])
+    if prm.solinit == 1:
+        _, U0 = cio.read_initial_condition(os.path.join(case_dir, prm.fich_sol_init.strip()), mesh.nelt)
+    else:
+        U0 = initial_state(prm, mesh, geo["zm"])
+    s.set_state(U0)
+    snaps = SnapshotWriter(prm, mesh, geo["zm"], geo["surf"], case_dir)
+    snaps.check(0.0, s.dt, s.get_state)
+
+    freq = max(1, prm.freqaffich)
+    nt, dts = 0, []
+    t0 = time.perf_counter()
+    if verbose:
+        print(" Loop over time starts")
+    status = 0
+    while True:
+        # run in batches up to the next print; step by step close to a snapshot
+        dt = s.dt
+        n = freq - (nt % freq)
+        if snaps.pending:
+            room = snaps.next_time() - 1.5 * dt - s.tc
+            n = min(n, max(1, int(room / (4.0 * dt)) if room > 0 else 1))
+        if max_steps is not None:
+            n = min(n, max_steps - nt)
+            if n <= 0:
+                break
+        hist, done, st = s.step(n, raise_on_error=False)
+        dts.extend(hist[:done])
+        nt += done
+        if done and snaps.pending:
+            snaps.check(s.tc, hist[done - 1], s.get_state)
+        if verbose and done and nt % freq == 0:
+            print(" -----------------------------------------------")
+            print(" Iteration number ", nt, ", t = ", s.tc)
+            print(" -----------------------------------------------")
+        if st != 0:
+            print(" Erreur sur le pas de temps dt=", s.dt)
+            status = st
+            break
+        if done < n:
+            break
+    wall = time.perf_counter() - t0
+    U = s.get_state()
+    if verbose:
+        print(" Loop on time ends at iteration : ", nt)
+    if prm.solrestart == 1:
+        cio.write_restart(os.path.join(case_dir, "solution_elements_restart.txt"), s.tc, U)
+    if verbose:
+        print(" DUREE DU CALCUL :", wall, " Secondes ")
+    out = {"U": U, "tc": s.tc, "steps": nt, "dt_hist": np.array(dts), "status": status, "wall_s": wall,
+           "snapshots": snaps.written, "zm": geo["zm"]}
+    s.close()
+    return out
+
+
+def main(argv=None):
+    argv = sys.argv[1:] if argv is None else argv
+    if len(argv) != 1:
+        print("usage: python -m cuteflow_amd <case_dir>", file=sys.stderr)
+        return 2
+    r = run_case(argv[0])
+    return 0 if r["status"] == 0 else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

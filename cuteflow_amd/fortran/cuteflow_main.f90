@@ -1,0 +1,382 @@
+! Fortran driver of the St-Venant time step on one subdomain: the reference's
+! program (main.cuf, solve_finite_volumes in finite_volumes_method.cuf) with
+! the per-step kernels replaced by the HIP library through cuteflow_hip.
+!
+! Keeps the reference's inputs and outputs: the `donnees.f` namelist
+! (global_data.cuf:65-94), the text mesh (maillage.cuf:196-396), the
+! boundary_table_<meshfile> file (maillage.cuf:55-79), the initial condition
+! file (initial_condition.cuf:82-102) or the dam-line / plane initial
+! condition (initial_condition.cuf:4-80), the status block every freqaffich
+! steps (finite_volumes_method.cuf:345-368), the final legacy VTK, element
+! text and restart files (pre_post_traitement.cuf:167-207, :407-512).
+program cuteflow_main
+  use, intrinsic :: iso_c_binding
+  use cuteflow_hip
+  implicit none
+
+  integer, parameter :: dp = c_double
+
+  ! ---- namelist variables and defaults (global_data.cuf:31-62) ----
+  integer :: friction = 0, is_override_manning = 0, diffusion = 0
+  real(dp) :: gp = 9.81d0, override_manning = 0.d0
+  character(len=60) :: meshfile = '', meshfile_path = './'
+  integer :: elt_bound = 0, nombre_entrees = 1, nombre_sorties = 1, is_cgns = 0
+  character(len=200) :: fich_sol_init = ''
+  integer :: solinit = 0, eqlin_barrage = 0, plan = 0, sortie_finale_bluekenue = 0
+  real(dp) :: h_amont = 0, u_amont = 0, v_amont = 0, h_aval = 0, u_aval = 0, v_aval = 0
+  real(dp) :: xplan = 0, yplan = 0, zplan = 1, xpoint = 0, ypoint = 0, zpoint = 0
+  real(dp) :: x1eqbar = 0, x2eqbar = 0, y1eqbar = 0, y2eqbar = 0
+  character(len=30) :: inlet = 'inflow', outlet = 'fixedheight', loi_debitglob = ''
+  integer :: iflux = 1, ilimiteur = 1, iupwind = 1, fricimplic = 1, debit_var = 0
+  integer :: freqaffich = 1000, solrestart = 0
+  real(dp) :: sol_z_offset = 0.
+  real(dp) :: debitglob(20) = 0, h_sortie(20) = 0
+  character(len=100) :: inlet_name(20) = '', outlet_name(20) = ''
+  real(dp) :: ts = 1, tsstab = 0, cfl = 0.9d0, tolisec = 1d-5, tol_reg_perm = 0
+  real(dp) :: constant_dt = 0.
+  real(dp) :: tol = 1e-6, tolaffiche = 1e-2
+  integer :: visu_snapshots = 0, solvisu = 0, is_dt_constant = 0
+
+  namelist /donnees_namelist/ gp, is_override_manning, override_manning, is_cgns, meshfile_path, &
+       meshfile, elt_bound, nombre_entrees, nombre_sorties, eqlin_barrage, x1eqbar, x2eqbar, &
+       y1eqbar, y2eqbar, h_amont, u_amont, v_amont, h_aval, u_aval, v_aval, plan, xplan, yplan, &
+       zplan, xpoint, ypoint, zpoint, solinit, fich_sol_init, inlet, inlet_name, debitglob, debit_var, &
+       loi_debitglob, outlet, outlet_name, h_sortie, iflux, ilimiteur, iupwind, tolisec, friction, &
+       fricimplic, diffusion, ts, cfl, tol_reg_perm, is_dt_constant, constant_dt, freqaffich, &
+       tolaffiche, sol_z_offset, solrestart, solvisu, visu_snapshots, sortie_finale_bluekenue
+
+  ! ---- mesh ----
+  integer :: nnt, nelt, ndi, ndo, ndw, i, k, n, ios, st, done, nt, nbatch, tc_init
+  real(dp), allocatable :: coords(:, :), manning_nd(:), manning(:), u(:, :), zm(:), surf(:), cot(:)
+  integer, allocatable :: conn(:, :), ndinput(:), num_in(:), ndoutput(:), num_out(:), ndwall(:)
+  integer(c_int), allocatable, target :: conn_c(:), bnd(:)
+  real(c_double), allocatable, target :: coords_c(:), mann_c(:)
+  real(c_double), allocatable :: dt_hist(:)
+  character(len=1) :: tarati
+  character(len=260) :: file_bc
+  type(cf_params_t) :: prm
+  type(cf_mesh_t) :: msh
+  type(c_ptr) :: h
+  real(dp) :: cotemin, tc, dt, long_entree(20), x1, x2, x3, y1, y2, y3, v1, v2, hh, alpha, beta
+  integer(8) :: c0, c1, crate
+  logical :: ex
+
+  open(10, file='donnees.f', form='formatted', status='old')
+  read(10, nml=donnees_namelist)
+  close(10)
+  inlet = trim(adjustl(inlet))
+
+  ! ---- text mesh (maillage.cuf:196-396) ----
+  open(11, file=trim(adjustl(meshfile_path))//trim(adjustl(meshfile)), form='formatted', status='old')
+  read(11, *) tarati
+  read(11, *) nnt
+  allocate(coords(nnt, 3), manning_nd(nnt))
+  do i = 1, nnt
+     read(11, *) n, coords(i, 1), coords(i, 2), coords(i, 3), manning_nd(i)
+  end do
+  read(11, *) tarati
+  read(11, *) nelt
+  allocate(conn(nelt, 3), manning(nelt))
+  do i = 1, nelt
+     read(11, *) n, conn(i, 1), conn(i, 2), conn(i, 3), manning(i)
+  end do
+  read(11, *) tarati
+  if (nombre_entrees == 1) then
+     read(11, *) ndi
+     allocate(ndinput(ndi), num_in(ndi))
+     do i = 1, ndi
+        read(11, *) ndinput(i)
+        num_in(i) = 1
+     end do
+  else
+     read(11, *) ndi, nombre_entrees
+     allocate(ndinput(ndi), num_in(ndi))
+     do i = 1, ndi
+        read(11, *) ndinput(i), num_in(i)
+     end do
+  end if
+  read(11, *) tarati
+  if (nombre_sorties == 1) then
+     read(11, *) ndo
+     allocate(ndoutput(ndo), num_out(ndo))
+     do i = 1, ndo
+        read(11, *) ndoutput(i)
+        num_out(i) = 1
+     end do
+  else
+     read(11, *) ndo, nombre_sorties
+     allocate(ndoutput(ndo), num_out(ndo))
+     do i = 1, ndo
+        read(11, *) ndoutput(i), num_out(i)
+     end do
+  end if
+  read(11, *) tarati
+  read(11, *) ndw
+  allocate(ndwall(ndw))
+  do i = 1, ndw
+     read(11, *) ndwall(i)
+  end do
+  close(11)
+  write(*, *) nnt, ' nodes, ', nelt, ' elems, ', ndi, ' inlet nodes, ', ndo, ' outlet nodes, ', &
+       ndw, ' wall nodes'
+
+  ! C layouts: connectivity and coordinates cell-/node-major
+  allocate(conn_c(3 * nelt), coords_c(3 * nnt), mann_c(nelt), bnd(6 * nelt))
+  do i = 1, nelt
+     conn_c(3 * i - 2:3 * i) = conn(i, :)
+  end do
+  do i = 1, nnt
+     coords_c(3 * i - 2:3 * i) = coords(i, :)
+  end do
+  mann_c = manning
+
+  ! ---- boundary table: build and write, or read (maillage.cuf:42-79) ----
+  file_bc = 'boundary_table_'//trim(adjustl(meshfile))
+  if (elt_bound == 0) then
+     st = cf_build_boundary(nelt, nnt, conn_c, ndi, ndinput, num_in, ndo, ndoutput, num_out, ndw, &
+          ndwall, bnd)
+     call cf_check(st, 'cf_build_boundary')
+     open(12, file=trim(file_bc), status='unknown')
+     do i = 1, nelt
+        write(12, *) bnd(i), bnd(nelt + i), bnd(2 * nelt + i), bnd(3 * nelt + i), bnd(4 * nelt + i), &
+             bnd(5 * nelt + i)
+     end do
+     close(12)
+  else
+     open(12, file=trim(file_bc), status='old')
+     do i = 1, nelt
+        read(12, *) bnd(i), bnd(nelt + i), bnd(2 * nelt + i), bnd(3 * nelt + i), bnd(4 * nelt + i), &
+             bnd(5 * nelt + i)
+     end do
+     close(12)
+  end if
+
+  long_entree = 0
+  if (nombre_entrees > 0) then
+     st = cf_inlet_lengths(nelt, nnt, conn_c, coords_c, bnd, min(nombre_entrees, 20), long_entree)
+     call cf_check(st, 'cf_inlet_lengths')
+  end if
+
+  ! ---- parameters (data_transfer.cuf:12-96) ----
+  prm%gp = gp; prm%tolisec = tolisec; prm%tol = tol; prm%cfl = cfl; prm%ts = ts
+  prm%constant_dt = constant_dt; prm%override_manning = override_manning
+  prm%tol_reg_perm = tol_reg_perm
+  prm%iflux = iflux; prm%ilimiteur = ilimiteur; prm%friction = friction; prm%fricimplic = fricimplic
+  if (inlet == 'transm') then
+     prm%inlet = 1
+  else if (inlet == 'inflow') then
+     prm%inlet = 2
+  else
+     print *, "Bad inlet !"
+     error stop
+  end if
+  if (outlet == 'fixedheight') then
+     prm%outlet = 1
+  else if (outlet == 'transm') then
+     prm%outlet = 2
+  else
+     print *, "Bad outlet !"
+     error stop
+  end if
+  prm%is_dt_constant = is_dt_constant; prm%is_override_manning = is_override_manning
+  prm%nombre_entrees = nombre_entrees; prm%options = 0
+  prm%debitglob = debitglob; prm%h_sortie = h_sortie; prm%long_entree = long_entree
+  prm%hydro_n = 0; prm%hydro_t = 0; prm%hydro_q = 0
+
+  msh%nelt = nelt; msh%nnt = nnt
+  msh%conn = c_loc(conn_c); msh%coords = c_loc(coords_c); msh%boundary = c_loc(bnd)
+  msh%manning = c_loc(mann_c)
+  st = cf_create(msh, prm, h)
+  call cf_check(st, 'cf_create')
+
+  allocate(zm(nelt), surf(nelt), cot(nelt), u(3, nelt))
+  st = cf_get_geometry(h, surf, zm, cot, cotemin)
+  call cf_check(st, 'cf_get_geometry')
+  write(*, *) '======================================================================'
+  write(*, *) ' Parameters'
+  write(*, *) '======================================================================'
+  write(*, *) 'meshfile            = ', meshfile
+  write(*, *) 'cotemin             = ', cotemin
+  write(*, *) 'iflux               =', iflux
+  write(*, *) 'Fricimplic          =', fricimplic
+  write(*, *) 'tol                 =', tol
+  write(*, *) 'tolisec             =', tolisec
+  write(*, *) 'nombre entree       =', nombre_entrees
+  do i = 1, nombre_entrees
+     write(*, *) 'debitglob       ', i, " =", debitglob(i)
+     write(*, *) 'longueur entree ', i, " =", long_entree(i)
+  end do
+
+  ! ---- initial condition (initial_condition.cuf) ----
+  u = 0
+  if (solinit == 1) then
+     open(13, file=trim(fich_sol_init), status='old')
+     read(13, *) tc_init
+     do i = 1, nelt
+        read(13, *) u(1, i), u(2, i), u(3, i)
+     end do
+     close(13)
+  else
+     do i = 1, nelt
+        x1 = coords(conn(i, 1), 1); y1 = coords(conn(i, 1), 2)
+        x2 = coords(conn(i, 2), 1); y2 = coords(conn(i, 2), 2)
+        x3 = coords(conn(i, 3), 1); y3 = coords(conn(i, 3), 2)
+        v1 = (x1 + x2 + x3) / 3
+        v2 = (y1 + y2 + y3) / 3
+        if (eqlin_barrage == 1) then
+           alpha = (y1eqbar - y2eqbar) / (x1eqbar - x2eqbar)
+           beta = y1eqbar - alpha * x1eqbar
+           if (alpha * v1 - v2 + beta >= 0.00) then
+              hh = h_amont - zm(i)
+              if (hh > tolisec) then
+                 u(:, i) = [hh, hh * u_amont, hh * v_amont]
+              else
+                 u(:, i) = 0
+              end if
+           else
+              hh = h_aval - zm(i)
+              if (hh > tolisec) then
+                 u(:, i) = [hh, hh * u_aval, hh * v_aval]
+              else
+                 u(:, i) = 0
+              end if
+           end if
+        end if
+        if (plan == 1) then
+           hh = (-xplan * (xpoint - v1) - yplan * (ypoint - v2)) / zplan + zpoint - zm(i)
+           if (hh > tolisec) then
+              u(:, i) = [hh, hh * u_aval, hh * v_aval]
+           else
+              u(:, i) = 0
+           end if
+        end if
+     end do
+  end if
+  st = cf_set_state(h, u)
+  call cf_check(st, 'cf_set_state')
+
+  ! ---- time loop (finite_volumes_method.cuf:113-242), freqaffich steps per batch ----
+  nbatch = max(1, freqaffich)
+  allocate(dt_hist(nbatch + 1))
+  nt = 0
+  write(*, *) 'Loop over time starts'
+  call system_clock(c0, crate)
+  do
+     st = cf_step(h, nbatch, dt_hist, done)
+     nt = nt + done
+     if (st == CF_ERR_DT) then
+        st = cf_get_time(h, tc, dt)
+        print *, "Erreur sur le pas de temps dt=", dt
+        exit
+     end if
+     call cf_check(st, 'cf_step')
+     st = cf_get_time(h, tc, dt)
+     if (mod(nt, freqaffich) == 0 .and. done > 0) then
+        print *, '-----------------------------------------------'
+        write(*, *) 'Iteration number ', nt, ", t = ", tc
+        print *, '-----------------------------------------------'
+     end if
+     if (done < nbatch) exit
+  end do
+  call system_clock(c1)
+  write(*, *) 'Loop on time ends at iteration : ', nt
+
+  st = cf_get_state(h, u)
+  call cf_check(st, 'cf_get_state')
+
+  ! ---- outputs ----
+  if (solrestart == 1) then   ! save_sol_for_restart (pre_post_traitement.cuf:498-512)
+     open(14, file='solution_elements_restart.txt', status='unknown')
+     write(14, *) tc
+     do i = 1, nelt
+        write(14, '(4f16.6)') u(1, i), u(2, i), u(3, i)
+     end do
+     close(14)
+  end if
+  if (solvisu == 1 .or. solvisu == 5) then   ! simple_export, elements
+     open(15, file='solution_elements_simple_final.txt', status='unknown')
+     write(15, *) tc
+     do i = 1, nelt
+        write(15, '(4f16.6)') u(1, i), u(1, i) + zm(i) - sol_z_offset, u(2, i), u(3, i)
+     end do
+     close(15)
+  end if
+  if (solvisu == 1 .or. solvisu == 2) call vtk_export('solution-Paraview_final.vtk')
+
+  print *, '==========================================================================='
+  print *, '=========================   FIN DE LA SIMULATION   ========================'
+  print *, '==========================================================================='
+  print *, 'DUREE DU CALCUL :', real(c1 - c0) / real(crate), ' Secondes '
+  call cf_destroy(h)
+
+contains
+
+  ! paraview_export (pre_post_traitement.cuf:407-496)
+  subroutine vtk_export(fname)
+    character(len=*), intent(in) :: fname
+    integer :: nd, iel
+    real(dp) :: solu, solv
+    open(16, file=fname, status='unknown')
+    write(16, '(a)') "# vtk DataFile Version 1.0"
+    write(16, '(a)') "2D Unstructured Grid"
+    write(16, '(a)') "ASCII"
+    write(16, '(a)') ""
+    write(16, '(a)') "DATASET UNSTRUCTURED_GRID"
+    write(16, '(a,I10,a)') "POINTS", nnt, " float"
+    do nd = 1, nnt
+       write(16, '(3f16.6)') coords(nd, 1), coords(nd, 2), 0.
+    end do
+    write(16, '(a)') ""
+    write(16, '(a,I10,I10)') "CELLS", nelt, 4 * nelt
+    do iel = 1, nelt
+       write(16, '(I1,3I10)') 3, conn(iel, 1) - 1, conn(iel, 2) - 1, conn(iel, 3) - 1
+    end do
+    write(16, '(a)') ""
+    write(16, '(a,I10)') "CELL_TYPES", nelt
+    do iel = 1, nelt
+       write(16, '(a)') "5"
+    end do
+    write(16, '(a)') ""
+    write(16, '(a,I10)') "CELL_DATA", nelt
+    write(16, '(a)') "SCALARS eta float"
+    write(16, '(a)') "LOOKUP_TABLE default"
+    do iel = 1, nelt
+       write(16, '(f16.6)') u(1, iel) + zm(iel) - sol_z_offset
+    end do
+    write(16, '(a)') ""
+    write(16, '(a)') "SCALARS h float"
+    write(16, '(a)') "LOOKUP_TABLE default"
+    do iel = 1, nelt
+       write(16, '(f16.6)') u(1, iel)
+    end do
+    write(16, '(a)') ""
+    write(16, '(a)') "SCALARS b float"
+    write(16, '(a)') "LOOKUP_TABLE default"
+    do iel = 1, nelt
+       write(16, '(f16.6)') zm(iel) - sol_z_offset
+    end do
+    write(16, '(a)') ""
+    write(16, '(a)') "VECTORS velocity float"
+    do iel = 1, nelt
+       if (u(1, iel) > tolisec) then
+          solu = u(2, iel) / u(1, iel)
+          solv = u(3, iel) / u(1, iel)
+       else
+          solu = 0.d0
+          solv = 0.d0
+       end if
+       write(16, '(3f16.6)') solu, solv, 0.
+    end do
+    write(16, '(a)') ""
+    write(16, '(a)') "VECTORS flux float"
+    do iel = 1, nelt
+       if (u(1, iel) > tolisec) then
+          write(16, '(3f16.6)') u(2, iel), u(3, iel), 0.
+       else
+          write(16, '(3f16.6)') 0., 0., 0.
+       end if
+    end do
+    close(16)
+  end subroutine vtk_export
+
+end program cuteflow_main

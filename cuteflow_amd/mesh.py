@@ -156,12 +156,12 @@ def write_text_mesh(path: str, m: Mesh) -> None:
         fh.write(f"{m.nnt}\n")
         for i in range(m.nnt):
             x, y, z = m.coords[i]
-            fh.write(f"{i + 1} {x!r} {y!r} {z!r} {mnd[i]!r}\n")
+            fh.write(f"{i + 1} {float(x)!r} {float(y)!r} {float(z)!r} {float(mnd[i])!r}\n")
         fh.write("Table de connectivite\n")
         fh.write(f"{m.nelt}\n")
         for i in range(m.nelt):
             a, b, c = m.conn[i]
-            fh.write(f"{i + 1} {a} {b} {c} {m.manning[i]!r}\n")
+            fh.write(f"{i + 1} {a} {b} {c} {float(m.manning[i])!r}\n")
         for title, ids, num, n_io in (("Noeuds d'entree", m.ndinput, m.numero_ndinput, m.nombre_entrees),
                                       ("Noeuds de sortie", m.ndoutput, m.numero_ndoutput, m.nombre_sorties)):
             fh.write(title + "\n")

@@ -1,0 +1,89 @@
+"""Case drivers on the GPU: the Fortran host (amdflang, iso_c_binding) and the
+Python driver run the same case directory (donnees.f, text mesh, boundary
+table, dam-line initial condition) and agree with the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from cuteflow_amd import io as cio
+from cuteflow_amd import mesh as M
+from cuteflow_amd.build import FORTRAN_EXE
+from cuteflow_amd.driver import run_case
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DONNEES = """&DONNEES_NAMELIST
+  GP=9.81,
+  is_override_manning=0, override_manning=0.0,
+  is_cgns=0, meshfile_path='./', meshfile='reach.txt', elt_bound=0,
+  nombre_entrees=1, nombre_sorties=1,
+  eqlin_barrage=1, x1eqbar=150.0, x2eqbar=150.0001, y1eqbar=0.0, y2eqbar=200.0,
+  H_AMONT={h_amont}, U_AMONT=0, V_AMONT=0, H_AVAL={h_aval}, U_AVAL=0, V_AVAL=0,
+  plan=0, solinit=0, fich_sol_init='',
+  inlet='inflow', inlet_name="in", debitglob=120.0,
+  outlet="fixedheight", outlet_name="out", H_sortie={h_out},
+  IFLUX=3, ilimiteur=1, iupwind=1, tolisec=1.0E-03, friction=1, fricimplic=1,
+  TS={ts}, CFL=0.9, is_dt_constant=0, constant_dt=0.0, tol_reg_perm=0.0,
+  freqaffich=50, sol_z_offset=0., solrestart=1, solvisu=5, visu_snapshots=2,
+  sortie_finale_bluekenue=0 /
+"""
+
+
+def _make_case(tmp_path, ts=20.0):
+    r = M.river_mesh(nx=40, ny=10, length=400.0, width=200.0, channel_width=80.0, bar_len=100.0,
+                     field_corr=50.0, seed=11)
+    M.write_text_mesh(str(tmp_path / "reach.txt"), r.mesh)
+    z = r.mesh.coords[:, 2]
+    (tmp_path / "donnees.f").write_text(DONNEES.format(h_amont=float(z.max()) + 0.5, h_aval=float(z.max()) - 1.0,
+                                                       h_out=r.h_out, ts=ts))
+    return r.mesh
+
+
+def _oracle(mesh, case_dir):
+    from cuteflow_amd.params import read_donnees
+    from cuteflow_amd.solver import initial_state
+    p = read_donnees(os.path.join(case_dir, "donnees.f"))
+    prep = O.Prepared(mesh)
+    U0 = initial_state(p, mesh, prep.zm)
+    return p, prep, O.run(prep, p, U0, 100000)
+
+
+def test_python_driver_matches_oracle(tmp_path):
+    mesh = _make_case(tmp_path)
+    out = run_case(str(tmp_path), verbose=False)
+    p, prep, ref = _oracle(mesh, str(tmp_path))
+    assert out["status"] == 0 and ref["status"] == 0
+    assert out["steps"] == ref["steps"] and out["tc"] == ref["tc"]
+    assert np.array_equal(out["U"], ref["U"])
+    # boundary table file written like the reference
+    bnd = M.read_boundary_table(str(tmp_path / "boundary_table_reach.txt"))
+    assert np.array_equal(bnd, prep.boundary_fortran())
+    # restart file: tc then (4f16.6) rows
+    lines = open(tmp_path / "solution_elements_restart.txt").read().splitlines()
+    assert float(lines[0]) == out["tc"] and len(lines) == mesh.nelt + 1
+    assert lines[1] == cio._fmt(ref["U"][0])
+    # snapshots at 0, ts/2, ts
+    assert [k for k, _ in out["snapshots"]] == [0, 1, 2]
+
+
+def test_fortran_driver_matches_python(tmp_path):
+    if not os.path.exists(FORTRAN_EXE):
+        pytest.skip("Fortran driver not built")
+    mesh = _make_case(tmp_path)
+    proc = subprocess.run([FORTRAN_EXE], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert proc.returncode == 0, proc.stdout + proc.stderr
+    assert "FIN DE LA SIMULATION" in proc.stdout
+    fort_restart = open(tmp_path / "solution_elements_restart.txt").read().splitlines()
+    fort_bt = open(tmp_path / "boundary_table_reach.txt").read()
+    p, prep, ref = _oracle(mesh, str(tmp_path))
+    # same step count and state (to the 6 decimals of the restart format)
+    assert f"Loop on time ends at iteration :" in proc.stdout
+    nt = int(proc.stdout.split("Loop on time ends at iteration :")[1].split()[0])
+    assert nt == ref["steps"]
+    assert abs(float(fort_restart[0]) - ref["tc"]) <= 1e-12 * ref["tc"]
+    assert fort_restart[1:] == [cio._fmt(row) for row in ref["U"]]
+    assert M.read_boundary_table(str(tmp_path / "boundary_table_reach.txt")).shape == (2, 3, mesh.nelt)
+    assert fort_bt.splitlines()[0].split() == [str(v) for v in prep.boundary_fortran()[:, :, 0].reshape(-1)]
