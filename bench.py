@@ -77,6 +77,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--options", type=int, default=0,
                     help="cf_params.options bits: 1 keep the input cell order, 2 run every tile")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="before the timed region, run a device copy of this many MiB (3 times): a "
+                         "known-size kernel that calibrates HBM byte counters in a profile")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -100,6 +103,16 @@ def main():
     if args.warmup:
         s.step(args.warmup)
     log(f"bench: warmup {args.warmup} steps done, tc={s.tc:.3f}s")
+
+    if args.copy_probe_mb:
+        n = args.copy_probe_mb * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: copy probe {args.copy_probe_mb} MiB read + {args.copy_probe_mb} MiB written, 3 times")
+        del src, dst
 
     stream = torch.cuda.ExternalStream(s.stream)
     ev0 = torch.cuda.Event(enable_timing=True)

@@ -319,18 +319,24 @@ def river_mesh(nx: int = 3200, ny: int = 320, length: float = 20000.0, width: fl
                bar_amp: float = 0.5, bar_len: float = 300.0, field_sigma: float = 0.3,
                field_corr: float = 200.0, n_channel: float = 0.03, n_plain: float = 0.06,
                q_peak: float = 3000.0, t_ramp: float = 3600.0, jitter: float = 0.2,
-               seed: int = 1) -> River:
+               bank_width: float | None = None, seed: int = 1) -> River:
     """River reach with a floodplain: a `depth`-deep main channel along the
     centre line of a dry floodplain; bed = -slope*x + sinusoidal bars +
     seeded smooth random field; inflow on the channel section at x = 0,
-    fixed water level over the whole section at x = length, walls elsewhere."""
+    fixed water level over the whole section at x = length, walls elsewhere.
+
+    The channel banks slope linearly over `bank_width` (default 8 cells
+    across): a one-cell cliff makes the reference's WAF scheme (superbee
+    limiter) blow up where shallow floodplain water flows over the bank."""
     rng = np.random.default_rng(seed)
     x, y = _grid_nodes(nx, ny, length, width, jitter, rng)
     yc0 = width / 2.0
-    in_channel_node = np.abs(y - yc0) <= channel_width / 2.0 + 1e-9
+    if bank_width is None:
+        bank_width = 8.0 * width / ny
     z = -slope * x + bar_amp * np.sin(2.0 * np.pi * x / bar_len)
     z = z + smooth_random_field(x, y, field_sigma, field_corr, seed + 1000, (0.0, 0.0, length, width))
-    z = z - np.where(in_channel_node, depth, 0.0)
+    frac = np.clip((channel_width / 2.0 + bank_width / 2.0 - np.abs(y - yc0)) / bank_width, 0.0, 1.0)
+    z = z - depth * frac
     coords = np.stack([x, y, z], 1)
     conn = _grid_triangles(nx, ny)
     ids, bnodes = _boundary_nodes(nx, ny)
