@@ -98,6 +98,19 @@ SIGNATURES = {
     "cf_last_run_stats": (C.c_int, [_H, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _IP]),
     "cf_get_stream": (C.c_void_p, [_H]),
     "cf_set_stream": (C.c_int, [_H, C.c_void_p]),
+    "cf_batch_create": (C.c_int, [C.c_int, C.POINTER(CfMesh), C.POINTER(CfParams), C.POINTER(_H)]),
+    "cf_batch_size": (C.c_int, [_H]),
+    "cf_batch_set_state": (C.c_int, [_H, C.c_int, _DP]),
+    "cf_batch_get_state": (C.c_int, [_H, C.c_int, _DP]),
+    "cf_batch_set_state_device": (C.c_int, [_H, C.c_int, C.c_void_p]),
+    "cf_batch_get_state_device": (C.c_int, [_H, C.c_int, C.c_void_p]),
+    "cf_batch_get_time": (C.c_int, [_H, C.c_int, _DP, _DP]),
+    "cf_batch_step": (C.c_int, [_H, C.c_int, _DP, _IP, _IP]),
+    "cf_batch_step_finish": (C.c_int, [_H, _DP, _IP, _IP]),
+    "cf_batch_set_inflow": (C.c_int, [_H, C.c_int, _DP, C.c_int]),
+    "cf_batch_set_hydrograph": (C.c_int, [_H, C.c_int, C.c_int, C.c_int, _DP, _DP]),
+    "cf_batch_set_manning": (C.c_int, [_H, C.c_int, _DP]),
+    "cf_batch_get_geometry": (C.c_int, [_H, C.c_int, _DP, _DP, _DP, _DP]),
     "cf_debug_pow": (C.c_int, [C.c_int, _DP, C.c_double, _DP]),
 }
 

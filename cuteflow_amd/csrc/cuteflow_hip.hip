@@ -10,13 +10,18 @@
 //   - the new state goes to the other half of a ping-pong buffer (no
 //     vdlg0 <- vdlg copy, finite_volumes_method.cuf:230);
 //   - the CFL dt of the new state is reduced with wavefront shuffles and a
-//     workgroup LDS pass to one plain store per workgroup; a one-workgroup
-//     finalize kernel reduces those, publishes dt, tc and the stop/error
-//     status for the next step, so a run of steps needs no host
-//     synchronisation and no device-wide fence or atomic (on gfx950 an
-//     agent-scope fence writes back and invalidates the XCD's L2);
+//     workgroup LDS pass to one plain store per workgroup; a finalize kernel
+//     (one workgroup per instance) reduces those and publishes dt, tc and
+//     the stop/error status in a device-resident control block, so a run of
+//     steps needs no host synchronisation and no device-wide fence or atomic
+//     (on gfx950 an agent-scope fence writes back and invalidates the XCD's
+//     L2);
 //   - cells are stored along a Hilbert curve (256-cell tiles are compact
-//     patches) and tiles that are dry with dry neighbours are skipped.
+//     patches) and tiles that are dry with dry neighbours are skipped;
+//   - a handle holds one or more instances (meshes of any size, each padded
+//     to whole tiles, each with its own parameters, dt, time and status):
+//     one launch per step covers all of them, and a run of steps is
+//     captured once in a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,8 +33,8 @@
 #include <new>
 #include <vector>
 
-#include "../../include/cuteflow_hip.h"
 #include "cf_device.h"
+#include "cf_internal.h"
 #include "cf_prep.h"
 
 #pragma clang fp contract(off)
@@ -38,60 +43,11 @@ using namespace cf;
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / 64;
-
-// geometry arrays (structure of arrays, each nelt doubles)
-enum { G_N1 = 0, G_N2 = 3, G_LN = 6, G_SURF = 9, G_COT = 10, G_XC = 11, G_YC = 12, G_COUNT = 13 };
-
-struct KParams {
-    double gp, tolisec, tol, cfl, ts, constant_dt, override_manning;
-    double pad_dt;  // min(1e6, 1e5 if nelt % 1024 != 0): reduction padding
-    int ilimiteur, inlet, outlet, is_dt_constant, is_override_manning;
-    int reserved;
-    double debitglob[CF_MAX_IO];
-    double h_sortie[CF_MAX_IO];
-    double long_entree[CF_MAX_IO];
-    int hydro_n[CF_MAX_IO];
-};
-
-struct Ctrl {
-    int *hdr;          // [0] status (0 running, 1 ts reached, -1 dt/NaN failure), [1] steps done
-    double *partial;   // per workgroup: min CFL dt of the new state, NaN if a state is NaN
-    double *dt;        // dt[k] used by step k
-    double *tc;        // tc[k] at the start of step k
-    int *ran;          // ran[k]: tiles that ran step k (skipped tiles store +inf partials)
-};
-
-struct StepArgs {
-    const double4 *__restrict__ st_in;
-    double4 *__restrict__ st_out;
-    const int4 *__restrict__ nbr;
-    const double *__restrict__ geo;
-    const double4 *__restrict__ dfc;
-    const double *__restrict__ manning;
-    const double *__restrict__ hydro;  // t[20*64] then q[20*64]
-    Ctrl c;
-    // dry-tile skipping: a tile runs step s only if it or a neighbour tile
-    // held a wet cell after step s-1 or s-2 (stamp >= s-1), or always[tile]
-    unsigned *stamp;
-    const int *__restrict__ tile_off;
-    const int *__restrict__ tile_adj;
-    const unsigned char *__restrict__ always;
-    unsigned gstep;  // global number of step 0 of this launch
-    int skip;        // 1: dry-tile skipping enabled
-    int nelt;
-    int step;
-    int xcd_map;  // 1: consecutive tiles on the same XCD
-    int reserved;
-    KParams p;
-};
-
 __device__ double inflow_q(const KParams &p, const double *hydro, int id, double tc) {
-    int n = p.hydro_n[id];
+    const int n = p.hydro_n[id];
     if (n <= 0) return p.debitglob[id];
-    const double *t = hydro + id * CF_MAX_HYDRO;
-    const double *q = hydro + CF_MAX_IO * CF_MAX_HYDRO + id * CF_MAX_HYDRO;
+    const double *t = hydro + p.hydro_off + id * CF_MAX_HYDRO;
+    const double *q = hydro + p.hydro_off + CF_MAX_IO * CF_MAX_HYDRO + id * CF_MAX_HYDRO;
     if (tc <= t[0]) return q[0];
     for (int i = 0; i + 1 < n; i++)
         if (tc < t[i + 1]) return q[i] + (q[i + 1] - q[i]) * ((tc - t[i]) / (t[i + 1] - t[i]));
@@ -105,15 +61,6 @@ __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = dmin(v, __shfl_xor(v, off, 64));
     return v;
-}
-
-// Workgroups are dispatched round-robin over the 8 XCDs; remap the workgroup
-// id so that each XCD works on one contiguous range of tiles, whose
-// neighbour gathers then hit that XCD's own L2.
-__device__ __forceinline__ int tile_of(int b, int nblocks, int xcd_map) {
-    if (!xcd_map) return b;
-    const int q = nblocks >> 3, r = nblocks & 7, x = b & 7, k = b >> 3;
-    return x * q + (x < r ? x : r) + k;
 }
 
 // Flux of a face whose two reconstructed depths are both below eps: the
@@ -140,49 +87,54 @@ __device__ __forceinline__ void zero_flux(double n1, double n2, double flux[3]) 
 
 // FRIC: 0 no friction, 1 explicit (fricimplic 0), 2 fricimplic 1, 3 fricimplic 2
 template <int IFLUX, int FRIC>
-__global__ __launch_bounds__(kBlock) CF_STEP_OCCUPANCY void step_kernel(StepArgs a) {
+__global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs a) {
     __shared__ double s_min[kWaves];
-    if (a.c.hdr[0] != 0) return;  // stopped (ts reached) or failed earlier
-    const int nelt = a.nelt;
-    const int tile = tile_of(blockIdx.x, gridDim.x, a.xcd_map);
-    const unsigned s_glob = a.gstep + (unsigned)a.step;
+    const int tile = blockIdx.x;
+    const int b = a.tile_inst[tile];
+    const InstCtrl c = a.ctrl[b];
+    if (c.status != 0) {  // this instance has stopped (ts reached) or failed
+        if (threadIdx.x == 0) a.partial[tile] = __builtin_inf();
+        return;
+    }
     // A tile whose cells and neighbour tiles' cells were all dry after the
     // last two steps would recompute (0, 0, 0) for every cell (all faces
     // dry/dry, no source, clamp), and both state buffers already hold that:
     // it is skipped.  Reading a stamp raised concurrently only makes the
     // tile run, which is always correct.
-    if (a.skip && !a.always[tile] && a.stamp[tile] + 1u < s_glob) {
-        if (threadIdx.x == 0) a.c.partial[blockIdx.x] = __builtin_inf();  // no dt contribution
+    if (a.skip && !a.always[tile] && a.stamp[tile] + 1u < c.gstep) {
+        if (threadIdx.x == 0) a.partial[tile] = __builtin_inf();  // no dt contribution
         return;
     }
-    const int i = tile * kBlock + threadIdx.x;
-    int wet_local = 0;
-    const double dt = a.c.dt[a.step];
-    const double tc = a.c.tc[a.step];
-    const KParams &p = a.p;
+    const double4 *__restrict__ st_in = a.in_x ? a.in_x : a.st[(c.par + a.step) & 1];
+    double4 *__restrict__ st_out = a.out_x ? a.out_x : a.st[(c.par + a.step + 1) & 1];
+    const int nelt = a.ncell;  // geometry stride
+    const int i = tile * kTile + threadIdx.x;
+    const double dt = c.dt;
+    const double tc = c.tc;
+    const KParams &p = a.kp[b];
     const double g = p.gp, tls = p.tolisec;
     double dti = 1000000.0;
-    int isnan_local = 0;
+    int isnan_local = 0, wet_local = 0;
 
-    if (i < nelt) {
-        const double4 u0 = a.st_in[i];
+    if (i < a.inst_end[b]) {
+        const double4 u0 = st_in[i];
         const int4 nb = a.nbr[i];
         const double zml = u0.w;
         // first ring: the three neighbour states, gathered once
         double4 unb[3];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            const int c = nb_at(nb, j);
-            unb[j] = c >= 0 ? a.st_in[c] : u0;
+            const int cc = nb_at(nb, j);
+            unb[j] = cc >= 0 ? st_in[cc] : u0;
         }
         double4 dfi = make_double4(0.0, 0.0, 0.0, 0.0);
         if (IFLUX == 3) dfi = a.dfc[i];
         double fl[3][3], fn1[3], fn2[3], fln[3];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-            const double n1 = a.geo[(G_N1 + j) * nelt + i];
-            const double n2 = a.geo[(G_N2 + j) * nelt + i];
-            const double ln = a.geo[(G_LN + j) * nelt + i];
+            const double n1 = a.geo[(G_N1 + j) * (size_t)nelt + i];
+            const double n2 = a.geo[(G_N2 + j) * (size_t)nelt + i];
+            const double ln = a.geo[(G_LN + j) * (size_t)nelt + i];
             fn1[j] = n1; fn2[j] = n2; fln[j] = ln;
             const int code = nb_at(nb, j);
             double hautl, vitul, vitvl;
@@ -257,7 +209,7 @@ __global__ __launch_bounds__(kBlock) CF_STEP_OCCUPANCY void step_kernel(StepArgs
                         const int ka = (tjv + 1) % 3, kb = (tjv + 2) % 3;
                         const int e1 = nb_at(nbk, ka), e2 = nb_at(nbk, kb);
                         const int kw1 = e1 >= 0 ? e1 : kv, kw2 = e2 >= 0 ? e2 : kv;
-                        const double4 g1 = a.st_in[kw1], g2 = a.st_in[kw2];
+                        const double4 g1 = st_in[kw1], g2 = st_in[kw2];
                         double g1h, g1u, g1w, g2h, g2u, g2w;
                         rot(tls, n1, n2, g1.x, g1.y, g1.z, g1h, g1u, g1w);
                         rot(tls, n1, n2, g2.x, g2.y, g2.z, g2h, g2u, g2w);
@@ -331,7 +283,7 @@ __global__ __launch_bounds__(kBlock) CF_STEP_OCCUPANCY void step_kernel(StepArgs
                 sb2 = sb2 + (((g / 2.0) * (hijet * hijet - q1 * q1)) * fln[j]) * fn2[j];
             }
         }
-        const double surf = a.geo[G_SURF * nelt + i];
+        const double surf = a.geo[G_SURF * (size_t)nelt + i];
         double r0, r1, r2;
         if (FRIC == 0) {
             // update without friction (finite_volumes_method.cuf:294)
@@ -391,7 +343,7 @@ __global__ __launch_bounds__(kBlock) CF_STEP_OCCUPANCY void step_kernel(StepArgs
         double hu = u0.y + dt * r1;
         double hv = u0.z + dt * r2;
         if (h <= tls) { h = 0.0; hu = 0.0; hv = 0.0; }
-        a.st_out[i] = make_double4(h, hu, hv, zml);
+        st_out[i] = make_double4(h, hu, hv, zml);
         isnan_local = (h != h) || (hu != hu) || (hv != hv);
         wet_local = (h != 0.0) || isnan_local;
         // cflcond on the new state (cfl_condition.cuf:20-36)
@@ -399,7 +351,7 @@ __global__ __launch_bounds__(kBlock) CF_STEP_OCCUPANCY void step_kernel(StepArgs
             const double uu = hu / h, vv = hv / h;
             const double nv = sqrt(uu * uu + vv * vv);
             const double vmax = dmax(0.01, nv + sqrt(g * fabs(h)));
-            dti = dmin((p.cfl * a.geo[G_COT * nelt + i]) / vmax, 1000000.0);
+            dti = dmin((p.cfl * a.geo[G_COT * (size_t)nelt + i]) / vmax, 1000000.0);
         }
         if (p.is_dt_constant == 1) dti = p.constant_dt;
     }
@@ -413,20 +365,20 @@ __global__ __launch_bounds__(kBlock) CF_STEP_OCCUPANCY void step_kernel(StepArgs
         double bm = s_min[0];
 #pragma unroll
         for (int w = 1; w < kWaves; w++) bm = dmin(bm, s_min[w]);
-        a.c.partial[blockIdx.x] = anynan ? __builtin_nan("") : bm;
+        a.partial[tile] = anynan ? __builtin_nan("") : bm;
     }
     // a wet tile keeps itself and its neighbour tiles running for two steps
     if (a.skip && __syncthreads_or(wet_local)) {
-        const int o = a.tile_off[tile], n = a.tile_off[tile + 1] - o;
-        for (int k = threadIdx.x; k <= n; k += kBlock)
-            a.stamp[k == 0 ? tile : a.tile_adj[o + k - 1]] = s_glob + 1u;
+        const int o = a.tile_adj_off[tile], n = a.tile_adj_off[tile + 1] - o;
+        for (int k = threadIdx.x; k <= n; k += kTile)
+            a.stamp[k == 0 ? tile : a.tile_adj[o + k - 1]] = c.gstep + 1u;
     }
 }
 
-// Reduces a set of per-workgroup minima (NaN marks a NaN state) starting
-// from `pad`; thread 0 gets the result and the NaN flag.
-__device__ __forceinline__ double block_min_1024(const double *__restrict__ part, int n, double pad, int &nan,
-                                                 int &nskip) {
+// Reduces per-tile minima (NaN marks a NaN state, +inf a skipped tile)
+// starting from `pad`; every thread gets the result, the NaN flag and the
+// number of skipped tiles.
+__device__ double block_min_1024(const double *__restrict__ part, int n, double pad, int &nan, int &nskip) {
     __shared__ double s[1024 / 64];
     __shared__ int sk[1024 / 64];
     double m = pad;
@@ -454,48 +406,57 @@ __device__ __forceinline__ double block_min_1024(const double *__restrict__ part
     return r;
 }
 
-// End of a step: dt of the new state (the reference's reduction, cfl_condition.cuf:68-104),
-// tc += dt, dt sanity check (:129-132) and the loop condition (finite_volumes_method.cuf:123).
-__global__ __launch_bounds__(1024) void finalize_kernel(Ctrl c, int nparts, int step, double pad,
-                                                         double ts, double tol) {
-    if (c.hdr[0] != 0) return;
+// End of a step, one workgroup per instance: dt of the new state (the
+// reference's reduction, cfl_condition.cuf:68-104), tc += dt, the dt range
+// check (:129-132) and the loop condition (finite_volumes_method.cuf:123).
+__global__ __launch_bounds__(1024) void finalize_kernel(InstCtrl *ctrl, const KParams *kp, const double *partial,
+                                                         const int *inst_tile, double *dt_hist, double *tc_hist,
+                                                         int *ran, int hist_stride, int step) {
+    const int b = blockIdx.x;
+    InstCtrl *c = ctrl + b;
+    if (c->status != 0) return;
+    const int t0 = inst_tile[b], nt = inst_tile[b + 1] - t0;
     int nan, nskip;
-    const double dtn = block_min_1024(c.partial, nparts, pad, nan, nskip);
+    const double dtn = block_min_1024(partial + t0, nt, kp[b].pad_dt, nan, nskip);
     if (threadIdx.x == 0) {
-        const double tcn = c.tc[step] + c.dt[step];
-        c.dt[step + 1] = dtn;
-        c.tc[step + 1] = tcn;
-        c.ran[step] = nparts - nskip;
-        c.hdr[1] = step + 1;
-        if (nan || !(dtn <= 9999.0 && dtn >= kDtLow)) c.hdr[0] = -1;
-        else if (!(((tcn - ts) + 2.0 * tol) < tol)) c.hdr[0] = 1;
+        const double tcn = c->tc + c->dt;
+        c->dt = dtn;
+        c->tc = tcn;
+        c->steps = step + 1;
+        c->gstep += 1u;
+        dt_hist[(size_t)b * hist_stride + step + 1] = dtn;
+        tc_hist[(size_t)b * hist_stride + step + 1] = tcn;
+        ran[(size_t)b * hist_stride + step] = nt - nskip;
+        if (nan || !(dtn <= 9999.0 && dtn >= kDtLow)) c->status = -1;
+        else if (!(((tcn - kp[b].ts) + 2.0 * kp[b].tol) < kp[b].tol)) c->status = 1;
     }
 }
 
-// dt of the current state (finite_volumes_method.cuf:113-116), into *out.
-__global__ __launch_bounds__(1024) void reduce_dt_kernel(const double *part, int nparts, double pad, double *out) {
-    int nan, nskip;
-    const double m = block_min_1024(part, nparts, pad, nan, nskip);
-    if (threadIdx.x == 0) *out = nan ? __builtin_nan("") : m;
+// Start of a run: the buffer parity moves past the previous run's steps,
+// the loop condition is evaluated (a failed instance stays failed).
+__global__ void run_init_kernel(InstCtrl *ctrl, const KParams *kp, double *dt_hist, double *tc_hist,
+                                int hist_stride, int ninst) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ninst) return;
+    InstCtrl *c = ctrl + b;
+    c->par = (c->par + c->steps) & 1;
+    c->steps = 0;
+    if (c->status >= 0) c->status = (((c->tc - kp[b].ts) + 2.0 * kp[b].tol) < kp[b].tol) ? 0 : 1;
+    dt_hist[(size_t)b * hist_stride] = c->dt;
+    tc_hist[(size_t)b * hist_stride] = c->tc;
 }
 
-// Prepares the control block for a run of steps.
-__global__ void ctrl_init_kernel(Ctrl c, double dt0, double tc0, double ts, double tol) {
-    c.dt[0] = dt0;
-    c.tc[0] = tc0;
-    c.hdr[1] = 0;
-    c.hdr[0] = (((tc0 - ts) + 2.0 * tol) < tol) ? 0 : 1;
-}
-
-// cflcond for the current state, one minimum per workgroup (cfl_condition.cuf:4-39);
-// like the reference, the initial dt does not look for NaN states
-__global__ __launch_bounds__(kBlock) void cfl_kernel(const double4 *__restrict__ st,
-                                                      const double *__restrict__ cot, int nelt,
-                                                      KParams p, double *__restrict__ part) {
+// cflcond for the current state of one instance, one minimum per tile
+// (cfl_condition.cuf:4-39); like the reference, the initial dt does not
+// look for NaN states.
+__global__ __launch_bounds__(kTile) void cfl_kernel(const double4 *__restrict__ st, const double *__restrict__ cot,
+                                                     int cell0, int cell_end, const KParams *kp, int b,
+                                                     double *__restrict__ part) {
     __shared__ double s_min[kWaves];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = cell0 + blockIdx.x * kTile + threadIdx.x;
+    const KParams &p = kp[b];
     double dti = 1000000.0;
-    if (i < nelt) {
+    if (i < cell_end) {
         const double4 u = st[i];
         const double h = u.x;
         if (h > p.tolisec) {
@@ -516,34 +477,48 @@ __global__ __launch_bounds__(kBlock) void cfl_kernel(const double4 *__restrict__
     }
 }
 
+// dt of the current state (finite_volumes_method.cuf:113-116); a new state
+// restarts the instance's clock.
+__global__ __launch_bounds__(1024) void reset_inst_kernel(const double *part, int nparts, const KParams *kp, int b,
+                                                           InstCtrl *ctrl) {
+    int nan, nskip;
+    const double m = block_min_1024(part, nparts, kp[b].pad_dt, nan, nskip);
+    if (threadIdx.x == 0) {
+        ctrl[b].dt = nan ? __builtin_nan("") : m;
+        ctrl[b].tc = 0.0;
+        ctrl[b].status = 0;
+    }
+}
 
 // Geometry of each cell (maillage.cuf:132-159, :1254-1331, flux_riemann.cuf:50-74).
-__global__ void geometry_kernel(const int *__restrict__ conn, const double *__restrict__ coords,
-                                int nelt, double *__restrict__ geo, double *__restrict__ zm_out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nelt) return;
-    const int s1 = conn[3 * i] - 1, s2 = conn[3 * i + 1] - 1, s3 = conn[3 * i + 2] - 1;
+__global__ void geometry_kernel(const int *__restrict__ conn, const double *__restrict__ coords, int n,
+                                int cell0, int stride, double *__restrict__ geo, double *__restrict__ zm_out) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const int i = cell0 + l;
+    const int s1 = conn[3 * l] - 1, s2 = conn[3 * l + 1] - 1, s3 = conn[3 * l + 2] - 1;
     const double vx[3] = {coords[3 * s1], coords[3 * s2], coords[3 * s3]};
     const double vy[3] = {coords[3 * s1 + 1], coords[3 * s2 + 1], coords[3 * s3 + 1]};
     const double vz[3] = {coords[3 * s1 + 2], coords[3 * s2 + 2], coords[3 * s3 + 2]};
     const double x1 = vx[0], y1 = vy[0], x2 = vx[1], y2 = vy[1], x3 = vx[2], y3 = vy[2];
-    geo[G_XC * nelt + i] = ((x1 + x2) + x3) / 3.0;
-    geo[G_YC * nelt + i] = ((y1 + y2) + y3) / 3.0;
+    const size_t S = (size_t)stride;
+    geo[G_XC * S + i] = ((x1 + x2) + x3) / 3.0;
+    geo[G_YC * S + i] = ((y1 + y2) + y3) / 3.0;
     {
         const double ca = sqrt((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2));
         const double cb = sqrt((x2 - x3) * (x2 - x3) + (y2 - y3) * (y2 - y3));
         const double cc = sqrt((x3 - x1) * (x3 - x1) + (y3 - y1) * (y3 - y1));
         const double pp = ((ca + cb) + cc) / 2.0;
         const double aire = sqrt(((pp * (pp - ca)) * (pp - cb)) * (pp - cc));
-        double cot = (0.5 * aire) / pp;
-        geo[G_COT * nelt + i] = kCote * cot;
+        const double cot = (0.5 * aire) / pp;
+        geo[G_COT * S + i] = kCote * cot;
     }
     {
         const double aa = sqrt((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
         const double bb = sqrt((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2));
         const double cc = sqrt((x1 - x3) * (x1 - x3) + (y1 - y3) * (y1 - y3));
         const double s = ((aa + bb) + cc) / 2.0;
-        geo[G_SURF * nelt + i] = sqrt(((s * (s - aa)) * (s - bb)) * (s - cc));
+        geo[G_SURF * S + i] = sqrt(((s * (s - aa)) * (s - bb)) * (s - cc));
         const double xg = ((x1 + x2) + x3) / 3.0, yg = ((y1 + y2) + y3) / 3.0;
         const double d1 = sqrt((x1 - xg) * (x1 - xg) + (y1 - yg) * (y1 - yg));
         const double d2 = sqrt((x2 - xg) * (x2 - xg) + (y2 - yg) * (y2 - yg));
@@ -553,32 +528,33 @@ __global__ void geometry_kernel(const int *__restrict__ conn, const double *__re
     for (int j = 0; j < 3; j++) {
         const int c1 = j, c2 = (j + 1) % 3, c3 = (j + 2) % 3;
         const double ax = vx[c1], ay = vy[c1], bx = vx[c2], by = vy[c2], cx = vx[c3], cy = vy[c3];
-        const double l = sqrt((bx - ax) * (bx - ax) + (by - ay) * (by - ay));
-        double n1 = (by - ay) / l, n2 = (ax - bx) / l;
+        const double l2 = sqrt((bx - ax) * (bx - ax) + (by - ay) * (by - ay));
+        double n1 = (by - ay) / l2, n2 = (ax - bx) / l2;
         const double orient = n1 * ((ax + bx) / 2.0 - ((ax + bx) + cx) / 3.0) +
                               n2 * ((ay + by) / 2.0 - ((ay + by) + cy) / 3.0);
         if (orient < 0.0) {
-            n1 = (ay - by) / l;
-            n2 = (bx - ax) / l;
+            n1 = (ay - by) / l2;
+            n2 = (bx - ax) / l2;
         }
-        geo[(G_N1 + j) * nelt + i] = n1;
-        geo[(G_N2 + j) * nelt + i] = n2;
-        geo[(G_LN + j) * nelt + i] = l;
+        geo[(G_N1 + j) * S + i] = n1;
+        geo[(G_N2 + j) * S + i] = n2;
+        geo[(G_LN + j) * S + i] = l2;
     }
 }
 
 // Centroid distance across each face (0 across a boundary).
-__global__ void face_dist_kernel(const int4 *__restrict__ nbr, const double *__restrict__ geo,
-                                 int nelt, double4 *__restrict__ dfc) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= nelt) return;
+__global__ void face_dist_kernel(const int4 *__restrict__ nbr, const double *__restrict__ geo, int cell0,
+                                 int cell_end, int stride, double4 *__restrict__ dfc) {
+    const int i = cell0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cell_end) return;
+    const size_t S = (size_t)stride;
     const int4 nb = nbr[i];
-    const double xi = geo[G_XC * nelt + i], yi = geo[G_YC * nelt + i];
+    const double xi = geo[G_XC * S + i], yi = geo[G_YC * S + i];
     double d[3];
     for (int j = 0; j < 3; j++) {
         const int k = nb_at(nb, j);
         const int kk = k >= 0 ? k : i;
-        const double xk = geo[G_XC * nelt + kk], yk = geo[G_YC * nelt + kk];
+        const double xk = geo[G_XC * S + kk], yk = geo[G_YC * S + kk];
         d[j] = sqrt((xi - xk) * (xi - xk) + (yi - yk) * (yi - yk));
     }
     dfc[i] = make_double4(d[0], d[1], d[2], 0.0);
@@ -587,18 +563,20 @@ __global__ void face_dist_kernel(const int4 *__restrict__ nbr, const double *__r
 // State in the caller's order and layout (3, nelt) <-> internal double4
 // (h, hu, hv, zm) in the internal cell order; perm: internal -> caller.
 __global__ void pack_state_kernel(const double *__restrict__ U, const double *__restrict__ zm,
-                                  const int *__restrict__ perm, int nelt, double4 *__restrict__ st) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nelt) {
+                                  const int *__restrict__ perm, int cell0, int n, double4 *__restrict__ st) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) {
+        const int i = cell0 + l;
         const size_t o = 3 * (size_t)perm[i];
         st[i] = make_double4(U[o], U[o + 1], U[o + 2], zm[i]);
     }
 }
 
-__global__ void unpack_state_kernel(const double4 *__restrict__ st, const int *__restrict__ perm,
-                                    int nelt, double *__restrict__ U) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < nelt) {
+__global__ void unpack_state_kernel(const double4 *__restrict__ st, const int *__restrict__ perm, int cell0, int n,
+                                    double *__restrict__ U) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) {
+        const int i = cell0 + l;
         const double4 s = st[i];
         const size_t o = 3 * (size_t)perm[i];
         U[o] = s.x;
@@ -619,50 +597,13 @@ __global__ void pow_kernel(const double *__restrict__ x, double y, int n, double
 
 }  // namespace
 
-struct cf_handle {
-    int nelt = 0, nnt = 0;
-    cf_params prm{};
-    KParams kp{};
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    double4 *st[2] = {nullptr, nullptr};
-    int cur = 0;
-    int4 *nbr = nullptr;
-    double *geo = nullptr;
-    double4 *dfc = nullptr;
-    double *manning = nullptr;
-    double *hydro = nullptr;
-    double *zm = nullptr;
-    double *stage = nullptr;
-    Ctrl c{};
-    int cap = 0;
-    double *dt0 = nullptr;     // initial dt of the current state
-    int nblocks = 0;           // workgroups of a step launch (one tile each)
-    int xcd_map = 0;
-    // internal cell order (internal -> caller) on the device and the host
-    int *perm = nullptr;
-    std::vector<int> hperm;
-    // dry-tile skipping
-    unsigned *stamp = nullptr;
-    int *tile_off = nullptr;
-    int *tile_adj = nullptr;
-    unsigned char *always = nullptr;
-    unsigned gstep = 0;        // steps run since creation
-    int skip = 1;
-    long long last_tiles_run = 0, last_tiles_total = 0;
-    double dt = 0.0, tc = 0.0;
-    int status = 0;
-    int fric = 0;
-    int pending = -1;  // steps enqueued by cf_step_async, not yet finished
-};
-
-#define HIPCHK(x)                                   \
-    do {                                            \
-        hipError_t e_ = (x);                        \
-        if (e_ != hipSuccess) {                     \
-            fprintf(stderr, "cuteflow_hip: %s failed: %s\n", #x, hipGetErrorString(e_)); \
-            return CF_ERR_HIP;                      \
-        }                                           \
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) {                                                            \
+            fprintf(stderr, "cuteflow_hip: %s failed: %s\n", #x, hipGetErrorString(e_));  \
+            return CF_ERR_HIP;                                                             \
+        }                                                                                  \
     } while (0)
 
 static int grid_of(int n, int b) { return (n + b - 1) / b; }
@@ -673,54 +614,73 @@ static void dfree(T *&ptr) {
     ptr = nullptr;
 }
 
-static void fill_kparams(cf_handle *h) {
-    const cf_params &p = h->prm;
-    KParams &k = h->kp;
+template <typename T>
+static bool dalloc(T *&p, size_t count) {
+    return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T)) == hipSuccess;
+}
+
+template <typename T>
+static bool h2d(T *dst, const T *src, size_t count, hipStream_t s) {
+    return count == 0 || hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, s) == hipSuccess;
+}
+
+static KParams make_kparams(const cf_params &p, int nelt, int hydro_off) {
+    KParams k{};
     k.gp = p.gp; k.tolisec = p.tolisec; k.tol = p.tol; k.cfl = p.cfl; k.ts = p.ts;
     k.constant_dt = p.constant_dt; k.override_manning = p.override_manning;
-    k.pad_dt = (h->nelt % 1024 != 0) ? 100000.0 : 1000000.0;
+    k.pad_dt = (nelt % 1024 != 0) ? 100000.0 : 1000000.0;
     k.ilimiteur = p.ilimiteur; k.inlet = p.inlet; k.outlet = p.outlet;
     k.is_dt_constant = p.is_dt_constant; k.is_override_manning = p.is_override_manning;
+    k.hydro_off = hydro_off;
     for (int i = 0; i < CF_MAX_IO; i++) {
         k.debitglob[i] = p.debitglob[i];
         k.h_sortie[i] = p.h_sortie[i];
         k.long_entree[i] = p.long_entree[i];
         k.hydro_n[i] = p.hydro_n[i];
     }
+    return k;
 }
 
-static int upload_hydro(cf_handle *h) {
-    std::vector<double> buf(2 * CF_MAX_IO * CF_MAX_HYDRO);
-    memcpy(buf.data(), h->prm.hydro_t, sizeof(double) * CF_MAX_IO * CF_MAX_HYDRO);
-    memcpy(buf.data() + CF_MAX_IO * CF_MAX_HYDRO, h->prm.hydro_q, sizeof(double) * CF_MAX_IO * CF_MAX_HYDRO);
-    HIPCHK(hipMemcpyAsync(h->hydro, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+static constexpr int kHydroPerInst = 2 * CF_MAX_IO * CF_MAX_HYDRO;
+
+static int upload_params(cf_handle *h, int b) {
+    std::vector<double> buf(kHydroPerInst);
+    memcpy(buf.data(), h->prm[b].hydro_t, sizeof(double) * CF_MAX_IO * CF_MAX_HYDRO);
+    memcpy(buf.data() + CF_MAX_IO * CF_MAX_HYDRO, h->prm[b].hydro_q, sizeof(double) * CF_MAX_IO * CF_MAX_HYDRO);
+    HIPCHK(hipMemcpyAsync(h->hydro + (size_t)b * kHydroPerInst, buf.data(), buf.size() * sizeof(double),
+                          hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->d_kp + b, &h->kp[b], sizeof(KParams), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return CF_OK;
 }
 
-static int compute_dt(cf_handle *h) {
-    hipLaunchKernelGGL(cfl_kernel, dim3(h->nblocks), dim3(kBlock), 0, h->stream,
-                       h->st[h->cur], h->geo + G_COT * h->nelt, h->nelt, h->kp, h->c.partial);
-    hipLaunchKernelGGL(reduce_dt_kernel, dim3(1), dim3(1024), 0, h->stream, h->c.partial, h->nblocks,
-                       h->kp.pad_dt, h->dt0);
+static void drop_graphs(cf_handle *h) {
+    for (auto &g : h->graphs) (void)hipGraphExecDestroy(g.exec);
+    h->graphs.clear();
+}
+
+// dt of instance b's current state; restarts its clock.
+static int compute_dt(cf_handle *h, int b) {
+    const int c0 = h->cell_off[b], nt = h->tile_off[b + 1] - h->tile_off[b];
+    const int cur = (h->hctrl[b].par + h->hctrl[b].steps) & 1;
+    double *part = h->partial + h->tile_off[b];
+    hipLaunchKernelGGL(cfl_kernel, dim3(nt), dim3(kTile), 0, h->stream, h->st[cur], h->geo + G_COT * (size_t)h->ncell,
+                       c0, c0 + h->nelt[b], h->d_kp, b, part);
+    hipLaunchKernelGGL(reset_inst_kernel, dim3(1), dim3(1024), 0, h->stream, part, nt, h->d_kp, b, h->ctrl);
     HIPCHK(hipGetLastError());
-    double dt = 0.0;
-    HIPCHK(hipMemcpyAsync(&dt, h->dt0, sizeof(dt), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(h->hctrl.data() + b, h->ctrl + b, sizeof(InstCtrl), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    h->dt = dt;
     return CF_OK;
 }
 
-static int ensure_ctrl(cf_handle *h, int nsteps) {
+static int ensure_hist(cf_handle *h, int nsteps) {
     if (nsteps <= h->cap) return CF_OK;
-    int cap = std::max(nsteps, 2 * h->cap);
-    if (h->c.dt) {
-        HIPCHK(hipStreamSynchronize(h->stream));
-        dfree(h->c.dt); dfree(h->c.tc); dfree(h->c.ran);
-    }
-    HIPCHK(hipMalloc(&h->c.dt, sizeof(double) * ((size_t)cap + 1)));
-    HIPCHK(hipMalloc(&h->c.tc, sizeof(double) * ((size_t)cap + 1)));
-    HIPCHK(hipMalloc(&h->c.ran, sizeof(int) * ((size_t)cap + 1)));
+    const int cap = std::max(nsteps, 2 * h->cap);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    dfree(h->dt_hist); dfree(h->tc_hist); dfree(h->ran);
+    drop_graphs(h);
+    const size_t n = (size_t)h->ninst * (size_t)(cap + 1);
+    if (!dalloc(h->dt_hist, n) || !dalloc(h->tc_hist, n) || !dalloc(h->ran, n)) return CF_ERR_NOMEM;
     h->cap = cap;
     return CF_OK;
 }
@@ -752,109 +712,174 @@ static int validate_params(const cf_params *p) {
 }
 
 static void destroy_buffers(cf_handle *h) {
+    drop_graphs(h);
     dfree(h->st[0]); dfree(h->st[1]); dfree(h->nbr); dfree(h->geo); dfree(h->dfc);
-    dfree(h->manning); dfree(h->hydro); dfree(h->zm); dfree(h->stage);
-    dfree(h->c.hdr); dfree(h->c.partial); dfree(h->c.dt); dfree(h->c.tc); dfree(h->c.ran);
-    dfree(h->dt0); dfree(h->perm); dfree(h->stamp); dfree(h->tile_off); dfree(h->tile_adj);
-    dfree(h->always);
+    dfree(h->manning); dfree(h->hydro); dfree(h->zm); dfree(h->stage); dfree(h->perm);
+    dfree(h->d_kp); dfree(h->ctrl); dfree(h->partial); dfree(h->stamp);
+    dfree(h->tile_adj_off); dfree(h->tile_adj); dfree(h->always);
+    dfree(h->tile_inst); dfree(h->inst_end); dfree(h->inst_tile);
+    dfree(h->dt_hist); dfree(h->tc_hist); dfree(h->ran);
 }
 
-template <typename T>
-static bool dalloc(T *&p, size_t count) {
-    return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T)) == hipSuccess;
-}
-
-template <typename T>
-static bool h2d(T *dst, const T *src, size_t count, hipStream_t s) {
-    return count == 0 || hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, s) == hipSuccess;
-}
-
-extern "C" int cf_create(const cf_mesh *mesh, const cf_params *params, cf_handle **out) {
-    if (!mesh || !params || !out || mesh->nelt <= 0 || mesh->nnt <= 0 || !mesh->conn ||
-        !mesh->coords || !mesh->boundary)
-        return CF_ERR_ARG;
-    int st = validate_params(params);
-    if (st) return st;
+extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params *params, cf_handle **out) {
+    if (ninst <= 0 || !meshes || !params || !out) return CF_ERR_ARG;
     *out = nullptr;
-    const int nelt = mesh->nelt;
-    const size_t N = (size_t)nelt;
-    if (N > (size_t)INT32_MAX / 4) return CF_ERR_ARG;
-    for (size_t i = 0; i < 3 * N; i++)
-        if (mesh->conn[i] < 1 || mesh->conn[i] > mesh->nnt) return CF_ERR_MESH;
-
-    cfprep::Prep prep;
-    st = cfprep::prepare(mesh, params, kBlock, (params->options & CF_OPT_KEEP_ORDER) != 0, prep);
-    if (st) return st;
-    std::vector<int4> nbr(N);
-    for (size_t i = 0; i < N; i++)
-        nbr[i] = make_int4(prep.nbr[i].c[0], prep.nbr[i].c[1], prep.nbr[i].c[2], prep.nbr[i].w);
-    std::vector<double> mann(N, 0.0);
-    if (mesh->manning)
-        for (size_t i = 0; i < N; i++) mann[i] = mesh->manning[prep.perm[i]];
-    const int ntiles = (int)prep.tile_always.size();
-
+    for (int b = 0; b < ninst; b++) {
+        const cf_mesh *m = meshes + b;
+        if (m->nelt <= 0 || m->nnt <= 0 || !m->conn || !m->coords || !m->boundary) return CF_ERR_ARG;
+        if ((size_t)m->nelt > (size_t)INT32_MAX / 8) return CF_ERR_ARG;
+        int st = validate_params(params + b);
+        if (st) return st;
+        // one kernel serves the whole batch: same scheme and friction mode
+        if (params[b].iflux != params[0].iflux || params[b].friction != params[0].friction ||
+            params[b].fricimplic != params[0].fricimplic || params[b].options != params[0].options)
+            return CF_ERR_ARG;
+        for (size_t i = 0; i < 3 * (size_t)m->nelt; i++)
+            if (m->conn[i] < 1 || m->conn[i] > m->nnt) return CF_ERR_MESH;
+    }
+    std::vector<cfprep::Prep> prep(ninst);
+    for (int b = 0; b < ninst; b++) {
+        int st = cfprep::prepare(meshes + b, params + b, kTile, (params[b].options & CF_OPT_KEEP_ORDER) != 0, prep[b]);
+        if (st) return st;
+    }
     cf_handle *h = new (std::nothrow) cf_handle();
     if (!h) return CF_ERR_NOMEM;
-    h->nelt = nelt;
-    h->nnt = mesh->nnt;
-    h->prm = *params;
-    h->fric = params->friction == 0 ? 0 : 1 + params->fricimplic;
-    h->nblocks = grid_of(nelt, kBlock);
-    h->hperm = prep.perm;
+    h->ninst = ninst;
+    h->iflux = params[0].iflux;
+    h->fric = params[0].friction == 0 ? 0 : 1 + params[0].fricimplic;
+    h->nelt.resize(ninst); h->nnt.resize(ninst); h->cell_off.resize(ninst); h->tile_off.resize(ninst + 1);
+    h->prm.assign(params, params + ninst);
+    h->kp.resize(ninst);
+    h->hperm.resize(ninst);
+    h->hctrl.assign(ninst, InstCtrl{});
+    h->last_run.assign(ninst, 0);
+    h->last_total.assign(ninst, 0);
     // skipping dry tiles is exact except with a constant dt (every cell then
     // contributes constant_dt to the minimum)
-    h->skip = (params->options & CF_OPT_NO_TILE_SKIP) || params->is_dt_constant == 1 ? 0 : 1;
-    fill_kparams(h);
+    h->skip = (params[0].options & CF_OPT_NO_TILE_SKIP) ? 0 : 1;
+    for (int b = 0; b < ninst; b++)
+        if (params[b].is_dt_constant == 1) h->skip = 0;
+    int cells = 0, tiles = 0;
+    for (int b = 0; b < ninst; b++) {
+        h->nelt[b] = meshes[b].nelt;
+        h->nnt[b] = meshes[b].nnt;
+        h->cell_off[b] = cells;
+        h->tile_off[b] = tiles;
+        const int nt = grid_of(meshes[b].nelt, kTile);
+        tiles += nt;
+        cells += nt * kTile;
+        h->max_cells = std::max(h->max_cells, meshes[b].nelt);
+        h->kp[b] = make_kparams(params[b], meshes[b].nelt, b * kHydroPerInst);
+        h->hperm[b] = prep[b].perm;
+    }
+    h->tile_off[ninst] = tiles;
+    h->ncell = cells;
+    h->ntiles = tiles;
+    const size_t N = (size_t)cells;
+
+    // global (batched) host tables
+    std::vector<int4> nbr(N, make_int4(-3, -3, -3, 0));
+    std::vector<double> mann(N, 0.0);
+    std::vector<int> perm(N, 0), tile_inst(tiles), inst_end(ninst), inst_tile(ninst + 1), adj_off(tiles + 1, 0), adj;
+    std::vector<unsigned char> always(tiles, 0);
+    for (int b = 0; b < ninst; b++) {
+        const int c0 = h->cell_off[b], t0 = h->tile_off[b];
+        for (int i = 0; i < h->nelt[b]; i++) {
+            const cfprep::Nbr &r = prep[b].nbr[i];
+            int c[3];
+            for (int k = 0; k < 3; k++) c[k] = r.c[k] >= 0 ? r.c[k] + c0 : r.c[k];
+            nbr[c0 + i] = make_int4(c[0], c[1], c[2], r.w);
+            perm[c0 + i] = prep[b].perm[i];
+            if (meshes[b].manning) mann[c0 + i] = meshes[b].manning[prep[b].perm[i]];
+        }
+        inst_end[b] = c0 + h->nelt[b];
+        inst_tile[b] = t0;
+        const int nt = h->tile_off[b + 1] - t0;
+        for (int t = 0; t < nt; t++) {
+            tile_inst[t0 + t] = b;
+            always[t0 + t] = prep[b].tile_always[t];
+            for (int q = prep[b].tile_off[t]; q < prep[b].tile_off[t + 1]; q++) adj.push_back(prep[b].tile_adj[q] + t0);
+            adj_off[t0 + t + 1] = (int)adj.size();
+        }
+    }
+    inst_tile[ninst] = tiles;
+
     int rc = CF_OK;
-    int *dconn = nullptr;
-    double *dcoords = nullptr;
+    std::vector<int *> dconn(ninst, nullptr);
+    std::vector<double *> dcoords(ninst, nullptr);
     do {
         if (hipStreamCreate(&h->stream) != hipSuccess) { rc = CF_ERR_HIP; break; }
         h->own_stream = true;
-        if (!dalloc(h->st[0], N) || !dalloc(h->st[1], N) || !dalloc(h->nbr, N) ||
-            !dalloc(h->geo, G_COUNT * N) || !dalloc(h->dfc, N) || !dalloc(h->manning, N) ||
-            !dalloc(h->hydro, 2 * CF_MAX_IO * CF_MAX_HYDRO) || !dalloc(h->zm, N) ||
-            !dalloc(h->stage, 3 * N) || !dalloc(h->c.hdr, 4) || !dalloc(h->dt0, 1) ||
-            !dalloc(h->c.partial, (size_t)h->nblocks) || !dalloc(h->perm, N) ||
-            !dalloc(h->stamp, (size_t)ntiles) || !dalloc(h->tile_off, prep.tile_off.size()) ||
-            !dalloc(h->tile_adj, prep.tile_adj.size()) || !dalloc(h->always, (size_t)ntiles) ||
-            !dalloc(dconn, 3 * N) || !dalloc(dcoords, 3 * (size_t)mesh->nnt)) {
+        if (!dalloc(h->st[0], N) || !dalloc(h->st[1], N) || !dalloc(h->nbr, N) || !dalloc(h->geo, G_COUNT * N) ||
+            !dalloc(h->dfc, N) || !dalloc(h->manning, N) || !dalloc(h->hydro, (size_t)ninst * kHydroPerInst) ||
+            !dalloc(h->zm, N) || !dalloc(h->stage, 3 * (size_t)h->max_cells) || !dalloc(h->perm, N) ||
+            !dalloc(h->d_kp, (size_t)ninst) || !dalloc(h->ctrl, (size_t)ninst) || !dalloc(h->partial, (size_t)tiles) ||
+            !dalloc(h->stamp, (size_t)tiles) || !dalloc(h->tile_adj_off, adj_off.size()) ||
+            !dalloc(h->tile_adj, adj.size()) || !dalloc(h->always, (size_t)tiles) ||
+            !dalloc(h->tile_inst, (size_t)tiles) || !dalloc(h->inst_end, (size_t)ninst) ||
+            !dalloc(h->inst_tile, (size_t)ninst + 1)) {
             rc = CF_ERR_NOMEM;
             break;
         }
-        if (!h2d(dconn, prep.conn.data(), 3 * N, h->stream) ||
-            !h2d(dcoords, mesh->coords, 3 * (size_t)mesh->nnt, h->stream) ||
-            !h2d(h->nbr, nbr.data(), N, h->stream) || !h2d(h->manning, mann.data(), N, h->stream) ||
-            !h2d(h->perm, prep.perm.data(), N, h->stream) ||
-            !h2d(h->tile_off, prep.tile_off.data(), prep.tile_off.size(), h->stream) ||
-            !h2d(h->tile_adj, prep.tile_adj.data(), prep.tile_adj.size(), h->stream) ||
-            !h2d(h->always, prep.tile_always.data(), (size_t)ntiles, h->stream) ||
-            hipMemsetAsync(h->c.hdr, 0, 4 * sizeof(int), h->stream) != hipSuccess ||
-            hipMemsetAsync(h->stamp, 0, (size_t)ntiles * sizeof(unsigned), h->stream) != hipSuccess ||
-            hipMemsetAsync(h->stage, 0, 3 * N * sizeof(double), h->stream) != hipSuccess) {
-            rc = CF_ERR_HIP;
-            break;
+        for (int b = 0; b < ninst && rc == CF_OK; b++)
+            if (!dalloc(dconn[b], 3 * (size_t)h->nelt[b]) || !dalloc(dcoords[b], 3 * (size_t)h->nnt[b])) rc = CF_ERR_NOMEM;
+        if (rc) break;
+        bool ok = h2d(h->nbr, nbr.data(), N, h->stream) && h2d(h->manning, mann.data(), N, h->stream) &&
+                  h2d(h->perm, perm.data(), N, h->stream) && h2d(h->tile_adj_off, adj_off.data(), adj_off.size(), h->stream) &&
+                  h2d(h->tile_adj, adj.data(), adj.size(), h->stream) && h2d(h->always, always.data(), (size_t)tiles, h->stream) &&
+                  h2d(h->tile_inst, tile_inst.data(), (size_t)tiles, h->stream) &&
+                  h2d(h->inst_end, inst_end.data(), (size_t)ninst, h->stream) &&
+                  h2d(h->inst_tile, inst_tile.data(), (size_t)ninst + 1, h->stream) &&
+                  h2d(h->ctrl, h->hctrl.data(), (size_t)ninst, h->stream) &&
+                  hipMemsetAsync(h->stamp, 0, (size_t)tiles * sizeof(unsigned), h->stream) == hipSuccess &&
+                  hipMemsetAsync(h->stage, 0, 3 * (size_t)h->max_cells * sizeof(double), h->stream) == hipSuccess &&
+                  hipMemsetAsync(h->geo, 0, G_COUNT * N * sizeof(double), h->stream) == hipSuccess &&
+                  hipMemsetAsync(h->zm, 0, N * sizeof(double), h->stream) == hipSuccess &&
+                  hipMemsetAsync(h->dfc, 0, N * sizeof(double4), h->stream) == hipSuccess;
+        for (int b = 0; b < ninst && ok; b++)
+            ok = h2d(dconn[b], prep[b].conn.data(), 3 * (size_t)h->nelt[b], h->stream) &&
+                 h2d(dcoords[b], meshes[b].coords, 3 * (size_t)h->nnt[b], h->stream);
+        if (!ok) { rc = CF_ERR_HIP; break; }
+        for (int b = 0; b < ninst; b++) {
+            const int c0 = h->cell_off[b], n = h->nelt[b];
+            hipLaunchKernelGGL(geometry_kernel, dim3(grid_of(n, 256)), dim3(256), 0, h->stream, dconn[b], dcoords[b], n,
+                               c0, h->ncell, h->geo, h->zm);
         }
-        hipLaunchKernelGGL(geometry_kernel, dim3(grid_of(nelt, 256)), dim3(256), 0, h->stream, dconn, dcoords, nelt, h->geo, h->zm);
-        hipLaunchKernelGGL(face_dist_kernel, dim3(grid_of(nelt, 256)), dim3(256), 0, h->stream, h->nbr, h->geo, nelt, h->dfc);
-        // zero state with the bed level in the 4th component, in both buffers
-        hipLaunchKernelGGL(pack_state_kernel, dim3(grid_of(nelt, 256)), dim3(256), 0, h->stream, h->stage, h->zm, h->perm, nelt, h->st[0]);
-        hipLaunchKernelGGL(pack_state_kernel, dim3(grid_of(nelt, 256)), dim3(256), 0, h->stream, h->stage, h->zm, h->perm, nelt, h->st[1]);
+        for (int b = 0; b < ninst; b++) {
+            const int c0 = h->cell_off[b], n = h->nelt[b];
+            hipLaunchKernelGGL(face_dist_kernel, dim3(grid_of(n, 256)), dim3(256), 0, h->stream, h->nbr, h->geo, c0,
+                               c0 + n, h->ncell, h->dfc);
+            // zero state with the bed level in the 4th component, in both buffers
+            for (int k = 0; k < 2; k++)
+                hipLaunchKernelGGL(pack_state_kernel, dim3(grid_of(n, 256)), dim3(256), 0, h->stream, h->stage, h->zm,
+                                   h->perm, c0, n, h->st[k]);
+        }
         if (hipGetLastError() != hipSuccess || hipStreamSynchronize(h->stream) != hipSuccess) { rc = CF_ERR_HIP; break; }
-        if ((rc = upload_hydro(h)) != CF_OK) break;
-        if ((rc = ensure_ctrl(h, 1024)) != CF_OK) break;
-        if ((rc = compute_dt(h)) != CF_OK) break;
+        for (int b = 0; b < ninst && rc == CF_OK; b++) rc = upload_params(h, b);
+        if (rc) break;
+        if ((rc = ensure_hist(h, 1024)) != CF_OK) break;
+        for (int b = 0; b < ninst && rc == CF_OK; b++) rc = compute_dt(h, b);
     } while (0);
-    dfree(dconn);
-    dfree(dcoords);
+    for (int b = 0; b < ninst; b++) {
+        dfree(dconn[b]);
+        dfree(dcoords[b]);
+    }
     if (rc != CF_OK) {
         destroy_buffers(h);
         if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
         delete h;
         return rc;
     }
+    {
+        const char *env = getenv("CUTEFLOW_NO_GRAPHS");
+        h->use_graphs = env && atoi(env) ? 0 : 1;
+    }
     *out = h;
     return CF_OK;
+}
+
+extern "C" int cf_create(const cf_mesh *mesh, const cf_params *params, cf_handle **out) {
+    return cf_batch_create(1, mesh, params, out);
 }
 
 extern "C" void cf_destroy(cf_handle *h) {
@@ -865,61 +890,76 @@ extern "C" void cf_destroy(cf_handle *h) {
     delete h;
 }
 
-extern "C" int cf_set_state_device(cf_handle *h, const double *dU) {
-    if (!h || !dU || h->pending >= 0) return CF_ERR_ARG;
-    hipLaunchKernelGGL(pack_state_kernel, dim3(grid_of(h->nelt, 256)), dim3(256), 0, h->stream, dU, h->zm,
-                       h->perm, h->nelt, h->st[h->cur]);
-    // every tile runs the next two steps (both state buffers become consistent)
-    const int ntiles = h->nblocks;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_of(ntiles, 256)), dim3(256), 0, h->stream, h->stamp, ntiles, h->gstep);
+extern "C" int cf_batch_size(cf_handle *h) { return h ? h->ninst : 0; }
+
+static bool bad_inst(cf_handle *h, int b) { return !h || b < 0 || b >= h->ninst || h->pending >= 0; }
+
+static int current_buffer(cf_handle *h, int b) { return (h->hctrl[b].par + h->hctrl[b].steps) & 1; }
+
+extern "C" int cf_batch_set_state_device(cf_handle *h, int b, const double *dU) {
+    if (bad_inst(h, b) || !dU) return CF_ERR_ARG;
+    const int c0 = h->cell_off[b], n = h->nelt[b];
+    hipLaunchKernelGGL(pack_state_kernel, dim3(grid_of(n, 256)), dim3(256), 0, h->stream, dU, h->zm, h->perm, c0, n,
+                       h->st[current_buffer(h, b)]);
+    // every tile of the instance runs the next two steps (both state
+    // buffers become consistent)
+    const int t0 = h->tile_off[b], nt = h->tile_off[b + 1] - t0;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_of(nt, 256)), dim3(256), 0, h->stream, h->stamp + t0, nt,
+                       h->hctrl[b].gstep);
     HIPCHK(hipGetLastError());
-    h->tc = 0.0;
-    h->status = 0;
-    return compute_dt(h);
+    return compute_dt(h, b);
 }
 
-extern "C" int cf_set_state(cf_handle *h, const double *U) {
-    if (!h || !U || h->pending >= 0) return CF_ERR_ARG;
-    HIPCHK(hipMemcpyAsync(h->stage, U, 3 * (size_t)h->nelt * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    return cf_set_state_device(h, h->stage);
+extern "C" int cf_batch_set_state(cf_handle *h, int b, const double *U) {
+    if (bad_inst(h, b) || !U) return CF_ERR_ARG;
+    HIPCHK(hipMemcpyAsync(h->stage, U, 3 * (size_t)h->nelt[b] * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    return cf_batch_set_state_device(h, b, h->stage);
 }
 
-extern "C" int cf_get_state_device(cf_handle *h, double *dU) {
-    if (!h || !dU || h->pending >= 0) return CF_ERR_ARG;
-    hipLaunchKernelGGL(unpack_state_kernel, dim3(grid_of(h->nelt, 256)), dim3(256), 0, h->stream, h->st[h->cur],
-                       h->perm, h->nelt, dU);
+extern "C" int cf_batch_get_state_device(cf_handle *h, int b, double *dU) {
+    if (bad_inst(h, b) || !dU) return CF_ERR_ARG;
+    const int c0 = h->cell_off[b], n = h->nelt[b];
+    hipLaunchKernelGGL(unpack_state_kernel, dim3(grid_of(n, 256)), dim3(256), 0, h->stream, h->st[current_buffer(h, b)],
+                       h->perm, c0, n, dU);
     HIPCHK(hipGetLastError());
     return CF_OK;
 }
 
-extern "C" int cf_get_state(cf_handle *h, double *U) {
-    if (!h || !U) return CF_ERR_ARG;
-    int rc = cf_get_state_device(h, h->stage);
+extern "C" int cf_batch_get_state(cf_handle *h, int b, double *U) {
+    if (bad_inst(h, b) || !U) return CF_ERR_ARG;
+    int rc = cf_batch_get_state_device(h, b, h->stage);
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(U, h->stage, 3 * (size_t)h->nelt * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(U, h->stage, 3 * (size_t)h->nelt[b] * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    return CF_OK;
+}
+
+extern "C" int cf_set_state_device(cf_handle *h, const double *dU) { return cf_batch_set_state_device(h, 0, dU); }
+extern "C" int cf_set_state(cf_handle *h, const double *U) { return cf_batch_set_state(h, 0, U); }
+extern "C" int cf_get_state_device(cf_handle *h, double *dU) { return cf_batch_get_state_device(h, 0, dU); }
+extern "C" int cf_get_state(cf_handle *h, double *U) { return cf_batch_get_state(h, 0, U); }
+
+extern "C" int cf_batch_get_time(cf_handle *h, int b, double *tc, double *dt) {
+    if (!h || b < 0 || b >= h->ninst) return CF_ERR_ARG;
+    if (tc) *tc = h->hctrl[b].tc;
+    if (dt) *dt = h->hctrl[b].dt;
     return CF_OK;
 }
 
 extern "C" int cf_initial_dt(cf_handle *h, double *dt) {
     if (!h || !dt) return CF_ERR_ARG;
-    *dt = h->dt;
+    *dt = h->hctrl[0].dt;
     return CF_OK;
 }
 
-extern "C" int cf_get_time(cf_handle *h, double *tc, double *dt) {
-    if (!h) return CF_ERR_ARG;
-    if (tc) *tc = h->tc;
-    if (dt) *dt = h->dt;
-    return CF_OK;
-}
+extern "C" int cf_get_time(cf_handle *h, double *tc, double *dt) { return cf_batch_get_time(h, 0, tc, dt); }
 
 template <int IFLUX, int FRIC>
-static void launch_step(cf_handle *h, StepArgs &a, int grid) {
-    hipLaunchKernelGGL((step_kernel<IFLUX, FRIC>), dim3(grid), dim3(kBlock), 0, h->stream, a);
+static void launch_step(const StepArgs &a, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((step_kernel<IFLUX, FRIC>), dim3(grid), dim3(kTile), 0, s, a);
 }
 
-typedef void (*step_launcher)(cf_handle *, StepArgs &, int);
+typedef void (*step_launcher)(const StepArgs &, int, hipStream_t);
 
 static step_launcher pick_launcher(int iflux, int fric) {
     static const step_launcher tab[2][4] = {
@@ -928,80 +968,138 @@ static step_launcher pick_launcher(int iflux, int fric) {
     return tab[iflux == 3 ? 1 : 0][fric];
 }
 
-extern "C" int cf_step_async(cf_handle *h, int nsteps) {
-    if (!h || nsteps < 0) return CF_ERR_ARG;
-    if (h->pending >= 0) return CF_ERR_ARG;  // previous run not finished
-    if (h->status < 0) return CF_ERR_DT;
-    int rc = ensure_ctrl(h, std::max(nsteps, 1));
-    if (rc) return rc;
-    h->pending = nsteps;
-    if (nsteps == 0) return CF_OK;
-    hipLaunchKernelGGL(ctrl_init_kernel, dim3(1), dim3(1), 0, h->stream, h->c, h->dt, h->tc, h->kp.ts,
-                       h->kp.tol);
-    StepArgs a;
-    a.geo = h->geo;
+static StepArgs base_args(cf_handle *h) {
+    StepArgs a{};
+    a.st[0] = h->st[0];
+    a.st[1] = h->st[1];
     a.nbr = h->nbr;
+    a.geo = h->geo;
     a.dfc = h->dfc;
     a.manning = h->manning;
     a.hydro = h->hydro;
-    a.c = h->c;
-    a.nelt = h->nelt;
-    a.xcd_map = h->xcd_map;
-    a.reserved = 0;
+    a.kp = h->d_kp;
+    a.ctrl = h->ctrl;
+    a.dt_hist = h->dt_hist;
+    a.tc_hist = h->tc_hist;
+    a.ran = h->ran;
+    a.partial = h->partial;
     a.stamp = h->stamp;
-    a.tile_off = h->tile_off;
+    a.tile_adj_off = h->tile_adj_off;
     a.tile_adj = h->tile_adj;
     a.always = h->always;
-    a.gstep = h->gstep;
+    a.tile_inst = h->tile_inst;
+    a.inst_end = h->inst_end;
+    a.inst_tile = h->inst_tile;
+    a.in_x = nullptr;
+    a.out_x = nullptr;
+    a.ncell = h->ncell;
+    a.hist_stride = h->cap + 1;
     a.skip = h->skip;
-    a.p = h->kp;
-    step_launcher L = pick_launcher(h->prm.iflux, h->fric);
+    return a;
+}
+
+// The launches of a run of nsteps steps, on stream s.
+static void enqueue_run(cf_handle *h, int nsteps, hipStream_t s) {
+    hipLaunchKernelGGL(run_init_kernel, dim3(grid_of(h->ninst, 256)), dim3(256), 0, s, h->ctrl, h->d_kp, h->dt_hist,
+                       h->tc_hist, h->cap + 1, h->ninst);
+    StepArgs a = base_args(h);
+    step_launcher L = pick_launcher(h->iflux, h->fric);
     for (int k = 0; k < nsteps; k++) {
         a.step = k;
-        a.st_in = h->st[(h->cur + k) & 1];
-        a.st_out = h->st[(h->cur + k + 1) & 1];
-        L(h, a, h->nblocks);
-        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, h->stream, h->c, h->nblocks, k,
-                           h->kp.pad_dt, h->kp.ts, h->kp.tol);
+        L(a, h->ntiles, s);
+        hipLaunchKernelGGL(finalize_kernel, dim3(h->ninst), dim3(1024), 0, s, h->ctrl, h->d_kp, h->partial,
+                           h->inst_tile, h->dt_hist, h->tc_hist, h->ran, h->cap + 1, k);
     }
+}
+
+// A run's launches are captured once per step count and replayed: every
+// argument is fixed (per-run state lives in the device control block).
+static int run_graph(cf_handle *h, int nsteps) {
+    for (auto &g : h->graphs)
+        if (g.nsteps == nsteps) {
+            HIPCHK(hipGraphLaunch(g.exec, h->stream));
+            return CF_OK;
+        }
+    hipGraph_t graph;
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_run(h, nsteps, h->stream);
+    HIPCHK(hipStreamEndCapture(h->stream, &graph));
+    hipGraphExec_t exec;
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(e);
+    if (h->graphs.size() >= 8) {
+        (void)hipGraphExecDestroy(h->graphs.front().exec);
+        h->graphs.erase(h->graphs.begin());
+    }
+    h->graphs.push_back({nsteps, {}, exec});
+    HIPCHK(hipGraphLaunch(exec, h->stream));
+    return CF_OK;
+}
+
+extern "C" int cf_step_async(cf_handle *h, int nsteps) {
+    if (!h || nsteps < 0) return CF_ERR_ARG;
+    if (h->pending >= 0) return CF_ERR_ARG;  // previous run not finished
+    bool any = false;
+    for (int b = 0; b < h->ninst; b++) any = any || h->hctrl[b].status >= 0;
+    if (!any) return CF_ERR_DT;
+    int rc = ensure_hist(h, std::max(nsteps, 1));
+    if (rc) return rc;
+    h->pending = nsteps;
+    if (nsteps == 0) return CF_OK;
+    // the null stream cannot be captured
+    if (h->use_graphs && h->stream && nsteps <= 4096) return run_graph(h, nsteps);
+    enqueue_run(h, nsteps, h->stream);
     HIPCHK(hipGetLastError());
     return CF_OK;
 }
 
-extern "C" int cf_step_finish(cf_handle *h, double *dt_hist, int *steps_done) {
-    if (!h) return CF_ERR_ARG;
-    if (steps_done) *steps_done = 0;
-    if (h->pending < 0) return CF_ERR_ARG;
+extern "C" int cf_batch_step_finish(cf_handle *h, double *dt_hist, int *steps_done, int *status) {
+    if (!h || h->pending < 0) return CF_ERR_ARG;
     const int nsteps = h->pending;
     h->pending = -1;
+    const int B = h->ninst, S = h->cap + 1;
     if (nsteps == 0) {
-        if (dt_hist) dt_hist[0] = h->dt;
+        for (int b = 0; b < B; b++) {
+            if (dt_hist) dt_hist[(size_t)b * (nsteps + 1)] = h->hctrl[b].dt;
+            if (steps_done) steps_done[b] = 0;
+            if (status) status[b] = h->hctrl[b].status < 0 ? CF_ERR_DT : CF_OK;
+        }
         return CF_OK;
     }
-    int hdr[4];
-    HIPCHK(hipMemcpyAsync(hdr, h->c.hdr, sizeof(hdr), hipMemcpyDeviceToHost, h->stream));
+    std::vector<InstCtrl> c(B);
+    HIPCHK(hipMemcpyAsync(c.data(), h->ctrl, B * sizeof(InstCtrl), hipMemcpyDeviceToHost, h->stream));
+    std::vector<double> dts((size_t)B * S);
+    std::vector<int> ran((size_t)B * S);
+    HIPCHK(hipMemcpyAsync(dts.data(), h->dt_hist, dts.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(ran.data(), h->ran, ran.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    const int done = hdr[1];
-    std::vector<double> dts((size_t)done + 1), tcs(1);
-    std::vector<int> ran((size_t)done + 1, 0);
-    HIPCHK(hipMemcpyAsync(dts.data(), h->c.dt, sizeof(double) * ((size_t)done + 1), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(tcs.data(), h->c.tc + done, sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    if (done > 0) HIPCHK(hipMemcpyAsync(ran.data(), h->c.ran, sizeof(int) * (size_t)done, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
-    h->last_tiles_run = 0;
-    for (int k = 0; k < done; k++) h->last_tiles_run += ran[k];
-    h->last_tiles_total = (long long)done * h->nblocks;
-    if (dt_hist) memcpy(dt_hist, dts.data(), sizeof(double) * ((size_t)done + 1));
-    h->cur = (h->cur + done) & 1;
-    h->gstep += (unsigned)done;
-    h->dt = dts[done];
-    h->tc = tcs[0];
-    if (steps_done) *steps_done = done;
-    if (hdr[0] < 0) {
-        h->status = -1;
-        return CF_ERR_DT;
+    int rc = CF_OK;
+    for (int b = 0; b < B; b++) {
+        const int done = c[b].steps;
+        if (dt_hist) memcpy(dt_hist + (size_t)b * (nsteps + 1), dts.data() + (size_t)b * S, sizeof(double) * ((size_t)done + 1));
+        if (steps_done) steps_done[b] = done;
+        if (status) status[b] = c[b].status < 0 ? CF_ERR_DT : CF_OK;
+        if (c[b].status < 0) rc = CF_ERR_DT;
+        long long run = 0;
+        for (int k = 0; k < done; k++) run += ran[(size_t)b * S + k];
+        h->last_run[b] = run;
+        h->last_total[b] = (long long)done * (h->tile_off[b + 1] - h->tile_off[b]);
     }
-    return CF_OK;
+    h->hctrl = c;
+    return rc;
+}
+
+extern "C" int cf_step_finish(cf_handle *h, double *dt_hist, int *steps_done) {
+    if (!h || h->pending < 0) return CF_ERR_ARG;
+    const int nsteps = h->pending;
+    std::vector<double> hist((size_t)h->ninst * (nsteps + 1));
+    std::vector<int> done(h->ninst), st(h->ninst);
+    int rc = cf_batch_step_finish(h, hist.data(), done.data(), st.data());
+    if (dt_hist) memcpy(dt_hist, hist.data(), sizeof(double) * ((size_t)done[0] + 1));
+    if (steps_done) *steps_done = done[0];
+    if (rc == CF_ERR_HIP) return rc;
+    return st[0];
 }
 
 extern "C" int cf_step(cf_handle *h, int nsteps, double *dt_hist, int *steps_done) {
@@ -1011,48 +1109,75 @@ extern "C" int cf_step(cf_handle *h, int nsteps, double *dt_hist, int *steps_don
     return cf_step_finish(h, dt_hist, steps_done);
 }
 
-extern "C" int cf_set_inflow(cf_handle *h, const double *debitglob, int n) {
-    if (!h || !debitglob || n < 0 || n > CF_MAX_IO) return CF_ERR_ARG;
-    for (int i = 0; i < n; i++) h->prm.debitglob[i] = debitglob[i];
-    fill_kparams(h);
+extern "C" int cf_batch_step(cf_handle *h, int nsteps, double *dt_hist, int *steps_done, int *status) {
+    int rc = cf_step_async(h, nsteps);
+    if (rc) return rc;
+    return cf_batch_step_finish(h, dt_hist, steps_done, status);
+}
+
+extern "C" int cf_last_run_stats(cf_handle *h, long long *tiles_run, long long *tiles_total, int *tile_cells) {
+    if (!h) return CF_ERR_ARG;
+    long long r = 0, t = 0;
+    for (int b = 0; b < h->ninst; b++) {
+        r += h->last_run[b];
+        t += h->last_total[b];
+    }
+    if (tiles_run) *tiles_run = r;
+    if (tiles_total) *tiles_total = t;
+    if (tile_cells) *tile_cells = kTile;
     return CF_OK;
 }
 
-extern "C" int cf_set_hydrograph(cf_handle *h, int inlet, int npts, const double *t, const double *q) {
-    if (!h || inlet < 1 || inlet > CF_MAX_IO || npts < 0 || npts > CF_MAX_HYDRO || (npts > 0 && (!t || !q)))
+extern "C" int cf_batch_set_inflow(cf_handle *h, int b, const double *debitglob, int n) {
+    if (bad_inst(h, b) || !debitglob || n < 0 || n > CF_MAX_IO) return CF_ERR_ARG;
+    for (int i = 0; i < n; i++) h->prm[b].debitglob[i] = debitglob[i];
+    h->kp[b] = make_kparams(h->prm[b], h->nelt[b], b * kHydroPerInst);
+    return upload_params(h, b);
+}
+
+extern "C" int cf_set_inflow(cf_handle *h, const double *debitglob, int n) { return cf_batch_set_inflow(h, 0, debitglob, n); }
+
+extern "C" int cf_batch_set_hydrograph(cf_handle *h, int b, int inlet, int npts, const double *t, const double *q) {
+    if (bad_inst(h, b) || inlet < 1 || inlet > CF_MAX_IO || npts < 0 || npts > CF_MAX_HYDRO || (npts > 0 && (!t || !q)))
         return CF_ERR_ARG;
     for (int i = 0; i + 1 < npts; i++)
         if (!(t[i + 1] > t[i])) return CF_ERR_ARG;
-    h->prm.hydro_n[inlet - 1] = npts;
+    h->prm[b].hydro_n[inlet - 1] = npts;
     for (int i = 0; i < npts; i++) {
-        h->prm.hydro_t[(inlet - 1) * CF_MAX_HYDRO + i] = t[i];
-        h->prm.hydro_q[(inlet - 1) * CF_MAX_HYDRO + i] = q[i];
+        h->prm[b].hydro_t[(inlet - 1) * CF_MAX_HYDRO + i] = t[i];
+        h->prm[b].hydro_q[(inlet - 1) * CF_MAX_HYDRO + i] = q[i];
     }
-    fill_kparams(h);
-    return upload_hydro(h);
+    h->kp[b] = make_kparams(h->prm[b], h->nelt[b], b * kHydroPerInst);
+    return upload_params(h, b);
 }
 
-extern "C" int cf_set_manning(cf_handle *h, const double *n) {
-    if (!h || !n || h->pending >= 0) return CF_ERR_ARG;
-    const size_t N = (size_t)h->nelt;
+extern "C" int cf_set_hydrograph(cf_handle *h, int inlet, int npts, const double *t, const double *q) {
+    return cf_batch_set_hydrograph(h, 0, inlet, npts, t, q);
+}
+
+extern "C" int cf_batch_set_manning(cf_handle *h, int b, const double *n) {
+    if (bad_inst(h, b) || !n) return CF_ERR_ARG;
+    const size_t N = (size_t)h->nelt[b];
     std::vector<double> m(N);
-    for (size_t i = 0; i < N; i++) m[i] = n[h->hperm[i]];
-    HIPCHK(hipMemcpyAsync(h->manning, m.data(), N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    for (size_t i = 0; i < N; i++) m[i] = n[h->hperm[b][i]];
+    HIPCHK(hipMemcpyAsync(h->manning + h->cell_off[b], m.data(), N * sizeof(double), hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     return CF_OK;
 }
 
-extern "C" int cf_get_geometry(cf_handle *h, double *surf, double *zm, double *cotemin_arr, double *cotemin) {
-    if (!h || h->pending >= 0) return CF_ERR_ARG;
-    const size_t N = (size_t)h->nelt;
+extern "C" int cf_set_manning(cf_handle *h, const double *n) { return cf_batch_set_manning(h, 0, n); }
+
+extern "C" int cf_batch_get_geometry(cf_handle *h, int b, double *surf, double *zm, double *cotemin_arr, double *cotemin) {
+    if (bad_inst(h, b)) return CF_ERR_ARG;
+    const size_t N = (size_t)h->nelt[b], c0 = (size_t)h->cell_off[b], S = (size_t)h->ncell;
     std::vector<double> s(N), z(N), cot(N);
-    HIPCHK(hipMemcpyAsync(s.data(), h->geo + G_SURF * N, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(z.data(), h->zm, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(cot.data(), h->geo + G_COT * N, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(s.data(), h->geo + G_SURF * S + c0, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(z.data(), h->zm + c0, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(cot.data(), h->geo + G_COT * S + c0, N * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     double m = 1000000.0;
     for (size_t i = 0; i < N; i++) {
-        const size_t o = (size_t)h->hperm[i];
+        const size_t o = (size_t)h->hperm[b][i];
         if (surf) surf[o] = s[i];
         if (zm) zm[o] = z[i];
         if (cotemin_arr) cotemin_arr[o] = cot[i];
@@ -1062,18 +1187,14 @@ extern "C" int cf_get_geometry(cf_handle *h, double *surf, double *zm, double *c
     return CF_OK;
 }
 
-extern "C" int cf_last_run_stats(cf_handle *h, long long *tiles_run, long long *tiles_total, int *tile_cells) {
-    if (!h) return CF_ERR_ARG;
-    if (tiles_run) *tiles_run = h->last_tiles_run;
-    if (tiles_total) *tiles_total = h->last_tiles_total;
-    if (tile_cells) *tile_cells = kBlock;
-    return CF_OK;
+extern "C" int cf_get_geometry(cf_handle *h, double *surf, double *zm, double *cotemin_arr, double *cotemin) {
+    return cf_batch_get_geometry(h, 0, surf, zm, cotemin_arr, cotemin);
 }
 
 extern "C" void *cf_get_stream(cf_handle *h) { return h ? (void *)h->stream : nullptr; }
 
 extern "C" int cf_set_stream(cf_handle *h, void *stream) {
-    if (!h) return CF_ERR_ARG;
+    if (!h || h->pending >= 0) return CF_ERR_ARG;
     HIPCHK(hipStreamSynchronize(h->stream));
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
     h->stream = (hipStream_t)stream;

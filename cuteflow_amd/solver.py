@@ -136,6 +136,134 @@ class Solver:
         self.close()
 
 
+class BatchSolver:
+    """Many independent subdomains advanced together (cf_batch_* entry
+    points): one launch per step covers every instance; each instance keeps
+    its own parameters, state, dt, time and status.  All instances share the
+    scheme (iflux), friction mode and options."""
+
+    def __init__(self, meshes, params, ts=None, options: int = 0):
+        lib = _lib.load()
+        self.lib = lib
+        if isinstance(params, Params):
+            params = [params] * len(meshes)
+        if len(params) != len(meshes) or not meshes:
+            raise ValueError("one Params per mesh")
+        self.meshes = list(meshes)
+        self.params = list(params)
+        n = len(meshes)
+        cms = (_lib.CfMesh * n)()
+        cps = (_lib.CfParams * n)()
+        keep = []
+        self.boundaries = []
+        for b, (m, p) in enumerate(zip(meshes, params)):
+            bt = m.boundary_table()
+            le = m.inlet_lengths(bt, p.nombre_entrees)
+            conn = np.ascontiguousarray(m.conn.reshape(-1), np.int32)
+            coords = np.ascontiguousarray(m.coords.reshape(-1), np.float64)
+            mann = np.ascontiguousarray(m.manning, np.float64)
+            keep += [bt, conn, coords, mann]
+            self.boundaries.append(bt)
+            cms[b].nelt = m.nelt
+            cms[b].nnt = m.nnt
+            cms[b].conn = _lib.iptr(conn)
+            cms[b].coords = _lib.dptr(coords)
+            cms[b].boundary = _lib.iptr(bt)
+            cms[b].manning = _lib.dptr(mann)
+            cps[b] = p.to_cf(le, ts=ts)
+            cps[b].options = int(options)
+        h = C.c_void_p()
+        _lib.check(lib.cf_batch_create(n, cms, cps, C.byref(h)), "cf_batch_create")
+        self.h = h
+        self.ninst = n
+        self.nelt = [m.nelt for m in meshes]
+
+    def set_state(self, b: int, U: np.ndarray) -> None:
+        U = np.ascontiguousarray(U, np.float64)
+        assert U.shape == (self.nelt[b], 3)
+        _lib.check(self.lib.cf_batch_set_state(self.h, b, _lib.dptr(U)), "cf_batch_set_state")
+
+    def get_state(self, b: int) -> np.ndarray:
+        U = np.zeros((self.nelt[b], 3))
+        _lib.check(self.lib.cf_batch_get_state(self.h, b, _lib.dptr(U)), "cf_batch_get_state")
+        return U
+
+    def set_state_device(self, b: int, ptr: int) -> None:
+        _lib.check(self.lib.cf_batch_set_state_device(self.h, b, C.c_void_p(ptr)), "cf_batch_set_state_device")
+
+    def get_state_device(self, b: int, ptr: int) -> None:
+        _lib.check(self.lib.cf_batch_get_state_device(self.h, b, C.c_void_p(ptr)), "cf_batch_get_state_device")
+
+    def time(self, b: int):
+        """(tc, dt) of instance b."""
+        t, d = C.c_double(), C.c_double()
+        _lib.check(self.lib.cf_batch_get_time(self.h, b, C.byref(t), C.byref(d)), "cf_batch_get_time")
+        return t.value, d.value
+
+    def set_manning(self, b: int, n: np.ndarray) -> None:
+        n = np.ascontiguousarray(n, np.float64)
+        assert n.shape == (self.nelt[b],)
+        _lib.check(self.lib.cf_batch_set_manning(self.h, b, _lib.dptr(n)), "cf_batch_set_manning")
+
+    def set_hydrograph(self, b: int, inlet: int, t, q) -> None:
+        t = np.ascontiguousarray(t, np.float64)
+        q = np.ascontiguousarray(q, np.float64)
+        _lib.check(self.lib.cf_batch_set_hydrograph(self.h, b, inlet, len(t), _lib.dptr(t), _lib.dptr(q)),
+                   "cf_batch_set_hydrograph")
+
+    def set_inflow(self, b: int, q) -> None:
+        q = np.ascontiguousarray(q, np.float64)
+        _lib.check(self.lib.cf_batch_set_inflow(self.h, b, _lib.dptr(q), len(q)), "cf_batch_set_inflow")
+
+    def geometry(self, b: int):
+        n = self.nelt[b]
+        surf, zm, cot = np.zeros(n), np.zeros(n), np.zeros(n)
+        cmin = C.c_double()
+        _lib.check(self.lib.cf_batch_get_geometry(self.h, b, _lib.dptr(surf), _lib.dptr(zm), _lib.dptr(cot),
+                                                  C.byref(cmin)), "cf_batch_get_geometry")
+        return {"surf": surf, "zm": zm, "cotemin_arr": cot, "cotemin": cmin.value}
+
+    def step(self, nsteps: int, raise_on_error: bool = False):
+        """Run up to nsteps steps of every running instance.  Returns
+        (dt_hist list per instance [:done+1], done array, status array)."""
+        B = self.ninst
+        hist = np.zeros((B, nsteps + 1))
+        done = np.zeros(B, np.int32)
+        status = np.zeros(B, np.int32)
+        st = self.lib.cf_batch_step(self.h, int(nsteps), _lib.dptr(hist), _lib.iptr(done), _lib.iptr(status))
+        if st != _lib.CF_OK and (raise_on_error or st != _lib.CF_ERR_DT):
+            raise _lib.CuteflowError(st, "cf_batch_step")
+        return [hist[b, : done[b] + 1] for b in range(B)], done, status
+
+    def last_run_stats(self) -> dict:
+        run, tot, cells = C.c_longlong(), C.c_longlong(), C.c_int()
+        _lib.check(self.lib.cf_last_run_stats(self.h, C.byref(run), C.byref(tot), C.byref(cells)),
+                   "cf_last_run_stats")
+        return {"tiles_run": run.value, "tiles_total": tot.value, "tile_cells": cells.value,
+                "active_fraction": run.value / tot.value if tot.value else 0.0}
+
+    @property
+    def stream(self) -> int:
+        return self.lib.cf_get_stream(self.h) or 0
+
+    def close(self) -> None:
+        if getattr(self, "h", None):
+            self.lib.cf_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
 def initial_state(params: Params, mesh: Mesh, zm: np.ndarray) -> np.ndarray:
     """set_initial_condition (initial_condition.cuf:4-80): dam line or plane
     initial condition from the centroid and the cell bed level zm."""

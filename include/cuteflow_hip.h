@@ -156,6 +156,38 @@ int cf_get_geometry(cf_handle *h, double *surf, double *zm, double *cotemin_arr,
 void *cf_get_stream(cf_handle *h);
 int cf_set_stream(cf_handle *h, void *stream);
 
+/* ---- Batches: many independent subdomains in one handle ----------------
+ *
+ * No reference counterpart: the reference runs one mesh per process
+ * (finite_volumes_method.cuf:15-257).  A batch handle holds ninst instances
+ * (meshes of any sizes, e.g. ensemble members or calibration candidates),
+ * each with its own parameters, state, dt, time and stop/error status; one
+ * launch per step advances all of them, and each instance's results are
+ * bit-identical to a single-instance handle on the same inputs.  All
+ * instances must share iflux, friction, fricimplic and options (one kernel
+ * variant serves the batch); other fields may differ.  The single-instance
+ * functions above act on instance 0.  Instance indices are 0-based. */
+int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params *params, cf_handle **out);
+int cf_batch_size(cf_handle *h);
+int cf_batch_set_state(cf_handle *h, int inst, const double *U);
+int cf_batch_get_state(cf_handle *h, int inst, double *U);
+int cf_batch_set_state_device(cf_handle *h, int inst, const double *dU);
+int cf_batch_get_state_device(cf_handle *h, int inst, double *dU);
+int cf_batch_get_time(cf_handle *h, int inst, double *tc, double *dt);
+/* Runs up to nsteps steps of every instance that has not stopped; an
+ * instance stops at its own ts or dt failure while the others go on.
+ * dt_hist (ninst x (nsteps+1), row per instance, may be NULL) as for
+ * cf_step; steps_done and status (ninst each, may be NULL) per instance,
+ * status CF_OK or CF_ERR_DT.  Returns CF_ERR_DT if any instance failed.
+ * cf_step_async + cf_batch_step_finish is the split form. */
+int cf_batch_step(cf_handle *h, int nsteps, double *dt_hist, int *steps_done, int *status);
+int cf_batch_step_finish(cf_handle *h, double *dt_hist, int *steps_done, int *status);
+int cf_batch_set_inflow(cf_handle *h, int inst, const double *debitglob, int n);
+int cf_batch_set_hydrograph(cf_handle *h, int inst, int inlet, int npts, const double *t, const double *q);
+int cf_batch_set_manning(cf_handle *h, int inst, const double *n);
+int cf_batch_get_geometry(cf_handle *h, int inst, double *surf, double *zm, double *cotemin_arr,
+                          double *cotemin);
+
 /* Elementwise pow used by the friction kernel, evaluated on the device for
  * n values (x > 0): lets tests compare the device pow bit-for-bit. */
 int cf_debug_pow(int n, const double *x, double y, double *out);

@@ -1,0 +1,127 @@
+"""Batched instances (cf_batch_*): each instance of a batch must give the
+same bits as the oracle run of that instance alone, whatever the other
+instances do (different sizes, boundary conditions, end times, failures,
+state resets), and runs replayed from a captured graph must match runs
+launched one kernel at a time."""
+import os
+
+import numpy as np
+import pytest
+
+from cuteflow_amd.solver import BatchSolver, Solver
+from oracle import oracle as O
+
+from cases import dam_case, river_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(a, b):
+    a = np.ascontiguousarray(a, np.float64)
+    b = np.ascontiguousarray(b, np.float64)
+    return a.shape == b.shape and np.array_equal(a.view(np.int64), b.view(np.int64))
+
+
+def _hllc_batch():
+    """HLLC without friction; walls/transmissive dam breaks of three sizes,
+    one with an early end time, one all dry (fails after one step), and a
+    river reach with an inflow hydrograph and a fixed outlet level."""
+    cases = []
+    m, p, U = dam_case(16)
+    cases.append((m, p, U))
+    m, p, U = dam_case(10, seed=1)
+    p.ts = 0.5
+    cases.append((m, p, U))
+    m, p, U = dam_case(8, seed=2)
+    U[:] = 0.0
+    cases.append((m, p, U))
+    m, p, U = river_case(40, 8, iflux=1, friction=0)
+    cases.append((m, p, U))
+    return cases
+
+
+def test_batch_mixed_instances_match_oracle():
+    cases = _hllc_batch()
+    nsteps = 120
+    refs = [O.run(O.Prepared(m), p, U, nsteps) for m, p, U in cases]
+    with BatchSolver([c[0] for c in cases], [c[1] for c in cases]) as bs:
+        for b, (_, _, U) in enumerate(cases):
+            bs.set_state(b, U)
+        hist, done, status = bs.step(nsteps)
+        for b, ref in enumerate(refs):
+            print(f"instance {b}: nelt {cases[b][0].nelt} done {done[b]} status {status[b]}", flush=True)
+            assert done[b] == ref["steps"], (b, done[b], ref["steps"])
+            assert (status[b] == 0) == (ref["status"] == 0), (b, status[b], ref["status"])
+            assert _same(hist[b], ref["dt_hist"]), b
+            assert _same(bs.get_state(b), ref["U"]), b
+            assert bs.time(b)[0] == ref["tc"], b
+    assert refs[1]["steps"] < nsteps  # stopped at its own ts
+    assert refs[2]["steps"] == 1 and refs[2]["status"] != 0  # all dry: dt error
+
+
+def test_batch_waf_friction_uneven_sizes_and_manning():
+    cases = [river_case(40, 8, seed=3), river_case(24, 6, seed=4, q_peak=300.0),
+             river_case(60, 12, seed=5, ilimiteur=2)]
+    # per-cell Manning on the second instance
+    m1 = cases[1][0]
+    rng = np.random.default_rng(7)
+    n1 = 0.02 + 0.04 * rng.random(m1.nelt)
+    m1.manning = n1.copy()
+    nsteps = 150
+    refs = [O.run(O.Prepared(m), p, U, nsteps) for m, p, U in cases]
+    with BatchSolver([c[0] for c in cases], [c[1] for c in cases]) as bs:
+        for b, (_, _, U) in enumerate(cases):
+            bs.set_state(b, U)
+        hist, done, status = bs.step(nsteps)
+        for b, ref in enumerate(refs):
+            print(f"instance {b}: nelt {cases[b][0].nelt} done {done[b]}", flush=True)
+            assert done[b] == ref["steps"] == nsteps
+            assert _same(hist[b], ref["dt_hist"]), b
+            assert _same(bs.get_state(b), ref["U"]), b
+
+
+def test_batch_reset_one_instance_mid_run():
+    """Resetting instance 1 restarts its clock; instance 0 goes on."""
+    cases = [river_case(30, 6, seed=8), river_case(20, 6, seed=9)]
+    with BatchSolver([c[0] for c in cases], [c[1] for c in cases]) as bs:
+        for b, (_, _, U) in enumerate(cases):
+            bs.set_state(b, U)
+        bs.step(30)
+        bs.set_state(1, cases[1][2])
+        assert bs.time(1)[0] == 0.0
+        bs.step(25)
+        bs.step(15)
+        r0 = O.run(O.Prepared(cases[0][0]), cases[0][1], cases[0][2], 70)
+        r1 = O.run(O.Prepared(cases[1][0]), cases[1][1], cases[1][2], 40)
+        assert _same(bs.get_state(0), r0["U"])
+        assert _same(bs.get_state(1), r1["U"])
+        assert bs.time(0)[0] == r0["tc"] and bs.time(1)[0] == r1["tc"]
+
+
+def test_graph_replay_matches_direct_launches():
+    """Runs of a repeated length replay one captured graph; the same runs
+    launched kernel by kernel (CUTEFLOW_NO_GRAPHS=1) give the same bits, and
+    both equal one oracle run of the total length."""
+    m, p, U0 = river_case(40, 8, seed=11)
+    out = []
+    for env in ("0", "1"):
+        os.environ["CUTEFLOW_NO_GRAPHS"] = env
+        try:
+            with Solver(m, p) as s:
+                s.set_state(U0)
+                hists = [s.step(20)[0][:-1] for _ in range(5)]
+                out.append((np.concatenate(hists), s.get_state(), s.tc))
+        finally:
+            os.environ.pop("CUTEFLOW_NO_GRAPHS", None)
+    ref = O.run(O.Prepared(m), p, U0, 100)
+    for hist, U, tc in out:
+        assert _same(hist, ref["dt_hist"][:-1])
+        assert _same(U, ref["U"])
+        assert tc == ref["tc"]
+
+
+def test_batch_rejects_mixed_schemes():
+    m0, p0, _ = dam_case(8)
+    m1, p1, _ = river_case(20, 6)
+    with pytest.raises(RuntimeError):
+        BatchSolver([m0, m1], [p0, p1])
