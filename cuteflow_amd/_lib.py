@@ -93,7 +93,7 @@ SIGNATURES = {
     "cf_step_finish": (C.c_int, [_H, _DP, _IP]),
     "cf_set_inflow": (C.c_int, [_H, _DP, C.c_int]),
     "cf_set_hydrograph": (C.c_int, [_H, C.c_int, C.c_int, _DP, _DP]),
-    "cf_set_manning": (C.c_int, [_H, _DP]),
+    "cf_set_manning": (C.c_int, [_H, C.c_void_p]),
     "cf_get_geometry": (C.c_int, [_H, _DP, _DP, _DP, _DP]),
     "cf_last_run_stats": (C.c_int, [_H, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), _IP]),
     "cf_get_stream": (C.c_void_p, [_H]),
@@ -109,8 +109,18 @@ SIGNATURES = {
     "cf_batch_step_finish": (C.c_int, [_H, _DP, _IP, _IP]),
     "cf_batch_set_inflow": (C.c_int, [_H, C.c_int, _DP, C.c_int]),
     "cf_batch_set_hydrograph": (C.c_int, [_H, C.c_int, C.c_int, C.c_int, _DP, _DP]),
-    "cf_batch_set_manning": (C.c_int, [_H, C.c_int, _DP]),
+    "cf_batch_set_manning": (C.c_int, [_H, C.c_int, C.c_void_p]),
     "cf_batch_get_geometry": (C.c_int, [_H, C.c_int, _DP, _DP, _DP, _DP]),
+    "cf_adj_set_gauges": (C.c_int, [_H, C.c_int, C.c_int, _IP]),
+    "cf_adj_forward": (C.c_int, [_H, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "cf_adj_num_samples": (C.c_int, [_H]),
+    "cf_adj_get_samples": (C.c_int, [_H, C.c_int, _DP, C.c_void_p]),
+    "cf_adj_set_seed": (C.c_int, [_H, C.c_int, _DP, C.c_void_p, C.c_void_p]),
+    "cf_adj_backward": (C.c_int, [_H]),
+    "cf_adj_get_gradient": (C.c_int, [_H, C.c_int, C.c_void_p, C.c_void_p]),
+    "cf_tlm_set_direction": (C.c_int, [_H, C.c_int, C.c_void_p, C.c_void_p]),
+    "cf_tlm_run": (C.c_int, [_H]),
+    "cf_tlm_get": (C.c_int, [_H, C.c_int, _DP, _DP, C.c_void_p, C.c_void_p, C.c_void_p]),
     "cf_debug_pow": (C.c_int, [C.c_int, _DP, C.c_double, _DP]),
 }
 
@@ -133,6 +143,13 @@ def load(path: str | None = None):
     if not os.path.exists(path):
         raise RuntimeError(
             f"{path} is missing: build it first (python -c 'import __graft_entry__ as g; g.build()')")
+    # One HIP runtime per process: torch loads its own libamdhip64.so.7; when
+    # it is loaded first the library binds to it (same soname), otherwise
+    # torch later finds a foreign runtime under its soname and sees no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

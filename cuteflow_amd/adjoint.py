@@ -1,0 +1,222 @@
+"""Gradients of gauge water levels with respect to Manning n (discrete
+adjoint of the HIP time step) for the calibration workflow.
+
+The reference calibrates n zone by zone with repeated forward runs driven by
+scripts (Calibration_manning/); here the derivative of the discrete step is
+exact (wet/dry switches, wave-speed cases, limiter regions and dt frozen as
+taken in the forward), computed by a reverse sweep on the GPU with
+checkpoint/recompute, and exposed to PyTorch as an autograd Function:
+
+    run = AdjointRun(solver, gauges, sample_every=10, checkpoint_every=100)
+    n_cells = zone_map(zone_of_cell, 8) @ n_zones          # differentiable
+    eta = GaugeLevels.apply(n_cells, run, 2000, [U0])       # (nsamp, ngauges)
+    loss = ((eta - eta_obs) ** 2).sum(); loss.backward()    # -> n_zones.grad
+
+For a batch (BatchSolver) the Function takes the instances' per-cell n
+concatenated and returns the instances' gauge series concatenated.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+
+
+def _ptr(a):
+    """(pointer, keep-alive) for a per-cell array: numpy (host) or a
+    contiguous float64 CUDA tensor (device)."""
+    if a is None:
+        return None, None
+    if hasattr(a, "data_ptr"):
+        import torch
+
+        t = a.detach().to(torch.float64).contiguous()
+        if t.is_cuda:
+            torch.cuda.current_stream().synchronize()
+            return C.c_void_p(t.data_ptr()), t
+        a = t.numpy()
+    a = np.ascontiguousarray(a, np.float64)
+    return a.ctypes.data_as(C.c_void_p), a
+
+
+class AdjointRun:
+    """Recorded forward runs and reverse sweeps on a Solver or BatchSolver.
+
+    gauges: for a Solver one array of cell indices (caller's order), for a
+    BatchSolver one array per instance.
+    """
+
+    def __init__(self, solver, gauges, sample_every: int = 10, checkpoint_every: int = 100,
+                 want_hmax: bool = False):
+        self.solver = solver
+        self.lib = solver.lib
+        self.h = solver.h
+        self.batch = hasattr(solver, "ninst")
+        self.ninst = solver.ninst if self.batch else 1
+        self.nelt = list(solver.nelt) if self.batch else [solver.nelt]
+        if not self.batch:
+            gauges = [gauges]
+        if len(gauges) != self.ninst:
+            raise ValueError("one gauge list per instance")
+        self.gauges = [np.ascontiguousarray(g, np.int32) for g in gauges]
+        for b, g in enumerate(self.gauges):
+            _lib.check(self.lib.cf_adj_set_gauges(self.h, b, len(g), _lib.iptr(g)), "cf_adj_set_gauges")
+        self.sample_every = int(sample_every)
+        self.checkpoint_every = int(checkpoint_every)
+        self.want_hmax = bool(want_hmax)
+        self.nsteps = 0
+        self.nsamp = 0
+
+    # -- forward ---------------------------------------------------------------
+    def set_manning(self, n_list) -> None:
+        for b, n in enumerate(n_list):
+            p, keep = _ptr(n)
+            _lib.check(self.lib.cf_batch_set_manning(self.h, b, p), "cf_batch_set_manning")
+            del keep
+
+    def set_state(self, U_list) -> None:
+        for b, U in enumerate(U_list):
+            U = np.ascontiguousarray(U, np.float64)
+            _lib.check(self.lib.cf_batch_set_state(self.h, b, _lib.dptr(U)), "cf_batch_set_state")
+
+    def forward(self, nsteps: int, U0=None, manning=None):
+        """Recorded run of nsteps steps (from U0 when given, else from the
+        current state).  Returns (eta per instance (nsamp, ngauges), hmax per
+        instance or None)."""
+        if manning is not None:
+            self.set_manning(manning)
+        if U0 is not None:
+            self.set_state(U0)
+        _lib.check(self.lib.cf_adj_forward(self.h, int(nsteps), self.sample_every, self.checkpoint_every,
+                                           int(self.want_hmax)), "cf_adj_forward")
+        self.nsteps = int(nsteps)
+        self.nsamp = self.lib.cf_adj_num_samples(self.h)
+        etas, hmaxs = [], []
+        for b in range(self.ninst):
+            eta = np.zeros((self.nsamp, len(self.gauges[b])))
+            hm = np.zeros(self.nelt[b]) if self.want_hmax else None
+            _lib.check(self.lib.cf_adj_get_samples(self.h, b, _lib.dptr(eta),
+                                                   hm.ctypes.data_as(C.c_void_p) if hm is not None else None),
+                       "cf_adj_get_samples")
+            etas.append(eta)
+            hmaxs.append(hm)
+        return etas, (hmaxs if self.want_hmax else None)
+
+    # -- reverse ---------------------------------------------------------------
+    def backward(self, g_eta, g_hmax=None, g_state=None):
+        """Cotangents of the outputs (per instance; None = 0) -> gradients
+        (per instance) with respect to per-cell n and the initial state."""
+        for b in range(self.ninst):
+            ge = None if g_eta is None or g_eta[b] is None else np.ascontiguousarray(g_eta[b], np.float64)
+            gh, kh = _ptr(None if g_hmax is None else g_hmax[b])
+            gs, ks = _ptr(None if g_state is None else g_state[b])
+            _lib.check(self.lib.cf_adj_set_seed(self.h, b, _lib.dptr(ge) if ge is not None else None, gh, gs),
+                       "cf_adj_set_seed")
+            del kh, ks
+        _lib.check(self.lib.cf_adj_backward(self.h), "cf_adj_backward")
+        gns, gus = [], []
+        for b in range(self.ninst):
+            gn = np.zeros(self.nelt[b])
+            gu = np.zeros((self.nelt[b], 3))
+            _lib.check(self.lib.cf_adj_get_gradient(self.h, b, gn.ctypes.data_as(C.c_void_p),
+                                                    gu.ctypes.data_as(C.c_void_p)), "cf_adj_get_gradient")
+            gns.append(gn)
+            gus.append(gu)
+        return gns, gus
+
+    # -- tangent-linear --------------------------------------------------------
+    def tlm(self, dn=None, dU0=None):
+        """Tangent-linear run over the recorded run along (dn, dU0) (per
+        instance lists; None = 0).  Returns per instance a dict with eta and
+        state (the run with the recorded dt replayed at the current n) and
+        d_eta, d_hmax, d_state."""
+        for b in range(self.ninst):
+            pn, kn = _ptr(None if dn is None else dn[b])
+            pu, ku = _ptr(None if dU0 is None else dU0[b])
+            _lib.check(self.lib.cf_tlm_set_direction(self.h, b, pn, pu), "cf_tlm_set_direction")
+            del kn, ku
+        _lib.check(self.lib.cf_tlm_run(self.h), "cf_tlm_run")
+        out = []
+        for b in range(self.ninst):
+            eta = np.zeros((self.nsamp, len(self.gauges[b])))
+            d_eta = np.zeros((self.nsamp, len(self.gauges[b])))
+            d_hmax = np.zeros(self.nelt[b]) if self.want_hmax else None
+            d_state = np.zeros((self.nelt[b], 3))
+            state = np.zeros((self.nelt[b], 3))
+            _lib.check(self.lib.cf_tlm_get(self.h, b, _lib.dptr(eta), _lib.dptr(d_eta),
+                                           d_hmax.ctypes.data_as(C.c_void_p) if d_hmax is not None else None,
+                                           d_state.ctypes.data_as(C.c_void_p), state.ctypes.data_as(C.c_void_p)),
+                       "cf_tlm_get")
+            out.append({"eta": eta, "d_eta": d_eta, "d_hmax": d_hmax, "d_state": d_state, "state": state})
+        return out
+
+
+def _split(t, sizes):
+    out, o = [], 0
+    for s in sizes:
+        out.append(t[o:o + s])
+        o += s
+    return out
+
+
+class GaugeLevels:
+    """torch.autograd.Function: per-cell Manning n (instances concatenated)
+    -> gauge water levels h + zm (instances' (nsamp, ngauges) blocks
+    flattened and concatenated), and the per-cell maximum depth over the
+    samples when the run keeps it."""
+
+    @staticmethod
+    def apply(n_cells, run: AdjointRun, nsteps: int, U0):
+        return _fn().apply(n_cells, run, nsteps, U0)
+
+
+_FN = None
+
+
+def _fn():
+    global _FN
+    if _FN is None:
+        import torch
+
+        class _GaugeLevels(torch.autograd.Function):
+            @staticmethod
+            def forward(ctx, n_cells, run, nsteps, U0):
+                ns = _split(n_cells.detach(), run.nelt)
+                etas, hmaxs = run.forward(nsteps, U0=U0, manning=ns)
+                ctx.run = run
+                ctx.device = n_cells.device
+                eta = torch.as_tensor(np.concatenate([e.reshape(-1) for e in etas]), device=n_cells.device)
+                if run.want_hmax:
+                    hm = torch.as_tensor(np.concatenate(hmaxs), device=n_cells.device)
+                    return eta, hm
+                return eta
+
+            @staticmethod
+            def backward(ctx, g_eta, g_hmax=None):
+                run = ctx.run
+                sizes = [run.nsamp * len(g) for g in run.gauges]
+                ge = [g.detach().cpu().numpy().reshape(run.nsamp, -1) if g.numel() else None
+                      for g in _split(g_eta, sizes)]
+                gh = None
+                if run.want_hmax and g_hmax is not None:
+                    gh = [g.detach().cpu().numpy() for g in _split(g_hmax, run.nelt)]
+                gns, _ = run.backward(ge, gh)
+                g = torch.as_tensor(np.concatenate(gns), device=ctx.device)
+                return g, None, None, None
+
+        _FN = _GaugeLevels
+    return _FN
+
+
+def zone_map(zone_of_cell, nzones: int, device=None):
+    """Dense one-hot (ncells, nzones) float64 matrix: n_cells = Z @ n_zones,
+    whose gradient Z^T g sums the per-cell gradients of each zone with a
+    fixed reduction order."""
+    import torch
+
+    z = np.asarray(zone_of_cell, np.int64)
+    Z = np.zeros((len(z), nzones))
+    Z[np.arange(len(z)), z] = 1.0
+    return torch.as_tensor(Z, device=device)

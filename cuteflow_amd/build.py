@@ -32,7 +32,8 @@ HIP_FLAGS = [
     "-Wall",
 ]
 
-SOURCES = [os.path.join(CSRC, "cuteflow_hip.hip"), os.path.join(CSRC, "cf_prep.cpp")]
+SOURCES = [os.path.join(CSRC, "cuteflow_hip.hip"), os.path.join(CSRC, "cf_adjoint.hip"),
+           os.path.join(CSRC, "cf_prep.cpp")]
 
 
 def _run(cmd: list[str], cwd: str | None = None) -> None:

@@ -1,0 +1,900 @@
+// Discrete adjoint and tangent-linear model of the time step (no reference
+// counterpart: the reference's Manning calibration, Calibration_manning/,
+// estimates n from repeated forward runs).
+//
+// Recorded forward: a run of nsteps steps that keeps dt and tc of every step
+// (the reverse sweep freezes them), a checkpoint of the state every K steps,
+// the water level h + zm at gauge cells every `every` steps and optionally
+// the per-cell maximum depth over those samples with the sample it occurred
+// at.
+//
+// Reverse sweep, one segment of K steps at a time from the last: the
+// segment is replayed from its checkpoint (same step kernel, dt/tc from the
+// record, every tile) keeping each step's input state and residual x; then
+// for each step, last to first,
+//   rev_kernel     one thread per cell: cell_reverse() writes the cotangent
+//                  of the cell's own old state, of its three neighbours'
+//                  (R1) and of its WAF second-ring upwind cells (R2), and
+//                  adds the Manning cotangent to gn;
+//   gather_kernel  one thread per cell: sums the contributions addressed to
+//                  it in a fixed order (own, the neighbours' R1 slots that
+//                  face it, its R2 entries by a CSR list) and the seeds of
+//                  the step.
+// No atomics: every gradient is reproducible bit for bit.
+//
+// Tangent-linear model: the step body evaluated on dual numbers, replaying
+// the recorded dt/tc, carrying the derivative of the state along a
+// direction (dn, dU0); its values are the forward's bits.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "cf_dual.h"
+#include "cf_internal.h"
+
+#pragma clang fp contract(off)
+
+using namespace cf;
+
+namespace cf {
+
+struct AdjState {
+    // configuration of the recorded run
+    int nsteps = 0, every = 1, K = 1, nseg = 0, nsamp = 0, want_hmax = 0;
+    bool recorded = false;
+    bool tlm_ready = false;
+    int gtotal = 0;
+    std::vector<std::vector<int>> gcells;  // per instance: caller-local gauge cells
+    std::vector<int> goff;                 // per instance: first gauge (ninst + 1)
+    int *d_gcell = nullptr;                // internal cell of each gauge
+    int *d_ginst = nullptr;                // instance of each gauge
+    int *d_goff = nullptr;
+    double *eta = nullptr, *geta = nullptr, *deta = nullptr;  // nsamp x gtotal
+    double *teta = nullptr;   // gauge levels of the tangent-linear run's values
+    double *hmax = nullptr, *ghmax = nullptr, *dhmax = nullptr;
+    int *argmax = nullptr;
+    double *dt = nullptr, *tc = nullptr;  // ninst x (nsteps + 1)
+    int *ran = nullptr;
+    double4 *ckpt = nullptr;  // nseg x ncell
+    double4 *seg = nullptr;   // K x ncell: states k0+1 .. k0+K of a segment
+    double *xs = nullptr;     // K x 3 x ncell: residual x of each step of a segment
+    double4 *lam[2] = {nullptr, nullptr};
+    double4 *gstate = nullptr;  // final-state seed
+    double4 *R1 = nullptr;      // 4 x ncell: neighbours 0..2, own
+    double4 *R2 = nullptr;      // 6 x ncell: face k, slot s at 2k + s
+    int *r2_off = nullptr, *r2_idx = nullptr;
+    double *gn = nullptr;
+    int lam_cur = 0;
+    // tangent-linear run
+    double4 *tst[2] = {nullptr, nullptr}, *tdst[2] = {nullptr, nullptr};
+    double4 *tdir = nullptr;  // tangent of the initial state
+    double *dman = nullptr;   // tangent of Manning n
+    int tlm_cur = 0;
+    bool r2_built = false;
+    int alloc_steps = 0, alloc_K = 0, alloc_nsamp = 0, alloc_gtotal = 0;
+};
+
+}  // namespace cf
+
+namespace {
+
+struct RevArgs {
+    const double4 *st;
+    const int4 *nbr;
+    const double4 *dfc;
+    const double *geo;
+    const double *manning;
+    const double *hydro;
+    const KParams *kp;
+    const int *tile_inst;
+    const int *inst_end;
+    const double *xs;
+    const double4 *lam;
+    double4 *R1;
+    double4 *R2;
+    double *gn;
+    const double *dt;
+    const double *tc;
+    int hstride;
+    int step;
+    int ncell;
+};
+
+// The reverse body needs about twice the registers of the forward one; one
+// workgroup of 256 threads per CU slot keeps it from spilling.
+template <int IFLUX, int FRIC>
+__global__ __launch_bounds__(kTile) void rev_kernel(RevArgs r) {
+    const int tile = blockIdx.x;
+    const int b = r.tile_inst[tile];
+    const int i = tile * kTile + threadIdx.x;
+    if (i >= r.inst_end[b]) return;
+    const DevLd ld{r.st, r.nbr, r.dfc, r.geo, r.manning, r.ncell};
+    const size_t hix = (size_t)b * r.hstride + r.step;
+    const double x[3] = {r.xs[i], r.xs[(size_t)r.ncell + i], r.xs[2 * (size_t)r.ncell + i]};
+    const double4 l4 = r.lam[i];
+    const double lam[3] = {l4.x, l4.y, l4.z};
+    double own[3], r1[3][3], r2[3][2][3], gn;
+    cell_reverse<IFLUX, FRIC>(ld, r.kp[b], r.hydro, i, r.dt[hix], r.tc[hix], x, lam, own, r1, r2, gn);
+#pragma unroll
+    for (int k = 0; k < 3; k++) r.R1[4 * (size_t)i + k] = make_double4(r1[k][0], r1[k][1], r1[k][2], 0.0);
+    r.R1[4 * (size_t)i + 3] = make_double4(own[0], own[1], own[2], 0.0);
+    if (IFLUX == 3) {
+        const int mask = r.nbr[i].w >> kR2Shift;
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+#pragma unroll
+            for (int s = 0; s < 2; s++)
+                if (mask & (1 << (2 * k + s)))
+                    r.R2[6 * (size_t)i + 2 * k + s] = make_double4(r2[k][s][0], r2[k][s][1], r2[k][s][2], 0.0);
+    }
+    r.gn[i] += gn;
+}
+
+__device__ __forceinline__ void acc4(double4 &a, const double4 &b) {
+    a.x += b.x;
+    a.y += b.y;
+    a.z += b.z;
+}
+
+// Cotangent of each cell's old state: the sum of the contributions the
+// reverse kernel addressed to it, in a fixed order; plus the hmax seed of
+// sample `sidx` (when hseed) and the final-state seed (when gstate).
+template <bool R2ON>
+__global__ void gather_kernel(const int4 *__restrict__ nbr, const double4 *__restrict__ R1,
+                              const double4 *__restrict__ R2, const int *__restrict__ r2_off,
+                              const int *__restrict__ r2_idx, const int *__restrict__ tile_inst,
+                              const int *__restrict__ inst_end, const int *__restrict__ argmax,
+                              const double *__restrict__ ghmax, int sidx, double4 *__restrict__ lam_out) {
+    const int tile = blockIdx.x;
+    const int b = tile_inst[tile];
+    const int c = tile * kTile + threadIdx.x;
+    if (c >= inst_end[b]) return;
+    double4 acc = R1[4 * (size_t)c + 3];
+    const int4 nb = nbr[c];
+    const int nbv[3] = {nb.x, nb.y, nb.z};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int i = nbv[k];
+        if (i >= 0) acc4(acc, R1[4 * (size_t)i + ((nb.w >> (2 * k)) & 3)]);
+    }
+    if (R2ON)
+        for (int e = r2_off[c]; e < r2_off[c + 1]; e++) acc4(acc, R2[r2_idx[e]]);
+    if (argmax && argmax[c] == sidx) acc.x += ghmax[c];
+    acc.w = 0.0;
+    lam_out[c] = acc;
+}
+
+// lam = final-state seed + hmax seed of the last sample (cells not seeded: 0)
+__global__ void lam_init_kernel(const int *__restrict__ tile_inst, const int *__restrict__ inst_end,
+                                const double4 *__restrict__ gstate, const int *__restrict__ argmax,
+                                const double *__restrict__ ghmax, int sidx, double4 *__restrict__ lam) {
+    const int tile = blockIdx.x;
+    const int b = tile_inst[tile];
+    const int c = tile * kTile + threadIdx.x;
+    double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (c < inst_end[b]) {
+        v = gstate[c];
+        v.w = 0.0;
+        if (argmax && argmax[c] == sidx) v.x += ghmax[c];
+    }
+    lam[c] = v;
+}
+
+// Gauge seeds of sample sidx, one thread per instance (gauges of an
+// instance added in order).
+__global__ void gauge_seed_kernel(const int *__restrict__ gcell, const int *__restrict__ goff,
+                                  const double *__restrict__ geta, int gtotal, int sidx, int ninst,
+                                  double4 *__restrict__ lam) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ninst) return;
+    for (int g = goff[b]; g < goff[b + 1]; g++) lam[gcell[g]].x += geta[(size_t)sidx * gtotal + g];
+}
+
+// Current state buffer of the instance of `tile` during a run.
+__device__ __forceinline__ const double4 *cur_state(double4 *const *st, const InstCtrl *ctrl, int b) {
+    const InstCtrl &c = ctrl[b];
+    return st[(c.par + c.steps) & 1];
+}
+
+struct StPair {
+    double4 *s[2];
+};
+
+__global__ void sample_gauges_kernel(StPair st, const InstCtrl *ctrl, const int *__restrict__ gcell,
+                                     const int *__restrict__ ginst, int gtotal, int sidx, double *__restrict__ eta) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= gtotal) return;
+    const double4 u = cur_state(st.s, ctrl, ginst[g])[gcell[g]];
+    eta[(size_t)sidx * gtotal + g] = u.x + u.w;
+}
+
+__global__ void sample_hmax_kernel(StPair st, const InstCtrl *ctrl, const int *__restrict__ tile_inst,
+                                   const int *__restrict__ inst_end, int sidx, double *__restrict__ hmax,
+                                   int *__restrict__ argmax) {
+    const int tile = blockIdx.x;
+    const int b = tile_inst[tile];
+    const int c = tile * kTile + threadIdx.x;
+    if (c >= inst_end[b]) return;
+    const double h = cur_state(st.s, ctrl, b)[c].x;
+    if (sidx == 0 || h > hmax[c]) {
+        hmax[c] = h;
+        argmax[c] = sidx;
+    }
+}
+
+__global__ void ckpt_kernel(StPair st, const InstCtrl *ctrl, const int *__restrict__ tile_inst,
+                            const int *__restrict__ inst_end, double4 *__restrict__ dst) {
+    const int tile = blockIdx.x;
+    const int b = tile_inst[tile];
+    const int c = tile * kTile + threadIdx.x;
+    if (c >= inst_end[b]) return;
+    dst[c] = cur_state(st.s, ctrl, b)[c];
+}
+
+// ---- tangent-linear model ------------------------------------------------
+
+struct TlmLd {
+    const double4 *st_, *dst_;
+    const int4 *nbr_;
+    const double4 *dfc_;
+    const double *geo_, *man_, *dman_;
+    int stride;
+    __device__ __forceinline__ St<Dual> state(int c) const {
+        const double4 v = st_[c], d = dst_[c];
+        return {Dual(v.x, d.x), Dual(v.y, d.y), Dual(v.z, d.z), v.w};
+    }
+    __device__ __forceinline__ I4 nbr(int c) const {
+        const int4 v = nbr_[c];
+        return {v.x, v.y, v.z, v.w};
+    }
+    __device__ __forceinline__ D4 dfc(int c) const {
+        const double4 v = dfc_[c];
+        return {v.x, v.y, v.z, v.w};
+    }
+    __device__ __forceinline__ double geo(int comp, int c) const { return geo_[comp * (size_t)stride + c]; }
+    __device__ __forceinline__ Dual manning(int c) const { return Dual(man_[c], dman_[c]); }
+};
+
+struct TlmArgs {
+    const double4 *st_in, *dst_in;
+    double4 *st_out, *dst_out;
+    const int4 *nbr;
+    const double4 *dfc;
+    const double *geo;
+    const double *manning, *dman;
+    const double *hydro;
+    const KParams *kp;
+    const int *tile_inst, *inst_end;
+    const double *dt, *tc;
+    int hstride, step, ncell;
+};
+
+template <int IFLUX, int FRIC>
+__global__ __launch_bounds__(kTile) void tlm_kernel(TlmArgs a) {
+    const int tile = blockIdx.x;
+    const int b = a.tile_inst[tile];
+    const int i = tile * kTile + threadIdx.x;
+    if (i >= a.inst_end[b]) return;
+    const TlmLd ld{a.st_in, a.dst_in, a.nbr, a.dfc, a.geo, a.manning, a.dman, a.ncell};
+    const size_t hix = (size_t)b * a.hstride + a.step;
+    Dual o[3];
+    cell_update<Dual, IFLUX, FRIC>(ld, a.kp[b], a.hydro, i, a.dt[hix], a.tc[hix], o, (Dual *)nullptr);
+    const double zm = a.st_in[i].w;
+    a.st_out[i] = make_double4(o[0].v, o[1].v, o[2].v, zm);
+    a.dst_out[i] = make_double4(o[0].d, o[1].d, o[2].d, 0.0);
+}
+
+__global__ void tlm_sample_kernel(const double4 *__restrict__ st, const double4 *__restrict__ dst,
+                                  const int *__restrict__ gcell, int gtotal, int sidx, double *__restrict__ teta,
+                                  double *__restrict__ deta) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= gtotal) return;
+    const double4 u = st[gcell[g]];
+    teta[(size_t)sidx * gtotal + g] = u.x + u.w;
+    deta[(size_t)sidx * gtotal + g] = dst[gcell[g]].x;
+}
+
+__global__ void tlm_hmax_kernel(const double4 *__restrict__ dst, const int *__restrict__ tile_inst,
+                                const int *__restrict__ inst_end, const int *__restrict__ argmax, int sidx,
+                                double *__restrict__ dhmax) {
+    const int tile = blockIdx.x;
+    const int b = tile_inst[tile];
+    const int c = tile * kTile + threadIdx.x;
+    if (c >= inst_end[b]) return;
+    if (argmax[c] == sidx) dhmax[c] = dst[c].x;
+}
+
+// ---- small helpers -------------------------------------------------------
+
+__global__ void fill_d4_kernel(double4 *p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
+__global__ void fill_d_kernel(double *p, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+__global__ void copy_d4_kernel(const double4 *__restrict__ src, double4 *__restrict__ dst, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+// caller order (local index perm[c]) -> internal cell c, per-cell scalar
+__global__ void scalar_in_kernel(const double *__restrict__ src, const int *__restrict__ perm, int c0, int n,
+                                 double *__restrict__ dst) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) dst[c0 + l] = src[perm[c0 + l]];
+}
+
+__global__ void scalar_out_kernel(const double *__restrict__ src, const int *__restrict__ perm, int c0, int n,
+                                  double *__restrict__ dst) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) dst[perm[c0 + l]] = src[c0 + l];
+}
+
+// (h, hu, hv) per cell in caller order <-> double4 (zm slot left 0)
+__global__ void state3_in_kernel(const double *__restrict__ src, const int *__restrict__ perm, int c0, int n,
+                                 double4 *__restrict__ dst) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) {
+        const size_t o = 3 * (size_t)perm[c0 + l];
+        dst[c0 + l] = make_double4(src[o], src[o + 1], src[o + 2], 0.0);
+    }
+}
+
+__global__ void state3_out_kernel(const double4 *__restrict__ src, const int *__restrict__ perm, int c0, int n,
+                                  double *__restrict__ dst) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) {
+        const double4 v = src[c0 + l];
+        const size_t o = 3 * (size_t)perm[c0 + l];
+        dst[o] = v.x;
+        dst[o + 1] = v.y;
+        dst[o + 2] = v.z;
+    }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// host side
+
+static void fill0(double4 *p, size_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(fill_d4_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+}
+static void fill0(double *p, size_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(fill_d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+}
+
+void cf::adj_free(cf_handle *h) {
+    AdjState *A = h ? h->adj : nullptr;
+    if (!A) return;
+    dfree(A->d_gcell); dfree(A->d_ginst); dfree(A->d_goff);
+    dfree(A->eta); dfree(A->geta); dfree(A->deta); dfree(A->teta);
+    dfree(A->hmax); dfree(A->ghmax); dfree(A->dhmax); dfree(A->argmax);
+    dfree(A->dt); dfree(A->tc); dfree(A->ran);
+    dfree(A->ckpt); dfree(A->seg); dfree(A->xs);
+    dfree(A->lam[0]); dfree(A->lam[1]); dfree(A->gstate);
+    dfree(A->R1); dfree(A->R2); dfree(A->r2_off); dfree(A->r2_idx); dfree(A->gn);
+    dfree(A->tst[0]); dfree(A->tst[1]); dfree(A->tdst[0]); dfree(A->tdst[1]); dfree(A->tdir); dfree(A->dman);
+    delete A;
+    h->adj = nullptr;
+}
+
+static AdjState *adj_of(cf_handle *h) {
+    if (!h->adj) {
+        h->adj = new (std::nothrow) AdjState();
+        if (h->adj) {
+            h->adj->gcells.resize(h->ninst);
+            h->adj->goff.assign(h->ninst + 1, 0);
+        }
+    }
+    return h->adj;
+}
+
+// Second-ring adjoint targets of the WAF stencil: marks in nbr.w which
+// (face, slot) of each cell addresses a cell through R2 and builds the
+// inverse lists (target cell -> R2 entries), both from the topology alone
+// with the same rules as the forward's upwind choice.
+static int build_r2(cf_handle *h, AdjState *A) {
+    if (A->r2_built) return CF_OK;
+    const int N = h->ncell;
+    std::vector<int> tgt(6 * (size_t)N, -1);
+    std::vector<int4> &nb = h->hnbr;
+    for (int b = 0; b < h->ninst; b++) {
+        const int c0 = h->cell_off[b], c1 = c0 + h->nelt[b];
+        const int outlet = h->prm[b].outlet;
+        for (int i = c0; i < c1; i++) {
+            const int4 v = nb[i];
+            const int c[3] = {v.x, v.y, v.z};
+            int mask = 0;
+            for (int j = 0; j < 3; j++) {
+                int kv = -1;
+                if (c[j] >= 0) kv = c[j];
+                else if (((-c[j]) & 3) == 2 && outlet == 2) kv = i;
+                if (kv < 0) continue;
+                const int ja = (j + 1) % 3, jb = (j + 2) % 3;
+                const bool own1 = c[ja] < 0, own2 = c[jb] < 0;
+                const int tjv = (v.w >> (2 * j)) & 3;
+                const int4 vk = (kv == i) ? v : nb[kv];
+                const int ck[3] = {vk.x, vk.y, vk.z};
+                const int e1 = ck[(tjv + 1) % 3], e2 = ck[(tjv + 2) % 3];
+                const int kw1 = e1 >= 0 ? e1 : kv, kw2 = e2 >= 0 ? e2 : kv;
+                if ((own1 && own2) || (kw1 == kv && kw2 == kv)) continue;
+                if (kw1 != kv) {
+                    mask |= 1 << (2 * j);
+                    tgt[6 * (size_t)i + 2 * j] = kw1;
+                }
+                if (kw2 != kv) {
+                    mask |= 1 << (2 * j + 1);
+                    tgt[6 * (size_t)i + 2 * j + 1] = kw2;
+                }
+            }
+            nb[i].w = (v.w & ((1 << kR2Shift) - 1)) | (mask << kR2Shift);
+        }
+    }
+    std::vector<int> off(N + 1, 0);
+    for (size_t e = 0; e < tgt.size(); e++)
+        if (tgt[e] >= 0) off[tgt[e] + 1]++;
+    for (int c = 0; c < N; c++) off[c + 1] += off[c];
+    std::vector<int> idx(off[N]), fillp(off.begin(), off.end() - 1);
+    for (size_t e = 0; e < tgt.size(); e++)
+        if (tgt[e] >= 0) idx[fillp[tgt[e]]++] = (int)e;
+    if (!dalloc(A->r2_off, off.size()) || !dalloc(A->r2_idx, idx.size()) || !dalloc(A->R2, 6 * (size_t)N))
+        return CF_ERR_NOMEM;
+    if (!h2d(A->r2_off, off.data(), off.size(), h->stream) || !h2d(A->r2_idx, idx.data(), idx.size(), h->stream) ||
+        !h2d(h->nbr, nb.data(), nb.size(), h->stream))
+        return CF_ERR_HIP;
+    fill0(A->R2, 6 * (size_t)N, h->stream);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    A->r2_built = true;
+    return CF_OK;
+}
+
+extern "C" int cf_adj_set_gauges(cf_handle *h, int inst, int n, const int *cells) {
+    if (!h || inst < 0 || inst >= h->ninst || n < 0 || (n > 0 && !cells) || h->pending >= 0) return CF_ERR_ARG;
+    AdjState *A = adj_of(h);
+    if (!A) return CF_ERR_NOMEM;
+    for (int g = 0; g < n; g++)
+        if (cells[g] < 0 || cells[g] >= h->nelt[inst]) return CF_ERR_ARG;
+    A->gcells[inst].assign(cells, cells + n);
+    A->recorded = false;
+    A->tlm_ready = false;
+    return CF_OK;
+}
+
+static int upload_gauges(cf_handle *h, AdjState *A) {
+    const int B = h->ninst;
+    A->goff.assign(B + 1, 0);
+    for (int b = 0; b < B; b++) A->goff[b + 1] = A->goff[b] + (int)A->gcells[b].size();
+    A->gtotal = A->goff[B];
+    std::vector<int> gc(std::max(A->gtotal, 1)), gi(std::max(A->gtotal, 1));
+    for (int b = 0; b < B; b++) {
+        std::vector<int> inv(h->nelt[b]);
+        for (int l = 0; l < h->nelt[b]; l++) inv[h->hperm[b][l]] = l;
+        for (size_t g = 0; g < A->gcells[b].size(); g++) {
+            gc[A->goff[b] + g] = h->cell_off[b] + inv[A->gcells[b][g]];
+            gi[A->goff[b] + g] = b;
+        }
+    }
+    dfree(A->d_gcell); dfree(A->d_ginst); dfree(A->d_goff);
+    if (!dalloc(A->d_gcell, gc.size()) || !dalloc(A->d_ginst, gi.size()) || !dalloc(A->d_goff, (size_t)B + 1))
+        return CF_ERR_NOMEM;
+    if (!h2d(A->d_gcell, gc.data(), gc.size(), h->stream) || !h2d(A->d_ginst, gi.data(), gi.size(), h->stream) ||
+        !h2d(A->d_goff, A->goff.data(), (size_t)B + 1, h->stream))
+        return CF_ERR_HIP;
+    return CF_OK;
+}
+
+// (Re)allocates the storage of a recorded run of nsteps steps.
+static int adj_alloc(cf_handle *h, AdjState *A) {
+    const size_t N = (size_t)h->ncell, B = (size_t)h->ninst;
+    const size_t S = (size_t)A->nsteps + 1;
+    if (A->alloc_steps != A->nsteps || A->alloc_K != A->K) {
+        dfree(A->dt); dfree(A->tc); dfree(A->ran); dfree(A->ckpt); dfree(A->seg); dfree(A->xs);
+        if (!dalloc(A->dt, B * S) || !dalloc(A->tc, B * S) || !dalloc(A->ran, B * S) ||
+            !dalloc(A->ckpt, (size_t)A->nseg * N) || !dalloc(A->seg, (size_t)A->K * N) ||
+            !dalloc(A->xs, (size_t)A->K * 3 * N))
+            return CF_ERR_NOMEM;
+        A->alloc_steps = A->nsteps;
+        A->alloc_K = A->K;
+    }
+    if (A->alloc_nsamp != A->nsamp || A->alloc_gtotal != A->gtotal) {
+        dfree(A->eta); dfree(A->geta); dfree(A->deta); dfree(A->teta);
+        const size_t ne = (size_t)std::max(A->nsamp, 1) * (size_t)std::max(A->gtotal, 1);
+        if (!dalloc(A->eta, ne) || !dalloc(A->geta, ne) || !dalloc(A->deta, ne) || !dalloc(A->teta, ne))
+            return CF_ERR_NOMEM;
+        A->alloc_nsamp = A->nsamp;
+        A->alloc_gtotal = A->gtotal;
+    }
+    if (!A->lam[0]) {
+        if (!dalloc(A->lam[0], N) || !dalloc(A->lam[1], N) || !dalloc(A->gstate, N) || !dalloc(A->R1, 4 * N) ||
+            !dalloc(A->gn, N) || !dalloc(A->hmax, N) || !dalloc(A->ghmax, N) || !dalloc(A->dhmax, N) ||
+            !dalloc(A->argmax, N))
+            return CF_ERR_NOMEM;
+        fill0(A->R1, 4 * N, h->stream);
+    }
+    return CF_OK;
+}
+
+extern "C" int cf_adj_forward(cf_handle *h, int nsteps, int sample_every, int checkpoint_every, int want_hmax) {
+    if (!h || nsteps <= 0 || sample_every <= 0 || checkpoint_every <= 0 || h->pending >= 0) return CF_ERR_ARG;
+    for (int b = 0; b < h->ninst; b++)
+        if (h->hctrl[b].status != 0 || h->prm[b].is_override_manning) return CF_ERR_ARG;
+    AdjState *A = adj_of(h);
+    if (!A) return CF_ERR_NOMEM;
+    A->recorded = false;
+    A->tlm_ready = false;
+    A->nsteps = nsteps;
+    A->every = sample_every;
+    A->K = std::min(checkpoint_every, nsteps);
+    A->nseg = (nsteps + A->K - 1) / A->K;
+    A->nsamp = nsteps / sample_every;
+    A->want_hmax = want_hmax ? 1 : 0;
+    int rc = upload_gauges(h, A);
+    if (rc) return rc;
+    if ((rc = adj_alloc(h, A)) != CF_OK) return rc;
+    if (h->iflux == 3 && (rc = build_r2(h, A)) != CF_OK) return rc;
+    // seeds default to zero for instances the caller does not seed
+    fill0(A->geta, (size_t)std::max(A->nsamp, 1) * (size_t)std::max(A->gtotal, 1), h->stream);
+    fill0(A->gstate, (size_t)h->ncell, h->stream);
+    fill0(A->ghmax, (size_t)h->ncell, h->stream);
+
+    hipStream_t s = h->stream;
+    const int S = nsteps + 1;
+    const StPair sp{{h->st[0], h->st[1]}};
+    launch_run_init(h, A->dt, A->tc, S, s);
+    hipLaunchKernelGGL(ckpt_kernel, dim3(h->ntiles), dim3(kTile), 0, s, sp, h->ctrl, h->tile_inst, h->inst_end,
+                       A->ckpt);
+    StepArgs a = base_args(h);
+    a.dt_hist = A->dt;
+    a.tc_hist = A->tc;
+    a.ran = A->ran;
+    a.hist_stride = S;
+    for (int k = 0; k < nsteps; k++) {
+        a.step = k;
+        launch_step(a, h->iflux, h->fric, h->ntiles, s);
+        launch_finalize(h, A->dt, A->tc, A->ran, S, k, s);
+        if ((k + 1) % sample_every == 0) {
+            const int sidx = (k + 1) / sample_every - 1;
+            if (A->gtotal)
+                hipLaunchKernelGGL(sample_gauges_kernel, dim3(grid_of(A->gtotal, 256)), dim3(256), 0, s, sp, h->ctrl,
+                                   A->d_gcell, A->d_ginst, A->gtotal, sidx, A->eta);
+            if (A->want_hmax)
+                hipLaunchKernelGGL(sample_hmax_kernel, dim3(h->ntiles), dim3(kTile), 0, s, sp, h->ctrl, h->tile_inst,
+                                   h->inst_end, sidx, A->hmax, A->argmax);
+        }
+        if ((k + 1) % A->K == 0 && k + 1 < nsteps)
+            hipLaunchKernelGGL(ckpt_kernel, dim3(h->ntiles), dim3(kTile), 0, s, sp, h->ctrl, h->tile_inst,
+                               h->inst_end, A->ckpt + (size_t)((k + 1) / A->K) * h->ncell);
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<InstCtrl> c(h->ninst);
+    HIPCHK(hipMemcpyAsync(c.data(), h->ctrl, c.size() * sizeof(InstCtrl), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    h->hctrl = c;
+    for (int b = 0; b < h->ninst; b++) {
+        h->last_run[b] = 0;
+        h->last_total[b] = 0;
+        if (c[b].steps != nsteps) {
+            fprintf(stderr, "cuteflow_hip: recorded run of instance %d stopped after %d of %d steps\n", b,
+                    c[b].steps, nsteps);
+            return c[b].status < 0 ? CF_ERR_DT : CF_ERR_ARG;
+        }
+    }
+    A->recorded = true;
+    return CF_OK;
+}
+
+extern "C" int cf_adj_num_samples(cf_handle *h) { return (h && h->adj && h->adj->recorded) ? h->adj->nsamp : 0; }
+
+static bool bad_adj(cf_handle *h, int inst) {
+    return !h || !h->adj || !h->adj->recorded || inst < 0 || inst >= h->ninst || h->pending >= 0;
+}
+
+// Copies a per-cell internal-order device array of instance b to a caller
+// pointer (host or device) in the caller's order.
+static int out_scalar(cf_handle *h, int b, const double *src, double *dst) {
+    const int c0 = h->cell_off[b], n = h->nelt[b];
+    hipLaunchKernelGGL(scalar_out_kernel, dim3(grid_of(n, 256)), dim3(256), 0, h->stream, src, h->perm, c0, n,
+                       h->stage);
+    HIPCHK(hipMemcpyAsync(dst, h->stage, sizeof(double) * (size_t)n, hipMemcpyDefault, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CF_OK;
+}
+
+static int in_scalar(cf_handle *h, int b, const double *src, double *dst) {
+    const int c0 = h->cell_off[b], n = h->nelt[b];
+    HIPCHK(hipMemcpyAsync(h->stage, src, sizeof(double) * (size_t)n, hipMemcpyDefault, h->stream));
+    hipLaunchKernelGGL(scalar_in_kernel, dim3(grid_of(n, 256)), dim3(256), 0, h->stream, h->stage, h->perm, c0, n,
+                       dst);
+    HIPCHK(hipGetLastError());
+    return CF_OK;
+}
+
+static int out_state3(cf_handle *h, int b, const double4 *src, double *dst) {
+    const int c0 = h->cell_off[b], n = h->nelt[b];
+    hipLaunchKernelGGL(state3_out_kernel, dim3(grid_of(n, 256)), dim3(256), 0, h->stream, src, h->perm, c0, n,
+                       h->stage);
+    HIPCHK(hipMemcpyAsync(dst, h->stage, sizeof(double) * 3 * (size_t)n, hipMemcpyDefault, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CF_OK;
+}
+
+static int in_state3(cf_handle *h, int b, const double *src, double4 *dst) {
+    const int c0 = h->cell_off[b], n = h->nelt[b];
+    HIPCHK(hipMemcpyAsync(h->stage, src, sizeof(double) * 3 * (size_t)n, hipMemcpyDefault, h->stream));
+    hipLaunchKernelGGL(state3_in_kernel, dim3(grid_of(n, 256)), dim3(256), 0, h->stream, h->stage, h->perm, c0, n,
+                       dst);
+    HIPCHK(hipGetLastError());
+    return CF_OK;
+}
+
+// The (nsamp x ngauges) block of instance inst of a sample-major array.
+static int get_samples(cf_handle *h, int inst, const double *src, double *dst) {
+    AdjState *A = h->adj;
+    if (A->nsamp <= 0 || A->gtotal <= 0) return CF_OK;
+    const int ng = A->goff[inst + 1] - A->goff[inst];
+    std::vector<double> all((size_t)A->nsamp * A->gtotal);
+    HIPCHK(hipMemcpyAsync(all.data(), src, all.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int s = 0; s < A->nsamp; s++)
+        for (int g = 0; g < ng; g++) dst[(size_t)s * ng + g] = all[(size_t)s * A->gtotal + A->goff[inst] + g];
+    return CF_OK;
+}
+
+extern "C" int cf_adj_get_samples(cf_handle *h, int inst, double *eta, double *hmax) {
+    if (bad_adj(h, inst)) return CF_ERR_ARG;
+    AdjState *A = h->adj;
+    if (eta) {
+        int rc = get_samples(h, inst, A->eta, eta);
+        if (rc) return rc;
+    }
+    if (hmax) {
+        if (!A->want_hmax) return CF_ERR_ARG;
+        return out_scalar(h, inst, A->hmax, hmax);
+    }
+    return CF_OK;
+}
+
+extern "C" int cf_adj_set_seed(cf_handle *h, int inst, const double *g_eta, const double *g_hmax,
+                               const double *g_state) {
+    if (bad_adj(h, inst)) return CF_ERR_ARG;
+    AdjState *A = h->adj;
+    if (g_hmax && !A->want_hmax) return CF_ERR_ARG;
+    const int c0 = h->cell_off[inst], n = h->nelt[inst];
+    const int ng = A->goff[inst + 1] - A->goff[inst];
+    if (ng > 0 && A->nsamp > 0) {
+        // seeds of this instance's gauges: strided columns of the sample-major array
+        std::vector<double> all((size_t)A->nsamp * A->gtotal);
+        HIPCHK(hipMemcpyAsync(all.data(), A->geta, all.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int s = 0; s < A->nsamp; s++)
+            for (int g = 0; g < ng; g++)
+                all[(size_t)s * A->gtotal + A->goff[inst] + g] = g_eta ? g_eta[(size_t)s * ng + g] : 0.0;
+        HIPCHK(hipMemcpyAsync(A->geta, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    }
+    if (A->want_hmax) {
+        if (g_hmax) {
+            int rc = in_scalar(h, inst, g_hmax, A->ghmax);
+            if (rc) return rc;
+        } else {
+            fill0(A->ghmax + c0, (size_t)n, h->stream);
+        }
+    }
+    if (g_state) {
+        int rc = in_state3(h, inst, g_state, A->gstate);
+        if (rc) return rc;
+    } else {
+        fill0(A->gstate + c0, (size_t)n, h->stream);
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CF_OK;
+}
+
+template <int IFLUX, int FRIC>
+static void launch_rev_t(const RevArgs &r, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((rev_kernel<IFLUX, FRIC>), dim3(grid), dim3(kTile), 0, s, r);
+}
+
+static void launch_rev(const RevArgs &r, int iflux, int fric, int grid, hipStream_t s) {
+    typedef void (*fn)(const RevArgs &, int, hipStream_t);
+    static const fn tab[2][4] = {{launch_rev_t<1, 0>, launch_rev_t<1, 1>, launch_rev_t<1, 2>, launch_rev_t<1, 3>},
+                                 {launch_rev_t<3, 0>, launch_rev_t<3, 1>, launch_rev_t<3, 2>, launch_rev_t<3, 3>}};
+    tab[iflux == 3 ? 1 : 0][fric](r, grid, s);
+}
+
+extern "C" int cf_adj_backward(cf_handle *h) {
+    if (!h || !h->adj || !h->adj->recorded || h->pending >= 0) return CF_ERR_ARG;
+    AdjState *A = h->adj;
+    hipStream_t s = h->stream;
+    const size_t N = (size_t)h->ncell;
+    const int S = A->nsteps + 1;
+    const int *am = A->want_hmax ? A->argmax : nullptr;
+    // lambda of the final state: its seed + the seeds of the last sample
+    const int last = (A->nsteps % A->every == 0) ? A->nsteps / A->every - 1 : -1;
+    int cur = 0;
+    hipLaunchKernelGGL(lam_init_kernel, dim3(h->ntiles), dim3(kTile), 0, s, h->tile_inst, h->inst_end, A->gstate, am,
+                       A->ghmax, last, A->lam[cur]);
+    if (last >= 0 && A->gtotal)
+        hipLaunchKernelGGL(gauge_seed_kernel, dim3(grid_of(h->ninst, 64)), dim3(64), 0, s, A->d_gcell, A->d_goff,
+                           A->geta, A->gtotal, last, h->ninst, A->lam[cur]);
+    fill0(A->gn, N, s);
+
+    StepArgs a = base_args(h);
+    a.dt_fix = A->dt;
+    a.tc_fix = A->tc;
+    a.hist_stride = S;
+    a.skip = 0;
+    RevArgs r{};
+    r.nbr = h->nbr;
+    r.dfc = h->dfc;
+    r.geo = h->geo;
+    r.manning = h->manning;
+    r.hydro = h->hydro;
+    r.kp = h->d_kp;
+    r.tile_inst = h->tile_inst;
+    r.inst_end = h->inst_end;
+    r.R1 = A->R1;
+    r.R2 = A->R2;
+    r.gn = A->gn;
+    r.dt = A->dt;
+    r.tc = A->tc;
+    r.hstride = S;
+    r.ncell = h->ncell;
+    for (int sg = A->nseg - 1; sg >= 0; sg--) {
+        const int k0 = sg * A->K, k1 = std::min(k0 + A->K, A->nsteps);
+        const double4 *ck = A->ckpt + (size_t)sg * N;
+        // replay: state k0+m+1 -> seg[m], residual of step k0+m -> xs[m]
+        for (int k = k0; k < k1; k++) {
+            a.step = k;
+            a.in_x = (k == k0) ? ck : A->seg + (size_t)(k - k0 - 1) * N;
+            a.out_x = A->seg + (size_t)(k - k0) * N;
+            a.xs = A->xs + (size_t)(k - k0) * 3 * N;
+            launch_step(a, h->iflux, h->fric, h->ntiles, s);
+        }
+        for (int k = k1 - 1; k >= k0; k--) {
+            r.step = k;
+            r.st = (k == k0) ? ck : A->seg + (size_t)(k - k0 - 1) * N;
+            r.xs = A->xs + (size_t)(k - k0) * 3 * N;
+            r.lam = A->lam[cur];
+            launch_rev(r, h->iflux, h->fric, h->ntiles, s);
+            const int sidx = (k >= 1 && k % A->every == 0) ? k / A->every - 1 : -1;
+            if (h->iflux == 3)
+                hipLaunchKernelGGL(gather_kernel<true>, dim3(h->ntiles), dim3(kTile), 0, s, h->nbr, A->R1, A->R2,
+                                   A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, am, A->ghmax, sidx, A->lam[1 - cur]);
+            else
+                hipLaunchKernelGGL(gather_kernel<false>, dim3(h->ntiles), dim3(kTile), 0, s, h->nbr, A->R1, A->R2,
+                                   A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, am, A->ghmax, sidx, A->lam[1 - cur]);
+            if (sidx >= 0 && A->gtotal)
+                hipLaunchKernelGGL(gauge_seed_kernel, dim3(grid_of(h->ninst, 64)), dim3(64), 0, s, A->d_gcell,
+                                   A->d_goff, A->geta, A->gtotal, sidx, h->ninst, A->lam[1 - cur]);
+            cur = 1 - cur;
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    A->lam_cur = cur;
+    return CF_OK;
+}
+
+extern "C" int cf_adj_get_gradient(cf_handle *h, int inst, double *g_manning, double *g_state0) {
+    if (bad_adj(h, inst)) return CF_ERR_ARG;
+    AdjState *A = h->adj;
+    if (g_manning) {
+        int rc = out_scalar(h, inst, A->gn, g_manning);
+        if (rc) return rc;
+    }
+    if (g_state0) return out_state3(h, inst, A->lam[A->lam_cur], g_state0);
+    return CF_OK;
+}
+
+extern "C" int cf_tlm_set_direction(cf_handle *h, int inst, const double *dn, const double *dU0) {
+    if (bad_adj(h, inst)) return CF_ERR_ARG;
+    AdjState *A = h->adj;
+    const size_t N = (size_t)h->ncell;
+    if (!A->dman) {
+        if (!dalloc(A->dman, N) || !dalloc(A->tst[0], N) || !dalloc(A->tst[1], N) || !dalloc(A->tdst[0], N) ||
+            !dalloc(A->tdst[1], N) || !dalloc(A->tdir, N))
+            return CF_ERR_NOMEM;
+        fill0(A->dman, N, h->stream);
+        fill0(A->tdir, N, h->stream);
+    }
+    const int c0 = h->cell_off[inst], n = h->nelt[inst];
+    int rc;
+    if (dn) {
+        if ((rc = in_scalar(h, inst, dn, A->dman)) != CF_OK) return rc;
+    } else {
+        fill0(A->dman + c0, (size_t)n, h->stream);
+    }
+    if (dU0) {
+        if ((rc = in_state3(h, inst, dU0, A->tdir)) != CF_OK) return rc;
+    } else {
+        fill0(A->tdir + c0, (size_t)n, h->stream);
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    A->tlm_ready = false;
+    return CF_OK;
+}
+
+template <int IFLUX, int FRIC>
+static void launch_tlm_t(const TlmArgs &t, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((tlm_kernel<IFLUX, FRIC>), dim3(grid), dim3(kTile), 0, s, t);
+}
+
+extern "C" int cf_tlm_run(cf_handle *h) {
+    if (!h || !h->adj || !h->adj->recorded || !h->adj->dman || h->pending >= 0) return CF_ERR_ARG;
+    AdjState *A = h->adj;
+    hipStream_t s = h->stream;
+    const size_t N = (size_t)h->ncell;
+    typedef void (*fn)(const TlmArgs &, int, hipStream_t);
+    static const fn tab[2][4] = {{launch_tlm_t<1, 0>, launch_tlm_t<1, 1>, launch_tlm_t<1, 2>, launch_tlm_t<1, 3>},
+                                 {launch_tlm_t<3, 0>, launch_tlm_t<3, 1>, launch_tlm_t<3, 2>, launch_tlm_t<3, 3>}};
+    const fn L = tab[h->iflux == 3 ? 1 : 0][h->fric];
+    // start: the recorded initial state and the direction's initial tangent
+    hipLaunchKernelGGL(copy_d4_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, A->ckpt, A->tst[0], N);
+    hipLaunchKernelGGL(copy_d4_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, A->tdir, A->tdst[0], N);
+    TlmArgs t{};
+    t.nbr = h->nbr;
+    t.dfc = h->dfc;
+    t.geo = h->geo;
+    t.manning = h->manning;
+    t.dman = A->dman;
+    t.hydro = h->hydro;
+    t.kp = h->d_kp;
+    t.tile_inst = h->tile_inst;
+    t.inst_end = h->inst_end;
+    t.dt = A->dt;
+    t.tc = A->tc;
+    t.hstride = A->nsteps + 1;
+    t.ncell = h->ncell;
+    double4 *vb[2] = {A->tst[0], A->tst[1]};
+    double4 *db[2] = {A->tdst[0], A->tdst[1]};
+    if (A->want_hmax) fill0(A->dhmax, N, s);
+    for (int k = 0; k < A->nsteps; k++) {
+        t.step = k;
+        t.st_in = vb[k & 1];
+        t.dst_in = db[k & 1];
+        t.st_out = vb[(k + 1) & 1];
+        t.dst_out = db[(k + 1) & 1];
+        L(t, h->ntiles, s);
+        if ((k + 1) % A->every == 0) {
+            const int sidx = (k + 1) / A->every - 1;
+            if (A->gtotal)
+                hipLaunchKernelGGL(tlm_sample_kernel, dim3(grid_of(A->gtotal, 256)), dim3(256), 0, s,
+                                   vb[(k + 1) & 1], db[(k + 1) & 1], A->d_gcell, A->gtotal, sidx, A->teta, A->deta);
+            if (A->want_hmax)
+                hipLaunchKernelGGL(tlm_hmax_kernel, dim3(h->ntiles), dim3(kTile), 0, s, db[(k + 1) & 1], h->tile_inst,
+                                   h->inst_end, A->argmax, sidx, A->dhmax);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    A->tlm_cur = A->nsteps & 1;
+    A->tlm_ready = true;
+    return CF_OK;
+}
+
+extern "C" int cf_tlm_get(cf_handle *h, int inst, double *eta, double *d_eta, double *d_hmax, double *d_state,
+                          double *state) {
+    if (bad_adj(h, inst) || !h->adj->tlm_ready) return CF_ERR_ARG;
+    AdjState *A = h->adj;
+    int rc;
+    if (eta && (rc = get_samples(h, inst, A->teta, eta)) != CF_OK) return rc;
+    if (d_eta && (rc = get_samples(h, inst, A->deta, d_eta)) != CF_OK) return rc;
+    if (d_hmax) {
+        if (!A->want_hmax) return CF_ERR_ARG;
+        if ((rc = out_scalar(h, inst, A->dhmax, d_hmax)) != CF_OK) return rc;
+    }
+    double4 *db[2] = {A->tdst[0], A->tdst[1]};
+    double4 *vb[2] = {A->tst[0], A->tst[1]};
+    if (d_state && (rc = out_state3(h, inst, db[A->tlm_cur], d_state)) != CF_OK) return rc;
+    if (state && (rc = out_state3(h, inst, vb[A->tlm_cur], state)) != CF_OK) return rc;
+    return CF_OK;
+}

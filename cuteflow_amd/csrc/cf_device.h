@@ -1,11 +1,19 @@
-// Device-side numerics of the St-Venant step: project pow, HLLC and WAF
-// Riemann solvers, limiter.  Every expression keeps the operation order of
-// the reference Fortran (flux_riemann.cuf, source_terms.cuf) so that the
-// results are bit-identical to a CPU evaluation of the same formulas; the
-// translation unit is compiled with -ffp-contract=off.
+// Numerics of the St-Venant step: project pow, HLLC and WAF Riemann solvers,
+// limiter, rotation to the face frame; and their reverse (adjoint) forms.
+//
+// Every expression keeps the operation order of the reference Fortran
+// (flux_riemann.cuf, source_terms.cuf) so that the results are bit-identical
+// to a CPU evaluation of the same formulas; translation units are compiled
+// with -ffp-contract=off.  The forward functions are templated on the scalar
+// type: double for the step, Dual for the tangent-linear model (same value
+// bits, plus the directional derivative).  The *_rev functions are the
+// hand-written reverse of the forward ones for fixed branches: given the
+// cotangents of the outputs they accumulate the cotangents of the inputs.
 #pragma once
-#include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "cf_dual.h"
+#include "cf_hd.h"
 
 #pragma clang fp contract(off)
 
@@ -17,10 +25,12 @@ constexpr double kEpsHllc = (double)1e-8f;   // flux_riemann.cuf:926
 constexpr double kDtLow = (double)1e-12f;    // cfl_condition.cuf:129
 constexpr double kCote = (double)1.8f;       // maillage.cuf:157
 
-__device__ __forceinline__ double dmax(double a, double b) { return a > b ? a : b; }
-__device__ __forceinline__ double dmin(double a, double b) { return a < b ? a : b; }
+template <class T>
+CF_HD inline T dmax(T a, T b) { return a > b ? a : b; }
+template <class T>
+CF_HD inline T dmin(T a, T b) { return a < b ? a : b; }
 
-__device__ __forceinline__ void two_prod(double a, double b, double &p, double &e) {
+CF_HD inline void two_prod(double a, double b, double &p, double &e) {
     const double split = 134217729.0;  // 2^27 + 1 (Veltkamp)
     p = a * b;
     double t = split * a;
@@ -30,7 +40,7 @@ __device__ __forceinline__ void two_prod(double a, double b, double &p, double &
     e = ((ah * bh - p) + ah * bl + al * bh) + al * bl;
 }
 
-__device__ __forceinline__ void two_sum(double a, double b, double &s, double &e) {
+CF_HD inline void two_sum(double a, double b, double &s, double &e) {
     s = a + b;
     double bb = s - a;
     e = (a - (s - bb)) + (b - bb);
@@ -38,7 +48,7 @@ __device__ __forceinline__ void two_sum(double a, double b, double &s, double &e
 
 // x^y for x > 0: double-double log (atanh series) and Cody-Waite exp.
 // Uses only +,-,*,/, floor and bit operations: same bits as the host build.
-__device__ inline double cf_pow(double x, double y) {
+CF_HD inline double cf_pow(double x, double y) {
     const double ln2_hi = 0x1.62e42ffp-1;
     const double ln2_lo = -0x1.718432a1b0e26p-35;
     const double inv_ln2 = 0x1.71547652b82fep+0;
@@ -116,27 +126,33 @@ __device__ inline double cf_pow(double x, double y) {
     return ex;
 }
 
+// d(x^y) = y x^y / x dx
+CF_HD inline Dual cf_pow(Dual x, double y) {
+    const double p = cf_pow(x.v, y);
+    return Dual(p, x.d * ((y * p) / x.v));
+}
+
 // Wave speeds and HLLC flux in the face frame (flux_riemann.cuf:930-1020 for
 // HLLC, :753-845 for the first half of WAF).  Returns false for a dry/dry
 // face: the reference leaves al, ar, pql, pqr unset there; the flux is zero.
-struct Hllc {
-    double f1[3], f2[3], fh[3];
-    double sl, sr, set;
+template <class T>
+struct HllcT {
+    T f1[3], f2[3], fh[3];
+    T sl, sr, set;
 };
 
-__device__ __forceinline__ bool hllc_core(double gp, double eps, double hl, double ul,
-                                          double vl, double hr, double ur, double vr,
-                                          Hllc &o) {
+template <class T>
+CF_HD inline bool hllc_core(double gp, double eps, T hl, T ul, T vl, T hr, T ur, T vr, HllcT<T> &o) {
     if (hl < eps && hr < eps) return false;
-    double al = sqrt(gp * hl);
-    double ar = sqrt(gp * hr);
-    double het = (hl + hr) / 2.0 - (((ur - ul) * (hl + hr)) / (al + ar)) / 4.0;
-    double uet = (ul + ur) / 2.0 - ((hr - hl) * (al + ar)) / (hl + hr);
-    double pql, pqr, sl, sr, set;
+    T al = vsqrt(gp * hl);
+    T ar = vsqrt(gp * hr);
+    T het = (hl + hr) / 2.0 - (((ur - ul) * (hl + hr)) / (al + ar)) / 4.0;
+    T uet = (ul + ur) / 2.0 - ((hr - hl) * (al + ar)) / (hl + hr);
+    T pql, pqr, sl, sr, set;
     if (het < hl) pql = 1.0;
-    else pql = (hl > eps) ? sqrt(((0.5 * (het + hl)) * het) / (hl * hl)) : 0.0;
+    else pql = (hl > eps) ? vsqrt(((0.5 * (het + hl)) * het) / (hl * hl)) : T(0.0);
     if (het < hr) pqr = 1.0;
-    else pqr = (hr > eps) ? sqrt(((0.5 * (het + hr)) * het) / (hr * hr)) : 0.0;
+    else pqr = (hr > eps) ? vsqrt(((0.5 * (het + hr)) * het) / (hr * hr)) : T(0.0);
     if (hl > eps) {
         sl = ul - al * pql;
         if (hr > eps) {
@@ -149,11 +165,11 @@ __device__ __forceinline__ bool hllc_core(double gp, double eps, double hl, doub
         sl = ur - 2.0 * ar;
         sr = ur + ar;
     }
-    double denom = hr * (ur - sr) - hl * (ul - sl);
-    if (fabs(denom) < eps) set = uet;
+    T denom = hr * (ur - sr) - hl * (ul - sl);
+    if (vabs(denom) < eps) set = uet;
     else set = ((sl * hr) * (ur - sr) - (sr * hl) * (ul - sl)) / denom;
-    double q1[3] = {hl, hl * ul, hl * vl};
-    double q2[3] = {hr, hr * ur, hr * vr};
+    T q1[3] = {hl, hl * ul, hl * vl};
+    T q2[3] = {hr, hr * ur, hr * vr};
     o.f1[0] = hl * ul;
     o.f1[1] = hl * (ul * ul) + ((gp * hl) * hl) / 2.0;
     o.f1[2] = (hl * ul) * vl;
@@ -178,11 +194,10 @@ __device__ __forceinline__ bool hllc_core(double gp, double eps, double hl, doub
 }
 
 // solver_Riemann_hllc (flux_riemann.cuf:902-1026): flux in the global frame.
-__device__ __forceinline__ void hllc_flux(double gp, double n1, double n2, double hl,
-                                          double ul, double vl, double hr, double ur,
-                                          double vr, double flux[3]) {
-    Hllc o;
-    double fh0 = 0.0, fh1 = 0.0, fh2 = 0.0;
+template <class T>
+CF_HD inline void hllc_flux(double gp, double n1, double n2, T hl, T ul, T vl, T hr, T ur, T vr, T flux[3]) {
+    HllcT<T> o;
+    T fh0 = 0.0, fh1 = 0.0, fh2 = 0.0;
     if (hllc_core(gp, kEpsHllc, hl, ul, vl, hr, ur, vr, o)) {
         fh0 = o.fh[0]; fh1 = o.fh[1]; fh2 = o.fh[2];
     }
@@ -192,8 +207,9 @@ __device__ __forceinline__ void hllc_flux(double gp, double n1, double n2, doubl
 }
 
 // limiter (flux_riemann.cuf:465-524); unknown ilimiteur -> 1.
-__device__ __forceinline__ double limiter(int ilim, double r, double c) {
-    double ac = fabs(c);
+template <class T>
+CF_HD inline T limiter(int ilim, T r, T c) {
+    T ac = vabs(c);
     switch (ilim) {
     case 0: return 1.0;
     case 1:
@@ -217,38 +233,36 @@ __device__ __forceinline__ double limiter(int ilim, double r, double c) {
 }
 
 // solver_Riemann_waf (flux_riemann.cuf:720-900).
-__device__ __forceinline__ void waf_flux(double gp, int ilim, double n1, double n2,
-                                         double hl, double ul, double vl, double hr,
-                                         double ur, double vr, double hl_upw,
-                                         double vl_upw, double hr_upw, double vr_upw,
-                                         double d_mrl, double d_mkvol, double d_mkvoisin,
-                                         double dt, double flux[3]) {
+template <class T>
+CF_HD inline void waf_flux(double gp, int ilim, double n1, double n2, T hl, T ul, T vl, T hr, T ur, T vr,
+                           T hl_upw, T vl_upw, T hr_upw, T vr_upw, double d_mrl, double d_mkvol,
+                           double d_mkvoisin, double dt, T flux[3]) {
     const double eps = kEpsWaf;
-    Hllc o;
-    double fw0 = 0.0, fw1 = 0.0, fw2 = 0.0;
+    HllcT<T> o;
+    T fw0 = 0.0, fw1 = 0.0, fw2 = 0.0;
     if (hllc_core(gp, eps, hl, ul, vl, hr, ur, vr, o)) {
         double dtoverdx = dt / d_mrl;
-        double cr = o.sr * dtoverdx;
-        double cl = o.sl * dtoverdx;
-        double cet = o.set * dtoverdx;
-        double dlr = (hr - hl) / d_mrl;
-        double dll = (hr - hl) / d_mrl;
-        double dlet = (vr - vl) / d_mrl;
-        double dur = (o.sr > 0.0) ? (hl - hl_upw) / d_mkvol : (hr_upw - hr) / d_mkvoisin;
-        double dul = (o.sl > 0.0) ? (hl - hl_upw) / d_mkvol : (hr_upw - hr) / d_mkvoisin;
-        double duet = (o.set > 0.0) ? (vl - vl_upw) / d_mkvol : (vr_upw - vr) / d_mkvoisin;
-        double rl = dul / (dll + eps);
-        double rr = dur / (dlr + eps);
-        double ret = duet / (dlet + eps);
-        double lim_l = limiter(ilim, rl, cl);
-        double lim_r = limiter(ilim, rr, cr);
-        double lim_et = limiter(ilim, ret, cet);
-        double sgl = copysign(1.0, o.sl), sgr = copysign(1.0, o.sr), sge = copysign(1.0, o.set);
-        double wl = (1.0 + sgl * lim_l) / 2.0;
-        double wlet = (1.0 + sge * lim_et) / 2.0;
-        double wlr = (sgr * lim_r - sgl * lim_l) / 2.0;
-        double wr = (1.0 - sgr * lim_r) / 2.0;
-        double wret = (1.0 - sge * lim_et) / 2.0;
+        T cr = o.sr * dtoverdx;
+        T cl = o.sl * dtoverdx;
+        T cet = o.set * dtoverdx;
+        T dlr = (hr - hl) / d_mrl;
+        T dll = (hr - hl) / d_mrl;
+        T dlet = (vr - vl) / d_mrl;
+        T dur = (o.sr > 0.0) ? (hl - hl_upw) / d_mkvol : (hr_upw - hr) / d_mkvoisin;
+        T dul = (o.sl > 0.0) ? (hl - hl_upw) / d_mkvol : (hr_upw - hr) / d_mkvoisin;
+        T duet = (o.set > 0.0) ? (vl - vl_upw) / d_mkvol : (vr_upw - vr) / d_mkvoisin;
+        T rl = dul / (dll + eps);
+        T rr = dur / (dlr + eps);
+        T ret = duet / (dlet + eps);
+        T lim_l = limiter(ilim, rl, cl);
+        T lim_r = limiter(ilim, rr, cr);
+        T lim_et = limiter(ilim, ret, cet);
+        double sgl = copysign(1.0, val(o.sl)), sgr = copysign(1.0, val(o.sr)), sge = copysign(1.0, val(o.set));
+        T wl = (1.0 + sgl * lim_l) / 2.0;
+        T wlet = (1.0 + sge * lim_et) / 2.0;
+        T wlr = (sgr * lim_r - sgl * lim_l) / 2.0;
+        T wr = (1.0 - sgr * lim_r) / 2.0;
+        T wret = (1.0 - sge * lim_et) / 2.0;
         fw0 = (wl * o.f1[0] + wlr * o.fh[0]) + wr * o.f2[0];
         fw1 = (wl * o.f1[1] + wlr * o.fh[1]) + wr * o.f2[1];
         fw2 = (wlet * vl + wret * vr) * fw0;
@@ -260,9 +274,8 @@ __device__ __forceinline__ void waf_flux(double gp, int ilim, double n1, double 
 
 // Rotation to the face frame with the cell dry threshold
 // (flux_riemann.cuf:105-112, :186-193, :240-258).
-__device__ __forceinline__ void rot(double tolisec, double n1, double n2, double h0,
-                                    double hu, double hv, double &h, double &un,
-                                    double &ut) {
+template <class T>
+CF_HD inline void rot(double tolisec, double n1, double n2, T h0, T hu, T hv, T &h, T &un, T &ut) {
     h = h0;
     if (h > tolisec) {
         un = (hu * n1 + hv * n2) / h;
@@ -272,6 +285,341 @@ __device__ __forceinline__ void rot(double tolisec, double n1, double n2, double
         un = 0.0;
         ut = 0.0;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Reverse (adjoint) forms.  Each takes the forward inputs (the forward
+// intermediates are recomputed), the cotangents of the forward outputs, and
+// adds the cotangents of the inputs to the accumulators it is given.
+
+// Reverse of rot for a cell state (h0, hu, hv): cotangents of (h, un, ut).
+CF_HD inline void rot_rev(double tolisec, double n1, double n2, double h0, double hu, double hv, double bh,
+                          double bun, double but, double &bh0, double &bhu, double &bhv) {
+    if (!(h0 > tolisec)) return;  // all outputs are constants
+    const double un = (hu * n1 + hv * n2) / h0;
+    const double ut = (hv * n1 - hu * n2) / h0;
+    bhu += (bun * n1 - but * n2) / h0;
+    bhv += (bun * n2 + but * n1) / h0;
+    bh0 += bh - (bun * un + but * ut) / h0;
+}
+
+// Reverse of hllc_core.  Output cotangents: fh[0..2], f1[0..1], f2[0..1],
+// sl, sr, set (the first-order solver uses only fh; WAF uses all of them).
+// g[0..5] accumulates the cotangents of (hl, ul, vl, hr, ur, vr).
+CF_HD inline void hllc_core_rev(double gp, double eps, double hl, double ul, double vl, double hr, double ur,
+                                double vr, const double bfh[3], const double bf1in[2], const double bf2in[2],
+                                double bsl, double bsr, double bset, double g[6]) {
+    if (hl < eps && hr < eps) return;
+    // forward intermediates
+    const double al = sqrt(gp * hl), ar = sqrt(gp * hr);
+    const double S = hl + hr, A = al + ar;
+    const double het = S / 2.0 - (((ur - ul) * S) / A) / 4.0;
+    const double uet = (ul + ur) / 2.0 - ((hr - hl) * A) / S;
+    const bool pql_var = !(het < hl) && hl > eps;
+    const bool pqr_var = !(het < hr) && hr > eps;
+    const double pql = (het < hl) ? 1.0 : (pql_var ? sqrt(((0.5 * (het + hl)) * het) / (hl * hl)) : 0.0);
+    const double pqr = (het < hr) ? 1.0 : (pqr_var ? sqrt(((0.5 * (het + hr)) * het) / (hr * hr)) : 0.0);
+    double sl, sr;
+    int sb;  // wave-speed case
+    if (hl > eps) {
+        if (hr > eps) {
+            sl = ul - al * pql;
+            sr = ur + ar * pqr;
+            sb = 0;
+        } else {
+            sl = ul - al;
+            sr = ul + 2.0 * al;
+            sb = 1;
+        }
+    } else {
+        sl = ur - 2.0 * ar;
+        sr = ur + ar;
+        sb = 2;
+    }
+    const double denom = hr * (ur - sr) - hl * (ul - sl);
+    const bool set_uet = fabs(denom) < eps;
+    const double set = set_uet ? uet : ((sl * hr) * (ur - sr) - (sr * hl) * (ul - sl)) / denom;
+    const double f1[3] = {hl * ul, hl * (ul * ul) + ((gp * hl) * hl) / 2.0, (hl * ul) * vl};
+    const double f2[3] = {hr * ur, hr * (ur * ur) + ((gp * hr) * hr) / 2.0, (hr * ur) * vr};
+    const double q1[2] = {hl, hl * ul};
+    const double q2[2] = {hr, hr * ur};
+
+    double bhl = 0.0, bul = 0.0, bvl = 0.0, bhr = 0.0, bur = 0.0, bvr = 0.0;
+    double bf1[3] = {bf1in[0], bf1in[1], 0.0}, bf2[3] = {bf2in[0], bf2in[1], 0.0};
+    double bq1[2] = {0.0, 0.0}, bq2[2] = {0.0, 0.0};
+    // fh
+    if (sl > 0.0) {
+        for (int k = 0; k < 3; k++) bf1[k] += bfh[k];
+    } else if (sr < 0.0) {
+        for (int k = 0; k < 3; k++) bf2[k] += bfh[k];
+    } else {
+        const double D = (sr - sl) + eps;
+        double fh[2];
+        for (int k = 0; k < 2; k++) fh[k] = ((sr * f1[k] - sl * f2[k]) + (sl * sr) * (q2[k] - q1[k])) / D;
+        double bfk[2] = {bfh[0], bfh[1]};
+        if (set > 0.0) {
+            bfk[0] += bfh[2] * vl;
+            bvl += bfh[2] * fh[0];
+        } else {
+            bfk[0] += bfh[2] * vr;
+            bvr += bfh[2] * fh[0];
+        }
+        double bD = 0.0;
+        for (int k = 0; k < 2; k++) {
+            const double bN = bfk[k] / D;
+            const double dq = q2[k] - q1[k];
+            bD -= bN * fh[k];
+            bsr += bN * f1[k] + bN * sl * dq;
+            bsl += -bN * f2[k] + bN * sr * dq;
+            bf1[k] += bN * sr;
+            bf2[k] -= bN * sl;
+            bq2[k] += bN * (sl * sr);
+            bq1[k] -= bN * (sl * sr);
+        }
+        bsr += bD;
+        bsl -= bD;
+    }
+    // set (used by the WAF weights only)
+    double bA = 0.0, bS = 0.0;
+    if (bset != 0.0) {
+        if (set_uet) {
+            const double E = (hr - hl) * A;
+            bul += bset / 2.0;
+            bur += bset / 2.0;
+            const double bE = -bset / S;
+            bS += bset * E / (S * S);
+            bhr += bE * A;
+            bhl -= bE * A;
+            bA += bE * (hr - hl);
+        } else {
+            const double bnum = bset / denom;
+            const double bden = -bset * set / denom;
+            bsl += bnum * (hr * (ur - sr)) + bnum * (sr * hl);
+            bhr += bnum * (sl * (ur - sr));
+            bur += bnum * (sl * hr);
+            bsr += -bnum * (sl * hr) - bnum * (hl * (ul - sl));
+            bhl += -bnum * (sr * (ul - sl));
+            bul += -bnum * (sr * hl);
+            bhr += bden * (ur - sr);
+            bur += bden * hr;
+            bsr -= bden * hr;
+            bhl -= bden * (ul - sl);
+            bul -= bden * hl;
+            bsl += bden * hl;
+        }
+    }
+    // wave speeds
+    double bal = 0.0, bar = 0.0, bpql = 0.0, bpqr = 0.0;
+    if (sb == 0) {
+        bul += bsl;
+        bal -= bsl * pql;
+        bpql -= bsl * al;
+        bur += bsr;
+        bar += bsr * pqr;
+        bpqr += bsr * ar;
+    } else if (sb == 1) {
+        bul += bsl + bsr;
+        bal += -bsl + 2.0 * bsr;
+    } else {
+        bur += bsl + bsr;
+        bar += -2.0 * bsl + bsr;
+    }
+    // pql, pqr: sqrt(Q), Q = (0.5 (het + h)) het / h^2
+    double bhet = 0.0;
+    if (pql_var && bpql != 0.0) {
+        const double H2 = hl * hl, Q = ((0.5 * (het + hl)) * het) / H2;
+        const double bP = (bpql / (2.0 * pql)) / H2;
+        bhet += bP * (het + 0.5 * hl);
+        bhl += bP * (0.5 * het) - (bP * Q) * (2.0 * hl);
+    }
+    if (pqr_var && bpqr != 0.0) {
+        const double H2 = hr * hr, Q = ((0.5 * (het + hr)) * het) / H2;
+        const double bP = (bpqr / (2.0 * pqr)) / H2;
+        bhet += bP * (het + 0.5 * hr);
+        bhr += bP * (0.5 * het) - (bP * Q) * (2.0 * hr);
+    }
+    // het = S/2 - ((ur - ul) S / A) / 4
+    {
+        const double W = ((ur - ul) * S) / A;
+        const double bW = -bhet / 4.0;
+        bS += bhet / 2.0 + bW * ((ur - ul) / A);
+        bur += bW * (S / A);
+        bul -= bW * (S / A);
+        bA -= bW * (W / A);
+    }
+    bhl += bS;
+    bhr += bS;
+    bal += bA;
+    bar += bA;
+    if (al > 0.0) bhl += bal * (gp / (2.0 * al));
+    if (ar > 0.0) bhr += bar * (gp / (2.0 * ar));
+    // f1, f2, q1, q2
+    bhl += bf1[0] * ul + bf1[1] * (ul * ul + gp * hl) + bf1[2] * (ul * vl) + bq1[0] + bq1[1] * ul;
+    bul += bf1[0] * hl + bf1[1] * (2.0 * hl * ul) + bf1[2] * (hl * vl) + bq1[1] * hl;
+    bvl += bf1[2] * (hl * ul);
+    bhr += bf2[0] * ur + bf2[1] * (ur * ur + gp * hr) + bf2[2] * (ur * vr) + bq2[0] + bq2[1] * ur;
+    bur += bf2[0] * hr + bf2[1] * (2.0 * hr * ur) + bf2[2] * (hr * vr) + bq2[1] * hr;
+    bvr += bf2[2] * (hr * ur);
+    g[0] += bhl;
+    g[1] += bul;
+    g[2] += bvl;
+    g[3] += bhr;
+    g[4] += bur;
+    g[5] += bvr;
+}
+
+// Cotangent of a global-frame flux (f0, n1 f1 - n2 f2, n2 f1 + n1 f2) mapped
+// to the face-frame components.
+CF_HD inline void unrotate_rev(double n1, double n2, const double bflux[3], double b[3]) {
+    b[0] = bflux[0];
+    b[1] = n1 * bflux[1] + n2 * bflux[2];
+    b[2] = -n2 * bflux[1] + n1 * bflux[2];
+}
+
+// Reverse of hllc_flux: g[0..5] += cotangents of (hl, ul, vl, hr, ur, vr).
+CF_HD inline void hllc_flux_rev(double gp, double n1, double n2, double hl, double ul, double vl, double hr,
+                                double ur, double vr, const double bflux[3], double g[6]) {
+    double bfh[3];
+    unrotate_rev(n1, n2, bflux, bfh);
+    const double z2[2] = {0.0, 0.0};
+    hllc_core_rev(gp, kEpsHllc, hl, ul, vl, hr, ur, vr, bfh, z2, z2, 0.0, 0.0, 0.0, g);
+}
+
+// Value and partial derivatives (with respect to r and c) of limiter().
+CF_HD inline double limiter_d(int ilim, double r, double c, double &dr, double &dc) {
+    const double ac = fabs(c);
+    // d|c|/dc with the dual type's convention at c = 0 (+1)
+    const double sc = c < 0.0 ? -1.0 : 1.0;
+    double dac = 0.0;
+    double v;
+    dr = 0.0;
+    switch (ilim) {
+    case 1:
+        if (r <= 0.0) v = 1.0;
+        else if (r > 0.0 && r < 0.5) { v = 1.0 - (2.0 * (1.0 - ac)) * r; dr = -(2.0 * (1.0 - ac)); dac = 2.0 * r; }
+        else if (r > 0.5 && r < 1.0) { v = ac; dac = 1.0; }
+        else if (r > 1.0 && r < 2.0) { v = 1.0 - (1.0 - ac) * r; dr = -(1.0 - ac); dac = r; }
+        else { v = 2.0 * ac - 1.0; dac = 2.0; }
+        break;
+    case 2:
+        if (r <= 0.0) v = 1.0;
+        else {
+            v = 1.0 - ((2.0 * (1.0 - ac)) * r) / (1.0 + r);
+            dr = -(2.0 * (1.0 - ac)) / ((1.0 + r) * (1.0 + r));
+            dac = (2.0 * r) / (1.0 + r);
+        }
+        break;
+    case 3:
+        if (r <= 0.0) v = 1.0;
+        else {
+            const double q = (r * (1.0 + r)) / (1.0 + r * r);
+            v = 1.0 - (((1.0 - ac) * r) * (1.0 + r)) / (1.0 + r * r);
+            dr = -(1.0 - ac) * ((1.0 + 2.0 * r - r * r) / ((1.0 + r * r) * (1.0 + r * r)));
+            dac = q;
+        }
+        break;
+    case 4:
+        if (r <= 0.0) v = 1.0;
+        else if (r <= 1.0) { v = 1.0 - (1.0 - ac) * r; dr = -(1.0 - ac); dac = r; }
+        else { v = ac; dac = 1.0; }
+        break;
+    case 0:
+    default: v = 1.0; break;
+    }
+    dc = dac * sc;
+    return v;
+}
+
+// Reverse of waf_flux.  g[0..9] += cotangents of
+// (hl, ul, vl, hr, ur, vr, hl_upw, vl_upw, hr_upw, vr_upw).
+CF_HD inline void waf_flux_rev(double gp, int ilim, double n1, double n2, double hl, double ul, double vl, double hr,
+                               double ur, double vr, double hl_upw, double vl_upw, double hr_upw, double vr_upw,
+                               double d_mrl, double d_mkvol, double d_mkvoisin, double dt, const double bflux[3],
+                               double g[10]) {
+    const double eps = kEpsWaf;
+    HllcT<double> o;
+    if (!hllc_core(gp, eps, hl, ul, vl, hr, ur, vr, o)) return;
+    double bfw[3];
+    unrotate_rev(n1, n2, bflux, bfw);
+    const double dtoverdx = dt / d_mrl;
+    const double cr = o.sr * dtoverdx, cl = o.sl * dtoverdx, cet = o.set * dtoverdx;
+    const double dlr = (hr - hl) / d_mrl, dll = (hr - hl) / d_mrl, dlet = (vr - vl) / d_mrl;
+    const double dur = (o.sr > 0.0) ? (hl - hl_upw) / d_mkvol : (hr_upw - hr) / d_mkvoisin;
+    const double dul = (o.sl > 0.0) ? (hl - hl_upw) / d_mkvol : (hr_upw - hr) / d_mkvoisin;
+    const double duet = (o.set > 0.0) ? (vl - vl_upw) / d_mkvol : (vr_upw - vr) / d_mkvoisin;
+    const double rl = dul / (dll + eps), rr = dur / (dlr + eps), ret = duet / (dlet + eps);
+    double drl, dcl, drr, dcr, dret, dcet;
+    const double lim_l = limiter_d(ilim, rl, cl, drl, dcl);
+    const double lim_r = limiter_d(ilim, rr, cr, drr, dcr);
+    const double lim_et = limiter_d(ilim, ret, cet, dret, dcet);
+    const double sgl = copysign(1.0, o.sl), sgr = copysign(1.0, o.sr), sge = copysign(1.0, o.set);
+    const double wl = (1.0 + sgl * lim_l) / 2.0;
+    const double wlet = (1.0 + sge * lim_et) / 2.0;
+    const double wlr = (sgr * lim_r - sgl * lim_l) / 2.0;
+    const double wr = (1.0 - sgr * lim_r) / 2.0;
+    const double wret = (1.0 - sge * lim_et) / 2.0;
+    const double fw0 = (wl * o.f1[0] + wlr * o.fh[0]) + wr * o.f2[0];
+
+    double bhl = 0.0, bul = 0.0, bvl = 0.0, bhr = 0.0, bur = 0.0, bvr = 0.0;
+    double bhlu = 0.0, bvlu = 0.0, bhru = 0.0, bvru = 0.0;
+    // fw2 = (wlet vl + wret vr) fw0
+    const double V = wlet * vl + wret * vr;
+    const double bV = bfw[2] * fw0;
+    const double bfw0 = bfw[0] + bfw[2] * V;
+    const double bwlet = bV * vl, bwret = bV * vr;
+    bvl += bV * wlet;
+    bvr += bV * wret;
+    // fw0, fw1
+    const double bwl = bfw0 * o.f1[0] + bfw[1] * o.f1[1];
+    const double bwlr = bfw0 * o.fh[0] + bfw[1] * o.fh[1];
+    const double bwr = bfw0 * o.f2[0] + bfw[1] * o.f2[1];
+    const double bf1[2] = {bfw0 * wl, bfw[1] * wl};
+    const double bfh[3] = {bfw0 * wlr, bfw[1] * wlr, 0.0};
+    const double bf2[2] = {bfw0 * wr, bfw[1] * wr};
+    // weights -> limiter values
+    const double blim_l = (bwl * sgl - bwlr * sgl) / 2.0;
+    const double blim_r = (bwlr * sgr - bwr * sgr) / 2.0;
+    const double blim_et = (bwlet * sge - bwret * sge) / 2.0;
+    // limiters -> (r, c)
+    const double brl = blim_l * drl, brr = blim_r * drr, bret = blim_et * dret;
+    const double bcl = blim_l * dcl, bcr = blim_r * dcr, bcet = blim_et * dcet;
+    // r = du / (dl + eps)
+    const double bdul = brl / (dll + eps), bdur = brr / (dlr + eps), bduet = bret / (dlet + eps);
+    const double bdll = -brl * (rl / (dll + eps)), bdlr = -brr * (rr / (dlr + eps));
+    const double bdlet = -bret * (ret / (dlet + eps));
+    bhr += (bdll + bdlr) / d_mrl;
+    bhl -= (bdll + bdlr) / d_mrl;
+    bvr += bdlet / d_mrl;
+    bvl -= bdlet / d_mrl;
+    if (o.sr > 0.0) {
+        bhl += bdur / d_mkvol;
+        bhlu -= bdur / d_mkvol;
+    } else {
+        bhru += bdur / d_mkvoisin;
+        bhr -= bdur / d_mkvoisin;
+    }
+    if (o.sl > 0.0) {
+        bhl += bdul / d_mkvol;
+        bhlu -= bdul / d_mkvol;
+    } else {
+        bhru += bdul / d_mkvoisin;
+        bhr -= bdul / d_mkvoisin;
+    }
+    if (o.set > 0.0) {
+        bvl += bduet / d_mkvol;
+        bvlu -= bduet / d_mkvol;
+    } else {
+        bvru += bduet / d_mkvoisin;
+        bvr -= bduet / d_mkvoisin;
+    }
+    double gg[6] = {bhl, bul, bvl, bhr, bur, bvr};
+    hllc_core_rev(gp, eps, hl, ul, vl, hr, ur, vr, bfh, bf1, bf2, bcl * dtoverdx, bcr * dtoverdx, bcet * dtoverdx,
+                  gg);
+    for (int k = 0; k < 6; k++) g[k] += gg[k];
+    g[6] += bhlu;
+    g[7] += bvlu;
+    g[8] += bhru;
+    g[9] += bvru;
 }
 
 }  // namespace cf

@@ -1,34 +1,22 @@
 // Shared definitions of the HIP translation units: device data layout,
-// per-instance parameters and control block, and the solver handle.
+// per-instance control block, kernel arguments, the solver handle and small
+// host helpers.
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <cstdio>
 #include <vector>
 
 #include "../../include/cuteflow_hip.h"
+#include "cf_cell.h"
 
 namespace cf {
 
 // cells per tile = threads per workgroup of the step kernel
 constexpr int kTile = 256;
 constexpr int kWaves = kTile / 64;
-
-// geometry arrays (structure of arrays, stride = padded cell count)
-enum { G_N1 = 0, G_N2 = 3, G_LN = 6, G_SURF = 9, G_COT = 10, G_XC = 11, G_YC = 12, G_COUNT = 13 };
-
-// Per-instance constants (the reference's `constant` memory,
-// global_data_device.cuf, set by data_transfer.cuf:12-96).
-struct KParams {
-    double gp, tolisec, tol, cfl, ts, constant_dt, override_manning;
-    double pad_dt;  // min(1e6, 1e5 if nelt % 1024 != 0): the reduction's padding
-    int ilimiteur, inlet, outlet, is_dt_constant, is_override_manning;
-    int hydro_off;  // offset of this instance's hydrograph tables in the hydro array
-    double debitglob[CF_MAX_IO];
-    double h_sortie[CF_MAX_IO];
-    double long_entree[CF_MAX_IO];
-    int hydro_n[CF_MAX_IO];
-};
 
 // Per-instance control block, resident on the device across runs.
 struct InstCtrl {
@@ -60,14 +48,45 @@ struct StepArgs {
     const int *__restrict__ tile_inst;   // instance of each tile
     const int *__restrict__ inst_end;    // per instance: one past its last real cell
     const int *__restrict__ inst_tile;   // per instance: first tile; [ninst] = total
-    // explicit buffers (adjoint recompute): when set, the step reads in and writes out
+    // Replay of a recorded run (reverse sweep): when dt_fix is set the step
+    // takes dt and tc of step `step` from dt_fix/tc_fix, reads in_x, writes
+    // out_x and the residual x to xs (3 x ncell), and skips the dt reduction.
+    const double *dt_fix;
+    const double *tc_fix;
     const double4 *in_x;
     double4 *out_x;
+    double *xs;
     int ncell;        // padded cell count (geo stride)
     int hist_stride;
-    int step;         // index of this step in the run
+    int step;         // index of this step in the run (absolute in a replay)
     int skip;         // dry-tile skipping enabled
 };
+
+// Device loader of the cell body: states, neighbours, geometry.
+struct DevLd {
+    const double4 *st_;
+    const int4 *nbr_;
+    const double4 *dfc_;
+    const double *geo_;
+    const double *man_;
+    int stride;
+    __device__ __forceinline__ St<double> state(int c) const {
+        const double4 v = st_[c];
+        return {v.x, v.y, v.z, v.w};
+    }
+    __device__ __forceinline__ I4 nbr(int c) const {
+        const int4 v = nbr_[c];
+        return {v.x, v.y, v.z, v.w};
+    }
+    __device__ __forceinline__ D4 dfc(int c) const {
+        const double4 v = dfc_[c];
+        return {v.x, v.y, v.z, v.w};
+    }
+    __device__ __forceinline__ double geo(int comp, int c) const { return geo_[comp * (size_t)stride + c]; }
+    __device__ __forceinline__ double manning(int c) const { return man_[c]; }
+};
+
+struct AdjState;  // reverse-sweep storage (cf_adjoint.hip)
 
 }  // namespace cf
 
@@ -79,6 +98,7 @@ struct cf_handle {
     std::vector<cf_params> prm;
     std::vector<cf::KParams> kp;
     std::vector<std::vector<int>> hperm;               // per instance: internal -> caller (local)
+    std::vector<int4> hnbr;                            // host copy of the neighbour codes
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int iflux = 1, fric = 0, skip = 1;
@@ -107,12 +127,53 @@ struct cf_handle {
     std::vector<long long> last_run, last_total;
     int pending = -1;
     int max_cells = 0;
-    // graphs of whole runs, keyed by (steps, parity pattern)
+    // graphs of whole runs, keyed by the step count
     struct Graph {
         int nsteps;
-        std::vector<int> par;
         hipGraphExec_t exec;
     };
     std::vector<Graph> graphs;
     int use_graphs = 1;
+    cf::AdjState *adj = nullptr;
 };
+
+// ---- host helpers shared by the translation units ----------------------
+
+#define HIPCHK(x)                                                                          \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) {                                                            \
+            fprintf(stderr, "cuteflow_hip: %s failed: %s\n", #x, hipGetErrorString(e_));  \
+            return CF_ERR_HIP;                                                             \
+        }                                                                                  \
+    } while (0)
+
+namespace cf {
+
+inline int grid_of(int n, int b) { return (n + b - 1) / b; }
+
+template <typename T>
+inline void dfree(T *&ptr) {
+    if (ptr) (void)hipFree((void *)ptr);
+    ptr = nullptr;
+}
+
+template <typename T>
+inline bool dalloc(T *&p, size_t count) {
+    return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T)) == hipSuccess;
+}
+
+template <typename T>
+inline bool h2d(T *dst, const T *src, size_t count, hipStream_t s) {
+    return count == 0 || hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, s) == hipSuccess;
+}
+
+// defined in cuteflow_hip.hip
+StepArgs base_args(cf_handle *h);
+void launch_step(const StepArgs &a, int iflux, int fric, int grid, hipStream_t s);
+void launch_finalize(cf_handle *h, double *dt_hist, double *tc_hist, int *ran, int stride, int step, hipStream_t s);
+void launch_run_init(cf_handle *h, double *dt_hist, double *tc_hist, int stride, hipStream_t s);
+// defined in cf_adjoint.hip
+void adj_free(cf_handle *h);
+
+}  // namespace cf

@@ -43,34 +43,12 @@ using namespace cf;
 
 namespace {
 
-__device__ double inflow_q(const KParams &p, const double *hydro, int id, double tc) {
-    const int n = p.hydro_n[id];
-    if (n <= 0) return p.debitglob[id];
-    const double *t = hydro + p.hydro_off + id * CF_MAX_HYDRO;
-    const double *q = hydro + p.hydro_off + CF_MAX_IO * CF_MAX_HYDRO + id * CF_MAX_HYDRO;
-    if (tc <= t[0]) return q[0];
-    for (int i = 0; i + 1 < n; i++)
-        if (tc < t[i + 1]) return q[i] + (q[i + 1] - q[i]) * ((tc - t[i]) / (t[i + 1] - t[i]));
-    return q[n - 1];
-}
-
-__device__ __forceinline__ int nb_at(const int4 &v, int j) { return j == 0 ? v.x : (j == 1 ? v.y : v.z); }
-__device__ __forceinline__ double d4_at(const double4 &v, int j) { return j == 0 ? v.x : (j == 1 ? v.y : v.z); }
+__device__ __forceinline__ int nb4_at(const int4 &v, int j) { return j == 0 ? v.x : (j == 1 ? v.y : v.z); }
 
 __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) v = dmin(v, __shfl_xor(v, off, 64));
     return v;
-}
-
-// Flux of a face whose two reconstructed depths are both below eps: the
-// Riemann solvers return a zero flux there; same operations as their
-// rotation back to the global frame.
-__device__ __forceinline__ void zero_flux(double n1, double n2, double flux[3]) {
-    const double z = 0.0;
-    flux[0] = z;
-    flux[1] = n1 * z - n2 * z;
-    flux[2] = n2 * z + n1 * z;
 }
 
 // Occupancy target of the step kernel (waves per SIMD): 4 waves caps the
@@ -92,270 +70,60 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
     const int tile = blockIdx.x;
     const int b = a.tile_inst[tile];
     const InstCtrl c = a.ctrl[b];
-    if (c.status != 0) {  // this instance has stopped (ts reached) or failed
-        if (threadIdx.x == 0) a.partial[tile] = __builtin_inf();
-        return;
+    const bool replay = a.dt_fix != nullptr;
+    if (!replay) {
+        if (c.status != 0) {  // this instance has stopped (ts reached) or failed
+            if (threadIdx.x == 0) a.partial[tile] = __builtin_inf();
+            return;
+        }
+        // A tile whose cells and neighbour tiles' cells were all dry after
+        // the last two steps would recompute (0, 0, 0) for every cell (all
+        // faces dry/dry, no source, clamp), and both state buffers already
+        // hold that: it is skipped.  Reading a stamp raised concurrently only
+        // makes the tile run, which is always correct.
+        if (a.skip && !a.always[tile] && a.stamp[tile] + 1u < c.gstep) {
+            if (threadIdx.x == 0) a.partial[tile] = __builtin_inf();  // no dt contribution
+            return;
+        }
     }
-    // A tile whose cells and neighbour tiles' cells were all dry after the
-    // last two steps would recompute (0, 0, 0) for every cell (all faces
-    // dry/dry, no source, clamp), and both state buffers already hold that:
-    // it is skipped.  Reading a stamp raised concurrently only makes the
-    // tile run, which is always correct.
-    if (a.skip && !a.always[tile] && a.stamp[tile] + 1u < c.gstep) {
-        if (threadIdx.x == 0) a.partial[tile] = __builtin_inf();  // no dt contribution
-        return;
-    }
-    const double4 *__restrict__ st_in = a.in_x ? a.in_x : a.st[(c.par + a.step) & 1];
-    double4 *__restrict__ st_out = a.out_x ? a.out_x : a.st[(c.par + a.step + 1) & 1];
-    const int nelt = a.ncell;  // geometry stride
+    const double4 *__restrict__ st_in = replay ? a.in_x : a.st[(c.par + a.step) & 1];
+    double4 *__restrict__ st_out = replay ? a.out_x : a.st[(c.par + a.step + 1) & 1];
     const int i = tile * kTile + threadIdx.x;
-    const double dt = c.dt;
-    const double tc = c.tc;
+    const size_t hix = (size_t)b * a.hist_stride + a.step;
+    const double dt = replay ? a.dt_fix[hix] : c.dt;
+    const double tc = replay ? a.tc_fix[hix] : c.tc;
     const KParams &p = a.kp[b];
-    const double g = p.gp, tls = p.tolisec;
+    const double tls = p.tolisec;
     double dti = 1000000.0;
     int isnan_local = 0, wet_local = 0;
 
     if (i < a.inst_end[b]) {
-        const double4 u0 = st_in[i];
-        const int4 nb = a.nbr[i];
-        const double zml = u0.w;
-        // first ring: the three neighbour states, gathered once
-        double4 unb[3];
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const int cc = nb_at(nb, j);
-            unb[j] = cc >= 0 ? st_in[cc] : u0;
-        }
-        double4 dfi = make_double4(0.0, 0.0, 0.0, 0.0);
-        if (IFLUX == 3) dfi = a.dfc[i];
-        double fl[3][3], fn1[3], fn2[3], fln[3];
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            const double n1 = a.geo[(G_N1 + j) * (size_t)nelt + i];
-            const double n2 = a.geo[(G_N2 + j) * (size_t)nelt + i];
-            const double ln = a.geo[(G_LN + j) * (size_t)nelt + i];
-            fn1[j] = n1; fn2[j] = n2; fln[j] = ln;
-            const int code = nb_at(nb, j);
-            double hautl, vitul, vitvl;
-            rot(tls, n1, n2, u0.x, u0.y, u0.z, hautl, vitul, vitvl);
-            double flux[3] = {0.0, 0.0, 0.0};
-            int kv = -1;
-            if (code >= 0) {
-                kv = code;
-            } else {
-                const int cls = (-code) & 3, id = (-code) >> 2;
-                if (cls == 1) {  // inlet (flux_riemann.cuf:115-139)
-                    if (p.inlet == 1) {
-                        flux[0] = hautl * vitul;
-                        flux[1] = n1 * (hautl * (vitul * vitul) + (g * (hautl * hautl)) / 2.0) - ((n2 * hautl) * vitul) * vitvl;
-                        flux[2] = n2 * (hautl * (vitul * vitul) + (g * (hautl * hautl)) / 2.0) + ((n1 * hautl) * vitul) * vitvl;
-                    } else if (p.inlet == 2) {
-                        const double dn = -(inflow_q(p, a.hydro, id - 1, tc) / p.long_entree[id - 1]);
-                        flux[0] = dn;
-                        flux[1] = n1 * ((dn * dn) / (hautl + tls) + (g * (hautl * hautl)) / 2.0);
-                        flux[2] = n2 * ((dn * dn) / (hautl + tls) + (g * (hautl * hautl)) / 2.0);
-                    }
-                } else if (cls == 3) {  // wall (:141-145)
-                    flux[1] = ((n1 * g) * (hautl * hautl)) / 2.0;
-                    flux[2] = ((n2 * g) * (hautl * hautl)) / 2.0;
-                } else if (cls == 2) {
-                    if (p.outlet == 1) {  // fixed level (:147-163)
-                        if (vitul >= 0.0 && vitul / sqrt(g * hautl) < 1.0) {
-                            hautl = dmax(0.0, p.h_sortie[id - 1] - zml);
-                            flux[0] = hautl * vitul;
-                            flux[1] = n1 * (hautl * (vitul * vitul) + (g * (hautl * hautl)) / 2.0) - ((n2 * hautl) * vitul) * vitvl;
-                            flux[2] = n2 * (hautl * (vitul * vitul) + (g * (hautl * hautl)) / 2.0) + ((n1 * hautl) * vitul) * vitvl;
-                        } else {
-                            flux[1] = ((n1 * g) * (hautl * hautl)) / 2.0;
-                            flux[2] = ((n2 * g) * (hautl * hautl)) / 2.0;
-                        }
-                    } else {  // transmissive: Riemann problem with itself (:166-171)
-                        kv = i;
-                    }
-                }
-            }
-            if (kv >= 0) {  // interior face (:166-409)
-                const double4 ur4 = (kv == i) ? u0 : unb[j];
-                const double zmr = ur4.w;
-                const double zij = dmax(zml, zmr);
-                double hijet = dmax(0.0, (hautl + zml) - zij);
-                double hautr, vitur, vitvr;
-                rot(tls, n1, n2, ur4.x, ur4.y, ur4.z, hautr, vitur, vitvr);
-                if (IFLUX == 1) {
-                    double hjiet = dmax(0.0, (ur4.x + zmr) - zij);
-                    if (hijet < tls) { hijet = 0.0; vitul = 0.0; vitvl = 0.0; }
-                    if (hjiet < tls) { hjiet = 0.0; vitur = 0.0; vitvr = 0.0; }
-                    hllc_flux(g, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr, flux);
-                } else {
-                    // reconstructed depths of flux_riemann.cuf:371-394 (the
-                    // values solver_Riemann_waf receives)
-                    double hjiet = dmax(0.0, (hautr + zmr) - zij);
-                    if (hijet < tls) { hijet = 0.0; vitul = 0.0; vitvl = 0.0; }
-                    if (hjiet < tls) { hjiet = 0.0; vitur = 0.0; vitvr = 0.0; }
-                    if (hijet < kEpsWaf && hjiet < kEpsWaf) {
-                        zero_flux(n1, n2, flux);  // dry/dry: no upwind data needed
-                    } else {
-                        const double d_mrl = d4_at(dfi, j);
-                        const int ja = (j + 1) % 3, jb = (j + 2) % 3;
-                        const bool own1 = nb_at(nb, ja) < 0, own2 = nb_at(nb, jb) < 0;
-                        const double4 s1 = own1 ? u0 : unb[ja];
-                        const double4 s2 = own2 ? u0 : unb[jb];
-                        double h1, u1, w1, h2, u2, w2;
-                        rot(tls, n1, n2, s1.x, s1.y, s1.z, h1, u1, w1);
-                        rot(tls, n1, n2, s2.x, s2.y, s2.z, h2, u2, w2);
-                        const int tjv = (nb.w >> (2 * j)) & 3;
-                        const int4 nbk = (kv == i) ? nb : a.nbr[kv];
-                        const int ka = (tjv + 1) % 3, kb = (tjv + 2) % 3;
-                        const int e1 = nb_at(nbk, ka), e2 = nb_at(nbk, kb);
-                        const int kw1 = e1 >= 0 ? e1 : kv, kw2 = e2 >= 0 ? e2 : kv;
-                        const double4 g1 = st_in[kw1], g2 = st_in[kw2];
-                        double g1h, g1u, g1w, g2h, g2u, g2w;
-                        rot(tls, n1, n2, g1.x, g1.y, g1.z, g1h, g1u, g1w);
-                        rot(tls, n1, n2, g2.x, g2.y, g2.z, g2h, g2u, g2w);
-                        double hl_upw, vl_upw, hr_upw, vr_upw, zml_upw, zmr_upw, d_mkvol, d_mkvoisin;
-                        if ((own1 && own2) || (kw1 == kv && kw2 == kv)) {
-                            hl_upw = hautl; vl_upw = vitvl;
-                            hr_upw = hautr; vr_upw = vitvr;
-                            d_mkvol = d_mrl; d_mkvoisin = d_mrl;
-                            zml_upw = zml; zmr_upw = zmr;
-                        } else {
-                            if (own1) {
-                                hl_upw = h2; vl_upw = w2;
-                                d_mkvol = d4_at(dfi, jb);
-                                zml_upw = s2.w;
-                            } else if (own2) {
-                                hl_upw = h1; vl_upw = w1;
-                                d_mkvol = d4_at(dfi, ja);
-                                zml_upw = s1.w;
-                            } else {
-                                const double d1 = d4_at(dfi, ja), d2 = d4_at(dfi, jb);
-                                hl_upw = (d1 * h1 + d2 * h2) / (d1 + d2);
-                                vl_upw = (d1 * w1 + d2 * w2) / (d1 + d2);
-                                zml_upw = (d1 * s1.w + d2 * s2.w) / (d1 + d2);
-                                d_mkvol = (d1 + d2) / 2.0;
-                            }
-                            const double4 dfk = (kv == i) ? dfi : a.dfc[kv];
-                            if (kw1 == kv) {
-                                hr_upw = g2h; vr_upw = g2w;
-                                d_mkvoisin = d4_at(dfk, kb);
-                                zmr_upw = g2.w;
-                            } else if (kw2 == kv) {
-                                hr_upw = g1h; vr_upw = g1w;
-                                d_mkvoisin = d4_at(dfk, ka);
-                                zmr_upw = g1.w;
-                            } else {
-                                const double d1 = d4_at(dfk, ka), d2 = d4_at(dfk, kb);
-                                hr_upw = (d1 * g1h + d2 * g2h) / (d1 + d2);
-                                vr_upw = (d1 * g1w + d2 * g2w) / (d1 + d2);
-                                zmr_upw = (d1 * g1.w + d2 * g2.w) / (d1 + d2);
-                                d_mkvoisin = (d1 + d2) / 2.0;
-                            }
-                        }
-                        // upwind hydrostatic reconstruction (flux_riemann.cuf:373,376,384-400)
-                        hl_upw = dmax(0.0, (hl_upw + zml_upw) - zij);
-                        hr_upw = dmax(0.0, (hr_upw + zmr_upw) - zij);
-                        if (hl_upw < tls) { hl_upw = 0.0; vl_upw = 0.0; }
-                        if (hr_upw < tls) { hr_upw = 0.0; vr_upw = 0.0; }
-                        waf_flux(g, p.ilimiteur, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr,
-                                 hl_upw, vl_upw, hr_upw, vr_upw, d_mrl, d_mkvol, d_mkvoisin, dt, flux);
-                    }
-                }
-            }
-            fl[j][0] = flux[0] * ln;
-            fl[j][1] = flux[1] * ln;
-            fl[j][2] = flux[2] * ln;
-        }
-        // face sum (flux_riemann.cuf:411-426)
-        const double resc0 = (fl[0][0] + fl[1][0]) + fl[2][0];
-        const double resc1 = (fl[0][1] + fl[1][1]) + fl[2][1];
-        const double resc2 = (fl[0][2] + fl[1][2]) + fl[2][2];
-
-        // bed slope source over interior faces (source_terms.cuf:50-100)
-        const double q1 = u0.x, q2 = u0.y, q3 = u0.z;
-        double sb1 = 0.0, sb2 = 0.0;
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            if (nb_at(nb, j) >= 0) {
-                const double zij = dmax(zml, unb[j].w);
-                const double hijet = dmax(0.0, (q1 + zml) - zij);
-                sb1 = sb1 + (((g / 2.0) * (hijet * hijet - q1 * q1)) * fln[j]) * fn1[j];
-                sb2 = sb2 + (((g / 2.0) * (hijet * hijet - q1 * q1)) * fln[j]) * fn2[j];
-            }
-        }
-        const double surf = a.geo[G_SURF * (size_t)nelt + i];
-        double r0, r1, r2;
-        if (FRIC == 0) {
-            // update without friction (finite_volumes_method.cuf:294)
-            r0 = (0.0 - resc0) / surf;
-            r1 = (sb1 - resc1) / surf;
-            r2 = (sb2 - resc2) / surf;
-        } else {
-            // friction (source_terms.cuf:102-209)
-            const double tol = p.tol;
-            const double mann = p.is_override_manning ? p.override_manning : a.manning[i];
-            double sf1 = 0.0, sf2 = 0.0, vitu = 0.0, vitv = 0.0, normvit = 0.0;
-            if (q1 > tls) {
-                vitu = q2 / q1;
-                vitv = q3 / q1;
-                normvit = sqrt(vitu * vitu + vitv * vitv);
-                if (normvit < tol) normvit = tol;
-                const double p13 = cf_pow(q1, -1.0 / 3.0);
-                sf1 = -(((((g * mann) * mann) * vitu) * normvit) * surf * p13);
-                sf2 = -(((((g * mann) * mann) * vitv) * normvit) * surf * p13);
-            }
-            const double x0 = ((0.0 + 0.0) - resc0) / surf;
-            const double x1 = ((sb1 + sf1) - resc1) / surf;
-            const double x2 = ((sb2 + sf2) - resc2) / surf;
-            double a0 = 1.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 1.0, a5 = 0.0, a6 = 0.0, a7 = 0.0, a8 = 1.0;
-            if (FRIC == 2 && normvit > tol && q1 > tls) {
-                const double dtgn2h = ((((dt / 2.0) * g) * mann) * mann) * cf_pow(q1, -4.0 / 3.0);
-                const double f21 = -((((7.0 / 3.0) * dtgn2h) * vitu) * normvit);
-                const double f22 = 1.0 + (dtgn2h * (2.0 * (vitu * vitu) + vitv * vitv)) / normvit;
-                const double f23 = ((dtgn2h * vitu) * vitv) / normvit;
-                const double f31 = -((((7.0 / 3.0) * dtgn2h) * vitv) * normvit);
-                const double f32 = f23;
-                const double f33 = 1.0 + (dtgn2h * (vitu * vitu + 2.0 * (vitv * vitv))) / normvit;
-                const double det = f22 * f33 - f23 * f32;
-                a3 = -(f21 * f33 - f23 * f31) / det;
-                a4 = f33 / det;
-                a5 = -f23 / det;
-                a6 = (f21 * f32 - f22 * f31) / det;
-                a7 = -f32 / det;
-                a8 = f22 / det;
-            } else if (FRIC == 3 && normvit > tol && q1 > tls) {
-                const double beta = -(((((dt * g) * mann) * mann) * normvit) * cf_pow(q1, -4.0 / 3.0));
-                const double det = (1.0 - beta) * (1.0 - beta);
-                a4 = (1.0 - beta) / det;
-                a8 = (1.0 - beta) / det;
-            }
-            if (FRIC == 1) {
-                r0 = x0; r1 = x1; r2 = x2;
-            } else {
-                // res = afm1 * resini (finite_volumes_method.cuf:286-289)
-                r0 = (a0 * x0 + a1 * x1) + a2 * x2;
-                r1 = (a3 * x0 + a4 * x1) + a5 * x2;
-                r2 = (a6 * x0 + a7 * x1) + a8 * x2;
-            }
-        }
-        // update and wet/dry clamp (finite_volumes_method.cuf:297-303)
-        double h = u0.x + dt * r0;
-        double hu = u0.y + dt * r1;
-        double hv = u0.z + dt * r2;
-        if (h <= tls) { h = 0.0; hu = 0.0; hv = 0.0; }
+        const DevLd ld{st_in, a.nbr, a.dfc, a.geo, a.manning, a.ncell};
+        const double zml = st_in[i].w;
+        double o[3], xo[3];
+        cell_update<double, IFLUX, FRIC>(ld, p, a.hydro, i, dt, tc, o, a.xs ? xo : nullptr);
+        const double h = o[0], hu = o[1], hv = o[2];
         st_out[i] = make_double4(h, hu, hv, zml);
+        if (replay) {
+            if (a.xs) {
+                a.xs[i] = xo[0];
+                a.xs[(size_t)a.ncell + i] = xo[1];
+                a.xs[2 * (size_t)a.ncell + i] = xo[2];
+            }
+            return;
+        }
         isnan_local = (h != h) || (hu != hu) || (hv != hv);
         wet_local = (h != 0.0) || isnan_local;
         // cflcond on the new state (cfl_condition.cuf:20-36)
         if (h > tls) {
             const double uu = hu / h, vv = hv / h;
             const double nv = sqrt(uu * uu + vv * vv);
-            const double vmax = dmax(0.01, nv + sqrt(g * fabs(h)));
-            dti = dmin((p.cfl * a.geo[G_COT * (size_t)nelt + i]) / vmax, 1000000.0);
+            const double vmax = dmax(0.01, nv + sqrt(p.gp * fabs(h)));
+            dti = dmin((p.cfl * a.geo[G_COT * (size_t)a.ncell + i]) / vmax, 1000000.0);
         }
         if (p.is_dt_constant == 1) dti = p.constant_dt;
     }
-
+    if (replay) return;
     // per-workgroup dt minimum: wavefront shuffles, then LDS across waves;
     // one plain store per workgroup (reduced by finalize_kernel)
     const double m = wave_min(dti);
@@ -500,46 +268,12 @@ __global__ void geometry_kernel(const int *__restrict__ conn, const double *__re
     const double vx[3] = {coords[3 * s1], coords[3 * s2], coords[3 * s3]};
     const double vy[3] = {coords[3 * s1 + 1], coords[3 * s2 + 1], coords[3 * s3 + 1]};
     const double vz[3] = {coords[3 * s1 + 2], coords[3 * s2 + 2], coords[3 * s3 + 2]};
-    const double x1 = vx[0], y1 = vy[0], x2 = vx[1], y2 = vy[1], x3 = vx[2], y3 = vy[2];
+    double g[G_COUNT], zm;
+    cell_geometry(vx, vy, vz, g, zm);
     const size_t S = (size_t)stride;
-    geo[G_XC * S + i] = ((x1 + x2) + x3) / 3.0;
-    geo[G_YC * S + i] = ((y1 + y2) + y3) / 3.0;
-    {
-        const double ca = sqrt((x1 - x2) * (x1 - x2) + (y1 - y2) * (y1 - y2));
-        const double cb = sqrt((x2 - x3) * (x2 - x3) + (y2 - y3) * (y2 - y3));
-        const double cc = sqrt((x3 - x1) * (x3 - x1) + (y3 - y1) * (y3 - y1));
-        const double pp = ((ca + cb) + cc) / 2.0;
-        const double aire = sqrt(((pp * (pp - ca)) * (pp - cb)) * (pp - cc));
-        const double cot = (0.5 * aire) / pp;
-        geo[G_COT * S + i] = kCote * cot;
-    }
-    {
-        const double aa = sqrt((x2 - x1) * (x2 - x1) + (y2 - y1) * (y2 - y1));
-        const double bb = sqrt((x3 - x2) * (x3 - x2) + (y3 - y2) * (y3 - y2));
-        const double cc = sqrt((x1 - x3) * (x1 - x3) + (y1 - y3) * (y1 - y3));
-        const double s = ((aa + bb) + cc) / 2.0;
-        geo[G_SURF * S + i] = sqrt(((s * (s - aa)) * (s - bb)) * (s - cc));
-        const double xg = ((x1 + x2) + x3) / 3.0, yg = ((y1 + y2) + y3) / 3.0;
-        const double d1 = sqrt((x1 - xg) * (x1 - xg) + (y1 - yg) * (y1 - yg));
-        const double d2 = sqrt((x2 - xg) * (x2 - xg) + (y2 - yg) * (y2 - yg));
-        const double d3 = sqrt((x3 - xg) * (x3 - xg) + (y3 - yg) * (y3 - yg));
-        zm_out[i] = ((d1 * vz[0] + d2 * vz[1]) + d3 * vz[2]) / ((d1 + d2) + d3);
-    }
-    for (int j = 0; j < 3; j++) {
-        const int c1 = j, c2 = (j + 1) % 3, c3 = (j + 2) % 3;
-        const double ax = vx[c1], ay = vy[c1], bx = vx[c2], by = vy[c2], cx = vx[c3], cy = vy[c3];
-        const double l2 = sqrt((bx - ax) * (bx - ax) + (by - ay) * (by - ay));
-        double n1 = (by - ay) / l2, n2 = (ax - bx) / l2;
-        const double orient = n1 * ((ax + bx) / 2.0 - ((ax + bx) + cx) / 3.0) +
-                              n2 * ((ay + by) / 2.0 - ((ay + by) + cy) / 3.0);
-        if (orient < 0.0) {
-            n1 = (ay - by) / l2;
-            n2 = (bx - ax) / l2;
-        }
-        geo[(G_N1 + j) * S + i] = n1;
-        geo[(G_N2 + j) * S + i] = n2;
-        geo[(G_LN + j) * S + i] = l2;
-    }
+#pragma unroll
+    for (int k = 0; k < G_COUNT; k++) geo[k * S + i] = g[k];
+    zm_out[i] = zm;
 }
 
 // Centroid distance across each face (0 across a boundary).
@@ -552,7 +286,7 @@ __global__ void face_dist_kernel(const int4 *__restrict__ nbr, const double *__r
     const double xi = geo[G_XC * S + i], yi = geo[G_YC * S + i];
     double d[3];
     for (int j = 0; j < 3; j++) {
-        const int k = nb_at(nb, j);
+        const int k = nb4_at(nb, j);
         const int kk = k >= 0 ? k : i;
         const double xk = geo[G_XC * S + kk], yk = geo[G_YC * S + kk];
         d[j] = sqrt((xi - xk) * (xi - xk) + (yi - yk) * (yi - yk));
@@ -585,6 +319,13 @@ __global__ void unpack_state_kernel(const double4 *__restrict__ st, const int *_
     }
 }
 
+// per-cell scalar in the caller's order -> internal cell order
+__global__ void gather_scalar_kernel(const double *__restrict__ src, const int *__restrict__ perm, int cell0, int n,
+                                     double *__restrict__ dst) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) dst[cell0 + l] = src[perm[cell0 + l]];
+}
+
 __global__ void fill_u32_kernel(unsigned *p, int n, unsigned v) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
@@ -596,33 +337,6 @@ __global__ void pow_kernel(const double *__restrict__ x, double y, int n, double
 }
 
 }  // namespace
-
-#define HIPCHK(x)                                                                          \
-    do {                                                                                   \
-        hipError_t e_ = (x);                                                               \
-        if (e_ != hipSuccess) {                                                            \
-            fprintf(stderr, "cuteflow_hip: %s failed: %s\n", #x, hipGetErrorString(e_));  \
-            return CF_ERR_HIP;                                                             \
-        }                                                                                  \
-    } while (0)
-
-static int grid_of(int n, int b) { return (n + b - 1) / b; }
-
-template <typename T>
-static void dfree(T *&ptr) {
-    if (ptr) (void)hipFree((void *)ptr);
-    ptr = nullptr;
-}
-
-template <typename T>
-static bool dalloc(T *&p, size_t count) {
-    return hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T)) == hipSuccess;
-}
-
-template <typename T>
-static bool h2d(T *dst, const T *src, size_t count, hipStream_t s) {
-    return count == 0 || hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyHostToDevice, s) == hipSuccess;
-}
 
 static KParams make_kparams(const cf_params &p, int nelt, int hydro_off) {
     KParams k{};
@@ -874,6 +588,7 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
         const char *env = getenv("CUTEFLOW_NO_GRAPHS");
         h->use_graphs = env && atoi(env) ? 0 : 1;
     }
+    h->hnbr.swap(nbr);
     *out = h;
     return CF_OK;
 }
@@ -885,6 +600,7 @@ extern "C" int cf_create(const cf_mesh *mesh, const cf_params *params, cf_handle
 extern "C" void cf_destroy(cf_handle *h) {
     if (!h) return;
     (void)hipStreamSynchronize(h->stream);
+    adj_free(h);
     destroy_buffers(h);
     if (h->own_stream && h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -955,7 +671,7 @@ extern "C" int cf_initial_dt(cf_handle *h, double *dt) {
 extern "C" int cf_get_time(cf_handle *h, double *tc, double *dt) { return cf_batch_get_time(h, 0, tc, dt); }
 
 template <int IFLUX, int FRIC>
-static void launch_step(const StepArgs &a, int grid, hipStream_t s) {
+static void launch_step_t(const StepArgs &a, int grid, hipStream_t s) {
     hipLaunchKernelGGL((step_kernel<IFLUX, FRIC>), dim3(grid), dim3(kTile), 0, s, a);
 }
 
@@ -963,12 +679,27 @@ typedef void (*step_launcher)(const StepArgs &, int, hipStream_t);
 
 static step_launcher pick_launcher(int iflux, int fric) {
     static const step_launcher tab[2][4] = {
-        {launch_step<1, 0>, launch_step<1, 1>, launch_step<1, 2>, launch_step<1, 3>},
-        {launch_step<3, 0>, launch_step<3, 1>, launch_step<3, 2>, launch_step<3, 3>}};
+        {launch_step_t<1, 0>, launch_step_t<1, 1>, launch_step_t<1, 2>, launch_step_t<1, 3>},
+        {launch_step_t<3, 0>, launch_step_t<3, 1>, launch_step_t<3, 2>, launch_step_t<3, 3>}};
     return tab[iflux == 3 ? 1 : 0][fric];
 }
 
-static StepArgs base_args(cf_handle *h) {
+void cf::launch_step(const StepArgs &a, int iflux, int fric, int grid, hipStream_t s) {
+    pick_launcher(iflux, fric)(a, grid, s);
+}
+
+void cf::launch_finalize(cf_handle *h, double *dt_hist, double *tc_hist, int *ran, int stride, int step,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(finalize_kernel, dim3(h->ninst), dim3(1024), 0, s, h->ctrl, h->d_kp, h->partial, h->inst_tile,
+                       dt_hist, tc_hist, ran, stride, step);
+}
+
+void cf::launch_run_init(cf_handle *h, double *dt_hist, double *tc_hist, int stride, hipStream_t s) {
+    hipLaunchKernelGGL(run_init_kernel, dim3(grid_of(h->ninst, 256)), dim3(256), 0, s, h->ctrl, h->d_kp, dt_hist,
+                       tc_hist, stride, h->ninst);
+}
+
+StepArgs cf::base_args(cf_handle *h) {
     StepArgs a{};
     a.st[0] = h->st[0];
     a.st[1] = h->st[1];
@@ -990,8 +721,11 @@ static StepArgs base_args(cf_handle *h) {
     a.tile_inst = h->tile_inst;
     a.inst_end = h->inst_end;
     a.inst_tile = h->inst_tile;
+    a.dt_fix = nullptr;
+    a.tc_fix = nullptr;
     a.in_x = nullptr;
     a.out_x = nullptr;
+    a.xs = nullptr;
     a.ncell = h->ncell;
     a.hist_stride = h->cap + 1;
     a.skip = h->skip;
@@ -1000,15 +734,13 @@ static StepArgs base_args(cf_handle *h) {
 
 // The launches of a run of nsteps steps, on stream s.
 static void enqueue_run(cf_handle *h, int nsteps, hipStream_t s) {
-    hipLaunchKernelGGL(run_init_kernel, dim3(grid_of(h->ninst, 256)), dim3(256), 0, s, h->ctrl, h->d_kp, h->dt_hist,
-                       h->tc_hist, h->cap + 1, h->ninst);
+    launch_run_init(h, h->dt_hist, h->tc_hist, h->cap + 1, s);
     StepArgs a = base_args(h);
     step_launcher L = pick_launcher(h->iflux, h->fric);
     for (int k = 0; k < nsteps; k++) {
         a.step = k;
         L(a, h->ntiles, s);
-        hipLaunchKernelGGL(finalize_kernel, dim3(h->ninst), dim3(1024), 0, s, h->ctrl, h->d_kp, h->partial,
-                           h->inst_tile, h->dt_hist, h->tc_hist, h->ran, h->cap + 1, k);
+        launch_finalize(h, h->dt_hist, h->tc_hist, h->ran, h->cap + 1, k, s);
     }
 }
 
@@ -1032,7 +764,7 @@ static int run_graph(cf_handle *h, int nsteps) {
         (void)hipGraphExecDestroy(h->graphs.front().exec);
         h->graphs.erase(h->graphs.begin());
     }
-    h->graphs.push_back({nsteps, {}, exec});
+    h->graphs.push_back({nsteps, exec});
     HIPCHK(hipGraphLaunch(exec, h->stream));
     return CF_OK;
 }
@@ -1157,10 +889,12 @@ extern "C" int cf_set_hydrograph(cf_handle *h, int inlet, int npts, const double
 
 extern "C" int cf_batch_set_manning(cf_handle *h, int b, const double *n) {
     if (bad_inst(h, b) || !n) return CF_ERR_ARG;
-    const size_t N = (size_t)h->nelt[b];
-    std::vector<double> m(N);
-    for (size_t i = 0; i < N; i++) m[i] = n[h->hperm[b][i]];
-    HIPCHK(hipMemcpyAsync(h->manning + h->cell_off[b], m.data(), N * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    const int N = h->nelt[b];
+    // host or device pointer: staged, then gathered into the internal order
+    HIPCHK(hipMemcpyAsync(h->stage, n, (size_t)N * sizeof(double), hipMemcpyDefault, h->stream));
+    hipLaunchKernelGGL(gather_scalar_kernel, dim3(grid_of(N, 256)), dim3(256), 0, h->stream, h->stage, h->perm,
+                       h->cell_off[b], N, h->manning);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
     return CF_OK;
 }

@@ -143,7 +143,8 @@ int cf_last_run_stats(cf_handle *h, long long *tiles_run, long long *tiles_total
 /* Inflow discharges debitglob(1:n) and hydrographs. */
 int cf_set_inflow(cf_handle *h, const double *debitglob, int n);
 int cf_set_hydrograph(cf_handle *h, int inlet, int npts, const double *t, const double *q);
-/* Per-cell Manning n (manning_nelt) */
+/* Per-cell Manning n (manning_nelt, nelt doubles in the caller's order; host
+ * or device pointer). */
 int cf_set_manning(cf_handle *h, const double *n);
 
 /* Geometry computed on the device: surf, zm, cotemin_arr (each nelt, may be
@@ -187,6 +188,48 @@ int cf_batch_set_hydrograph(cf_handle *h, int inst, int inlet, int npts, const d
 int cf_batch_set_manning(cf_handle *h, int inst, const double *n);
 int cf_batch_get_geometry(cf_handle *h, int inst, double *surf, double *zm, double *cotemin_arr,
                           double *cotemin);
+
+/* ---- Discrete adjoint and tangent-linear model --------------------------
+ *
+ * No reference counterpart: the reference calibrates Manning n with
+ * repeated forward runs (Calibration_manning/).  These entry points give
+ * exact derivatives of the discrete time step (wet/dry switches, wave-speed
+ * cases, limiter regions and dt frozen as taken in the forward) of the
+ * water levels h + zm at gauge cells and of the per-cell maximum depth, with
+ * respect to per-cell Manning n and the initial state.  Per-cell arrays
+ * (nelt or 3*nelt doubles, caller's cell order) may be host or device
+ * pointers; sample arrays (nsamp x ngauges, row per sample) are host
+ * pointers.  Requires is_override_manning == 0 in every instance. */
+
+/* Gauge cells of instance inst (0-based, caller's order). */
+int cf_adj_set_gauges(cf_handle *h, int inst, int n, const int *cells);
+/* Recorded forward: advances every instance by exactly nsteps steps (an
+ * instance that stops earlier is an error) from its current state, keeping
+ * dt and tc of every step, a checkpoint every checkpoint_every steps, the
+ * gauge levels after every sample_every-th step (nsamp = nsteps /
+ * sample_every samples) and, when want_hmax, the per-cell maximum depth over
+ * those samples. */
+int cf_adj_forward(cf_handle *h, int nsteps, int sample_every, int checkpoint_every, int want_hmax);
+int cf_adj_num_samples(cf_handle *h);
+int cf_adj_get_samples(cf_handle *h, int inst, double *eta, double *hmax);
+/* Cotangents of the outputs of instance inst: gauge levels (nsamp x ng),
+ * hmax (nelt) and the final state (3*nelt); NULL = 0. */
+int cf_adj_set_seed(cf_handle *h, int inst, const double *g_eta, const double *g_hmax, const double *g_state);
+/* Reverse sweep over the recorded run: segments of checkpoint_every steps
+ * replayed from their checkpoints, then reversed step by step. */
+int cf_adj_backward(cf_handle *h);
+/* Gradient of instance inst with respect to per-cell Manning n (nelt) and
+ * to the initial state (3*nelt); either may be NULL. */
+int cf_adj_get_gradient(cf_handle *h, int inst, double *g_manning, double *g_state0);
+/* Tangent-linear run over the recorded run along (dn, dU0) (NULL = 0),
+ * from the recorded initial state with the recorded dt and tc at the current
+ * Manning n: the gauge levels (eta) and final state (state) of that run (at
+ * the recorded n: the recorded forward's bits) and their directional
+ * derivatives, and that of hmax. */
+int cf_tlm_set_direction(cf_handle *h, int inst, const double *dn, const double *dU0);
+int cf_tlm_run(cf_handle *h);
+int cf_tlm_get(cf_handle *h, int inst, double *eta, double *d_eta, double *d_hmax, double *d_state,
+               double *state);
 
 /* Elementwise pow used by the friction kernel, evaluated on the device for
  * n values (x > 0): lets tests compare the device pow bit-for-bit. */

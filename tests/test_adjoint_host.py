@@ -1,0 +1,207 @@
+"""CPU checks of the derivative code of the time step (cuteflow_amd/csrc/
+cf_device.h and cf_cell.h) through a g++ build of the same sources
+(tests/adjoint_host/host_adjoint.cpp):
+
+- the host build of the templated forward body is bit-identical to the C
+  oracle (an independent restatement of the reference kernels);
+- face level: the hand-written reverse of the HLLC and WAF fluxes against the
+  dual-number forward mode, <w, J v> = <J^T w, v>, on random states covering
+  the wave-speed cases and every limiter region;
+- mesh level: the reverse sweep against the tangent-linear sweep over many
+  steps (dot-product test) for HLLC and WAF, every friction mode and the
+  boundary types, and the tangent-linear model against central finite
+  differences of the forward.
+"""
+import numpy as np
+import pytest
+
+from adjoint_host.harness import Host, _DP, _p, build
+from oracle import oracle as O
+
+from cases import dam_case, lake_case, river_case
+
+
+@pytest.fixture(scope="module")
+def hs(tmp_path_factory):
+    return build(tmp_path_factory.mktemp("hs"))
+
+
+# ---------------------------------------------------------------------------
+# forward: host build of the templated cell body == oracle, bit for bit
+
+@pytest.mark.parametrize("case", ["dam", "river_waf", "river_hllc_f2"])
+def test_host_forward_matches_oracle_bits(hs, case):
+    if case == "dam":
+        m, p, U0 = dam_case(12)
+    elif case == "river_waf":
+        m, p, U0 = river_case(24, 6)
+    else:
+        m, p, U0 = river_case(24, 6, iflux=1, fricimplic=2)
+    ref = O.run(O.Prepared(m), p, U0, 60)
+    H = Host(hs, m, p)
+    U, hist, done = H.forward(U0, 60)
+    H.close()
+    assert done == ref["steps"] == 60
+    assert np.array_equal(hist.view(np.int64), ref["dt_hist"].view(np.int64))
+    assert np.array_equal(U.view(np.int64), ref["U"].view(np.int64))
+
+
+# ---------------------------------------------------------------------------
+# face level: reverse vs forward mode
+
+def _face_cases(rng, iflux, n):
+    """Random face-frame states: wet/wet with sub- and supercritical
+    velocities of both signs, one side dry, and upwind depths around the
+    face depths (so that every limiter ratio region occurs)."""
+    for _ in range(n):
+        hl, hr = rng.uniform(0.05, 5.0, 2)
+        if rng.random() < 0.15:
+            hl = 0.0
+        elif rng.random() < 0.15:
+            hr = 0.0
+        ul, ur = rng.normal(0.0, 2.5, 2)
+        vl, vr = rng.normal(0.0, 1.0, 2)
+        hlu = max(0.0, hl + rng.normal(0.0, 0.6))
+        hru = max(0.0, hr + rng.normal(0.0, 0.6))
+        vlu, vru = vl + rng.normal(0.0, 0.5), vr + rng.normal(0.0, 0.5)
+        x = np.array([hl, ul, vl, hr, ur, vr, hlu, vlu, hru, vru])
+        if iflux == 1:
+            x[6:] = 0.0
+        yield x
+
+
+@pytest.mark.parametrize("iflux,ilim", [(1, 0), (3, 0), (3, 1), (3, 2), (3, 3), (3, 4)])
+def test_face_reverse_matches_forward_mode(hs, iflux, ilim):
+    rng = np.random.default_rng(100 + 10 * iflux + ilim)
+    out = np.zeros(5)
+    worst = 0.0
+    for x in _face_cases(rng, iflux, 400):
+        th = rng.uniform(0, 2 * np.pi)
+        n1, n2 = np.cos(th), np.sin(th)
+        d = rng.normal(size=10)
+        if iflux == 1:
+            d[6:] = 0.0
+        # directions only along variables that are not frozen zeros
+        d[:3] *= x[0] > 0
+        d[3:6] *= x[3] > 0
+        d[6:8] *= x[6] > 0
+        d[8:] *= x[8] > 0
+        w = rng.normal(size=3)
+        dm, dk, dv = rng.uniform(1.0, 3.0, 3)
+        hs.hs_face_dot(iflux, ilim, 9.81, n1, n2, _p(x), _p(d), _p(w), dm, dk, dv, rng.uniform(0.01, 0.3),
+                       out.ctypes.data_as(_DP))
+        fwd, rev = out[0], out[1]
+        scale = max(1.0, abs(fwd), np.max(np.abs(out[2:])))
+        worst = max(worst, abs(fwd - rev) / scale)
+    assert worst < 1e-12, worst
+
+
+# ---------------------------------------------------------------------------
+# mesh level: adjoint vs tangent-linear vs finite differences
+
+def _mesh_case(name):
+    if name == "dam_hllc":
+        m, p, U0 = dam_case(10)
+    elif name == "dam_hllc_transm_f1":
+        m, p, U0 = dam_case(10, seed=4)
+        p.friction, p.fricimplic = 1, 1
+        m.manning[:] = 0.03
+    elif name == "dam_hllc_f0":
+        # explicit friction (fricimplic 0) is unstable on the thin layers of
+        # the river reach; the dam break stays stable
+        m, p, U0 = dam_case(10, seed=2)
+        p.friction, p.fricimplic = 1, 0
+        m.manning[:] = 0.03
+    elif name == "river_hllc_f1":
+        m, p, U0 = river_case(20, 6, iflux=1, fricimplic=1)
+    elif name == "river_hllc_f2":
+        m, p, U0 = river_case(20, 6, iflux=1, fricimplic=2)
+    elif name.startswith("river_waf_l"):
+        m, p, U0 = river_case(20, 6, ilimiteur=int(name[-1]))
+    elif name == "river_hllc_transm":
+        # transmissive outlet (a Riemann problem of the cell with itself);
+        # with WAF the zero centroid distance of that face makes the
+        # reference's flux 0/0, so the outlet is tested with HLLC
+        m, p, U0 = river_case(20, 6, iflux=1, outlet="transm")
+    elif name == "river_waf_f2":
+        m, p, U0 = river_case(20, 6, fricimplic=2)
+    elif name == "lake_hllc":
+        # lake at rest over a rough bed with dry islands (walls).  With WAF
+        # this state is ill-conditioned for derivatives: the two face depths
+        # agree to the last bit, so the limiter ratio du / (dl + 1e-16)
+        # (flux_riemann.cuf:461-463) has derivatives of order 1e16 that
+        # cancel only in exact arithmetic; HLLC has no such ratio.
+        m, p = lake_case(8, iflux=1)
+        zm = O.Prepared(m).zm
+        U0 = np.zeros((m.nelt, 3))
+        U0[:, 0] = np.maximum(0.0, 0.2 - zm)
+    else:
+        raise KeyError(name)
+    return m, p, U0
+
+
+MESH_CASES = ["dam_hllc", "dam_hllc_transm_f1", "dam_hllc_f0", "river_hllc_f1", "river_hllc_f2",
+              "river_waf_l1", "river_waf_l2", "river_waf_l3", "river_waf_l4", "river_hllc_transm",
+              "river_waf_f2", "lake_hllc"]
+
+
+@pytest.mark.parametrize("name", MESH_CASES)
+def test_adjoint_dot_tangent_linear(hs, name):
+    """<lam, dU_T> (tangent-linear) == <g_n, dn> + <g_U0, dU0> (reverse)."""
+    m, p, U0 = _mesh_case(name)
+    H = Host(hs, m, p)
+    nsteps = 40
+    U, hist, done = H.forward(U0, nsteps)
+    assert done == nsteps
+    rng = np.random.default_rng(7)
+    lam = rng.normal(size=(m.nelt, 3))
+    dn = rng.normal(size=m.nelt) * 0.01
+    dU0 = rng.normal(size=(m.nelt, 3)) * 0.01
+    dU0[U0[:, 0] <= p.tolisec] = 0.0  # dry cells: the threshold freezes them
+    dU, Ut = H.tlm(dn, dU0)
+    assert np.array_equal(Ut.view(np.int64), U.view(np.int64))  # TLM values = forward bits
+    gn, gU, bad = H.adjoint(lam)
+    H.close()
+    assert bad == 0
+    lhs = float(np.sum(lam * dU))
+    rhs = float(np.sum(gn * dn) + np.sum(gU * dU0))
+    scale = float(np.sum(np.abs(lam * dU))) + 1e-300
+    assert abs(lhs - rhs) <= 1e-11 * scale, (lhs, rhs, scale)
+    if p.friction and np.any(U0[:, 1:] != 0.0) or name.startswith("river"):
+        assert np.any(gn != 0.0)  # flowing water: friction depends on n
+
+
+@pytest.mark.parametrize("name", ["river_hllc_f1", "river_waf_l1", "river_waf_f2"])
+def test_tangent_linear_matches_central_differences(hs, name):
+    """Directional derivative of a smooth functional of the final state
+    along a Manning perturbation: TLM vs central differences (step 1e-6
+    relative to n) of the forward with the recorded dt sequence, i.e. of the
+    map the derivative code differentiates (dt frozen).  The forward at the
+    perturbed n with the recorded dt is the value part of a tangent-linear
+    run."""
+    m, p, U0 = _mesh_case(name)
+    H = Host(hs, m, p)
+    nsteps = 30
+    n0 = m.manning.copy()
+    rng = np.random.default_rng(3)
+    dn = n0 * rng.uniform(-1.0, 1.0, m.nelt)
+    w = rng.uniform(0.5, 1.5, (m.nelt, 3))
+    _, _, done = H.forward(U0, nsteps)
+    assert done == nsteps
+    zero_n, zero_u = np.zeros(m.nelt), np.zeros_like(U0)
+    dU, _ = H.tlm(dn, zero_u)
+    d_tlm = float(np.sum(w * dU))
+    # The WAF limiters are piecewise (flux_riemann.cuf:465-524): a central
+    # difference whose step straddles a kink of the limiter of some face at
+    # some step measures a different slope, so the best of three steps is
+    # held to the bar.
+    errs = []
+    for eps in (1e-6, 1e-7, 1e-8):
+        H.set_manning(n0 + eps * dn)
+        _, Up = H.tlm(zero_n, zero_u)
+        H.set_manning(n0 - eps * dn)
+        _, Um = H.tlm(zero_n, zero_u)
+        d_fd = float(np.sum(w * (Up - Um))) / (2 * eps)
+        errs.append(abs(d_fd - d_tlm) / abs(d_tlm))
+    H.close()
+    assert min(errs) <= 1e-6, errs
