@@ -19,6 +19,8 @@ CF_MAX_HYDRO = 64
 CF_OPT_KEEP_ORDER = 1
 CF_OPT_NO_TILE_SKIP = 2
 
+CF_ADJ_FREEZE_LIMITERS = 1
+
 CF_OK = 0
 CF_ERR_ARG = -1
 CF_ERR_HIP = -2
@@ -111,6 +113,7 @@ SIGNATURES = {
     "cf_batch_set_hydrograph": (C.c_int, [_H, C.c_int, C.c_int, C.c_int, _DP, _DP]),
     "cf_batch_set_manning": (C.c_int, [_H, C.c_int, C.c_void_p]),
     "cf_batch_get_geometry": (C.c_int, [_H, C.c_int, _DP, _DP, _DP, _DP]),
+    "cf_adj_set_flags": (C.c_int, [_H, C.c_int]),
     "cf_adj_set_gauges": (C.c_int, [_H, C.c_int, C.c_int, _IP]),
     "cf_adj_forward": (C.c_int, [_H, C.c_int, C.c_int, C.c_int, C.c_int]),
     "cf_adj_num_samples": (C.c_int, [_H]),

@@ -45,11 +45,14 @@ class AdjointRun:
     """Recorded forward runs and reverse sweeps on a Solver or BatchSolver.
 
     gauges: for a Solver one array of cell indices (caller's order), for a
-    BatchSolver one array per instance.
+    BatchSolver one array per instance.  freeze_limiters: derivatives with the
+    WAF limiter values held at their forward values (CF_ADJ_FREEZE_LIMITERS);
+    the exact derivative of the WAF step grows without bound over long runs
+    through water at rest or uniform flow.
     """
 
     def __init__(self, solver, gauges, sample_every: int = 10, checkpoint_every: int = 100,
-                 want_hmax: bool = False):
+                 want_hmax: bool = False, freeze_limiters: bool = False):
         self.solver = solver
         self.lib = solver.lib
         self.h = solver.h
@@ -61,6 +64,8 @@ class AdjointRun:
         if len(gauges) != self.ninst:
             raise ValueError("one gauge list per instance")
         self.gauges = [np.ascontiguousarray(g, np.int32) for g in gauges]
+        _lib.check(self.lib.cf_adj_set_flags(self.h, _lib.CF_ADJ_FREEZE_LIMITERS if freeze_limiters else 0),
+                   "cf_adj_set_flags")
         for b, g in enumerate(self.gauges):
             _lib.check(self.lib.cf_adj_set_gauges(self.h, b, len(g), _lib.iptr(g)), "cf_adj_set_gauges")
         self.sample_every = int(sample_every)

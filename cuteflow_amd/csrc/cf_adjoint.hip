@@ -68,6 +68,7 @@ struct AdjState {
     double4 *R2 = nullptr;      // 6 x ncell: face k, slot s at 2k + s
     int *r2_off = nullptr, *r2_idx = nullptr;
     double *gn = nullptr;
+    unsigned char *tile_ran = nullptr;  // nsteps x ntiles: tile ran in the recorded run
     int lam_cur = 0;
     // tangent-linear run
     double4 *tst[2] = {nullptr, nullptr}, *tdst[2] = {nullptr, nullptr};
@@ -99,6 +100,7 @@ struct RevArgs {
     double *gn;
     const double *dt;
     const double *tc;
+    const unsigned char *ran;  // this step's row of tile activity
     int hstride;
     int step;
     int ncell;
@@ -111,7 +113,9 @@ __global__ __launch_bounds__(kTile) void rev_kernel(RevArgs r) {
     const int tile = blockIdx.x;
     const int b = r.tile_inst[tile];
     const int i = tile * kTile + threadIdx.x;
-    if (i >= r.inst_end[b]) return;
+    // a tile skipped in the recorded run holds dry cells with dry
+    // neighbours: every contribution of its reverse is zero
+    if (i >= r.inst_end[b] || !r.ran[tile]) return;
     const DevLd ld{r.st, r.nbr, r.dfc, r.geo, r.manning, r.ncell};
     const size_t hix = (size_t)b * r.hstride + r.step;
     const double x[3] = {r.xs[i], r.xs[(size_t)r.ncell + i], r.xs[2 * (size_t)r.ncell + i]};
@@ -141,28 +145,36 @@ __device__ __forceinline__ void acc4(double4 &a, const double4 &b) {
 }
 
 // Cotangent of each cell's old state: the sum of the contributions the
-// reverse kernel addressed to it, in a fixed order; plus the hmax seed of
-// sample `sidx` (when hseed) and the final-state seed (when gstate).
+// reverse kernel addressed to it, in a fixed order (contributions of tiles
+// that did not run at this step are zero and not read), plus the hmax seed
+// of sample `sidx`.
 template <bool R2ON>
 __global__ void gather_kernel(const int4 *__restrict__ nbr, const double4 *__restrict__ R1,
                               const double4 *__restrict__ R2, const int *__restrict__ r2_off,
                               const int *__restrict__ r2_idx, const int *__restrict__ tile_inst,
-                              const int *__restrict__ inst_end, const int *__restrict__ argmax,
-                              const double *__restrict__ ghmax, int sidx, double4 *__restrict__ lam_out) {
+                              const int *__restrict__ inst_end, const unsigned char *__restrict__ ran,
+                              const int *__restrict__ argmax, const double *__restrict__ ghmax, int sidx,
+                              double4 *__restrict__ lam_out) {
     const int tile = blockIdx.x;
     const int b = tile_inst[tile];
     const int c = tile * kTile + threadIdx.x;
     if (c >= inst_end[b]) return;
-    double4 acc = R1[4 * (size_t)c + 3];
-    const int4 nb = nbr[c];
-    const int nbv[3] = {nb.x, nb.y, nb.z};
+    double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (ran[tile]) {
+        acc = R1[4 * (size_t)c + 3];
+        const int4 nb = nbr[c];
+        const int nbv[3] = {nb.x, nb.y, nb.z};
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const int i = nbv[k];
-        if (i >= 0) acc4(acc, R1[4 * (size_t)i + ((nb.w >> (2 * k)) & 3)]);
+        for (int k = 0; k < 3; k++) {
+            const int i = nbv[k];
+            if (i >= 0 && ran[i / kTile]) acc4(acc, R1[4 * (size_t)i + ((nb.w >> (2 * k)) & 3)]);
+        }
+        if (R2ON)
+            for (int e = r2_off[c]; e < r2_off[c + 1]; e++) {
+                const int src = r2_idx[e];
+                if (ran[(src / 6) / kTile]) acc4(acc, R2[src]);
+            }
     }
-    if (R2ON)
-        for (int e = r2_off[c]; e < r2_off[c + 1]; e++) acc4(acc, R2[r2_idx[e]]);
     if (argmax && argmax[c] == sidx) acc.x += ghmax[c];
     acc.w = 0.0;
     lam_out[c] = acc;
@@ -381,7 +393,7 @@ void cf::adj_free(cf_handle *h) {
     dfree(A->dt); dfree(A->tc); dfree(A->ran);
     dfree(A->ckpt); dfree(A->seg); dfree(A->xs);
     dfree(A->lam[0]); dfree(A->lam[1]); dfree(A->gstate);
-    dfree(A->R1); dfree(A->R2); dfree(A->r2_off); dfree(A->r2_idx); dfree(A->gn);
+    dfree(A->R1); dfree(A->R2); dfree(A->r2_off); dfree(A->r2_idx); dfree(A->gn); dfree(A->tile_ran);
     dfree(A->tst[0]); dfree(A->tst[1]); dfree(A->tdst[0]); dfree(A->tdst[1]); dfree(A->tdir); dfree(A->dman);
     delete A;
     h->adj = nullptr;
@@ -457,6 +469,16 @@ static int build_r2(cf_handle *h, AdjState *A) {
     return CF_OK;
 }
 
+extern "C" int cf_adj_set_flags(cf_handle *h, int flags) {
+    if (!h || h->pending >= 0 || (flags & ~CF_ADJ_FREEZE_LIMITERS)) return CF_ERR_ARG;
+    h->ad_flags = flags;
+    for (int b = 0; b < h->ninst; b++) h->kp[b].ad_flags = flags;
+    HIPCHK(hipMemcpyAsync(h->d_kp, h->kp.data(), h->kp.size() * sizeof(KParams), hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (h->adj) h->adj->tlm_ready = false;
+    return CF_OK;
+}
+
 extern "C" int cf_adj_set_gauges(cf_handle *h, int inst, int n, const int *cells) {
     if (!h || inst < 0 || inst >= h->ninst || n < 0 || (n > 0 && !cells) || h->pending >= 0) return CF_ERR_ARG;
     AdjState *A = adj_of(h);
@@ -497,10 +519,10 @@ static int adj_alloc(cf_handle *h, AdjState *A) {
     const size_t N = (size_t)h->ncell, B = (size_t)h->ninst;
     const size_t S = (size_t)A->nsteps + 1;
     if (A->alloc_steps != A->nsteps || A->alloc_K != A->K) {
-        dfree(A->dt); dfree(A->tc); dfree(A->ran); dfree(A->ckpt); dfree(A->seg); dfree(A->xs);
+        dfree(A->dt); dfree(A->tc); dfree(A->ran); dfree(A->ckpt); dfree(A->seg); dfree(A->xs); dfree(A->tile_ran);
         if (!dalloc(A->dt, B * S) || !dalloc(A->tc, B * S) || !dalloc(A->ran, B * S) ||
             !dalloc(A->ckpt, (size_t)A->nseg * N) || !dalloc(A->seg, (size_t)A->K * N) ||
-            !dalloc(A->xs, (size_t)A->K * 3 * N))
+            !dalloc(A->xs, (size_t)A->K * 3 * N) || !dalloc(A->tile_ran, (size_t)A->nsteps * h->ntiles))
             return CF_ERR_NOMEM;
         A->alloc_steps = A->nsteps;
         A->alloc_K = A->K;
@@ -556,6 +578,7 @@ extern "C" int cf_adj_forward(cf_handle *h, int nsteps, int sample_every, int ch
     a.dt_hist = A->dt;
     a.tc_hist = A->tc;
     a.ran = A->ran;
+    a.tile_ran = A->tile_ran;
     a.hist_stride = S;
     for (int k = 0; k < nsteps; k++) {
         a.step = k;
@@ -732,6 +755,7 @@ extern "C" int cf_adj_backward(cf_handle *h) {
     a.tc_fix = A->tc;
     a.hist_stride = S;
     a.skip = 0;
+    a.tile_ran = A->tile_ran;
     RevArgs r{};
     r.nbr = h->nbr;
     r.dfc = h->dfc;
@@ -764,14 +788,17 @@ extern "C" int cf_adj_backward(cf_handle *h) {
             r.st = (k == k0) ? ck : A->seg + (size_t)(k - k0 - 1) * N;
             r.xs = A->xs + (size_t)(k - k0) * 3 * N;
             r.lam = A->lam[cur];
+            r.ran = A->tile_ran + (size_t)k * h->ntiles;
             launch_rev(r, h->iflux, h->fric, h->ntiles, s);
             const int sidx = (k >= 1 && k % A->every == 0) ? k / A->every - 1 : -1;
             if (h->iflux == 3)
                 hipLaunchKernelGGL(gather_kernel<true>, dim3(h->ntiles), dim3(kTile), 0, s, h->nbr, A->R1, A->R2,
-                                   A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, am, A->ghmax, sidx, A->lam[1 - cur]);
+                                   A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, r.ran, am, A->ghmax, sidx,
+                                   A->lam[1 - cur]);
             else
                 hipLaunchKernelGGL(gather_kernel<false>, dim3(h->ntiles), dim3(kTile), 0, s, h->nbr, A->R1, A->R2,
-                                   A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, am, A->ghmax, sidx, A->lam[1 - cur]);
+                                   A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, r.ran, am, A->ghmax, sidx,
+                                   A->lam[1 - cur]);
             if (sidx >= 0 && A->gtotal)
                 hipLaunchKernelGGL(gauge_seed_kernel, dim3(grid_of(h->ninst, 64)), dim3(64), 0, s, A->d_gcell,
                                    A->d_goff, A->geta, A->gtotal, sidx, h->ninst, A->lam[1 - cur]);

@@ -29,6 +29,7 @@ struct KParams {
     double pad_dt;  // min(1e6, 1e5 if nelt % 1024 != 0): the reduction's padding
     int ilimiteur, inlet, outlet, is_dt_constant, is_override_manning;
     int hydro_off;  // offset of this instance's hydrograph tables in the hydro array
+    int ad_flags;   // derivative runs: bit 0 = WAF limiter values frozen
     double debitglob[CF_MAX_IO];
     double h_sortie[CF_MAX_IO];
     double long_entree[CF_MAX_IO];
@@ -278,7 +279,8 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
                     if (hl_upw < tls) { hl_upw = 0.0; vl_upw = 0.0; }
                     if (hr_upw < tls) { hr_upw = 0.0; vr_upw = 0.0; }
                     waf_flux(g, p.ilimiteur, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr,
-                             hl_upw, vl_upw, hr_upw, vr_upw, d_mrl, d_mkvol, d_mkvoisin, dt, flux);
+                             hl_upw, vl_upw, hr_upw, vr_upw, d_mrl, d_mkvol, d_mkvoisin, dt, flux,
+                             (p.ad_flags & 1) != 0);
                 }
             }
         }
@@ -698,7 +700,7 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
                 const double fhru = zru ? 0.0 : hru, fvru = zru ? 0.0 : vr0;
                 double gg[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
                 waf_flux_rev(g, p.ilimiteur, n1, n2, fhl, ful, fvl, fhr, fur, fvr, fhlu, fvlu, fhru, fvru, d_mrl,
-                             d_mkvol, d_mkvoisin, dt, bf, gg);
+                             d_mkvol, d_mkvoisin, dt, bf, gg, (p.ad_flags & 1) != 0);
                 // face states
                 double bvl_post = 0.0, bvr_post = 0.0;
                 if (!zl) {

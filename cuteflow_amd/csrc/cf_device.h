@@ -233,10 +233,12 @@ CF_HD inline T limiter(int ilim, T r, T c) {
 }
 
 // solver_Riemann_waf (flux_riemann.cuf:720-900).
+// freeze_lim (derivative runs only): the limiter values are taken as
+// constants, like the other switches of the forward; values are unchanged.
 template <class T>
 CF_HD inline void waf_flux(double gp, int ilim, double n1, double n2, T hl, T ul, T vl, T hr, T ur, T vr,
                            T hl_upw, T vl_upw, T hr_upw, T vr_upw, double d_mrl, double d_mkvol,
-                           double d_mkvoisin, double dt, T flux[3]) {
+                           double d_mkvoisin, double dt, T flux[3], bool freeze_lim = false) {
     const double eps = kEpsWaf;
     HllcT<T> o;
     T fw0 = 0.0, fw1 = 0.0, fw2 = 0.0;
@@ -257,6 +259,11 @@ CF_HD inline void waf_flux(double gp, int ilim, double n1, double n2, T hl, T ul
         T lim_l = limiter(ilim, rl, cl);
         T lim_r = limiter(ilim, rr, cr);
         T lim_et = limiter(ilim, ret, cet);
+        if (freeze_lim) {
+            lim_l = T(val(lim_l));
+            lim_r = T(val(lim_r));
+            lim_et = T(val(lim_et));
+        }
         double sgl = copysign(1.0, val(o.sl)), sgr = copysign(1.0, val(o.sr)), sge = copysign(1.0, val(o.set));
         T wl = (1.0 + sgl * lim_l) / 2.0;
         T wlet = (1.0 + sge * lim_et) / 2.0;
@@ -535,7 +542,7 @@ CF_HD inline double limiter_d(int ilim, double r, double c, double &dr, double &
 CF_HD inline void waf_flux_rev(double gp, int ilim, double n1, double n2, double hl, double ul, double vl, double hr,
                                double ur, double vr, double hl_upw, double vl_upw, double hr_upw, double vr_upw,
                                double d_mrl, double d_mkvol, double d_mkvoisin, double dt, const double bflux[3],
-                               double g[10]) {
+                               double g[10], bool freeze_lim = false) {
     const double eps = kEpsWaf;
     HllcT<double> o;
     if (!hllc_core(gp, eps, hl, ul, vl, hr, ur, vr, o)) return;
@@ -580,9 +587,10 @@ CF_HD inline void waf_flux_rev(double gp, int ilim, double n1, double n2, double
     const double blim_l = (bwl * sgl - bwlr * sgl) / 2.0;
     const double blim_r = (bwlr * sgr - bwr * sgr) / 2.0;
     const double blim_et = (bwlet * sge - bwret * sge) / 2.0;
-    // limiters -> (r, c)
-    const double brl = blim_l * drl, brr = blim_r * drr, bret = blim_et * dret;
-    const double bcl = blim_l * dcl, bcr = blim_r * dcr, bcet = blim_et * dcet;
+    // limiters -> (r, c); nothing when the limiter values are frozen
+    const double fz = freeze_lim ? 0.0 : 1.0;
+    const double brl = fz * blim_l * drl, brr = fz * blim_r * drr, bret = fz * blim_et * dret;
+    const double bcl = fz * blim_l * dcl, bcr = fz * blim_r * dcr, bcet = fz * blim_et * dcet;
     // r = du / (dl + eps)
     const double bdul = brl / (dll + eps), bdur = brr / (dlr + eps), bduet = bret / (dlet + eps);
     const double bdll = -brl * (rl / (dll + eps)), bdlr = -brr * (rr / (dlr + eps));

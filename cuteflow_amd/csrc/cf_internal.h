@@ -56,6 +56,10 @@ struct StepArgs {
     const double4 *in_x;
     double4 *out_x;
     double *xs;
+    // Tile activity of a recorded run, row per step, one byte per tile: the
+    // recorded forward writes whether each tile ran; a replay copies the
+    // tiles that did not run (their cells are dry and stay so).
+    unsigned char *tile_ran;
     int ncell;        // padded cell count (geo stride)
     int hist_stride;
     int step;         // index of this step in the run (absolute in a replay)
@@ -135,6 +139,7 @@ struct cf_handle {
     std::vector<Graph> graphs;
     int use_graphs = 1;
     cf::AdjState *adj = nullptr;
+    int ad_flags = 0;                  // CF_ADJ_* bits of the derivative runs
 };
 
 // ---- host helpers shared by the translation units ----------------------

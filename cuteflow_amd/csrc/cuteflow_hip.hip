@@ -71,9 +71,13 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
     const int b = a.tile_inst[tile];
     const InstCtrl c = a.ctrl[b];
     const bool replay = a.dt_fix != nullptr;
+    unsigned char *ran_row = a.tile_ran ? a.tile_ran + (size_t)a.step * gridDim.x : nullptr;
     if (!replay) {
         if (c.status != 0) {  // this instance has stopped (ts reached) or failed
-            if (threadIdx.x == 0) a.partial[tile] = __builtin_inf();
+            if (threadIdx.x == 0) {
+                a.partial[tile] = __builtin_inf();
+                if (ran_row) ran_row[tile] = 0;
+            }
             return;
         }
         // A tile whose cells and neighbour tiles' cells were all dry after
@@ -82,9 +86,18 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
         // hold that: it is skipped.  Reading a stamp raised concurrently only
         // makes the tile run, which is always correct.
         if (a.skip && !a.always[tile] && a.stamp[tile] + 1u < c.gstep) {
-            if (threadIdx.x == 0) a.partial[tile] = __builtin_inf();  // no dt contribution
+            if (threadIdx.x == 0) {
+                a.partial[tile] = __builtin_inf();  // no dt contribution
+                if (ran_row) ran_row[tile] = 0;
+            }
             return;
         }
+        if (ran_row && threadIdx.x == 0) ran_row[tile] = 1;
+    } else if (ran_row && !ran_row[tile]) {
+        // skipped in the recorded run: the state passes through unchanged
+        const int ic = tile * kTile + threadIdx.x;
+        a.out_x[ic] = a.in_x[ic];
+        return;
     }
     const double4 *__restrict__ st_in = replay ? a.in_x : a.st[(c.par + a.step) & 1];
     double4 *__restrict__ st_out = replay ? a.out_x : a.st[(c.par + a.step + 1) & 1];
@@ -726,6 +739,7 @@ StepArgs cf::base_args(cf_handle *h) {
     a.in_x = nullptr;
     a.out_x = nullptr;
     a.xs = nullptr;
+    a.tile_ran = nullptr;
     a.ncell = h->ncell;
     a.hist_stride = h->cap + 1;
     a.skip = h->skip;
@@ -864,6 +878,7 @@ extern "C" int cf_batch_set_inflow(cf_handle *h, int b, const double *debitglob,
     if (bad_inst(h, b) || !debitglob || n < 0 || n > CF_MAX_IO) return CF_ERR_ARG;
     for (int i = 0; i < n; i++) h->prm[b].debitglob[i] = debitglob[i];
     h->kp[b] = make_kparams(h->prm[b], h->nelt[b], b * kHydroPerInst);
+    h->kp[b].ad_flags = h->ad_flags;
     return upload_params(h, b);
 }
 
@@ -880,6 +895,7 @@ extern "C" int cf_batch_set_hydrograph(cf_handle *h, int b, int inlet, int npts,
         h->prm[b].hydro_q[(inlet - 1) * CF_MAX_HYDRO + i] = q[i];
     }
     h->kp[b] = make_kparams(h->prm[b], h->nelt[b], b * kHydroPerInst);
+    h->kp[b].ad_flags = h->ad_flags;
     return upload_params(h, b);
 }
 

@@ -201,6 +201,17 @@ int cf_batch_get_geometry(cf_handle *h, int inst, double *surf, double *zm, doub
  * pointers; sample arrays (nsamp x ngauges, row per sample) are host
  * pointers.  Requires is_override_manning == 0 in every instance. */
 
+/* Derivative options (bits; 0 = the exact derivative of the discrete step).
+ * CF_ADJ_FREEZE_LIMITERS: the WAF limiter values are taken as constants, like
+ * the wet/dry switches and dt.  Where two face depths agree to rounding (water
+ * at rest, uniform flow) the limiter ratio du / (dl + 1e-16)
+ * (flux_riemann.cuf:461-463) is rounding noise with derivatives of order
+ * 1e16, and over thousands of steps the exact derivative of the WAF step
+ * grows without bound; frozen limiters give the derivative of the step with
+ * the limiter values of the forward.  The forward is unchanged. */
+#define CF_ADJ_FREEZE_LIMITERS 1
+int cf_adj_set_flags(cf_handle *h, int flags);
+
 /* Gauge cells of instance inst (0-based, caller's order). */
 int cf_adj_set_gauges(cf_handle *h, int inst, int n, const int *cells);
 /* Recorded forward: advances every instance by exactly nsteps steps (an

@@ -25,6 +25,7 @@ def build(outdir):
     lib.hs_create.restype = C.c_void_p
     lib.hs_create.argtypes = [C.POINTER(_lib.CfMesh), C.POINTER(_lib.CfParams)]
     lib.hs_destroy.argtypes = [C.c_void_p]
+    lib.hs_set_ad_flags.argtypes = [C.c_void_p, C.c_int]
     lib.hs_set_manning.argtypes = [C.c_void_p, _DP]
     lib.hs_forward.restype = C.c_int
     lib.hs_forward.argtypes = [C.c_void_p, C.c_int, _DP, _DP, _DP]
@@ -60,6 +61,9 @@ class Host:
         self.h = lib.hs_create(C.byref(cm), C.byref(self.cfp))
         assert self.h
         self.n = mesh.nelt
+
+    def set_ad_flags(self, flags):
+        self.lib.hs_set_ad_flags(self.h, int(flags))
 
     def set_manning(self, n):
         n = np.ascontiguousarray(n, np.float64)

@@ -246,6 +246,8 @@ void *hs_create(const cf_mesh *m, const cf_params *p) {
 
 void hs_destroy(void *h) { delete (HS *)h; }
 
+void hs_set_ad_flags(void *h, int flags) { ((HS *)h)->kp.ad_flags = flags; }
+
 void hs_set_manning(void *h, const double *n) {
     HS *s = (HS *)h;
     for (int i = 0; i < s->N; i++) s->man[i] = n[i];

@@ -171,6 +171,39 @@ def test_adjoint_dot_tangent_linear(hs, name):
         assert np.any(gn != 0.0)  # flowing water: friction depends on n
 
 
+@pytest.mark.parametrize("name", ["river_waf_l1", "river_waf_l2", "river_waf_l3", "river_waf_l4",
+                                  "river_waf_f2", "lake_waf"])
+def test_adjoint_dot_tangent_linear_frozen_limiters(hs, name):
+    """Same dot-product test with the WAF limiter values frozen
+    (CF_ADJ_FREEZE_LIMITERS), including the lake at rest that the exact
+    derivative cannot handle."""
+    if name == "lake_waf":
+        m, p = lake_case(8, iflux=3)
+        zm = O.Prepared(m).zm
+        U0 = np.zeros((m.nelt, 3))
+        U0[:, 0] = np.maximum(0.0, 0.2 - zm)
+    else:
+        m, p, U0 = _mesh_case(name)
+    H = Host(hs, m, p)
+    H.set_ad_flags(1)
+    U, _, done = H.forward(U0, 40)
+    assert done == 40
+    rng = np.random.default_rng(17)
+    lam = rng.normal(size=(m.nelt, 3))
+    dn = rng.normal(size=m.nelt) * 0.01
+    dU0 = rng.normal(size=(m.nelt, 3)) * 0.01
+    dU0[U0[:, 0] <= p.tolisec] = 0.0
+    dU, Ut = H.tlm(dn, dU0)
+    assert np.array_equal(Ut.view(np.int64), U.view(np.int64))
+    gn, gU, bad = H.adjoint(lam)
+    H.close()
+    assert bad == 0
+    lhs = float(np.sum(lam * dU))
+    rhs = float(np.sum(gn * dn) + np.sum(gU * dU0))
+    scale = float(np.sum(np.abs(lam * dU))) + 1e-300
+    assert abs(lhs - rhs) <= 1e-11 * scale, (lhs, rhs, scale)
+
+
 @pytest.mark.parametrize("name", ["river_hllc_f1", "river_waf_l1", "river_waf_f2"])
 def test_tangent_linear_matches_central_differences(hs, name):
     """Directional derivative of a smooth functional of the final state
