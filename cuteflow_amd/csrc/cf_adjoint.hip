@@ -76,7 +76,14 @@ struct AdjState {
     double *dman = nullptr;   // tangent of Manning n
     int tlm_cur = 0;
     bool r2_built = false;
+    bool gauges_dirty = true;
     int alloc_steps = 0, alloc_K = 0, alloc_nsamp = 0, alloc_gtotal = 0;
+    // captured launch sequences (recorded forward, reverse sweep, TLM run)
+    struct Graph {
+        int kind, nsteps, every, K, hmax;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
 };
 
 }  // namespace cf
@@ -106,10 +113,17 @@ struct RevArgs {
     int ncell;
 };
 
-// The reverse body needs about twice the registers of the forward one; one
-// workgroup of 256 threads per CU slot keeps it from spilling.
+// Occupancy target of the reverse kernel (waves per SIMD): the first-order
+// reverse fits 168 VGPRs (3 waves) with a few spills and runs 8 % faster
+// than at 2 waves on MI355X; the WAF reverse needs 256 (1 wave).
+template <int IFLUX>
+struct RevWaves {
+    static constexpr int value = IFLUX == 1 ? 3 : 1;
+};
+
 template <int IFLUX, int FRIC>
-__global__ __launch_bounds__(kTile) void rev_kernel(RevArgs r) {
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(RevWaves<IFLUX>::value, 8))) void rev_kernel(
+    RevArgs r) {
     const int tile = blockIdx.x;
     const int b = r.tile_inst[tile];
     const int i = tile * kTile + threadIdx.x;
@@ -147,18 +161,21 @@ __device__ __forceinline__ void acc4(double4 &a, const double4 &b) {
 // Cotangent of each cell's old state: the sum of the contributions the
 // reverse kernel addressed to it, in a fixed order (contributions of tiles
 // that did not run at this step are zero and not read), plus the hmax seed
-// of sample `sidx`.
+// of sample `sidx`.  A tile that did not run gets only its seeds, and only
+// when the reverse of the previous step will read them (ran_prev; null at
+// the first step, whose cotangents are the initial-state gradient).
 template <bool R2ON>
 __global__ void gather_kernel(const int4 *__restrict__ nbr, const double4 *__restrict__ R1,
                               const double4 *__restrict__ R2, const int *__restrict__ r2_off,
                               const int *__restrict__ r2_idx, const int *__restrict__ tile_inst,
                               const int *__restrict__ inst_end, const unsigned char *__restrict__ ran,
-                              const int *__restrict__ argmax, const double *__restrict__ ghmax, int sidx,
-                              double4 *__restrict__ lam_out) {
+                              const unsigned char *__restrict__ ran_prev, const int *__restrict__ argmax,
+                              const double *__restrict__ ghmax, int sidx, double4 *__restrict__ lam_out) {
     const int tile = blockIdx.x;
     const int b = tile_inst[tile];
     const int c = tile * kTile + threadIdx.x;
     if (c >= inst_end[b]) return;
+    if (!ran[tile] && ran_prev && !ran_prev[tile]) return;  // nobody reads this tile's cotangents
     double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
     if (ran[tile]) {
         acc = R1[4 * (size_t)c + 3];
@@ -384,9 +401,15 @@ static void fill0(double *p, size_t n, hipStream_t s) {
     if (n) hipLaunchKernelGGL(fill_d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
 }
 
+static void drop_adj_graphs(AdjState *A) {
+    for (auto &g : A->graphs) (void)hipGraphExecDestroy(g.exec);
+    A->graphs.clear();
+}
+
 void cf::adj_free(cf_handle *h) {
     AdjState *A = h ? h->adj : nullptr;
     if (!A) return;
+    drop_adj_graphs(A);
     dfree(A->d_gcell); dfree(A->d_ginst); dfree(A->d_goff);
     dfree(A->eta); dfree(A->geta); dfree(A->deta); dfree(A->teta);
     dfree(A->hmax); dfree(A->ghmax); dfree(A->dhmax); dfree(A->argmax);
@@ -466,6 +489,7 @@ static int build_r2(cf_handle *h, AdjState *A) {
     fill0(A->R2, 6 * (size_t)N, h->stream);
     HIPCHK(hipStreamSynchronize(h->stream));
     A->r2_built = true;
+    drop_adj_graphs(A);
     return CF_OK;
 }
 
@@ -486,12 +510,16 @@ extern "C" int cf_adj_set_gauges(cf_handle *h, int inst, int n, const int *cells
     for (int g = 0; g < n; g++)
         if (cells[g] < 0 || cells[g] >= h->nelt[inst]) return CF_ERR_ARG;
     A->gcells[inst].assign(cells, cells + n);
+    A->gauges_dirty = true;
     A->recorded = false;
     A->tlm_ready = false;
     return CF_OK;
 }
 
 static int upload_gauges(cf_handle *h, AdjState *A) {
+    if (!A->gauges_dirty) return CF_OK;
+    drop_adj_graphs(A);
+    A->gauges_dirty = false;
     const int B = h->ninst;
     A->goff.assign(B + 1, 0);
     for (int b = 0; b < B; b++) A->goff[b + 1] = A->goff[b] + (int)A->gcells[b].size();
@@ -519,6 +547,7 @@ static int adj_alloc(cf_handle *h, AdjState *A) {
     const size_t N = (size_t)h->ncell, B = (size_t)h->ninst;
     const size_t S = (size_t)A->nsteps + 1;
     if (A->alloc_steps != A->nsteps || A->alloc_K != A->K) {
+        drop_adj_graphs(A);
         dfree(A->dt); dfree(A->tc); dfree(A->ran); dfree(A->ckpt); dfree(A->seg); dfree(A->xs); dfree(A->tile_ran);
         if (!dalloc(A->dt, B * S) || !dalloc(A->tc, B * S) || !dalloc(A->ran, B * S) ||
             !dalloc(A->ckpt, (size_t)A->nseg * N) || !dalloc(A->seg, (size_t)A->K * N) ||
@@ -528,6 +557,7 @@ static int adj_alloc(cf_handle *h, AdjState *A) {
         A->alloc_K = A->K;
     }
     if (A->alloc_nsamp != A->nsamp || A->alloc_gtotal != A->gtotal) {
+        drop_adj_graphs(A);
         dfree(A->eta); dfree(A->geta); dfree(A->deta); dfree(A->teta);
         const size_t ne = (size_t)std::max(A->nsamp, 1) * (size_t)std::max(A->gtotal, 1);
         if (!dalloc(A->eta, ne) || !dalloc(A->geta, ne) || !dalloc(A->deta, ne) || !dalloc(A->teta, ne))
@@ -536,6 +566,7 @@ static int adj_alloc(cf_handle *h, AdjState *A) {
         A->alloc_gtotal = A->gtotal;
     }
     if (!A->lam[0]) {
+        drop_adj_graphs(A);
         if (!dalloc(A->lam[0], N) || !dalloc(A->lam[1], N) || !dalloc(A->gstate, N) || !dalloc(A->R1, 4 * N) ||
             !dalloc(A->gn, N) || !dalloc(A->hmax, N) || !dalloc(A->ghmax, N) || !dalloc(A->dhmax, N) ||
             !dalloc(A->argmax, N))
@@ -543,6 +574,68 @@ static int adj_alloc(cf_handle *h, AdjState *A) {
         fill0(A->R1, 4 * N, h->stream);
     }
     return CF_OK;
+}
+
+// Runs a launch sequence of the current recorded-run configuration: captured
+// in a hipGraph on first use and replayed afterwards (the sequences' kernel
+// arguments are fixed buffers; per-run data lives in device memory).
+template <class F>
+static int run_captured(cf_handle *h, AdjState *A, int kind, F enqueue) {
+    hipStream_t s = h->stream;
+    if (!h->use_graphs || !s) {
+        enqueue(h, A, s);
+        HIPCHK(hipGetLastError());
+        return CF_OK;
+    }
+    for (auto &g : A->graphs)
+        if (g.kind == kind && g.nsteps == A->nsteps && g.every == A->every && g.K == A->K &&
+            g.hmax == A->want_hmax) {
+            HIPCHK(hipGraphLaunch(g.exec, s));
+            return CF_OK;
+        }
+    hipGraph_t graph;
+    HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    enqueue(h, A, s);
+    HIPCHK(hipStreamEndCapture(s, &graph));
+    hipGraphExec_t exec;
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HIPCHK(e);
+    A->graphs.push_back({kind, A->nsteps, A->every, A->K, A->want_hmax, exec});
+    HIPCHK(hipGraphLaunch(exec, s));
+    return CF_OK;
+}
+
+// Recorded forward: steps with the tile activity, samples and checkpoints.
+static void enqueue_forward(cf_handle *h, AdjState *A, hipStream_t s) {
+    const int S = A->nsteps + 1;
+    const StPair sp{{h->st[0], h->st[1]}};
+    launch_run_init(h, A->dt, A->tc, S, s);
+    hipLaunchKernelGGL(ckpt_kernel, dim3(h->ntiles), dim3(kTile), 0, s, sp, h->ctrl, h->tile_inst, h->inst_end,
+                       A->ckpt);
+    StepArgs a = base_args(h);
+    a.dt_hist = A->dt;
+    a.tc_hist = A->tc;
+    a.ran = A->ran;
+    a.tile_ran = A->tile_ran;
+    a.hist_stride = S;
+    for (int k = 0; k < A->nsteps; k++) {
+        a.step = k;
+        launch_step(a, h->iflux, h->fric, h->ntiles, s);
+        launch_finalize(h, A->dt, A->tc, A->ran, S, k, s);
+        if ((k + 1) % A->every == 0) {
+            const int sidx = (k + 1) / A->every - 1;
+            if (A->gtotal)
+                hipLaunchKernelGGL(sample_gauges_kernel, dim3(grid_of(A->gtotal, 256)), dim3(256), 0, s, sp, h->ctrl,
+                                   A->d_gcell, A->d_ginst, A->gtotal, sidx, A->eta);
+            if (A->want_hmax)
+                hipLaunchKernelGGL(sample_hmax_kernel, dim3(h->ntiles), dim3(kTile), 0, s, sp, h->ctrl, h->tile_inst,
+                                   h->inst_end, sidx, A->hmax, A->argmax);
+        }
+        if ((k + 1) % A->K == 0 && k + 1 < A->nsteps)
+            hipLaunchKernelGGL(ckpt_kernel, dim3(h->ntiles), dim3(kTile), 0, s, sp, h->ctrl, h->tile_inst,
+                               h->inst_end, A->ckpt + (size_t)((k + 1) / A->K) * h->ncell);
+    }
 }
 
 extern "C" int cf_adj_forward(cf_handle *h, int nsteps, int sample_every, int checkpoint_every, int want_hmax) {
@@ -568,36 +661,9 @@ extern "C" int cf_adj_forward(cf_handle *h, int nsteps, int sample_every, int ch
     fill0(A->gstate, (size_t)h->ncell, h->stream);
     fill0(A->ghmax, (size_t)h->ncell, h->stream);
 
+    int rc2 = run_captured(h, A, 0, enqueue_forward);
+    if (rc2) return rc2;
     hipStream_t s = h->stream;
-    const int S = nsteps + 1;
-    const StPair sp{{h->st[0], h->st[1]}};
-    launch_run_init(h, A->dt, A->tc, S, s);
-    hipLaunchKernelGGL(ckpt_kernel, dim3(h->ntiles), dim3(kTile), 0, s, sp, h->ctrl, h->tile_inst, h->inst_end,
-                       A->ckpt);
-    StepArgs a = base_args(h);
-    a.dt_hist = A->dt;
-    a.tc_hist = A->tc;
-    a.ran = A->ran;
-    a.tile_ran = A->tile_ran;
-    a.hist_stride = S;
-    for (int k = 0; k < nsteps; k++) {
-        a.step = k;
-        launch_step(a, h->iflux, h->fric, h->ntiles, s);
-        launch_finalize(h, A->dt, A->tc, A->ran, S, k, s);
-        if ((k + 1) % sample_every == 0) {
-            const int sidx = (k + 1) / sample_every - 1;
-            if (A->gtotal)
-                hipLaunchKernelGGL(sample_gauges_kernel, dim3(grid_of(A->gtotal, 256)), dim3(256), 0, s, sp, h->ctrl,
-                                   A->d_gcell, A->d_ginst, A->gtotal, sidx, A->eta);
-            if (A->want_hmax)
-                hipLaunchKernelGGL(sample_hmax_kernel, dim3(h->ntiles), dim3(kTile), 0, s, sp, h->ctrl, h->tile_inst,
-                                   h->inst_end, sidx, A->hmax, A->argmax);
-        }
-        if ((k + 1) % A->K == 0 && k + 1 < nsteps)
-            hipLaunchKernelGGL(ckpt_kernel, dim3(h->ntiles), dim3(kTile), 0, s, sp, h->ctrl, h->tile_inst,
-                               h->inst_end, A->ckpt + (size_t)((k + 1) / A->K) * h->ncell);
-    }
-    HIPCHK(hipGetLastError());
     std::vector<InstCtrl> c(h->ninst);
     HIPCHK(hipMemcpyAsync(c.data(), h->ctrl, c.size() * sizeof(InstCtrl), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -733,10 +799,9 @@ static void launch_rev(const RevArgs &r, int iflux, int fric, int grid, hipStrea
     tab[iflux == 3 ? 1 : 0][fric](r, grid, s);
 }
 
-extern "C" int cf_adj_backward(cf_handle *h) {
-    if (!h || !h->adj || !h->adj->recorded || h->pending >= 0) return CF_ERR_ARG;
-    AdjState *A = h->adj;
-    hipStream_t s = h->stream;
+// Reverse sweep: segments from the last, each replayed from its checkpoint
+// and reversed step by step.
+static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
     const size_t N = (size_t)h->ncell;
     const int S = A->nsteps + 1;
     const int *am = A->want_hmax ? A->argmax : nullptr;
@@ -791,23 +856,30 @@ extern "C" int cf_adj_backward(cf_handle *h) {
             r.ran = A->tile_ran + (size_t)k * h->ntiles;
             launch_rev(r, h->iflux, h->fric, h->ntiles, s);
             const int sidx = (k >= 1 && k % A->every == 0) ? k / A->every - 1 : -1;
+            const unsigned char *ran_prev = k > 0 ? A->tile_ran + (size_t)(k - 1) * h->ntiles : nullptr;
             if (h->iflux == 3)
                 hipLaunchKernelGGL(gather_kernel<true>, dim3(h->ntiles), dim3(kTile), 0, s, h->nbr, A->R1, A->R2,
-                                   A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, r.ran, am, A->ghmax, sidx,
-                                   A->lam[1 - cur]);
+                                   A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, r.ran, ran_prev, am, A->ghmax,
+                                   sidx, A->lam[1 - cur]);
             else
                 hipLaunchKernelGGL(gather_kernel<false>, dim3(h->ntiles), dim3(kTile), 0, s, h->nbr, A->R1, A->R2,
-                                   A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, r.ran, am, A->ghmax, sidx,
-                                   A->lam[1 - cur]);
+                                   A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, r.ran, ran_prev, am, A->ghmax,
+                                   sidx, A->lam[1 - cur]);
             if (sidx >= 0 && A->gtotal)
                 hipLaunchKernelGGL(gauge_seed_kernel, dim3(grid_of(h->ninst, 64)), dim3(64), 0, s, A->d_gcell,
                                    A->d_goff, A->geta, A->gtotal, sidx, h->ninst, A->lam[1 - cur]);
             cur = 1 - cur;
         }
     }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(s));
-    A->lam_cur = cur;
+}
+
+extern "C" int cf_adj_backward(cf_handle *h) {
+    if (!h || !h->adj || !h->adj->recorded || h->pending >= 0) return CF_ERR_ARG;
+    AdjState *A = h->adj;
+    int rc = run_captured(h, A, 1, enqueue_backward);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    A->lam_cur = A->nsteps & 1;  // the buffer holding the initial-state cotangent
     return CF_OK;
 }
 
@@ -827,6 +899,7 @@ extern "C" int cf_tlm_set_direction(cf_handle *h, int inst, const double *dn, co
     AdjState *A = h->adj;
     const size_t N = (size_t)h->ncell;
     if (!A->dman) {
+        drop_adj_graphs(A);
         if (!dalloc(A->dman, N) || !dalloc(A->tst[0], N) || !dalloc(A->tst[1], N) || !dalloc(A->tdst[0], N) ||
             !dalloc(A->tdst[1], N) || !dalloc(A->tdir, N))
             return CF_ERR_NOMEM;
@@ -855,10 +928,9 @@ static void launch_tlm_t(const TlmArgs &t, int grid, hipStream_t s) {
     hipLaunchKernelGGL((tlm_kernel<IFLUX, FRIC>), dim3(grid), dim3(kTile), 0, s, t);
 }
 
-extern "C" int cf_tlm_run(cf_handle *h) {
-    if (!h || !h->adj || !h->adj->recorded || !h->adj->dman || h->pending >= 0) return CF_ERR_ARG;
-    AdjState *A = h->adj;
-    hipStream_t s = h->stream;
+// Tangent-linear run over the recorded dt and tc from the recorded initial
+// state.
+static void enqueue_tlm(cf_handle *h, AdjState *A, hipStream_t s) {
     const size_t N = (size_t)h->ncell;
     typedef void (*fn)(const TlmArgs &, int, hipStream_t);
     static const fn tab[2][4] = {{launch_tlm_t<1, 0>, launch_tlm_t<1, 1>, launch_tlm_t<1, 2>, launch_tlm_t<1, 3>},
@@ -901,8 +973,14 @@ extern "C" int cf_tlm_run(cf_handle *h) {
                                    h->inst_end, A->argmax, sidx, A->dhmax);
         }
     }
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(s));
+}
+
+extern "C" int cf_tlm_run(cf_handle *h) {
+    if (!h || !h->adj || !h->adj->recorded || !h->adj->dman || h->pending >= 0) return CF_ERR_ARG;
+    AdjState *A = h->adj;
+    int rc = run_captured(h, A, 2, enqueue_tlm);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
     A->tlm_cur = A->nsteps & 1;
     A->tlm_ready = true;
     return CF_OK;

@@ -445,7 +445,8 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
         }
         if (FRIC == 2 && normvit > tol && q1 > tls) {
             mat = true;
-            p43 = cf_pow(q1, -4.0 / 3.0);
+            // q1^(-4/3) = q1^(-1/3) / q1: enters only linear coefficients here
+            p43 = p13 / q1;
             dtgn2h = ((((dt / 2.0) * g) * mann) * mann) * p43;
             f21 = -((((7.0 / 3.0) * dtgn2h) * vitu) * normvit);
             f22 = 1.0 + (dtgn2h * (2.0 * (vitu * vitu) + vitv * vitv)) / normvit;
@@ -462,7 +463,7 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
             a8 = f22 / det;
         } else if (FRIC == 3 && normvit > tol && q1 > tls) {
             mat = true;
-            p43 = cf_pow(q1, -4.0 / 3.0);
+            p43 = p13 / q1;
             beta = -(((((dt * g) * mann) * mann) * normvit) * p43);
             det = (1.0 - beta) * (1.0 - beta);
             a4 = (1.0 - beta) / det;
@@ -491,9 +492,10 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
     if (FRIC == 2 && mat) {
         const double ba3 = br1 * x0, ba4 = br1 * x1, ba5 = br1 * x2;
         const double ba6 = br2 * x0, ba7 = br2 * x1, ba8 = br2 * x2;
-        const double bdet = -(ba3 * a3 + ba4 * a4 + ba5 * a5 + ba6 * a6 + ba7 * a7 + ba8 * a8) / det;
-        const double bN3 = ba3 / det, bN4 = ba4 / det, bN5 = ba5 / det;
-        const double bN6 = ba6 / det, bN7 = ba7 / det, bN8 = ba8 / det;
+        const double idet = 1.0 / det;
+        const double bdet = -(ba3 * a3 + ba4 * a4 + ba5 * a5 + ba6 * a6 + ba7 * a7 + ba8 * a8) * idet;
+        const double bN3 = ba3 * idet, bN4 = ba4 * idet, bN5 = ba5 * idet;
+        const double bN6 = ba6 * idet, bN7 = ba7 * idet, bN8 = ba8 * idet;
         double bf21 = 0.0, bf22 = 0.0, bf23 = 0.0, bf31 = 0.0, bf32 = 0.0, bf33 = 0.0;
         // a3 = -(f21 f33 - f23 f31)/det, a4 = f33/det, a5 = -f23/det,
         // a6 = (f21 f32 - f22 f31)/det, a7 = -f32/det, a8 = f22/det
@@ -504,14 +506,14 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
         bf32 += bN6 * f21 - bN7 - bdet * f23;
         bf22 += -bN6 * f31 + bN8 + bdet * f33;
         bf23 += bf32;  // f32 = f23
-        const double D = dtgn2h, u = vitu, v = vitv, w = normvit;
+        const double D = dtgn2h, u = vitu, v = vitv, w = normvit, iw = 1.0 / normvit;
         const double c73 = 7.0 / 3.0;
         const double e22 = 2.0 * (u * u) + v * v, e33 = u * u + 2.0 * (v * v);
-        const double bD = -c73 * (bf21 * u * w + bf31 * v * w) + bf22 * (e22 / w) + bf23 * ((u * v) / w) +
-                          bf33 * (e33 / w);
-        bvitu += -c73 * bf21 * D * w + bf22 * D * (4.0 * u) / w + bf23 * D * v / w + bf33 * D * (2.0 * u) / w;
-        bvitv += -c73 * bf31 * D * w + bf22 * D * (2.0 * v) / w + bf23 * D * u / w + bf33 * D * (4.0 * v) / w;
-        bnv += -c73 * (bf21 * D * u + bf31 * D * v) - (bf22 * D * e22 + bf23 * D * (u * v) + bf33 * D * e33) / (w * w);
+        const double bD = -c73 * (bf21 * u * w + bf31 * v * w) + (bf22 * e22 + bf23 * (u * v) + bf33 * e33) * iw;
+        const double Diw = D * iw;
+        bvitu += -c73 * bf21 * D * w + Diw * (bf22 * (4.0 * u) + bf23 * v + bf33 * (2.0 * u));
+        bvitv += -c73 * bf31 * D * w + Diw * (bf22 * (2.0 * v) + bf23 * u + bf33 * (4.0 * v));
+        bnv += -c73 * (bf21 * D * u + bf31 * D * v) - (bf22 * e22 + bf23 * (u * v) + bf33 * e33) * (Diw * iw);
         // D = (dt/2 g m m) q1^(-4/3)
         bm += bD * (((dt / 2.0) * g) * (2.0 * mann)) * p43;
         bq1 += bD * D * (-4.0 / 3.0) / q1;
@@ -525,8 +527,9 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
         bq1 += bbeta * beta * (-4.0 / 3.0) / q1;
     }
     // x = ((sb + sf) - resc) / surf
-    const double bres[3] = {-bx0 / surf, -bx1 / surf, -bx2 / surf};
-    const double bs1 = bx1 / surf, bs2 = bx2 / surf;  // sb and sf
+    const double isurf = 1.0 / surf;
+    const double bres[3] = {-bx0 * isurf, -bx1 * isurf, -bx2 * isurf};
+    const double bs1 = bx1 * isurf, bs2 = bx2 * isurf;  // sb and sf
     if (FRIC != 0 && q1 > tls) {
         // sf = -(g m m) u w surf p13
         const double K = (g * mann) * mann;
@@ -535,14 +538,16 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
         bvitv += -bs2 * (K * normvit * surf * p13);
         bnv += -(bs1 * vitu + bs2 * vitv) * (K * surf * p13);
         const double bp13 = -(bs1 * vitu + bs2 * vitv) * (K * normvit * surf);
-        bq1 += bp13 * (-1.0 / 3.0) * (p13 / q1);
+        const double iq1 = 1.0 / q1;
+        bq1 += bp13 * (-1.0 / 3.0) * (p13 * iq1);
         if (!floored) {
-            bvitu += bnv * (vitu / normvit);
-            bvitv += bnv * (vitv / normvit);
+            const double inv = 1.0 / normvit;
+            bvitu += bnv * (vitu * inv);
+            bvitv += bnv * (vitv * inv);
         }
-        bq2 += bvitu / q1;
-        bq3 += bvitv / q1;
-        bq1 -= (bvitu * vitu + bvitv * vitv) / q1;
+        bq2 += bvitu * iq1;
+        bq3 += bvitv * iq1;
+        bq1 -= (bvitu * vitu + bvitv * vitv) * iq1;
     }
     gn = bm;
 

@@ -303,11 +303,13 @@ CF_HD inline void rot(double tolisec, double n1, double n2, T h0, T hu, T hv, T 
 CF_HD inline void rot_rev(double tolisec, double n1, double n2, double h0, double hu, double hv, double bh,
                           double bun, double but, double &bh0, double &bhu, double &bhv) {
     if (!(h0 > tolisec)) return;  // all outputs are constants
-    const double un = (hu * n1 + hv * n2) / h0;
-    const double ut = (hv * n1 - hu * n2) / h0;
-    bhu += (bun * n1 - but * n2) / h0;
-    bhv += (bun * n2 + but * n1) / h0;
-    bh0 += bh - (bun * un + but * ut) / h0;
+    // derivative-only arithmetic: one reciprocal (no branch depends on it)
+    const double inv = 1.0 / h0;
+    const double un = (hu * n1 + hv * n2) * inv;
+    const double ut = (hv * n1 - hu * n2) * inv;
+    bhu += (bun * n1 - but * n2) * inv;
+    bhv += (bun * n2 + but * n1) * inv;
+    bh0 += bh - (bun * un + but * ut) * inv;
 }
 
 // Reverse of hllc_core.  Output cotangents: fh[0..2], f1[0..1], f2[0..1],
@@ -360,9 +362,9 @@ CF_HD inline void hllc_core_rev(double gp, double eps, double hl, double ul, dou
     } else if (sr < 0.0) {
         for (int k = 0; k < 3; k++) bf2[k] += bfh[k];
     } else {
-        const double D = (sr - sl) + eps;
+        const double D = (sr - sl) + eps, invD = 1.0 / D;
         double fh[2];
-        for (int k = 0; k < 2; k++) fh[k] = ((sr * f1[k] - sl * f2[k]) + (sl * sr) * (q2[k] - q1[k])) / D;
+        for (int k = 0; k < 2; k++) fh[k] = ((sr * f1[k] - sl * f2[k]) + (sl * sr) * (q2[k] - q1[k])) * invD;
         double bfk[2] = {bfh[0], bfh[1]};
         if (set > 0.0) {
             bfk[0] += bfh[2] * vl;
@@ -373,7 +375,7 @@ CF_HD inline void hllc_core_rev(double gp, double eps, double hl, double ul, dou
         }
         double bD = 0.0;
         for (int k = 0; k < 2; k++) {
-            const double bN = bfk[k] / D;
+            const double bN = bfk[k] * invD;
             const double dq = q2[k] - q1[k];
             bD -= bN * fh[k];
             bsr += bN * f1[k] + bN * sl * dq;
@@ -446,13 +448,14 @@ CF_HD inline void hllc_core_rev(double gp, double eps, double hl, double ul, dou
         bhr += bP * (0.5 * het) - (bP * Q) * (2.0 * hr);
     }
     // het = S/2 - ((ur - ul) S / A) / 4
-    {
-        const double W = ((ur - ul) * S) / A;
-        const double bW = -bhet / 4.0;
-        bS += bhet / 2.0 + bW * ((ur - ul) / A);
-        bur += bW * (S / A);
-        bul -= bW * (S / A);
-        bA -= bW * (W / A);
+    if (bhet != 0.0) {
+        const double invA = 1.0 / A;
+        const double W = ((ur - ul) * S) * invA;
+        const double bW = -bhet * 0.25;
+        bS += bhet * 0.5 + bW * ((ur - ul) * invA);
+        bur += bW * (S * invA);
+        bul -= bW * (S * invA);
+        bA -= bW * (W * invA);
     }
     bhl += bS;
     bhr += bS;
