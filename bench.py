@@ -1,16 +1,33 @@
 #!/usr/bin/env python3
 """Benchmark of the FP64 finite-volume St-Venant time step on one MI355X.
 
-Workload (the river-reach configuration): a 20 km x 2 km reach meshed as
-3200 x 320 jittered quads split into 2,048,000 triangles in RCM order, bed
-slope 1/2000 + 0.5 m sinusoidal bars (300 m) + a seeded Gaussian-smoothed
-random field (sigma 0.3 m, 200 m correlation), a 3 m deep main channel in a
-floodplain that starts dry, Manning n 0.03 / 0.06, inflow ramped 0 ->
-3000 m3/s over 1 h, fixed outlet level, dry threshold 1e-3 m, WAF second
-order with the superbee limiter and semi-implicit friction (fricimplic 1).
+Headline (value, roofline, cpu_baseline): the river-reach configuration,
+a 20 km x 2 km reach meshed as 3200 x 320 jittered quads split into
+2,048,000 triangles in RCM order, bed slope 1/2000 + 0.5 m sinusoidal bars
+(300 m) + a seeded Gaussian-smoothed random field (sigma 0.3 m, 200 m
+correlation), a 3 m deep main channel in a floodplain that starts dry,
+Manning n 0.03 / 0.06, inflow ramped 0 -> 3000 m3/s over 1 h, fixed outlet
+level, dry threshold 1e-3 m, WAF second order with the superbee limiter and
+semi-implicit friction (fricimplic 1).  A step is one time step over the
+whole mesh.
 
-A step is one time step over the whole mesh (one launch of the fused step
-kernel).  Prints one JSON line on stdout; progress goes to stderr.
+Further sections of the same JSON line (each skippable):
+- "calibration": adjoint/forward cost ratio of a gauge-misfit gradient on a
+  100k-cell reach with 8 Manning zones, 20 gauges sampled every 10 steps
+  over 2000 steps, checkpoints every 100 steps; the gradient checked
+  against central differences and tangent-linear derivatives on each zone;
+  20 L-BFGS iterations from n 20 % above the truth;
+- "ensemble": 32 members of 50k-500k cells in one launch per step, 2000
+  steps, against the same 32 runs one after the other;
+- "batched_gradients": 16 members on a 1M-cell reach with per-cell
+  log-normal Manning fields and their own hydrographs, 3000 steps,
+  checkpoints every 150 steps, loss = misfit at 50 gauges + squared error of
+  the maximum flood extent; gradients per second, cost ratio to the batched
+  forward, and tangent-linear checks along 4 random directions.
+The gradient workloads use the first-order scheme (iflux 1), as the
+reference's own calibration runs do; see the "scheme" fields.
+
+Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
 
@@ -64,33 +81,19 @@ def traffic_from_profile(workload_key):
         return json.load(fh).get("hbm_bytes_per_launch")
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5000)
-    ap.add_argument("--warmup", type=int, default=100)
-    ap.add_argument("--nx", type=int, default=3200)
-    ap.add_argument("--ny", type=int, default=320)
-    ap.add_argument("--order", choices=["rcm", "natural", "random"], default="rcm")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0,
-                    help="target duration of the CPU-baseline sample (0 disables)")
-    ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--options", type=int, default=0,
-                    help="cf_params.options bits: 1 keep the input cell order, 2 run every tile")
-    ap.add_argument("--copy-probe-mb", type=int, default=0,
-                    help="before the timed region, run a device copy of this many MiB (3 times): a "
-                         "known-size kernel that calibrates HBM byte counters in a profile")
-    args = ap.parse_args()
+def sync():
+    import torch
 
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: the time step runs on one subdomain per GPU and "
-              "this build has no multi-GPU path; run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
+    torch.cuda.synchronize()
+
+
+# ---------------------------------------------------------------------------
+def bench_headline(args):
+    import ctypes as C
 
     import numpy as np
     import torch
 
-    torch.cuda.init()
     from cuteflow_amd import _lib
     from cuteflow_amd.solver import Solver
 
@@ -110,7 +113,7 @@ def main():
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)
-        torch.cuda.synchronize()
+        sync()
         log(f"bench: copy probe {args.copy_probe_mb} MiB read + {args.copy_probe_mb} MiB written, 3 times")
         del src, dst
 
@@ -118,16 +121,14 @@ def main():
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     hist = np.zeros(args.steps + 1)
-    import ctypes as C
-
     done = C.c_int()
-    torch.cuda.synchronize()
+    sync()
     w0 = time.perf_counter()
     ev0.record(stream)
     _lib.check(lib.cf_step_async(s.h, args.steps), "cf_step_async")
     ev1.record(stream)
     _lib.check(lib.cf_step_finish(s.h, _lib.dptr(hist), C.byref(done)), "cf_step_finish")
-    torch.cuda.synchronize()
+    sync()
     w1 = time.perf_counter()
     if done.value != args.steps:
         raise RuntimeError(f"run stopped after {done.value} of {args.steps} steps")
@@ -222,6 +223,254 @@ def main():
         }
         result["parity_check"] = {"steps": nsamp, "bit_identical": same}
     s.close()
+    return result
+
+
+# ---------------------------------------------------------------------------
+def bench_calibration(args):
+    """Zonal calibration reach: gradient cost, checks, L-BFGS."""
+    import numpy as np
+
+    from cuteflow_amd import calibration as CAL
+    from cuteflow_amd import scenarios as SC
+    from cuteflow_amd.adjoint import AdjointRun
+    from cuteflow_amd.solver import Solver
+
+    nsteps, every, K = 2000, 10, 100
+    sc = SC.calibration(100_000, ngauges=20, iflux=1)
+    N = sc.mesh.nelt
+    Z = np.zeros((N, 8))
+    Z[np.arange(N), sc.zone] = 1.0
+    out = {"workload": "100k-cell reach, 8 Manning zones (4 reaches x channel/floodplain), 20 channel gauges, "
+                       "levels every 10 steps over 2000 steps, checkpoints every 100 steps, HLLC + "
+                       "semi-implicit friction; observations from the hidden truth, gradient at +20 %",
+           "nelt": N, "scheme": "iflux 1 (HLLC), fricimplic 1"}
+    with Solver(sc.mesh, sc.params) as s:
+        s.set_state(sc.state)
+        s.step(nsteps)
+        times = []
+        for _ in range(2):
+            s.set_state(sc.state)
+            sync()
+            t = time.perf_counter()
+            s.step(nsteps)
+            sync()
+            times.append(time.perf_counter() - t)
+        t_fwd = min(times)
+        run = AdjointRun(s, sc.gauges, sample_every=every, checkpoint_every=K)
+        obs = run.forward(nsteps, U0=[sc.state], manning=[Z @ sc.n_zones])[0][0]
+        n0 = sc.n_zones * 1.2
+        grads, gtimes = [], []
+        for _ in range(3):
+            sync()
+            t = time.perf_counter()
+            eta = run.forward(nsteps, U0=[sc.state], manning=[Z @ n0])[0][0]
+            gn, _ = run.backward([2.0 * (eta - obs)])
+            sync()
+            gtimes.append(time.perf_counter() - t)
+            grads.append(Z.T @ gn[0])
+        t_grad = min(gtimes[1:])
+        g = grads[-1]
+        reproducible = all(np.array_equal(grads[0], x) for x in grads)
+        log(f"bench: calibration forward {t_fwd*1e3:.1f} ms, gradient {t_grad*1e3:.1f} ms "
+            f"(first {gtimes[0]*1e3:.1f} ms), ratio {t_grad / t_fwd:.2f}")
+        g_tlm = CAL.zonal_tlm_gradient(run, Z, obs)
+        g_fd = CAL.zonal_fd_gradient(run, Z, n0, obs, eps=1e-6)
+        g_fd7 = CAL.zonal_fd_gradient(run, Z, n0, obs, eps=1e-7)
+        rel = lambda a, b: (np.abs(a - b) / np.maximum(np.abs(b), 1e-300)).tolist()
+        log(f"bench: calibration gradient vs TLM {max(rel(g, g_tlm)):.1e}, vs FD(1e-6) {rel(g, g_fd)}")
+        out.update({
+            "forward_ms": t_fwd * 1e3,
+            "gradient_ms": t_grad * 1e3,
+            "gradient_first_call_ms": gtimes[0] * 1e3,
+            "adjoint_forward_ratio": t_grad / t_fwd,
+            "gradient_ratio_definition": "(recorded forward + reverse sweep with checkpoint recompute) / plain "
+                                         "forward of the same 2000 steps, steady state",
+            "speedup_vs_fd_zonal": 16.0 * t_fwd / t_grad,
+            "speedup_vs_fd_per_cell": 2.0 * N * t_fwd / t_grad,
+            "fd_cost_definition": "central differences need 2 plain forwards per parameter (8 zones: 16; "
+                                  "per-cell n: 2 x nelt)",
+            "gradient": g.tolist(),
+            "rel_err_vs_tlm": rel(g, g_tlm),
+            "rel_err_vs_fd_1e-6": rel(g, g_fd),
+            "rel_err_vs_fd_1e-7": rel(g, g_fd7),
+            "reproducible_bits": bool(reproducible),
+        })
+        if args.lbfgs_iters > 0:
+            t = time.perf_counter()
+            nz, hist, nev = CAL.zonal_lbfgs(run, sc.zone, n0, obs, nsteps, sc.state, iters=args.lbfgs_iters)
+            tl = time.perf_counter() - t
+            log(f"bench: L-BFGS {args.lbfgs_iters} iterations ({nev} evaluations) in {tl:.1f}s, "
+                f"misfit {hist[0]:.3e} -> {min(hist):.3e}")
+            out["lbfgs"] = {"iterations": args.lbfgs_iters, "evaluations": nev, "seconds": tl,
+                            "relative_misfit_start": hist[0], "relative_misfit_best": min(hist),
+                            "n_truth": sc.n_zones.tolist(), "n_start": n0.tolist(), "n_final": nz.tolist(),
+                            "max_rel_err_n_final": float(np.max(np.abs(nz - sc.n_zones) / sc.n_zones))}
+    return out
+
+
+# ---------------------------------------------------------------------------
+def bench_ensemble(args):
+    """32 uneven members, one launch per step vs one run after the other."""
+    from cuteflow_amd import scenarios as SC
+    from cuteflow_amd.solver import BatchSolver, Solver
+
+    nsteps = 2000
+    ens = SC.ensemble(32)
+    cells = sum(e.mesh.nelt for e in ens)
+    out = {"workload": "32 members of the 20 km x 2 km reach meshed at 50k-500k cells, peak inflow 500-4000 "
+                       "m3/s, n in 8 zones 0.02-0.08, WAF + semi-implicit friction, 2000 steps, own dt each",
+           "members": len(ens), "cells": cells, "sizes": [e.mesh.nelt for e in ens]}
+    with BatchSolver([e.mesh for e in ens], [e.params for e in ens]) as bs:
+        ts = []
+        for _ in range(2):
+            for b, e in enumerate(ens):
+                bs.set_state(b, e.state)
+            sync()
+            t = time.perf_counter()
+            hist, done, st = bs.step(nsteps)
+            sync()
+            ts.append(time.perf_counter() - t)
+        t_batch = ts[-1]
+        assert int(done.min()) == nsteps and int(st.max()) == 0
+        frac = bs.last_run_stats()["active_fraction"]
+    t_seq = 0.0
+    for e in ens:
+        with Solver(e.mesh, e.params) as s:
+            for k in range(2):
+                s.set_state(e.state)
+                sync()
+                t = time.perf_counter()
+                s.step(nsteps)
+                sync()
+                if k == 1:
+                    t_seq += time.perf_counter() - t
+    log(f"bench: ensemble batched {t_batch*1e3:.0f} ms, sequential {t_seq*1e3:.0f} ms, "
+        f"speedup {t_seq / t_batch:.2f}")
+    out.update({"batched_s": t_batch, "sequential_s": t_seq, "speedup": t_seq / t_batch,
+                "batched_cell_updates_per_s": cells * nsteps / t_batch,
+                "sequential_cell_updates_per_s": cells * nsteps / t_seq,
+                "active_tile_fraction": frac})
+    return out
+
+
+# ---------------------------------------------------------------------------
+def bench_field(args):
+    """16 x 1M per-cell Manning fields: batched gradients."""
+    import numpy as np
+
+    from cuteflow_amd import calibration as CAL
+    from cuteflow_amd import scenarios as SC
+    from cuteflow_amd.adjoint import AdjointRun
+    from cuteflow_amd.solver import BatchSolver
+
+    nsteps, every, K = 3000, 10, 150
+    mem = SC.field_ensemble(16, 1_000_000, ngauges=50, iflux=1)
+    B = len(mem)
+    out = {"workload": "16 members on a 1M-cell 20 km x 2 km reach, per-cell log-normal n (median 0.04, "
+                       "sigma_log 0.3, 500 m correlation), own hydrographs, 3000 steps, checkpoints every "
+                       "150 steps, loss = misfit at 50 gauges (every 10 steps) + squared error of the soft "
+                       "maximum flood extent; observations from the member's field, gradient at +20 %",
+           "members": B, "cells": sum(e.mesh.nelt for e in mem), "scheme": "iflux 1 (HLLC), fricimplic 1"}
+    with BatchSolver([e.mesh for e in mem], [e.params for e in mem]) as bs:
+        U0 = [e.state for e in mem]
+        surf = [bs.geometry(b)["surf"] for b in range(B)]
+        ts = []
+        for _ in range(2):
+            for b in range(B):
+                bs.set_state(b, U0[b])
+            sync()
+            t = time.perf_counter()
+            bs.step(nsteps)
+            sync()
+            ts.append(time.perf_counter() - t)
+        t_fwd = ts[-1]
+        run = AdjointRun(bs, [e.gauges for e in mem], sample_every=every, checkpoint_every=K, want_hmax=True)
+        obs, hobs = run.forward(nsteps, U0=U0, manning=[e.mesh.manning for e in mem])
+        a_obs = [CAL.soft_extent(hobs[b], surf[b])[0] for b in range(B)]
+        n0 = [1.2 * e.mesh.manning for e in mem]
+        scale = [max(1.0, 0.01 * a) for a in a_obs]
+
+        def gradient():
+            etas, hm = run.forward(nsteps, U0=U0, manning=n0)
+            ge, gh, loss = [], [], 0.0
+            for b in range(B):
+                ge.append(2.0 * (etas[b] - obs[b]))
+                le, dh = CAL.extent_loss(hm[b], surf[b], a_obs[b], scale[b])
+                gh.append(dh)
+                loss += CAL.gauge_misfit(etas[b], obs[b]) + le
+            gn, _ = run.backward(ge, gh)
+            return gn, etas, hm, loss
+
+        gt = []
+        for _ in range(2):
+            sync()
+            t = time.perf_counter()
+            gn, etas, hm, loss = gradient()
+            sync()
+            gt.append(time.perf_counter() - t)
+        t_grad = gt[-1]
+        log(f"bench: field ensemble forward {t_fwd*1e3:.0f} ms, gradient {t_grad*1e3:.0f} ms "
+            f"(first {gt[0]*1e3:.0f} ms), ratio {t_grad / t_fwd:.2f}, {B / t_grad:.2f} grads/s")
+        # tangent-linear checks along 4 random directions (per-cell smooth
+        # relative perturbations of every member's field)
+        rng = np.random.default_rng(3)
+        checks = []
+        for d in range(4):
+            dn = [n0[b] * rng.normal(0.0, 0.1, n0[b].shape) for b in range(B)]
+            tl = run.tlm(dn)
+            dl_tlm = 0.0
+            for b in range(B):
+                dl_tlm += float(np.sum(2.0 * (etas[b] - obs[b]) * tl[b]["d_eta"]))
+                _, dh = CAL.extent_loss(hm[b], surf[b], a_obs[b], scale[b])
+                dl_tlm += float(np.sum(dh * tl[b]["d_hmax"]))
+            dl_adj = float(sum(np.sum(gn[b] * dn[b]) for b in range(B)))
+            checks.append({"tlm": dl_tlm, "adjoint": dl_adj, "rel_err": abs(dl_adj - dl_tlm) / abs(dl_tlm)})
+        log(f"bench: field ensemble TLM checks {[round(c['rel_err'], 16) for c in checks]}")
+        out.update({"forward_s": t_fwd, "gradient_s": t_grad, "gradient_first_call_s": gt[0],
+                    "adjoint_forward_ratio": t_grad / t_fwd, "grads_per_s": B / t_grad,
+                    "loss": loss, "tlm_checks": checks,
+                    "speedup_vs_fd_per_cell": 2.0 * mem[0].mesh.nelt * t_fwd / B / (t_grad / B)})
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--nx", type=int, default=3200)
+    ap.add_argument("--ny", type=int, default=320)
+    ap.add_argument("--order", choices=["rcm", "natural", "random"], default="rcm")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+                    help="target duration of the CPU-baseline sample (0 disables)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--options", type=int, default=0,
+                    help="cf_params.options bits: 1 keep the input cell order, 2 run every tile")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="before the timed region, run a device copy of this many MiB (3 times): a "
+                         "known-size kernel that calibrates HBM byte counters in a profile")
+    ap.add_argument("--sections", default="calibration,ensemble,field",
+                    help="comma-separated extra sections (calibration, ensemble, field); empty for none")
+    ap.add_argument("--lbfgs-iters", type=int, default=20)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: the time step runs on one subdomain per GPU and "
+              "this build has no multi-GPU path; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    torch.cuda.init()
+    result = bench_headline(args)
+    sections = [x for x in args.sections.split(",") if x]
+    if "calibration" in sections:
+        result["calibration"] = bench_calibration(args)
+    if "ensemble" in sections:
+        result["ensemble"] = bench_ensemble(args)
+    if "field" in sections:
+        result["batched_gradients"] = bench_field(args)
     print(json.dumps(result), flush=True)
 
 
