@@ -18,6 +18,8 @@ CF_MAX_HYDRO = 64
 
 CF_OPT_KEEP_ORDER = 1
 CF_OPT_NO_TILE_SKIP = 2
+CF_OPT_TILE_128 = 4
+CF_OPT_TILE_64 = 8
 
 CF_ADJ_FREEZE_LIMITERS = 1
 

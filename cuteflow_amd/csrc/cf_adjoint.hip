@@ -108,6 +108,7 @@ struct RevArgs {
     const double *dt;
     const double *tc;
     const unsigned char *ran;  // this step's row of tile activity
+    int tile_cells;
     int hstride;
     int step;
     int ncell;
@@ -126,7 +127,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(RevWaves<
     RevArgs r) {
     const int tile = blockIdx.x;
     const int b = r.tile_inst[tile];
-    const int i = tile * kTile + threadIdx.x;
+    const int i = tile * (int)blockDim.x + threadIdx.x;
     // a tile skipped in the recorded run holds dry cells with dry
     // neighbours: every contribution of its reverse is zero
     if (i >= r.inst_end[b] || !r.ran[tile]) return;
@@ -173,7 +174,7 @@ __global__ void gather_kernel(const int4 *__restrict__ nbr, const double4 *__res
                               const double *__restrict__ ghmax, int sidx, double4 *__restrict__ lam_out) {
     const int tile = blockIdx.x;
     const int b = tile_inst[tile];
-    const int c = tile * kTile + threadIdx.x;
+    const int c = tile * (int)blockDim.x + threadIdx.x;
     if (c >= inst_end[b]) return;
     if (!ran[tile] && ran_prev && !ran_prev[tile]) return;  // nobody reads this tile's cotangents
     double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -184,12 +185,12 @@ __global__ void gather_kernel(const int4 *__restrict__ nbr, const double4 *__res
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const int i = nbv[k];
-            if (i >= 0 && ran[i / kTile]) acc4(acc, R1[4 * (size_t)i + ((nb.w >> (2 * k)) & 3)]);
+            if (i >= 0 && ran[i / (int)blockDim.x]) acc4(acc, R1[4 * (size_t)i + ((nb.w >> (2 * k)) & 3)]);
         }
         if (R2ON)
             for (int e = r2_off[c]; e < r2_off[c + 1]; e++) {
                 const int src = r2_idx[e];
-                if (ran[(src / 6) / kTile]) acc4(acc, R2[src]);
+                if (ran[(src / 6) / (int)blockDim.x]) acc4(acc, R2[src]);
             }
     }
     if (argmax && argmax[c] == sidx) acc.x += ghmax[c];
@@ -203,7 +204,7 @@ __global__ void lam_init_kernel(const int *__restrict__ tile_inst, const int *__
                                 const double *__restrict__ ghmax, int sidx, double4 *__restrict__ lam) {
     const int tile = blockIdx.x;
     const int b = tile_inst[tile];
-    const int c = tile * kTile + threadIdx.x;
+    const int c = tile * (int)blockDim.x + threadIdx.x;
     double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
     if (c < inst_end[b]) {
         v = gstate[c];
@@ -246,7 +247,7 @@ __global__ void sample_hmax_kernel(StPair st, const InstCtrl *ctrl, const int *_
                                    int *__restrict__ argmax) {
     const int tile = blockIdx.x;
     const int b = tile_inst[tile];
-    const int c = tile * kTile + threadIdx.x;
+    const int c = tile * (int)blockDim.x + threadIdx.x;
     if (c >= inst_end[b]) return;
     const double h = cur_state(st.s, ctrl, b)[c].x;
     if (sidx == 0 || h > hmax[c]) {
@@ -259,7 +260,7 @@ __global__ void ckpt_kernel(StPair st, const InstCtrl *ctrl, const int *__restri
                             const int *__restrict__ inst_end, double4 *__restrict__ dst) {
     const int tile = blockIdx.x;
     const int b = tile_inst[tile];
-    const int c = tile * kTile + threadIdx.x;
+    const int c = tile * (int)blockDim.x + threadIdx.x;
     if (c >= inst_end[b]) return;
     dst[c] = cur_state(st.s, ctrl, b)[c];
 }
@@ -299,14 +300,14 @@ struct TlmArgs {
     const KParams *kp;
     const int *tile_inst, *inst_end;
     const double *dt, *tc;
-    int hstride, step, ncell;
+    int hstride, step, ncell, tile_cells;
 };
 
 template <int IFLUX, int FRIC>
 __global__ __launch_bounds__(kTile) void tlm_kernel(TlmArgs a) {
     const int tile = blockIdx.x;
     const int b = a.tile_inst[tile];
-    const int i = tile * kTile + threadIdx.x;
+    const int i = tile * (int)blockDim.x + threadIdx.x;
     if (i >= a.inst_end[b]) return;
     const TlmLd ld{a.st_in, a.dst_in, a.nbr, a.dfc, a.geo, a.manning, a.dman, a.ncell};
     const size_t hix = (size_t)b * a.hstride + a.step;
@@ -332,7 +333,7 @@ __global__ void tlm_hmax_kernel(const double4 *__restrict__ dst, const int *__re
                                 double *__restrict__ dhmax) {
     const int tile = blockIdx.x;
     const int b = tile_inst[tile];
-    const int c = tile * kTile + threadIdx.x;
+    const int c = tile * (int)blockDim.x + threadIdx.x;
     if (c >= inst_end[b]) return;
     if (argmax[c] == sidx) dhmax[c] = dst[c].x;
 }
@@ -611,7 +612,7 @@ static void enqueue_forward(cf_handle *h, AdjState *A, hipStream_t s) {
     const int S = A->nsteps + 1;
     const StPair sp{{h->st[0], h->st[1]}};
     launch_run_init(h, A->dt, A->tc, S, s);
-    hipLaunchKernelGGL(ckpt_kernel, dim3(h->ntiles), dim3(kTile), 0, s, sp, h->ctrl, h->tile_inst, h->inst_end,
+    hipLaunchKernelGGL(ckpt_kernel, dim3(h->ntiles), dim3(h->tile), 0, s, sp, h->ctrl, h->tile_inst, h->inst_end,
                        A->ckpt);
     StepArgs a = base_args(h);
     a.dt_hist = A->dt;
@@ -629,11 +630,11 @@ static void enqueue_forward(cf_handle *h, AdjState *A, hipStream_t s) {
                 hipLaunchKernelGGL(sample_gauges_kernel, dim3(grid_of(A->gtotal, 256)), dim3(256), 0, s, sp, h->ctrl,
                                    A->d_gcell, A->d_ginst, A->gtotal, sidx, A->eta);
             if (A->want_hmax)
-                hipLaunchKernelGGL(sample_hmax_kernel, dim3(h->ntiles), dim3(kTile), 0, s, sp, h->ctrl, h->tile_inst,
+                hipLaunchKernelGGL(sample_hmax_kernel, dim3(h->ntiles), dim3(h->tile), 0, s, sp, h->ctrl, h->tile_inst,
                                    h->inst_end, sidx, A->hmax, A->argmax);
         }
         if ((k + 1) % A->K == 0 && k + 1 < A->nsteps)
-            hipLaunchKernelGGL(ckpt_kernel, dim3(h->ntiles), dim3(kTile), 0, s, sp, h->ctrl, h->tile_inst,
+            hipLaunchKernelGGL(ckpt_kernel, dim3(h->ntiles), dim3(h->tile), 0, s, sp, h->ctrl, h->tile_inst,
                                h->inst_end, A->ckpt + (size_t)((k + 1) / A->K) * h->ncell);
     }
 }
@@ -789,7 +790,7 @@ extern "C" int cf_adj_set_seed(cf_handle *h, int inst, const double *g_eta, cons
 
 template <int IFLUX, int FRIC>
 static void launch_rev_t(const RevArgs &r, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((rev_kernel<IFLUX, FRIC>), dim3(grid), dim3(kTile), 0, s, r);
+    hipLaunchKernelGGL((rev_kernel<IFLUX, FRIC>), dim3(grid), dim3(r.tile_cells), 0, s, r);
 }
 
 static void launch_rev(const RevArgs &r, int iflux, int fric, int grid, hipStream_t s) {
@@ -808,7 +809,7 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
     // lambda of the final state: its seed + the seeds of the last sample
     const int last = (A->nsteps % A->every == 0) ? A->nsteps / A->every - 1 : -1;
     int cur = 0;
-    hipLaunchKernelGGL(lam_init_kernel, dim3(h->ntiles), dim3(kTile), 0, s, h->tile_inst, h->inst_end, A->gstate, am,
+    hipLaunchKernelGGL(lam_init_kernel, dim3(h->ntiles), dim3(h->tile), 0, s, h->tile_inst, h->inst_end, A->gstate, am,
                        A->ghmax, last, A->lam[cur]);
     if (last >= 0 && A->gtotal)
         hipLaunchKernelGGL(gauge_seed_kernel, dim3(grid_of(h->ninst, 64)), dim3(64), 0, s, A->d_gcell, A->d_goff,
@@ -837,6 +838,7 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
     r.tc = A->tc;
     r.hstride = S;
     r.ncell = h->ncell;
+    r.tile_cells = h->tile;
     for (int sg = A->nseg - 1; sg >= 0; sg--) {
         const int k0 = sg * A->K, k1 = std::min(k0 + A->K, A->nsteps);
         const double4 *ck = A->ckpt + (size_t)sg * N;
@@ -858,11 +860,11 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
             const int sidx = (k >= 1 && k % A->every == 0) ? k / A->every - 1 : -1;
             const unsigned char *ran_prev = k > 0 ? A->tile_ran + (size_t)(k - 1) * h->ntiles : nullptr;
             if (h->iflux == 3)
-                hipLaunchKernelGGL(gather_kernel<true>, dim3(h->ntiles), dim3(kTile), 0, s, h->nbr, A->R1, A->R2,
+                hipLaunchKernelGGL(gather_kernel<true>, dim3(h->ntiles), dim3(h->tile), 0, s, h->nbr, A->R1, A->R2,
                                    A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, r.ran, ran_prev, am, A->ghmax,
                                    sidx, A->lam[1 - cur]);
             else
-                hipLaunchKernelGGL(gather_kernel<false>, dim3(h->ntiles), dim3(kTile), 0, s, h->nbr, A->R1, A->R2,
+                hipLaunchKernelGGL(gather_kernel<false>, dim3(h->ntiles), dim3(h->tile), 0, s, h->nbr, A->R1, A->R2,
                                    A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, r.ran, ran_prev, am, A->ghmax,
                                    sidx, A->lam[1 - cur]);
             if (sidx >= 0 && A->gtotal)
@@ -925,7 +927,7 @@ extern "C" int cf_tlm_set_direction(cf_handle *h, int inst, const double *dn, co
 
 template <int IFLUX, int FRIC>
 static void launch_tlm_t(const TlmArgs &t, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((tlm_kernel<IFLUX, FRIC>), dim3(grid), dim3(kTile), 0, s, t);
+    hipLaunchKernelGGL((tlm_kernel<IFLUX, FRIC>), dim3(grid), dim3(t.tile_cells), 0, s, t);
 }
 
 // Tangent-linear run over the recorded dt and tc from the recorded initial
@@ -953,6 +955,7 @@ static void enqueue_tlm(cf_handle *h, AdjState *A, hipStream_t s) {
     t.tc = A->tc;
     t.hstride = A->nsteps + 1;
     t.ncell = h->ncell;
+    t.tile_cells = h->tile;
     double4 *vb[2] = {A->tst[0], A->tst[1]};
     double4 *db[2] = {A->tdst[0], A->tdst[1]};
     if (A->want_hmax) fill0(A->dhmax, N, s);
@@ -969,7 +972,7 @@ static void enqueue_tlm(cf_handle *h, AdjState *A, hipStream_t s) {
                 hipLaunchKernelGGL(tlm_sample_kernel, dim3(grid_of(A->gtotal, 256)), dim3(256), 0, s,
                                    vb[(k + 1) & 1], db[(k + 1) & 1], A->d_gcell, A->gtotal, sidx, A->teta, A->deta);
             if (A->want_hmax)
-                hipLaunchKernelGGL(tlm_hmax_kernel, dim3(h->ntiles), dim3(kTile), 0, s, db[(k + 1) & 1], h->tile_inst,
+                hipLaunchKernelGGL(tlm_hmax_kernel, dim3(h->ntiles), dim3(h->tile), 0, s, db[(k + 1) & 1], h->tile_inst,
                                    h->inst_end, A->argmax, sidx, A->dhmax);
         }
     }

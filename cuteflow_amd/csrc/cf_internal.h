@@ -14,7 +14,9 @@
 
 namespace cf {
 
-// cells per tile = threads per workgroup of the step kernel
+// cells per tile = threads per workgroup of the per-cell kernels: 256 by
+// default, 128 or 64 by option (h->tile); kernels are compiled for the
+// largest and read the tile size from blockDim.x
 constexpr int kTile = 256;
 constexpr int kWaves = kTile / 64;
 
@@ -61,6 +63,7 @@ struct StepArgs {
     // tiles that did not run (their cells are dry and stay so).
     unsigned char *tile_ran;
     int ncell;        // padded cell count (geo stride)
+    int tile_cells;   // cells per tile = workgroup size
     int hist_stride;
     int step;         // index of this step in the run (absolute in a replay)
     int skip;         // dry-tile skipping enabled
@@ -106,6 +109,7 @@ struct cf_handle {
     hipStream_t stream = nullptr;
     bool own_stream = false;
     int iflux = 1, fric = 0, skip = 1;
+    int tile = cf::kTile;              // cells per tile (workgroup size)
     // device arrays
     double4 *st[2] = {nullptr, nullptr};
     int4 *nbr = nullptr;

@@ -64,8 +64,12 @@ __device__ __forceinline__ double wave_min(double v) {
 #endif
 
 // FRIC: 0 no friction, 1 explicit (fricimplic 0), 2 fricimplic 1, 3 fricimplic 2
+// `kp` is a.kp passed again as a kernel parameter: `__restrict__` only takes
+// effect on parameters, and with it the compiler keeps the per-instance
+// constants in scalar registers (scalar loads) instead of reloading them per
+// lane after the first store; that also frees vector registers (fewer spills).
 template <int IFLUX, int FRIC>
-__global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs a) {
+__global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs a, const KParams *__restrict__ kp) {
     __shared__ double s_min[kWaves];
     const int tile = blockIdx.x;
     const int b = a.tile_inst[tile];
@@ -92,20 +96,19 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
             }
             return;
         }
-        if (ran_row && threadIdx.x == 0) ran_row[tile] = 1;
     } else if (ran_row && !ran_row[tile]) {
         // skipped in the recorded run: the state passes through unchanged
-        const int ic = tile * kTile + threadIdx.x;
+        const int ic = tile * (int)blockDim.x + threadIdx.x;
         a.out_x[ic] = a.in_x[ic];
         return;
     }
     const double4 *__restrict__ st_in = replay ? a.in_x : a.st[(c.par + a.step) & 1];
     double4 *__restrict__ st_out = replay ? a.out_x : a.st[(c.par + a.step + 1) & 1];
-    const int i = tile * kTile + threadIdx.x;
+    const int i = tile * (int)blockDim.x + threadIdx.x;
     const size_t hix = (size_t)b * a.hist_stride + a.step;
     const double dt = replay ? a.dt_fix[hix] : c.dt;
     const double tc = replay ? a.tc_fix[hix] : c.tc;
-    const KParams &p = a.kp[b];
+    const KParams &p = kp[b];
     const double tls = p.tolisec;
     double dti = 1000000.0;
     int isnan_local = 0, wet_local = 0;
@@ -145,13 +148,16 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
     if (threadIdx.x == 0) {
         double bm = s_min[0];
 #pragma unroll
-        for (int w = 1; w < kWaves; w++) bm = dmin(bm, s_min[w]);
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) bm = dmin(bm, s_min[w]);
         a.partial[tile] = anynan ? __builtin_nan("") : bm;
+        // stored last: an early store would keep the uniform loads above
+        // from being scalar loads (they could alias it)
+        if (ran_row) ran_row[tile] = 1;
     }
     // a wet tile keeps itself and its neighbour tiles running for two steps
     if (a.skip && __syncthreads_or(wet_local)) {
         const int o = a.tile_adj_off[tile], n = a.tile_adj_off[tile + 1] - o;
-        for (int k = threadIdx.x; k <= n; k += kTile)
+        for (int k = threadIdx.x; k <= n; k += (int)blockDim.x)
             a.stamp[k == 0 ? tile : a.tile_adj[o + k - 1]] = c.gstep + 1u;
     }
 }
@@ -234,7 +240,7 @@ __global__ __launch_bounds__(kTile) void cfl_kernel(const double4 *__restrict__ 
                                                      int cell0, int cell_end, const KParams *kp, int b,
                                                      double *__restrict__ part) {
     __shared__ double s_min[kWaves];
-    const int i = cell0 + blockIdx.x * kTile + threadIdx.x;
+    const int i = cell0 + blockIdx.x * (int)blockDim.x + threadIdx.x;
     const KParams &p = kp[b];
     double dti = 1000000.0;
     if (i < cell_end) {
@@ -253,7 +259,7 @@ __global__ __launch_bounds__(kTile) void cfl_kernel(const double4 *__restrict__ 
     __syncthreads();
     if (threadIdx.x == 0) {
         double bm = s_min[0];
-        for (int w = 1; w < kWaves; w++) bm = dmin(bm, s_min[w]);
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) bm = dmin(bm, s_min[w]);
         part[blockIdx.x] = bm;
     }
 }
@@ -391,7 +397,7 @@ static int compute_dt(cf_handle *h, int b) {
     const int c0 = h->cell_off[b], nt = h->tile_off[b + 1] - h->tile_off[b];
     const int cur = (h->hctrl[b].par + h->hctrl[b].steps) & 1;
     double *part = h->partial + h->tile_off[b];
-    hipLaunchKernelGGL(cfl_kernel, dim3(nt), dim3(kTile), 0, h->stream, h->st[cur], h->geo + G_COT * (size_t)h->ncell,
+    hipLaunchKernelGGL(cfl_kernel, dim3(nt), dim3(h->tile), 0, h->stream, h->st[cur], h->geo + G_COT * (size_t)h->ncell,
                        c0, c0 + h->nelt[b], h->d_kp, b, part);
     hipLaunchKernelGGL(reset_inst_kernel, dim3(1), dim3(1024), 0, h->stream, part, nt, h->d_kp, b, h->ctrl);
     HIPCHK(hipGetLastError());
@@ -464,14 +470,19 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
         for (size_t i = 0; i < 3 * (size_t)m->nelt; i++)
             if (m->conn[i] < 1 || m->conn[i] > m->nnt) return CF_ERR_MESH;
     }
+    if ((params[0].options & ~(CF_OPT_KEEP_ORDER | CF_OPT_NO_TILE_SKIP | CF_OPT_TILE_128 | CF_OPT_TILE_64)) ||
+        ((params[0].options & CF_OPT_TILE_128) && (params[0].options & CF_OPT_TILE_64)))
+        return CF_ERR_ARG;
+    const int tile = (params[0].options & CF_OPT_TILE_64) ? 64 : (params[0].options & CF_OPT_TILE_128) ? 128 : kTile;
     std::vector<cfprep::Prep> prep(ninst);
     for (int b = 0; b < ninst; b++) {
-        int st = cfprep::prepare(meshes + b, params + b, kTile, (params[b].options & CF_OPT_KEEP_ORDER) != 0, prep[b]);
+        int st = cfprep::prepare(meshes + b, params + b, tile, (params[b].options & CF_OPT_KEEP_ORDER) != 0, prep[b]);
         if (st) return st;
     }
     cf_handle *h = new (std::nothrow) cf_handle();
     if (!h) return CF_ERR_NOMEM;
     h->ninst = ninst;
+    h->tile = tile;
     h->iflux = params[0].iflux;
     h->fric = params[0].friction == 0 ? 0 : 1 + params[0].fricimplic;
     h->nelt.resize(ninst); h->nnt.resize(ninst); h->cell_off.resize(ninst); h->tile_off.resize(ninst + 1);
@@ -492,9 +503,9 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
         h->nnt[b] = meshes[b].nnt;
         h->cell_off[b] = cells;
         h->tile_off[b] = tiles;
-        const int nt = grid_of(meshes[b].nelt, kTile);
+        const int nt = grid_of(meshes[b].nelt, tile);
         tiles += nt;
-        cells += nt * kTile;
+        cells += nt * tile;
         h->max_cells = std::max(h->max_cells, meshes[b].nelt);
         h->kp[b] = make_kparams(params[b], meshes[b].nelt, b * kHydroPerInst);
         h->hperm[b] = prep[b].perm;
@@ -685,7 +696,7 @@ extern "C" int cf_get_time(cf_handle *h, double *tc, double *dt) { return cf_bat
 
 template <int IFLUX, int FRIC>
 static void launch_step_t(const StepArgs &a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((step_kernel<IFLUX, FRIC>), dim3(grid), dim3(kTile), 0, s, a);
+    hipLaunchKernelGGL((step_kernel<IFLUX, FRIC>), dim3(grid), dim3(a.tile_cells), 0, s, a, a.kp);
 }
 
 typedef void (*step_launcher)(const StepArgs &, int, hipStream_t);
@@ -741,6 +752,7 @@ StepArgs cf::base_args(cf_handle *h) {
     a.xs = nullptr;
     a.tile_ran = nullptr;
     a.ncell = h->ncell;
+    a.tile_cells = h->tile;
     a.hist_stride = h->cap + 1;
     a.skip = h->skip;
     return a;
@@ -870,7 +882,7 @@ extern "C" int cf_last_run_stats(cf_handle *h, long long *tiles_run, long long *
     }
     if (tiles_run) *tiles_run = r;
     if (tiles_total) *tiles_total = t;
-    if (tile_cells) *tile_cells = kTile;
+    if (tile_cells) *tile_cells = h->tile;
     return CF_OK;
 }
 

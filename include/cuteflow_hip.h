@@ -30,6 +30,8 @@ extern "C" {
  * skipping. */
 #define CF_OPT_KEEP_ORDER 1      /* keep the caller's cell order on the device */
 #define CF_OPT_NO_TILE_SKIP 2    /* run every tile every step */
+#define CF_OPT_TILE_128 4        /* 128-cell tiles (workgroups) instead of 256 */
+#define CF_OPT_TILE_64 8         /* 64-cell tiles (one wavefront) */
 
 enum {
     CF_OK = 0,

@@ -146,10 +146,11 @@ def test_river_hllc_transmissive_outlet():
     _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
 
 
-@pytest.mark.parametrize("options", [0, 1, 2, 3])
+@pytest.mark.parametrize("options", [0, 1, 2, 3, 4, 8, 4 | 1, 8 | 2])
 def test_layout_options_identical(options):
-    """Internal Hilbert order and dry-tile skipping (on/off) give the oracle's
-    bits on a wetting floodplain over many steps."""
+    """Internal Hilbert order, dry-tile skipping (on/off) and the tile size
+    (256, 128, 64 cells) give the oracle's bits on a wetting floodplain over
+    many steps."""
     m, p, U0 = river_case(nx=120, ny=24, q_peak=2000.0, t_ramp=30.0)
     prep = O.Prepared(m)
     ref = O.run(prep, p, U0, 800)
@@ -163,7 +164,9 @@ def test_layout_options_identical(options):
     _compare(U, ref["U"], np.concatenate([hist[:-1], hist2]), ref["dt_hist"], p.tolisec)
     if options & 2:
         assert stats["active_fraction"] == 1.0
-    assert stats["tiles_total"] == 400 * ((m.nelt + 255) // 256)
+    tile = 64 if options & 8 else (128 if options & 4 else 256)
+    assert stats["tile_cells"] == tile
+    assert stats["tiles_total"] == 400 * ((m.nelt + tile - 1) // tile)
 
 
 def test_reordering_invariance():
