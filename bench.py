@@ -9,7 +9,9 @@ correlation), a 3 m deep main channel in a floodplain that starts dry,
 Manning n 0.03 / 0.06, inflow ramped 0 -> 3000 m3/s over 1 h, fixed outlet
 level, dry threshold 1e-3 m, WAF second order with the superbee limiter and
 semi-implicit friction (fricimplic 1).  A step is one time step over the
-whole mesh.
+whole mesh.  The schedule (cells per tile, dry-tile skipping, tile order
+over the XCDs) is picked by the autotuner before the run (--options auto),
+outside the timed region.
 
 Further sections of the same JSON line (each skippable):
 - "calibration": adjoint/forward cost ratio of a gauge-misfit gradient on a
@@ -18,7 +20,9 @@ Further sections of the same JSON line (each skippable):
   against central differences and tangent-linear derivatives on each zone;
   20 L-BFGS iterations from n 20 % above the truth;
 - "ensemble": 32 members of 50k-500k cells in one launch per step, 2000
-  steps, against the same 32 runs one after the other;
+  steps, with the schedule (cells per tile, dry-tile skipping, tile order
+  over the XCDs) picked by the autotuner, against the default schedule and
+  against the same 32 runs one after the other;
 - "batched_gradients": 16 members on a 1M-cell reach with per-cell
   log-normal Manning fields and their own hydrographs, 3000 steps,
   checkpoints every 150 steps, loss = misfit at 50 gauges + squared error of
@@ -95,12 +99,22 @@ def bench_headline(args):
     import torch
 
     from cuteflow_amd import _lib
-    from cuteflow_amd.solver import Solver
+    from cuteflow_amd.solver import Solver, autotune
 
     t0 = time.perf_counter()
     m, p, U0 = make_workload(args.nx, args.ny, args.order)
     log(f"bench: workload nelt={m.nelt} order={args.order} built in {time.perf_counter() - t0:.1f}s")
-    s = Solver(m, p, options=args.options)
+    schedule = {"autotuned": args.options == "auto"}
+    if args.options == "auto":
+        t0 = time.perf_counter()
+        options, table = autotune([m], [p], [U0], nsteps=args.tune_steps)
+        schedule.update({"tune_steps": args.tune_steps, "tune_seconds": time.perf_counter() - t0,
+                         "candidates": [dict(describe_options(o), options=o, ms=ms) for o, ms in table]})
+        log(f"bench: autotune {schedule['tune_seconds']:.1f} s, best {describe_options(options)}")
+    else:
+        options = int(args.options)
+    schedule.update(describe_options(options), options=options)
+    s = Solver(m, p, options=options)
     s.set_state(U0)
     lib = s.lib
     if args.warmup:
@@ -167,8 +181,9 @@ def bench_headline(args):
                         "0->3000 m3/s over 1 h, fixed outlet level, dry threshold 1e-3 m",
             "nelt": m.nelt,
             "order": args.order,
-            "device_order": "caller's" if args.options & 1 else "Hilbert curve of centroids",
-            "dry_tile_skipping": not (args.options & 2),
+            "device_order": "caller's" if options & 1 else "Hilbert curve of centroids",
+            "dry_tile_skipping": not (options & 2),
+            "schedule": schedule,
             "iflux": 3,
             "fricimplic": 1,
             "tc_end": s.tc,
@@ -310,30 +325,52 @@ def bench_calibration(args):
 
 
 # ---------------------------------------------------------------------------
+def describe_options(o):
+    """Schedule named by cf_params.options bits."""
+    from cuteflow_amd import _lib
+    return {"tile_cells": 64 if o & _lib.CF_OPT_TILE_64 else (128 if o & _lib.CF_OPT_TILE_128 else 256),
+            "skip_dry_tiles": not (o & _lib.CF_OPT_NO_TILE_SKIP), "xcd_contiguous": bool(o & _lib.CF_OPT_XCD_CONTIG)}
+
+
 def bench_ensemble(args):
-    """32 uneven members, one launch per step vs one run after the other."""
+    """32 uneven members, one launch per step with the autotuned schedule,
+    vs the default schedule and vs one run after the other."""
+    import numpy as np
+
     from cuteflow_amd import scenarios as SC
-    from cuteflow_amd.solver import BatchSolver, Solver
+    from cuteflow_amd.solver import BatchSolver, Solver, autotune
 
     nsteps = 2000
     ens = SC.ensemble(32)
     cells = sum(e.mesh.nelt for e in ens)
+    meshes, params, states = [e.mesh for e in ens], [e.params for e in ens], [e.state for e in ens]
     out = {"workload": "32 members of the 20 km x 2 km reach meshed at 50k-500k cells, peak inflow 500-4000 "
                        "m3/s, n in 8 zones 0.02-0.08, WAF + semi-implicit friction, 2000 steps, own dt each",
            "members": len(ens), "cells": cells, "sizes": [e.mesh.nelt for e in ens]}
-    with BatchSolver([e.mesh for e in ens], [e.params for e in ens]) as bs:
-        ts = []
-        for _ in range(2):
-            for b, e in enumerate(ens):
-                bs.set_state(b, e.state)
-            sync()
-            t = time.perf_counter()
-            hist, done, st = bs.step(nsteps)
-            sync()
-            ts.append(time.perf_counter() - t)
-        t_batch = ts[-1]
-        assert int(done.min()) == nsteps and int(st.max()) == 0
-        frac = bs.last_run_stats()["active_fraction"]
+    t = time.perf_counter()
+    best, table = autotune(meshes, params, states, nsteps=args.tune_steps)
+    t_tune = time.perf_counter() - t
+    log(f"bench: autotune {t_tune:.1f} s, best {describe_options(best)}")
+    out["autotune"] = {"steps": args.tune_steps, "seconds": t_tune, "best_options": best, "best": describe_options(best),
+                       "candidates": [dict(describe_options(o), options=o, ms=ms) for o, ms in table]}
+
+    def batched(options):
+        with BatchSolver(meshes, params, options=options) as bs:
+            ts = []
+            for _ in range(2):
+                for b, U in enumerate(states):
+                    bs.set_state(b, U)
+                sync()
+                t = time.perf_counter()
+                hist, done, st = bs.step(nsteps)
+                sync()
+                ts.append(time.perf_counter() - t)
+            assert int(done.min()) == nsteps and int(st.max()) == 0
+            return ts[-1], bs.last_run_stats()["active_fraction"], [bs.get_state(b) for b in (0, len(ens) - 1)]
+
+    t_batch, frac, u_best = batched(best)
+    t_default, _, u_default = batched(0)
+    same = all(np.array_equal(a, b) for a, b in zip(u_best, u_default))
     t_seq = 0.0
     for e in ens:
         with Solver(e.mesh, e.params) as s:
@@ -345,12 +382,12 @@ def bench_ensemble(args):
                 sync()
                 if k == 1:
                     t_seq += time.perf_counter() - t
-    log(f"bench: ensemble batched {t_batch*1e3:.0f} ms, sequential {t_seq*1e3:.0f} ms, "
-        f"speedup {t_seq / t_batch:.2f}")
-    out.update({"batched_s": t_batch, "sequential_s": t_seq, "speedup": t_seq / t_batch,
-                "batched_cell_updates_per_s": cells * nsteps / t_batch,
-                "sequential_cell_updates_per_s": cells * nsteps / t_seq,
-                "active_tile_fraction": frac})
+    log(f"bench: ensemble batched {t_batch*1e3:.0f} ms (default schedule {t_default*1e3:.0f} ms), "
+        f"sequential {t_seq*1e3:.0f} ms, speedup {t_seq / t_batch:.2f}")
+    out.update({"batched_s": t_batch, "batched_default_schedule_s": t_default, "sequential_s": t_seq,
+                "speedup": t_seq / t_batch, "batched_cell_updates_per_s": cells * nsteps / t_batch,
+                "sequential_cell_updates_per_s": cells * nsteps / t_seq, "active_tile_fraction": frac,
+                "tuned_equals_default_bits": bool(same)})
     return out
 
 
@@ -445,11 +482,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0,
                     help="target duration of the CPU-baseline sample (0 disables)")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--options", type=int, default=0,
-                    help="cf_params.options bits: 1 keep the input cell order, 2 run every tile")
+    ap.add_argument("--options", default="auto",
+                    help="'auto' (the autotuner picks the schedule) or cf_params.options bits: 1 keep the input "
+                         "cell order, 2 run every tile, 4 / 8 tiles of 128 / 64 cells, 16 XCD-contiguous tile order")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="before the timed region, run a device copy of this many MiB (3 times): a "
                          "known-size kernel that calibrates HBM byte counters in a profile")
+    ap.add_argument("--tune-steps", type=int, default=100, help="steps per autotuner candidate")
     ap.add_argument("--sections", default="calibration,ensemble,field",
                     help="comma-separated extra sections (calibration, ensemble, field); empty for none")
     ap.add_argument("--lbfgs-iters", type=int, default=20)

@@ -20,6 +20,8 @@ CF_OPT_KEEP_ORDER = 1
 CF_OPT_NO_TILE_SKIP = 2
 CF_OPT_TILE_128 = 4
 CF_OPT_TILE_64 = 8
+CF_OPT_XCD_CONTIG = 16
+CF_TUNE_NCAND = 12
 
 CF_ADJ_FREEZE_LIMITERS = 1
 
@@ -115,6 +117,8 @@ SIGNATURES = {
     "cf_batch_set_hydrograph": (C.c_int, [_H, C.c_int, C.c_int, C.c_int, _DP, _DP]),
     "cf_batch_set_manning": (C.c_int, [_H, C.c_int, C.c_void_p]),
     "cf_batch_get_geometry": (C.c_int, [_H, C.c_int, _DP, _DP, _DP, _DP]),
+    "cf_batch_autotune": (C.c_int, [C.c_int, C.POINTER(CfMesh), C.POINTER(CfParams), C.POINTER(C.c_void_p),
+                                    C.c_int, _IP, _DP]),
     "cf_adj_set_flags": (C.c_int, [_H, C.c_int]),
     "cf_adj_set_gauges": (C.c_int, [_H, C.c_int, C.c_int, _IP]),
     "cf_adj_forward": (C.c_int, [_H, C.c_int, C.c_int, C.c_int, C.c_int]),

@@ -10,16 +10,18 @@
 //
 // Reverse sweep, one segment of K steps at a time from the last: the
 // segment is replayed from its checkpoint (same step kernel, dt/tc from the
-// record, every tile) keeping each step's input state and residual x; then
-// for each step, last to first,
-//   rev_kernel     one thread per cell: cell_reverse() writes the cotangent
-//                  of the cell's own old state, of its three neighbours'
-//                  (R1) and of its WAF second-ring upwind cells (R2), and
-//                  adds the Manning cotangent to gn;
-//   gather_kernel  one thread per cell: sums the contributions addressed to
-//                  it in a fixed order (own, the neighbours' R1 slots that
-//                  face it, its R2 entries by a CSR list) and the seeds of
-//                  the step.
+// record, the tiles that ran in the record) keeping each step's input state
+// and residual x; then for each step k, last to first, one launch of
+//   revg_kernel    one thread per cell of the tiles that ran at step k:
+//                  sums the contributions addressed to its cell by the
+//                  reverse of step k + 1 in a fixed order (own, the
+//                  neighbours' R1 slots that face it, its R2 entries by a
+//                  CSR list) plus the seeds of the state after step k, then
+//                  cell_reverse() writes the cotangent of the cell's own old
+//                  state, of its three neighbours' (R1) and of its WAF
+//                  second-ring upwind cells (R2), and adds the Manning
+//                  cotangent to gn (R1/R2 double-buffered by step parity);
+// and a last gather gives the initial-state cotangent.
 // No atomics: every gradient is reproducible bit for bit.
 //
 // Tangent-linear model: the step body evaluated on dual numbers, replaying
@@ -51,6 +53,8 @@ struct AdjState {
     std::vector<std::vector<int>> gcells;  // per instance: caller-local gauge cells
     std::vector<int> goff;                 // per instance: first gauge (ninst + 1)
     int *d_gcell = nullptr;                // internal cell of each gauge
+    int *d_gfirst = nullptr;               // per cell: first gauge at the cell, -1 if none
+    int *d_gnext = nullptr;                // per gauge: next gauge at the same cell, -1 if none
     int *d_ginst = nullptr;                // instance of each gauge
     int *d_goff = nullptr;
     double *eta = nullptr, *geta = nullptr, *deta = nullptr;  // nsamp x gtotal
@@ -64,8 +68,8 @@ struct AdjState {
     double *xs = nullptr;     // K x 3 x ncell: residual x of each step of a segment
     double4 *lam[2] = {nullptr, nullptr};
     double4 *gstate = nullptr;  // final-state seed
-    double4 *R1 = nullptr;      // 4 x ncell: neighbours 0..2, own
-    double4 *R2 = nullptr;      // 6 x ncell: face k, slot s at 2k + s
+    double4 *R1[2] = {nullptr, nullptr};  // 4 x ncell: neighbours 0..2, own (by step parity)
+    double4 *R2[2] = {nullptr, nullptr};  // 6 x ncell: face k, slot s at 2k + s
     int *r2_off = nullptr, *r2_idx = nullptr;
     double *gn = nullptr;
     unsigned char *tile_ran = nullptr;  // nsteps x ntiles: tile ran in the recorded run
@@ -101,17 +105,32 @@ struct RevArgs {
     const int *tile_inst;
     const int *inst_end;
     const double *xs;
-    const double4 *lam;
-    double4 *R1;
+    const double4 *lam;        // cotangent of the state after the last step (when ran_next is null)
+    const double4 *R1in;       // contributions of the reverse of step k + 1
+    const double4 *R2in;
+    double4 *R1;               // contributions of this step's reverse
     double4 *R2;
+    const int *r2_off;
+    const int *r2_idx;
     double *gn;
     const double *dt;
     const double *tc;
-    const unsigned char *ran;  // this step's row of tile activity
+    const unsigned char *ran;       // this step's row of tile activity
+    const unsigned char *ran_next;  // step k + 1's row; null for the last step
+    // seeds of the state after this step: hmax of sample sidx, gauge
+    // levels of that sample (geta_row, null if none; gauges of a cell
+    // listed by gfirst/gnext in increasing index)
+    const int *argmax;
+    const double *ghmax;
+    const double *geta_row;
+    const int *gfirst;
+    const int *gnext;
+    int sidx;
     int tile_cells;
     int hstride;
     int step;
     int ncell;
+    int xcd;  // workgroup -> tile order (tile_of_block)
 };
 
 // Occupancy target of the reverse kernel (waves per SIMD): the first-order
@@ -122,22 +141,78 @@ struct RevWaves {
     static constexpr int value = IFLUX == 1 ? 3 : 1;
 };
 
+__device__ __forceinline__ void acc4(double4 &a, const double4 &b) {
+    a.x += b.x;
+    a.y += b.y;
+    a.z += b.z;
+}
+
+// Cotangent of cell c's state after step k: the contributions the reverse
+// of step k + 1 addressed to it, summed in a fixed order (own, the
+// neighbours' R1 slots that face it, its R2 entries by a CSR list;
+// contributions of tiles that did not run at step k + 1 are zero and not
+// read), then the seeds of that state (hmax of sample sidx, then the gauge
+// levels of that sample in increasing gauge index).  `tiles_ran` is the
+// activity row of step k + 1; a cell whose tile did not run gets its seeds
+// only.
+template <bool R2ON>
+__device__ __forceinline__ double4 gather_cell(int c, int tcells, const unsigned char *__restrict__ tiles_ran,
+                                               const int4 *__restrict__ nbr, const double4 *__restrict__ R1,
+                                               const double4 *__restrict__ R2, const int *__restrict__ r2_off,
+                                               const int *__restrict__ r2_idx, const int *__restrict__ argmax,
+                                               const double *__restrict__ ghmax, int sidx,
+                                               const double *__restrict__ geta_row, const int *__restrict__ gfirst,
+                                               const int *__restrict__ gnext) {
+    double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (tiles_ran[c / tcells]) {
+        acc = R1[4 * (size_t)c + 3];
+        const int4 nb = nbr[c];
+        const int nbv[3] = {nb.x, nb.y, nb.z};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int i = nbv[k];
+            if (i >= 0 && tiles_ran[i / tcells]) acc4(acc, R1[4 * (size_t)i + ((nb.w >> (2 * k)) & 3)]);
+        }
+        if (R2ON)
+            for (int e = r2_off[c]; e < r2_off[c + 1]; e++) {
+                const int src = r2_idx[e];
+                if (tiles_ran[(src / 6) / tcells]) acc4(acc, R2[src]);
+            }
+    }
+    if (argmax && argmax[c] == sidx) acc.x += ghmax[c];
+    if (geta_row)
+        for (int g = gfirst[c]; g >= 0; g = gnext[g]) acc.x += geta_row[g];
+    acc.w = 0.0;
+    return acc;
+}
+
+// Reverse of step k, one thread per cell of the tiles that ran at step k
+// (a tile skipped in the recorded run holds dry cells with dry neighbours:
+// every contribution of its reverse is zero).  The thread first forms its
+// cell's cotangent of the state after step k from the contributions of
+// step k + 1's reverse (gather_cell; for the last step it is read from
+// r.lam), then cell_reverse() gives the cotangent of the cell's own old
+// state, of its three neighbours' (R1) and of its WAF second-ring upwind
+// cells (R2), and the Manning cotangent added to gn.  R1/R2 alternate
+// between two buffers from step to step, so the gather of step k + 1 and
+// the reverse of step k share one launch.
 template <int IFLUX, int FRIC>
-__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(RevWaves<IFLUX>::value, 8))) void rev_kernel(
-    RevArgs r) {
-    const int tile = blockIdx.x;
+__global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(RevWaves<IFLUX>::value, 8))) void revg_kernel(
+    RevArgs r, const KParams *__restrict__ kp) {
+    const int tile = tile_of_block(r.xcd);
     const int b = r.tile_inst[tile];
     const int i = tile * (int)blockDim.x + threadIdx.x;
-    // a tile skipped in the recorded run holds dry cells with dry
-    // neighbours: every contribution of its reverse is zero
     if (i >= r.inst_end[b] || !r.ran[tile]) return;
+    const double4 l4 = r.ran_next ? gather_cell<IFLUX == 3>(i, (int)blockDim.x, r.ran_next, r.nbr, r.R1in, r.R2in,
+                                                            r.r2_off, r.r2_idx, r.argmax, r.ghmax, r.sidx,
+                                                            r.geta_row, r.gfirst, r.gnext)
+                                  : r.lam[i];
     const DevLd ld{r.st, r.nbr, r.dfc, r.geo, r.manning, r.ncell};
     const size_t hix = (size_t)b * r.hstride + r.step;
     const double x[3] = {r.xs[i], r.xs[(size_t)r.ncell + i], r.xs[2 * (size_t)r.ncell + i]};
-    const double4 l4 = r.lam[i];
     const double lam[3] = {l4.x, l4.y, l4.z};
     double own[3], r1[3][3], r2[3][2][3], gn;
-    cell_reverse<IFLUX, FRIC>(ld, r.kp[b], r.hydro, i, r.dt[hix], r.tc[hix], x, lam, own, r1, r2, gn);
+    cell_reverse<IFLUX, FRIC>(ld, kp[b], r.hydro, i, r.dt[hix], r.tc[hix], x, lam, own, r1, r2, gn);
 #pragma unroll
     for (int k = 0; k < 3; k++) r.R1[4 * (size_t)i + k] = make_double4(r1[k][0], r1[k][1], r1[k][2], 0.0);
     r.R1[4 * (size_t)i + 3] = make_double4(own[0], own[1], own[2], 0.0);
@@ -153,49 +228,20 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(RevWaves<
     r.gn[i] += gn;
 }
 
-__device__ __forceinline__ void acc4(double4 &a, const double4 &b) {
-    a.x += b.x;
-    a.y += b.y;
-    a.z += b.z;
-}
-
-// Cotangent of each cell's old state: the sum of the contributions the
-// reverse kernel addressed to it, in a fixed order (contributions of tiles
-// that did not run at this step are zero and not read), plus the hmax seed
-// of sample `sidx`.  A tile that did not run gets only its seeds, and only
-// when the reverse of the previous step will read them (ran_prev; null at
-// the first step, whose cotangents are the initial-state gradient).
+// Cotangent of the initial state: the gather of step 0's contributions
+// (the initial state carries no seeds), every cell.
 template <bool R2ON>
 __global__ void gather_kernel(const int4 *__restrict__ nbr, const double4 *__restrict__ R1,
                               const double4 *__restrict__ R2, const int *__restrict__ r2_off,
                               const int *__restrict__ r2_idx, const int *__restrict__ tile_inst,
                               const int *__restrict__ inst_end, const unsigned char *__restrict__ ran,
-                              const unsigned char *__restrict__ ran_prev, const int *__restrict__ argmax,
-                              const double *__restrict__ ghmax, int sidx, double4 *__restrict__ lam_out) {
+                              double4 *__restrict__ lam_out) {
     const int tile = blockIdx.x;
     const int b = tile_inst[tile];
     const int c = tile * (int)blockDim.x + threadIdx.x;
     if (c >= inst_end[b]) return;
-    if (!ran[tile] && ran_prev && !ran_prev[tile]) return;  // nobody reads this tile's cotangents
-    double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (ran[tile]) {
-        acc = R1[4 * (size_t)c + 3];
-        const int4 nb = nbr[c];
-        const int nbv[3] = {nb.x, nb.y, nb.z};
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const int i = nbv[k];
-            if (i >= 0 && ran[i / (int)blockDim.x]) acc4(acc, R1[4 * (size_t)i + ((nb.w >> (2 * k)) & 3)]);
-        }
-        if (R2ON)
-            for (int e = r2_off[c]; e < r2_off[c + 1]; e++) {
-                const int src = r2_idx[e];
-                if (ran[(src / 6) / (int)blockDim.x]) acc4(acc, R2[src]);
-            }
-    }
-    if (argmax && argmax[c] == sidx) acc.x += ghmax[c];
-    acc.w = 0.0;
-    lam_out[c] = acc;
+    lam_out[c] = gather_cell<R2ON>(c, (int)blockDim.x, ran, nbr, R1, R2, r2_off, r2_idx, nullptr, nullptr, -1,
+                                   nullptr, nullptr, nullptr);
 }
 
 // lam = final-state seed + hmax seed of the last sample (cells not seeded: 0)
@@ -300,12 +346,12 @@ struct TlmArgs {
     const KParams *kp;
     const int *tile_inst, *inst_end;
     const double *dt, *tc;
-    int hstride, step, ncell, tile_cells;
+    int hstride, step, ncell, tile_cells, xcd;
 };
 
 template <int IFLUX, int FRIC>
 __global__ __launch_bounds__(kTile) void tlm_kernel(TlmArgs a) {
-    const int tile = blockIdx.x;
+    const int tile = tile_of_block(a.xcd);
     const int b = a.tile_inst[tile];
     const int i = tile * (int)blockDim.x + threadIdx.x;
     if (i >= a.inst_end[b]) return;
@@ -407,17 +453,21 @@ static void drop_adj_graphs(AdjState *A) {
     A->graphs.clear();
 }
 
+void cf::adj_drop_graphs(cf_handle *h) {
+    if (h && h->adj) drop_adj_graphs(h->adj);
+}
+
 void cf::adj_free(cf_handle *h) {
     AdjState *A = h ? h->adj : nullptr;
     if (!A) return;
     drop_adj_graphs(A);
-    dfree(A->d_gcell); dfree(A->d_ginst); dfree(A->d_goff);
+    dfree(A->d_gcell); dfree(A->d_ginst); dfree(A->d_goff); dfree(A->d_gfirst); dfree(A->d_gnext);
     dfree(A->eta); dfree(A->geta); dfree(A->deta); dfree(A->teta);
     dfree(A->hmax); dfree(A->ghmax); dfree(A->dhmax); dfree(A->argmax);
     dfree(A->dt); dfree(A->tc); dfree(A->ran);
     dfree(A->ckpt); dfree(A->seg); dfree(A->xs);
     dfree(A->lam[0]); dfree(A->lam[1]); dfree(A->gstate);
-    dfree(A->R1); dfree(A->R2); dfree(A->r2_off); dfree(A->r2_idx); dfree(A->gn); dfree(A->tile_ran);
+    dfree(A->R1[0]); dfree(A->R1[1]); dfree(A->R2[0]); dfree(A->R2[1]); dfree(A->r2_off); dfree(A->r2_idx); dfree(A->gn); dfree(A->tile_ran);
     dfree(A->tst[0]); dfree(A->tst[1]); dfree(A->tdst[0]); dfree(A->tdst[1]); dfree(A->tdir); dfree(A->dman);
     delete A;
     h->adj = nullptr;
@@ -482,12 +532,14 @@ static int build_r2(cf_handle *h, AdjState *A) {
     std::vector<int> idx(off[N]), fillp(off.begin(), off.end() - 1);
     for (size_t e = 0; e < tgt.size(); e++)
         if (tgt[e] >= 0) idx[fillp[tgt[e]]++] = (int)e;
-    if (!dalloc(A->r2_off, off.size()) || !dalloc(A->r2_idx, idx.size()) || !dalloc(A->R2, 6 * (size_t)N))
+    if (!dalloc(A->r2_off, off.size()) || !dalloc(A->r2_idx, idx.size()) || !dalloc(A->R2[0], 6 * (size_t)N) ||
+        !dalloc(A->R2[1], 6 * (size_t)N))
         return CF_ERR_NOMEM;
     if (!h2d(A->r2_off, off.data(), off.size(), h->stream) || !h2d(A->r2_idx, idx.data(), idx.size(), h->stream) ||
         !h2d(h->nbr, nb.data(), nb.size(), h->stream))
         return CF_ERR_HIP;
-    fill0(A->R2, 6 * (size_t)N, h->stream);
+    fill0(A->R2[0], 6 * (size_t)N, h->stream);
+    fill0(A->R2[1], 6 * (size_t)N, h->stream);
     HIPCHK(hipStreamSynchronize(h->stream));
     A->r2_built = true;
     drop_adj_graphs(A);
@@ -534,12 +586,22 @@ static int upload_gauges(cf_handle *h, AdjState *A) {
             gi[A->goff[b] + g] = b;
         }
     }
-    dfree(A->d_gcell); dfree(A->d_ginst); dfree(A->d_goff);
-    if (!dalloc(A->d_gcell, gc.size()) || !dalloc(A->d_ginst, gi.size()) || !dalloc(A->d_goff, (size_t)B + 1))
+    // gauges of each cell as a list in increasing gauge index
+    std::vector<int> gfirst(h->ncell, -1), gnext(std::max(A->gtotal, 1), -1);
+    for (int g = A->gtotal - 1; g >= 0; g--) {
+        gnext[g] = gfirst[gc[g]];
+        gfirst[gc[g]] = g;
+    }
+    dfree(A->d_gcell); dfree(A->d_ginst); dfree(A->d_goff); dfree(A->d_gfirst); dfree(A->d_gnext);
+    if (!dalloc(A->d_gcell, gc.size()) || !dalloc(A->d_ginst, gi.size()) || !dalloc(A->d_goff, (size_t)B + 1) ||
+        !dalloc(A->d_gfirst, gfirst.size()) || !dalloc(A->d_gnext, gnext.size()))
         return CF_ERR_NOMEM;
     if (!h2d(A->d_gcell, gc.data(), gc.size(), h->stream) || !h2d(A->d_ginst, gi.data(), gi.size(), h->stream) ||
-        !h2d(A->d_goff, A->goff.data(), (size_t)B + 1, h->stream))
+        !h2d(A->d_goff, A->goff.data(), (size_t)B + 1, h->stream) ||
+        !h2d(A->d_gfirst, gfirst.data(), gfirst.size(), h->stream) ||
+        !h2d(A->d_gnext, gnext.data(), gnext.size(), h->stream))
         return CF_ERR_HIP;
+    HIPCHK(hipStreamSynchronize(h->stream));
     return CF_OK;
 }
 
@@ -568,11 +630,12 @@ static int adj_alloc(cf_handle *h, AdjState *A) {
     }
     if (!A->lam[0]) {
         drop_adj_graphs(A);
-        if (!dalloc(A->lam[0], N) || !dalloc(A->lam[1], N) || !dalloc(A->gstate, N) || !dalloc(A->R1, 4 * N) ||
-            !dalloc(A->gn, N) || !dalloc(A->hmax, N) || !dalloc(A->ghmax, N) || !dalloc(A->dhmax, N) ||
-            !dalloc(A->argmax, N))
+        if (!dalloc(A->lam[0], N) || !dalloc(A->lam[1], N) || !dalloc(A->gstate, N) || !dalloc(A->R1[0], 4 * N) ||
+            !dalloc(A->R1[1], 4 * N) || !dalloc(A->gn, N) || !dalloc(A->hmax, N) || !dalloc(A->ghmax, N) ||
+            !dalloc(A->dhmax, N) || !dalloc(A->argmax, N))
             return CF_ERR_NOMEM;
-        fill0(A->R1, 4 * N, h->stream);
+        fill0(A->R1[0], 4 * N, h->stream);
+        fill0(A->R1[1], 4 * N, h->stream);
     }
     return CF_OK;
 }
@@ -790,7 +853,7 @@ extern "C" int cf_adj_set_seed(cf_handle *h, int inst, const double *g_eta, cons
 
 template <int IFLUX, int FRIC>
 static void launch_rev_t(const RevArgs &r, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((rev_kernel<IFLUX, FRIC>), dim3(grid), dim3(r.tile_cells), 0, s, r);
+    hipLaunchKernelGGL((revg_kernel<IFLUX, FRIC>), dim3(grid), dim3(r.tile_cells), 0, s, r, r.kp);
 }
 
 static void launch_rev(const RevArgs &r, int iflux, int fric, int grid, hipStream_t s) {
@@ -801,19 +864,20 @@ static void launch_rev(const RevArgs &r, int iflux, int fric, int grid, hipStrea
 }
 
 // Reverse sweep: segments from the last, each replayed from its checkpoint
-// and reversed step by step.
+// and reversed step by step (revg_kernel: gather of step k + 1 and reverse
+// of step k in one launch), then the gather of step 0 gives the cotangent
+// of the initial state in lam[0].
 static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
     const size_t N = (size_t)h->ncell;
     const int S = A->nsteps + 1;
     const int *am = A->want_hmax ? A->argmax : nullptr;
     // lambda of the final state: its seed + the seeds of the last sample
     const int last = (A->nsteps % A->every == 0) ? A->nsteps / A->every - 1 : -1;
-    int cur = 0;
     hipLaunchKernelGGL(lam_init_kernel, dim3(h->ntiles), dim3(h->tile), 0, s, h->tile_inst, h->inst_end, A->gstate, am,
-                       A->ghmax, last, A->lam[cur]);
+                       A->ghmax, last, A->lam[1]);
     if (last >= 0 && A->gtotal)
         hipLaunchKernelGGL(gauge_seed_kernel, dim3(grid_of(h->ninst, 64)), dim3(64), 0, s, A->d_gcell, A->d_goff,
-                           A->geta, A->gtotal, last, h->ninst, A->lam[cur]);
+                           A->geta, A->gtotal, last, h->ninst, A->lam[1]);
     fill0(A->gn, N, s);
 
     StepArgs a = base_args(h);
@@ -831,48 +895,57 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
     r.kp = h->d_kp;
     r.tile_inst = h->tile_inst;
     r.inst_end = h->inst_end;
-    r.R1 = A->R1;
-    r.R2 = A->R2;
+    r.lam = A->lam[1];
+    r.r2_off = A->r2_off;
+    r.r2_idx = A->r2_idx;
     r.gn = A->gn;
     r.dt = A->dt;
     r.tc = A->tc;
+    r.argmax = am;
+    r.ghmax = A->ghmax;
+    r.gfirst = A->d_gfirst;
+    r.gnext = A->d_gnext;
     r.hstride = S;
     r.ncell = h->ncell;
     r.tile_cells = h->tile;
+    r.xcd = h->xcd;
     for (int sg = A->nseg - 1; sg >= 0; sg--) {
         const int k0 = sg * A->K, k1 = std::min(k0 + A->K, A->nsteps);
         const double4 *ck = A->ckpt + (size_t)sg * N;
-        // replay: state k0+m+1 -> seg[m], residual of step k0+m -> xs[m]
+        // replay: state k0+m+1 -> seg[m], residual of step k0+m -> xs[m].
+        // seg[m] last held step k0+K+m of the segment after this one (none
+        // for the first segment reversed: its content is unknown).
         for (int k = k0; k < k1; k++) {
             a.step = k;
             a.in_x = (k == k0) ? ck : A->seg + (size_t)(k - k0 - 1) * N;
             a.out_x = A->seg + (size_t)(k - k0) * N;
             a.xs = A->xs + (size_t)(k - k0) * 3 * N;
+            a.prev_use = (sg + 1 < A->nseg && k + A->K < A->nsteps)
+                             ? A->tile_ran + (size_t)(k + A->K) * h->ntiles : nullptr;
             launch_step(a, h->iflux, h->fric, h->ntiles, s);
         }
         for (int k = k1 - 1; k >= k0; k--) {
             r.step = k;
             r.st = (k == k0) ? ck : A->seg + (size_t)(k - k0 - 1) * N;
             r.xs = A->xs + (size_t)(k - k0) * 3 * N;
-            r.lam = A->lam[cur];
             r.ran = A->tile_ran + (size_t)k * h->ntiles;
+            r.ran_next = k + 1 < A->nsteps ? A->tile_ran + (size_t)(k + 1) * h->ntiles : nullptr;
+            r.R1in = A->R1[(k + 1) & 1];
+            r.R2in = A->R2[(k + 1) & 1];
+            r.R1 = A->R1[k & 1];
+            r.R2 = A->R2[k & 1];
+            // seeds of the state after step k (sample sidx)
+            r.sidx = ((k + 1) % A->every == 0) ? (k + 1) / A->every - 1 : -1;
+            r.geta_row = (r.sidx >= 0 && A->gtotal) ? A->geta + (size_t)r.sidx * A->gtotal : nullptr;
             launch_rev(r, h->iflux, h->fric, h->ntiles, s);
-            const int sidx = (k >= 1 && k % A->every == 0) ? k / A->every - 1 : -1;
-            const unsigned char *ran_prev = k > 0 ? A->tile_ran + (size_t)(k - 1) * h->ntiles : nullptr;
-            if (h->iflux == 3)
-                hipLaunchKernelGGL(gather_kernel<true>, dim3(h->ntiles), dim3(h->tile), 0, s, h->nbr, A->R1, A->R2,
-                                   A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, r.ran, ran_prev, am, A->ghmax,
-                                   sidx, A->lam[1 - cur]);
-            else
-                hipLaunchKernelGGL(gather_kernel<false>, dim3(h->ntiles), dim3(h->tile), 0, s, h->nbr, A->R1, A->R2,
-                                   A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, r.ran, ran_prev, am, A->ghmax,
-                                   sidx, A->lam[1 - cur]);
-            if (sidx >= 0 && A->gtotal)
-                hipLaunchKernelGGL(gauge_seed_kernel, dim3(grid_of(h->ninst, 64)), dim3(64), 0, s, A->d_gcell,
-                                   A->d_goff, A->geta, A->gtotal, sidx, h->ninst, A->lam[1 - cur]);
-            cur = 1 - cur;
         }
     }
+    if (h->iflux == 3)
+        hipLaunchKernelGGL(gather_kernel<true>, dim3(h->ntiles), dim3(h->tile), 0, s, h->nbr, A->R1[0], A->R2[0],
+                           A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, A->tile_ran, A->lam[0]);
+    else
+        hipLaunchKernelGGL(gather_kernel<false>, dim3(h->ntiles), dim3(h->tile), 0, s, h->nbr, A->R1[0], A->R2[0],
+                           A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, A->tile_ran, A->lam[0]);
 }
 
 extern "C" int cf_adj_backward(cf_handle *h) {
@@ -881,7 +954,7 @@ extern "C" int cf_adj_backward(cf_handle *h) {
     int rc = run_captured(h, A, 1, enqueue_backward);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
-    A->lam_cur = A->nsteps & 1;  // the buffer holding the initial-state cotangent
+    A->lam_cur = 0;  // the buffer holding the initial-state cotangent
     return CF_OK;
 }
 
@@ -956,6 +1029,7 @@ static void enqueue_tlm(cf_handle *h, AdjState *A, hipStream_t s) {
     t.hstride = A->nsteps + 1;
     t.ncell = h->ncell;
     t.tile_cells = h->tile;
+    t.xcd = h->xcd;
     double4 *vb[2] = {A->tst[0], A->tst[1]};
     double4 *db[2] = {A->tdst[0], A->tdst[1]};
     if (A->want_hmax) fill0(A->dhmax, N, s);

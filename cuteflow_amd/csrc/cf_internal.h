@@ -59,15 +59,33 @@ struct StepArgs {
     double4 *out_x;
     double *xs;
     // Tile activity of a recorded run, row per step, one byte per tile: the
-    // recorded forward writes whether each tile ran; a replay copies the
-    // tiles that did not run (their cells are dry and stay so).
+    // recorded forward writes whether each tile ran.  In a replay a tile
+    // that did not run holds dry cells with dry neighbours, whose state
+    // after the step is (+0, +0, +0, zm), as before it.  out_x is a reused
+    // buffer: prev_use is the activity row of the step it last held (null
+    // if unknown), and only tiles that ran then (or all, if unknown) are
+    // rewritten to (+0, +0, +0, zm).
     unsigned char *tile_ran;
+    const unsigned char *prev_use;
     int ncell;        // padded cell count (geo stride)
     int tile_cells;   // cells per tile = workgroup size
     int hist_stride;
     int step;         // index of this step in the run (absolute in a replay)
     int skip;         // dry-tile skipping enabled
+    int xcd;          // workgroup -> tile order: 0 identity, 1 XCD-contiguous
 };
+
+// Tile processed by this workgroup.  Workgroups are dispatched round-robin
+// over the 8 XCDs (each with its own L2); with xcd set, XCD x gets one
+// contiguous run of tiles instead of every 8th tile, so the neighbour
+// reads across tile edges stay in that XCD's L2.  A bijection on
+// [0, gridDim.x) for any grid size.
+__device__ inline int tile_of_block(int xcd) {
+    const int bid = blockIdx.x;
+    if (!xcd) return bid;
+    const int g = gridDim.x, x = bid & 7, q = g >> 3, r = g & 7;
+    return x * q + (x < r ? x : r) + (bid >> 3);
+}
 
 // Device loader of the cell body: states, neighbours, geometry.
 struct DevLd {
@@ -108,7 +126,7 @@ struct cf_handle {
     std::vector<int4> hnbr;                            // host copy of the neighbour codes
     hipStream_t stream = nullptr;
     bool own_stream = false;
-    int iflux = 1, fric = 0, skip = 1;
+    int iflux = 1, fric = 0, skip = 1, xcd = 0;
     int tile = cf::kTile;              // cells per tile (workgroup size)
     // device arrays
     double4 *st[2] = {nullptr, nullptr};
@@ -182,7 +200,9 @@ StepArgs base_args(cf_handle *h);
 void launch_step(const StepArgs &a, int iflux, int fric, int grid, hipStream_t s);
 void launch_finalize(cf_handle *h, double *dt_hist, double *tc_hist, int *ran, int stride, int step, hipStream_t s);
 void launch_run_init(cf_handle *h, double *dt_hist, double *tc_hist, int stride, hipStream_t s);
+int set_schedule(cf_handle *h, int skip, int xcd);
 // defined in cf_adjoint.hip
 void adj_free(cf_handle *h);
+void adj_drop_graphs(cf_handle *h);
 
 }  // namespace cf

@@ -71,7 +71,7 @@ __device__ __forceinline__ double wave_min(double v) {
 template <int IFLUX, int FRIC>
 __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs a, const KParams *__restrict__ kp) {
     __shared__ double s_min[kWaves];
-    const int tile = blockIdx.x;
+    const int tile = tile_of_block(a.xcd);
     const int b = a.tile_inst[tile];
     const InstCtrl c = a.ctrl[b];
     const bool replay = a.dt_fix != nullptr;
@@ -97,9 +97,13 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
             return;
         }
     } else if (ran_row && !ran_row[tile]) {
-        // skipped in the recorded run: the state passes through unchanged
-        const int ic = tile * (int)blockDim.x + threadIdx.x;
-        a.out_x[ic] = a.in_x[ic];
+        // skipped in the recorded run: dry cells stay (+0, +0, +0, zm); the
+        // output buffer already holds that unless the tile ran at the step
+        // it last held (or that step is unknown)
+        if (!a.prev_use || a.prev_use[tile]) {
+            const int ic = tile * (int)blockDim.x + threadIdx.x;
+            a.out_x[ic] = make_double4(0.0, 0.0, 0.0, a.in_x[ic].w);
+        }
         return;
     }
     const double4 *__restrict__ st_in = replay ? a.in_x : a.st[(c.par + a.step) & 1];
@@ -470,7 +474,8 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
         for (size_t i = 0; i < 3 * (size_t)m->nelt; i++)
             if (m->conn[i] < 1 || m->conn[i] > m->nnt) return CF_ERR_MESH;
     }
-    if ((params[0].options & ~(CF_OPT_KEEP_ORDER | CF_OPT_NO_TILE_SKIP | CF_OPT_TILE_128 | CF_OPT_TILE_64)) ||
+    if ((params[0].options & ~(CF_OPT_KEEP_ORDER | CF_OPT_NO_TILE_SKIP | CF_OPT_TILE_128 | CF_OPT_TILE_64 |
+                               CF_OPT_XCD_CONTIG)) ||
         ((params[0].options & CF_OPT_TILE_128) && (params[0].options & CF_OPT_TILE_64)))
         return CF_ERR_ARG;
     const int tile = (params[0].options & CF_OPT_TILE_64) ? 64 : (params[0].options & CF_OPT_TILE_128) ? 128 : kTile;
@@ -497,6 +502,7 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
     h->skip = (params[0].options & CF_OPT_NO_TILE_SKIP) ? 0 : 1;
     for (int b = 0; b < ninst; b++)
         if (params[b].is_dt_constant == 1) h->skip = 0;
+    h->xcd = (params[0].options & CF_OPT_XCD_CONTIG) ? 1 : 0;
     int cells = 0, tiles = 0;
     for (int b = 0; b < ninst; b++) {
         h->nelt[b] = meshes[b].nelt;
@@ -755,6 +761,7 @@ StepArgs cf::base_args(cf_handle *h) {
     a.tile_cells = h->tile;
     a.hist_stride = h->cap + 1;
     a.skip = h->skip;
+    a.xcd = h->xcd;
     return a;
 }
 
@@ -871,6 +878,33 @@ extern "C" int cf_batch_step(cf_handle *h, int nsteps, double *dt_hist, int *ste
     int rc = cf_step_async(h, nsteps);
     if (rc) return rc;
     return cf_batch_step_finish(h, dt_hist, steps_done, status);
+}
+
+// Switches dry-tile skipping and the workgroup -> tile order of a handle
+// (the autotuner's runtime knobs).  Captured graphs hold the old kernel
+// arguments and are dropped.  Stamps are not kept while skipping is off,
+// so turning it on restamps every tile to run the next two steps.
+int cf::set_schedule(cf_handle *h, int skip, int xcd) {
+    if (!h || h->pending >= 0) return CF_ERR_ARG;
+    skip = skip ? 1 : 0;
+    xcd = xcd ? 1 : 0;
+    for (int b = 0; b < h->ninst; b++)
+        if (h->prm[b].is_dt_constant == 1) skip = 0;
+    if (skip == h->skip && xcd == h->xcd) return CF_OK;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    if (skip && !h->skip)
+        for (int b = 0; b < h->ninst; b++) {
+            const int t0 = h->tile_off[b], nt = h->tile_off[b + 1] - t0;
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_of(nt, 256)), dim3(256), 0, h->stream, h->stamp + t0, nt,
+                               h->hctrl[b].gstep);
+        }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    drop_graphs(h);
+    adj_drop_graphs(h);
+    h->skip = skip;
+    h->xcd = xcd;
+    return CF_OK;
 }
 
 extern "C" int cf_last_run_stats(cf_handle *h, long long *tiles_run, long long *tiles_total, int *tile_cells) {

@@ -136,6 +136,72 @@ class Solver:
         self.close()
 
 
+def _batch_structs(meshes, params, ts, options):
+    """cf_mesh / cf_params arrays of a batch (boundary tables and inlet
+    lengths derived from each mesh), the numpy buffers they point into, and
+    the boundary tables."""
+    n = len(meshes)
+    cms = (_lib.CfMesh * n)()
+    cps = (_lib.CfParams * n)()
+    keep, boundaries = [], []
+    for b, (m, p) in enumerate(zip(meshes, params)):
+        bt = m.boundary_table()
+        le = m.inlet_lengths(bt, p.nombre_entrees)
+        conn = np.ascontiguousarray(m.conn.reshape(-1), np.int32)
+        coords = np.ascontiguousarray(m.coords.reshape(-1), np.float64)
+        mann = np.ascontiguousarray(m.manning, np.float64)
+        keep += [bt, conn, coords, mann]
+        boundaries.append(bt)
+        cms[b].nelt = m.nelt
+        cms[b].nnt = m.nnt
+        cms[b].conn = _lib.iptr(conn)
+        cms[b].coords = _lib.dptr(coords)
+        cms[b].boundary = _lib.iptr(bt)
+        cms[b].manning = _lib.dptr(mann)
+        cps[b] = p.to_cf(le, ts=ts)
+        cps[b].options = int(options)
+    return cms, cps, keep, boundaries
+
+
+def tune_candidate_options(k: int, base: int = 0) -> int:
+    """Options bits of autotuner candidate k (cf_batch_autotune's order)."""
+    o = base & ~(_lib.CF_OPT_TILE_128 | _lib.CF_OPT_TILE_64 | _lib.CF_OPT_NO_TILE_SKIP | _lib.CF_OPT_XCD_CONTIG)
+    o |= (0, _lib.CF_OPT_TILE_128, _lib.CF_OPT_TILE_64)[k // 4]
+    if k & 2:
+        o |= _lib.CF_OPT_NO_TILE_SKIP
+    if k & 1:
+        o |= _lib.CF_OPT_XCD_CONTIG
+    return o
+
+
+def autotune(meshes, params, states, nsteps: int = 100, ts=None, options: int = 0):
+    """Times the batch under every schedule candidate (cells per tile 256,
+    128, 64; dry-tile skipping on/off; XCD-contiguous tiles off/on) from the
+    given initial states and returns (best_options, table), table being a
+    list of (options, milliseconds or None) per candidate.  Pass
+    best_options to BatchSolver(options=...).  Results of a run do not depend
+    on the choice, only its speed."""
+    lib = _lib.load()
+    if isinstance(params, Params):
+        params = [params] * len(meshes)
+    if len(params) != len(meshes) or len(states) != len(meshes) or not meshes:
+        raise ValueError("one Params and one state per mesh")
+    cms, cps, keep, _ = _batch_structs(meshes, params, ts, options)
+    st = []
+    for m, U in zip(meshes, states):
+        U = np.ascontiguousarray(U, np.float64)
+        if U.shape != (m.nelt, 3):
+            raise ValueError("state shape must be (nelt, 3)")
+        st.append(U)
+    ptrs = (C.c_void_p * len(st))(*[U.ctypes.data for U in st])
+    best = C.c_int()
+    times = np.zeros(_lib.CF_TUNE_NCAND)
+    _lib.check(lib.cf_batch_autotune(len(meshes), cms, cps, ptrs, int(nsteps), C.byref(best), _lib.dptr(times)),
+               "cf_batch_autotune")
+    table = [(tune_candidate_options(k, options), float(t) if t >= 0 else None) for k, t in enumerate(times)]
+    return best.value, table
+
+
 class BatchSolver:
     """Many independent subdomains advanced together (cf_batch_* entry
     points): one launch per step covers every instance; each instance keeps
@@ -152,26 +218,7 @@ class BatchSolver:
         self.meshes = list(meshes)
         self.params = list(params)
         n = len(meshes)
-        cms = (_lib.CfMesh * n)()
-        cps = (_lib.CfParams * n)()
-        keep = []
-        self.boundaries = []
-        for b, (m, p) in enumerate(zip(meshes, params)):
-            bt = m.boundary_table()
-            le = m.inlet_lengths(bt, p.nombre_entrees)
-            conn = np.ascontiguousarray(m.conn.reshape(-1), np.int32)
-            coords = np.ascontiguousarray(m.coords.reshape(-1), np.float64)
-            mann = np.ascontiguousarray(m.manning, np.float64)
-            keep += [bt, conn, coords, mann]
-            self.boundaries.append(bt)
-            cms[b].nelt = m.nelt
-            cms[b].nnt = m.nnt
-            cms[b].conn = _lib.iptr(conn)
-            cms[b].coords = _lib.dptr(coords)
-            cms[b].boundary = _lib.iptr(bt)
-            cms[b].manning = _lib.dptr(mann)
-            cps[b] = p.to_cf(le, ts=ts)
-            cps[b].options = int(options)
+        cms, cps, keep, self.boundaries = _batch_structs(meshes, params, ts, options)
         h = C.c_void_p()
         _lib.check(lib.cf_batch_create(n, cms, cps, C.byref(h)), "cf_batch_create")
         self.h = h

@@ -32,6 +32,7 @@ extern "C" {
 #define CF_OPT_NO_TILE_SKIP 2    /* run every tile every step */
 #define CF_OPT_TILE_128 4        /* 128-cell tiles (workgroups) instead of 256 */
 #define CF_OPT_TILE_64 8         /* 64-cell tiles (one wavefront) */
+#define CF_OPT_XCD_CONTIG 16     /* each XCD (L2) runs one contiguous run of tiles */
 
 enum {
     CF_OK = 0,
@@ -190,6 +191,23 @@ int cf_batch_set_hydrograph(cf_handle *h, int inst, int inlet, int npts, const d
 int cf_batch_set_manning(cf_handle *h, int inst, const double *n);
 int cf_batch_get_geometry(cf_handle *h, int inst, double *surf, double *zm, double *cotemin_arr,
                           double *cotemin);
+
+/* Schedule autotuner.  No reference counterpart (the reference's launch
+ * shapes are fixed: 24-thread flux blocks, flux_riemann.cuf; 1024-thread
+ * reductions, cfl_condition.cuf:75).  Times nsteps steps of the batch
+ * (meshes, params as for cf_batch_create; states: ninst host pointers to
+ * (3, nelt) initial states) under each of CF_TUNE_NCAND schedules and
+ * returns in *best_options params[0].options with the tuned bits
+ * (CF_OPT_TILE_128, CF_OPT_TILE_64, CF_OPT_NO_TILE_SKIP, CF_OPT_XCD_CONTIG)
+ * set to the fastest; pass it as options to cf_batch_create.  Candidate k
+ * has cells per tile 256 >> (k / 4) (256, 128, 64), dry-tile skipping off
+ * if k & 2, XCD-contiguous tiles if k & 1.  times_ms (CF_TUNE_NCAND
+ * entries, may be NULL) receives each candidate's best time of the timed
+ * runs, or -1 if it was not run (a failure of that candidate).  The results
+ * of a run do not depend on the schedule: only its speed does. */
+#define CF_TUNE_NCAND 12
+int cf_batch_autotune(int ninst, const cf_mesh *meshes, const cf_params *params, const double *const *states,
+                      int nsteps, int *best_options, double *times_ms);
 
 /* ---- Discrete adjoint and tangent-linear model --------------------------
  *

@@ -227,8 +227,9 @@ def test_gpu_lbfgs_calibration_reduces_misfit():
 def test_gpu_reverse_tile_skipping_is_exact():
     """A reach with a dry floodplain: tiles the recorded forward skipped are
     skipped by the replay, reverse and gather kernels too.  Gradients with
-    256- and 64-cell tiles equal those of a run that processes every tile
-    (CF_OPT_NO_TILE_SKIP), value for value."""
+    256- and 64-cell tiles (and 128-cell tiles in XCD-contiguous order)
+    equal those of a run that processes every tile (CF_OPT_NO_TILE_SKIP),
+    value for value."""
     from cuteflow_amd import _lib
     from cuteflow_amd import scenarios as SC
 
@@ -240,7 +241,7 @@ def test_gpu_reverse_tile_skipping_is_exact():
     ge = rng.normal(size=(12, 20))
     gs = rng.normal(size=(m.nelt, 3))
     out = []
-    for opt in (0, _lib.CF_OPT_TILE_64, _lib.CF_OPT_NO_TILE_SKIP):
+    for opt in (0, _lib.CF_OPT_TILE_64, _lib.CF_OPT_TILE_128 | _lib.CF_OPT_XCD_CONTIG, _lib.CF_OPT_NO_TILE_SKIP):
         with Solver(m, p, options=opt) as s:
             s.set_state(U0)
             s.step(nsteps)
@@ -250,8 +251,8 @@ def test_gpu_reverse_tile_skipping_is_exact():
             gn, gU = run.backward([ge], None, [gs])
             out.append((frac, eta[0], gn[0], gU[0]))
     print("active fractions", [o[0] for o in out], flush=True)
-    assert out[0][0] < 1.0 and out[1][0] < 1.0 and out[2][0] == 1.0  # some tiles were skipped
-    for k in (0, 1):
-        assert _same(out[k][1], out[2][1])
-        assert np.array_equal(out[k][2], out[2][2])
-        assert np.array_equal(out[k][3], out[2][3])
+    assert all(o[0] < 1.0 for o in out[:3]) and out[3][0] == 1.0  # some tiles were skipped
+    for k in (0, 1, 2):
+        assert _same(out[k][1], out[3][1])
+        assert np.array_equal(out[k][2], out[3][2])
+        assert np.array_equal(out[k][3], out[3][3])

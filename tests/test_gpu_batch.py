@@ -125,3 +125,41 @@ def test_batch_rejects_mixed_schemes():
     m1, p1, _ = river_case(20, 6)
     with pytest.raises(RuntimeError):
         BatchSolver([m0, m1], [p0, p1])
+
+
+def test_autotune_picks_a_candidate_with_identical_results():
+    """cf_batch_autotune times every schedule candidate and returns one of
+    them; the batch built with the tuned options gives the same bits as the
+    default schedule (and the oracle), on instances of uneven sizes."""
+    from cuteflow_amd import _lib
+    from cuteflow_amd.solver import autotune, tune_candidate_options
+    cases = [river_case(60, 12, seed=21, q_peak=1500.0, t_ramp=30.0), river_case(30, 8, seed=22),
+             river_case(90, 10, seed=23, q_peak=800.0)]
+    meshes, params, states = [c[0] for c in cases], [c[1] for c in cases], [c[2] for c in cases]
+    best, table = autotune(meshes, params, states, nsteps=40)
+    print("autotune:", best, table, flush=True)
+    assert len(table) == _lib.CF_TUNE_NCAND
+    assert [o for o, _ in table] == [tune_candidate_options(k) for k in range(_lib.CF_TUNE_NCAND)]
+    assert all(t is not None and t > 0.0 for _, t in table)
+    k_best = min(range(len(table)), key=lambda k: table[k][1])
+    assert best == table[k_best][0]
+    nsteps = 200
+    refs = [O.run(O.Prepared(m), p, U, nsteps) for m, p, U in cases]
+    for options in (best, 0, _lib.CF_OPT_XCD_CONTIG | _lib.CF_OPT_TILE_64):
+        with BatchSolver(meshes, params, options=options) as bs:
+            for b, U in enumerate(states):
+                bs.set_state(b, U)
+            hist, done, status = bs.step(nsteps)
+            for b, ref in enumerate(refs):
+                assert done[b] == ref["steps"], (options, b)
+                assert _same(hist[b], ref["dt_hist"]), (options, b)
+                assert _same(bs.get_state(b), ref["U"]), (options, b)
+
+
+def test_autotune_rejects_bad_arguments():
+    from cuteflow_amd.solver import autotune
+    m, p, U = river_case(20, 6)
+    with pytest.raises(ValueError):
+        autotune([m], [p], [U[:-1]], nsteps=5)
+    with pytest.raises(RuntimeError):
+        autotune([m], [p], [U], nsteps=0)

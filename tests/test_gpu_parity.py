@@ -146,11 +146,12 @@ def test_river_hllc_transmissive_outlet():
     _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
 
 
-@pytest.mark.parametrize("options", [0, 1, 2, 3, 4, 8, 4 | 1, 8 | 2])
+@pytest.mark.parametrize("options", [0, 1, 2, 3, 4, 8, 4 | 1, 8 | 2, 16, 16 | 8, 16 | 4 | 2, 16 | 1])
 def test_layout_options_identical(options):
-    """Internal Hilbert order, dry-tile skipping (on/off) and the tile size
-    (256, 128, 64 cells) give the oracle's bits on a wetting floodplain over
-    many steps."""
+    """Internal Hilbert order, dry-tile skipping (on/off), the tile size
+    (256, 128, 64 cells) and the workgroup -> tile order (plain or
+    XCD-contiguous) give the oracle's bits on a wetting floodplain over many
+    steps."""
     m, p, U0 = river_case(nx=120, ny=24, q_peak=2000.0, t_ramp=30.0)
     prep = O.Prepared(m)
     ref = O.run(prep, p, U0, 800)
