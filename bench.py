@@ -260,7 +260,8 @@ def bench_calibration(args):
                        "levels every 10 steps over 2000 steps, checkpoints every 100 steps, HLLC + "
                        "semi-implicit friction; observations from the hidden truth, gradient at +20 %",
            "nelt": N, "scheme": "iflux 1 (HLLC), fricimplic 1"}
-    with Solver(sc.mesh, sc.params) as s:
+    options, out["schedule"] = section_options(args, [sc.mesh], [sc.params], [sc.state], "calibration")
+    with Solver(sc.mesh, sc.params, options=options) as s:
         s.set_state(sc.state)
         s.step(nsteps)
         times = []
@@ -330,6 +331,19 @@ def describe_options(o):
     from cuteflow_amd import _lib
     return {"tile_cells": 64 if o & _lib.CF_OPT_TILE_64 else (128 if o & _lib.CF_OPT_TILE_128 else 256),
             "skip_dry_tiles": not (o & _lib.CF_OPT_NO_TILE_SKIP), "xcd_contiguous": bool(o & _lib.CF_OPT_XCD_CONTIG)}
+
+
+def section_options(args, meshes, params, states, label):
+    """Schedule of a section's runs: the autotuner's pick (--options auto)
+    or the given options bits; returns (options, record)."""
+    if args.options != "auto":
+        return int(args.options), {"autotuned": False, "options": int(args.options)}
+    from cuteflow_amd.solver import autotune
+    t = time.perf_counter()
+    best, _ = autotune(meshes, params, states, nsteps=args.tune_steps)
+    dt = time.perf_counter() - t
+    log(f"bench: {label} autotune {dt:.1f} s, best {describe_options(best)}")
+    return best, dict(describe_options(best), autotuned=True, options=best, tune_seconds=dt)
 
 
 def bench_ensemble(args):
@@ -409,7 +423,9 @@ def bench_field(args):
                        "150 steps, loss = misfit at 50 gauges (every 10 steps) + squared error of the soft "
                        "maximum flood extent; observations from the member's field, gradient at +20 %",
            "members": B, "cells": sum(e.mesh.nelt for e in mem), "scheme": "iflux 1 (HLLC), fricimplic 1"}
-    with BatchSolver([e.mesh for e in mem], [e.params for e in mem]) as bs:
+    options, out["schedule"] = section_options(args, [e.mesh for e in mem], [e.params for e in mem],
+                                               [e.state for e in mem], "field ensemble")
+    with BatchSolver([e.mesh for e in mem], [e.params for e in mem], options=options) as bs:
         U0 = [e.state for e in mem]
         surf = [bs.geometry(b)["surf"] for b in range(B)]
         ts = []
@@ -471,6 +487,125 @@ def bench_field(args):
     return out
 
 
+def bench_scenarios(args):
+    """64 calibration scenarios on one 100k-cell reach in one batch: cost of
+    the batched zonal gradient (through the autograd Function) against the
+    batched forward and against central differences with plain forwards;
+    the gradient against central differences (recorded dt) and
+    tangent-linear directional derivatives."""
+    import numpy as np
+    import torch
+
+    from cuteflow_amd import calibration as CAL
+    from cuteflow_amd import scenarios as SC
+    from cuteflow_amd.adjoint import AdjointRun, GaugeLevels
+    from cuteflow_amd.solver import BatchSolver, Solver
+
+    nsteps, every, K, nz = 2000, 10, 100, 8
+    sc = SC.calibration_batch(args.nscen, 100_000, ngauges=20, iflux=1)
+    B = len(sc)
+    meshes, params, U0 = [x.mesh for x in sc], [x.params for x in sc], [x.state for x in sc]
+    n0 = [1.2 * x.n_zones for x in sc]
+    mann0 = [n0[b][sc[b].zone] for b in range(B)]
+    out = {"workload": f"{B} calibration scenarios (flood events) on one 100k-cell reach: own peak inflow "
+                       "500-4000 m3/s, own hidden n in 8 zones 0.025-0.05, 20 channel gauges each, levels every "
+                       "10 steps over 2000 steps, checkpoints every 100 steps, HLLC + semi-implicit friction, "
+                       "zonal gradient at +20 % through the autograd Function",
+           "scenarios": B, "cells": sum(m.nelt for m in meshes), "scheme": "iflux 1 (HLLC), fricimplic 1"}
+    for b in range(B):
+        meshes[b].manning = mann0[b]
+    options, out["schedule"] = section_options(args, meshes, params, U0, "scenarios")
+    dev = torch.device("cuda")
+    with BatchSolver(meshes, params, options=options) as bs:
+        run = AdjointRun(bs, [x.gauges for x in sc], sample_every=every, checkpoint_every=K)
+        obs, _ = run.forward(nsteps, U0=U0, manning=[x.n_zones[x.zone] for x in sc])
+        for b in range(B):
+            bs.set_manning(b, mann0[b])
+        ts = []
+        for _ in range(2):
+            for b in range(B):
+                bs.set_state(b, U0[b])
+            sync()
+            t = time.perf_counter()
+            _, done, st = bs.step(nsteps)
+            sync()
+            ts.append(time.perf_counter() - t)
+        assert int(done.min()) == nsteps and int(st.max()) == 0
+        t_fwd = ts[-1]
+        zidx = torch.tensor(np.concatenate([nz * b + sc[b].zone for b in range(B)]), device=dev)
+        obs_t = torch.tensor(np.concatenate([o.reshape(-1) for o in obs]), dtype=torch.float64, device=dev)
+
+        def gradient():
+            nzt = torch.tensor(np.concatenate(n0), dtype=torch.float64, device=dev, requires_grad=True)
+            eta = GaugeLevels.apply(nzt[zidx], run, nsteps, U0)
+            loss = ((eta - obs_t) ** 2).sum()
+            loss.backward()
+            return nzt.grad.view(B, nz).cpu().numpy(), float(loss)
+
+        gt = []
+        for _ in range(2):
+            sync()
+            t = time.perf_counter()
+            g, loss = gradient()
+            sync()
+            gt.append(time.perf_counter() - t)
+        t_grad = gt[-1]
+        log(f"bench: scenarios batched forward {t_fwd*1e3:.0f} ms, gradient {t_grad*1e3:.0f} ms, "
+            f"ratio {t_grad / t_fwd:.2f}")
+        # central differences with the plain forward: 2 x 8 runs per scenario
+        with Solver(sc[0].mesh, sc[0].params) as s1:
+            s1.set_manning(mann0[0])
+            t1 = []
+            for _ in range(3):
+                s1.set_state(U0[0])
+                sync()
+                t = time.perf_counter()
+                s1.step(nsteps)
+                sync()
+                t1.append(time.perf_counter() - t)
+        t_plain = min(t1[1:])
+        t_fd = 2 * nz * B * t_plain
+        log(f"bench: scenarios plain single-scenario forward {t_plain*1e3:.1f} ms; central differences "
+            f"{2 * nz * B} runs = {t_fd:.1f} s, {t_fd / t_grad:.1f}x the batched gradient")
+        # checks: central differences at the recorded dt (all scenarios at
+        # once, one zone and sign per tangent-linear value run) and
+        # tangent-linear derivatives along a random zonal direction
+        checks = {}
+        for eps in (1e-6, 1e-7):
+            vals = np.zeros((2, B, nz))
+            for z in range(nz):
+                for si, sgn in enumerate((1.0, -1.0)):
+                    nzp = [x.copy() for x in n0]
+                    for b in range(B):
+                        nzp[b][z] *= 1.0 + sgn * eps
+                    run.set_manning([nzp[b][sc[b].zone] for b in range(B)])
+                    tl = run.tlm()
+                    vals[si, :, z] = [CAL.gauge_misfit(tl[b]["eta"].reshape(-1), obs[b].reshape(-1))
+                                      for b in range(B)]
+            run.set_manning(mann0)
+            gfd = (vals[0] - vals[1]) / (2.0 * eps * np.array(n0))
+            scale = np.max(np.abs(gfd), axis=1, keepdims=True)
+            err = np.abs(g - gfd) / np.maximum(scale, 1e-300)
+            checks[f"fd_{eps:g}"] = {"max_rel_err_normwise": float(err.max()),
+                                     "median_rel_err_normwise": float(np.median(err)),
+                                     "fraction_within_1e-6": float(np.mean(err <= 1e-6))}
+        rng = np.random.default_rng(5)
+        dz = [rng.normal(size=nz) * x for x in n0]
+        tl = run.tlm([dz[b][sc[b].zone] for b in range(B)])
+        d_tlm = np.array([np.sum(2.0 * (tl[b]["eta"] - obs[b]) * tl[b]["d_eta"]) for b in range(B)])
+        d_adj = np.array([np.dot(g[b], dz[b]) for b in range(B)])
+        checks["tlm_max_rel_err"] = float(np.max(np.abs(d_adj - d_tlm) / np.abs(d_tlm)))
+        log(f"bench: scenarios checks {checks}")
+        out.update({"batched_forward_s": t_fwd, "gradient_s": t_grad, "gradient_first_call_s": gt[0],
+                    "adjoint_forward_ratio": t_grad / t_fwd, "grads_per_s": B / t_grad,
+                    "plain_single_forward_s": t_plain, "fd_plain_forward_s": t_fd,
+                    "speedup_vs_fd_plain_forward": t_fd / t_grad,
+                    "fd_cost_definition": f"central differences: 2 plain single-scenario forwards per zone and "
+                                          f"scenario ({2 * nz * B} runs), each timed as one such run",
+                    "loss": loss, "checks": checks})
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -489,8 +624,9 @@ def main():
                     help="before the timed region, run a device copy of this many MiB (3 times): a "
                          "known-size kernel that calibrates HBM byte counters in a profile")
     ap.add_argument("--tune-steps", type=int, default=100, help="steps per autotuner candidate")
-    ap.add_argument("--sections", default="calibration,ensemble,field",
-                    help="comma-separated extra sections (calibration, ensemble, field); empty for none")
+    ap.add_argument("--sections", default="calibration,ensemble,field,scenarios",
+                    help="comma-separated extra sections (calibration, ensemble, field, scenarios); empty for none")
+    ap.add_argument("--nscen", type=int, default=64, help="scenarios of the scenarios section")
     ap.add_argument("--lbfgs-iters", type=int, default=20)
     args = ap.parse_args()
 
@@ -508,6 +644,8 @@ def main():
         result["calibration"] = bench_calibration(args)
     if "ensemble" in sections:
         result["ensemble"] = bench_ensemble(args)
+    if "scenarios" in sections:
+        result["scenarios"] = bench_scenarios(args)
     if "field" in sections:
         result["batched_gradients"] = bench_field(args)
     print(json.dumps(result), flush=True)

@@ -117,6 +117,9 @@ SIGNATURES = {
     "cf_batch_set_hydrograph": (C.c_int, [_H, C.c_int, C.c_int, C.c_int, _DP, _DP]),
     "cf_batch_set_manning": (C.c_int, [_H, C.c_int, C.c_void_p]),
     "cf_batch_get_geometry": (C.c_int, [_H, C.c_int, _DP, _DP, _DP, _DP]),
+    "cf_set_discharge": (C.c_int, [_H, C.c_int]),
+    "cf_batch_get_discharge": (C.c_int, [_H, C.c_int, _DP, _DP, _IP]),
+    "cf_get_discharge": (C.c_int, [_H, _DP, _DP, _IP]),
     "cf_batch_autotune": (C.c_int, [C.c_int, C.POINTER(CfMesh), C.POINTER(CfParams), C.POINTER(C.c_void_p),
                                     C.c_int, _IP, _DP]),
     "cf_adj_set_flags": (C.c_int, [_H, C.c_int]),

@@ -678,6 +678,7 @@ static void enqueue_forward(cf_handle *h, AdjState *A, hipStream_t s) {
     hipLaunchKernelGGL(ckpt_kernel, dim3(h->ntiles), dim3(h->tile), 0, s, sp, h->ctrl, h->tile_inst, h->inst_end,
                        A->ckpt);
     StepArgs a = base_args(h);
+    a.q_io = nullptr;  // no discharge bookkeeping in recorded runs
     a.dt_hist = A->dt;
     a.tc_hist = A->tc;
     a.ran = A->ran;
@@ -704,6 +705,7 @@ static void enqueue_forward(cf_handle *h, AdjState *A, hipStream_t s) {
 
 extern "C" int cf_adj_forward(cf_handle *h, int nsteps, int sample_every, int checkpoint_every, int want_hmax) {
     if (!h || nsteps <= 0 || sample_every <= 0 || checkpoint_every <= 0 || h->pending >= 0) return CF_ERR_ARG;
+    if (h->iflux == 2) return CF_ERR_ARG;  // no reverse of the iflux 2 solver
     for (int b = 0; b < h->ninst; b++)
         if (h->hctrl[b].status != 0 || h->prm[b].is_override_manning) return CF_ERR_ARG;
     AdjState *A = adj_of(h);
@@ -881,6 +883,7 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
     fill0(A->gn, N, s);
 
     StepArgs a = base_args(h);
+    a.q_io = nullptr;
     a.dt_fix = A->dt;
     a.tc_fix = A->tc;
     a.hist_stride = S;

@@ -27,6 +27,7 @@ enum { G_N1 = 0, G_N2 = 3, G_LN = 6, G_SURF = 9, G_COT = 10, G_XC = 11, G_YC = 1
 struct KParams {
     double gp, tolisec, tol, cfl, ts, constant_dt, override_manning;
     double pad_dt;  // min(1e6, 1e5 if nelt % 1024 != 0): the reduction's padding
+    double tol_reg_perm;  // steady-state tolerance (<= 0: never steady)
     int ilimiteur, inlet, outlet, is_dt_constant, is_override_manning;
     int hydro_off;  // offset of this instance's hydrograph tables in the hydro array
     int ad_flags;   // derivative runs: bit 0 = WAF limiter values frozen
@@ -131,10 +132,13 @@ CF_HD inline void zero_flux(double n1, double n2, T flux[3]) {
 // New state of cell i after one step of length dt starting at time tc.
 // out = (h, hu, hv) after the wet/dry clamp; xout (may be null) = the
 // residual before the implicit friction matrix: res = afm1 * x
-// (finite_volumes_method.cuf:279-294).
+// (finite_volumes_method.cuf:279-294).  qio (may be null) = the cell's
+// outflow discharge (debit_sortie_arr, flux_riemann.cuf:161), inflow
+// discharge (debit_entree_arr, :125-126), each summed over its faces in
+// face order, and 1 if the cell has an inflow or outflow face, else 0.
 template <class T, int IFLUX, int FRIC, class L>
 CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro, int i, double dt, double tc,
-                              T out[3], T *xout) {
+                              T out[3], T *xout, double *qio = nullptr) {
     const double g = p.gp, tls = p.tolisec;
     const St<T> u0 = ld.state(i);
     const I4 nb = ld.nbr(i);
@@ -150,6 +154,8 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
     if (IFLUX == 3) dfi = ld.dfc(i);
     T fl[3][3];
     double fn1[3], fn2[3], fln[3];
+    double qof[3] = {0.0, 0.0, 0.0}, qif[3] = {0.0, 0.0, 0.0};
+    bool io = false;
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         const double n1 = ld.geo(G_N1 + j, i);
@@ -175,11 +181,14 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
                     flux[0] = dn;
                     flux[1] = n1 * ((dn * dn) / (hautl + tls) + (g * (hautl * hautl)) / 2.0);
                     flux[2] = n2 * ((dn * dn) / (hautl + tls) + (g * (hautl * hautl)) / 2.0);
+                    qif[j] = (-dn) * ln;
                 }
+                io = true;
             } else if (cls == 3) {  // wall (:141-145)
                 flux[1] = ((n1 * g) * (hautl * hautl)) / 2.0;
                 flux[2] = ((n2 * g) * (hautl * hautl)) / 2.0;
             } else if (cls == 2) {
+                io = true;
                 if (p.outlet == 1) {  // fixed level (:147-163)
                     if (vitul >= 0.0 && vitul / vsqrt(g * hautl) < 1.0) {
                         hautl = dmax(0.0, p.h_sortie[id - 1] - zml);
@@ -190,6 +199,7 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
                         flux[1] = ((n1 * g) * (hautl * hautl)) / 2.0;
                         flux[2] = ((n2 * g) * (hautl * hautl)) / 2.0;
                     }
+                    qof[j] = val(flux[0]) * ln;
                 } else {  // transmissive: Riemann problem with itself (:166-171)
                     kv = i;
                 }
@@ -202,11 +212,12 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
             T hijet = dmax(T(0.0), (hautl + zml) - zij);
             T hautr, vitur, vitvr;
             rot(tls, n1, n2, ur4.h, ur4.hu, ur4.hv, hautr, vitur, vitvr);
-            if (IFLUX == 1) {
+            if (IFLUX == 1 || IFLUX == 2) {  // :197-228: raw right depth
                 T hjiet = dmax(T(0.0), (ur4.h + zmr) - zij);
                 if (hijet < tls) { hijet = 0.0; vitul = 0.0; vitvl = 0.0; }
                 if (hjiet < tls) { hjiet = 0.0; vitur = 0.0; vitvr = 0.0; }
-                hllc_flux(g, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr, flux);
+                if (IFLUX == 1) hllc_flux(g, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr, flux);
+                else riadh_flux(g, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr, flux);
             } else {
                 // reconstructed depths of flux_riemann.cuf:371-394 (the
                 // values solver_Riemann_waf receives)
@@ -289,6 +300,11 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
         fl[j][2] = flux[2] * ln;
     }
     // face sum (flux_riemann.cuf:411-426)
+    if (qio) {
+        qio[0] = (qof[0] + qof[1]) + qof[2];
+        qio[1] = (qif[0] + qif[1]) + qif[2];
+        qio[2] = io ? 1.0 : 0.0;
+    }
     const T resc0 = (fl[0][0] + fl[1][0]) + fl[2][0];
     const T resc1 = (fl[0][1] + fl[1][1]) + fl[2][1];
     const T resc2 = (fl[0][2] + fl[1][2]) + fl[2][2];

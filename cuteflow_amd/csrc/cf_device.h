@@ -22,6 +22,8 @@ namespace cf {
 // Default-real literals of the reference are single precision, widened.
 constexpr double kEpsWaf = (double)1e-16f;   // flux_riemann.cuf:749
 constexpr double kEpsHllc = (double)1e-8f;   // flux_riemann.cuf:926
+constexpr double kEpsRiadh = (double)1e-6f;  // flux_riemann.cuf:1051
+constexpr double kQSteady = (double)1e-8f;   // finite_volumes_method.cuf:379
 constexpr double kDtLow = (double)1e-12f;    // cfl_condition.cuf:129
 constexpr double kCote = (double)1.8f;       // maillage.cuf:157
 
@@ -204,6 +206,70 @@ CF_HD inline void hllc_flux(double gp, double n1, double n2, T hl, T ul, T vl, T
     flux[0] = fh0;
     flux[1] = n1 * fh1 - n2 * fh2;
     flux[2] = n2 * fh1 + n1 * fh2;
+}
+
+// solver_Riemann_riadh (flux_riemann.cuf:1029-1182), iflux 2: the HLLC wave
+// speeds with eps = 1e-6f, then the flux of the region holding x/t = 0
+// among U_L, U*_L, U*_R, U_R, the star fluxes being F_K + S_K (U*_K - U_K)
+// with U*_K = h_K (S_K - u_K)/(S_K - S*) (1, S*, v_K) (weight 0 when
+// |S_K - S*| <= eps).  Both sides dry (:1055 leaves al, ar, pql, pqr
+// unset): zero flux.
+template <class T>
+CF_HD inline void riadh_flux(double gp, double n1, double n2, T hl, T ul, T vl, T hr, T ur, T vr, T flux[3]) {
+    const double eps = kEpsRiadh;
+    if (hl < eps && hr < eps) {
+        flux[0] = 0.0;
+        flux[1] = 0.0;
+        flux[2] = 0.0;
+        return;
+    }
+    T al = vsqrt(gp * hl);
+    T ar = vsqrt(gp * hr);
+    T het = (hl + hr) / 2.0 - (((ur - ul) * (hl + hr)) / (al + ar)) / 4.0;
+    T uet = (ul + ur) / 2.0 - ((hr - hl) * (al + ar)) / (hl + hr);
+    T pql, pqr, sl, sr, set;
+    if (het < hl) pql = 1.0;
+    else pql = (hl > eps) ? vsqrt(((0.5 * (het + hl)) * het) / (hl * hl)) : T(0.0);
+    if (het < hr) pqr = 1.0;
+    else pqr = (hr > eps) ? vsqrt(((0.5 * (het + hr)) * het) / (hr * hr)) : T(0.0);
+    if (hl > eps) {
+        sl = ul - al * pql;
+        if (hr > eps) {
+            sr = ur + ar * pqr;
+        } else {
+            sl = ul - al;
+            sr = ul + 2.0 * al;
+        }
+    } else {
+        sl = ur - 2.0 * ar;
+        sr = ur + ar;
+    }
+    T denom = hr * (ur - sr) - hl * (ul - sl);
+    if (vabs(denom) < eps) set = uet;
+    else set = ((sl * hr) * (ur - sr) - (sr * hl) * (ul - sl)) / denom;
+    T fx[3];
+    if (0.0 < sl) {
+        fx[0] = hl * ul;
+        fx[1] = hl * (ul * ul) + ((gp * hl) * hl) / 2.0;
+        fx[2] = (hl * ul) * vl;
+    } else if (0.0 < set && 0.0 > sl) {
+        const T pond = (vabs(sl - set) > eps) ? (hl * (sl - ul)) / (sl - set) : T(0.0);
+        fx[0] = hl * ul + sl * (pond - hl);
+        fx[1] = (hl * (ul * ul) + ((gp * hl) * hl) / 2.0) + sl * (pond * set - hl * ul);
+        fx[2] = (hl * ul) * vl + sl * (pond * vl - hl * vl);
+    } else if (0.0 > set && 0.0 < sr) {
+        const T pond = (vabs(sr - set) > eps) ? (hr * (sr - ur)) / (sr - set) : T(0.0);
+        fx[0] = hr * ur + sr * (pond - hr);
+        fx[1] = (hr * (ur * ur) + ((gp * hr) * hr) / 2.0) + sr * (pond * set - hr * ur);
+        fx[2] = (hr * ur) * vr + sr * (pond * vr - hr * vr);
+    } else {
+        fx[0] = hr * ur;
+        fx[1] = hr * (ur * ur) + ((gp * hr) * hr) / 2.0;
+        fx[2] = (hr * ur) * vr;
+    }
+    flux[0] = fx[0];
+    flux[1] = n1 * fx[1] - n2 * fx[2];
+    flux[2] = n2 * fx[1] + n1 * fx[2];
 }
 
 // limiter (flux_riemann.cuf:465-524); unknown ilimiteur -> 1.

@@ -24,10 +24,11 @@ constexpr int kWaves = kTile / 64;
 struct InstCtrl {
     double dt;        // dt of the current state (used by the next step)
     double tc;        // time of the current state
-    int status;       // 0 running, 1 ts reached, -1 dt/NaN failure
+    int status;       // 0 running, 1 ts reached, 2 steady state, -1 dt/NaN failure
     int steps;        // steps done in the current run
     unsigned gstep;   // steps done since creation (dry-tile stamps)
     int par;          // state buffer holding the current state at the run's start
+    int reg;          // consecutive steps with |Qin - Qout| / Qin < tol_reg_perm
 };
 
 struct StepArgs {
@@ -67,6 +68,11 @@ struct StepArgs {
     // rewritten to (+0, +0, +0, zm).
     unsigned char *tile_ran;
     const unsigned char *prev_use;
+    // Discharge bookkeeping (null when off): each cell with an inflow or
+    // outflow face writes its (outflow, inflow) discharge to slot
+    // io_slot[cell] of q_io.
+    double *q_io;
+    const int *io_slot;
     int ncell;        // padded cell count (geo stride)
     int tile_cells;   // cells per tile = workgroup size
     int hist_stride;
@@ -148,6 +154,15 @@ struct cf_handle {
     double *dt_hist = nullptr, *tc_hist = nullptr;
     int *ran = nullptr;
     int cap = 0;                       // steps per run the history arrays hold
+    // discharge bookkeeping (reference check_steady_state): on when asked
+    // for or when an instance has tol_reg_perm > 0
+    int discharge = 0;
+    int nio = 0;                       // cells with an inflow or outflow face
+    int *io_slot = nullptr;            // per cell: its slot (cells with an I/O face)
+    int *io_off = nullptr;             // per instance: first slot (ninst + 1)
+    double *q_io = nullptr;            // 2 per slot: outflow, inflow discharge of the step
+    double *qin_hist = nullptr, *qout_hist = nullptr;  // like dt_hist, when discharge is on
+    std::vector<std::vector<double>> last_qin, last_qout;  // per instance, last run
     // host mirror of the control block after the last finished run
     std::vector<cf::InstCtrl> hctrl;
     std::vector<long long> last_run, last_total;
@@ -198,7 +213,8 @@ inline bool h2d(T *dst, const T *src, size_t count, hipStream_t s) {
 // defined in cuteflow_hip.hip
 StepArgs base_args(cf_handle *h);
 void launch_step(const StepArgs &a, int iflux, int fric, int grid, hipStream_t s);
-void launch_finalize(cf_handle *h, double *dt_hist, double *tc_hist, int *ran, int stride, int step, hipStream_t s);
+void launch_finalize(cf_handle *h, double *dt_hist, double *tc_hist, int *ran, int stride, int step, hipStream_t s,
+                     double *qin_hist = nullptr, double *qout_hist = nullptr);
 void launch_run_init(cf_handle *h, double *dt_hist, double *tc_hist, int stride, hipStream_t s);
 int set_schedule(cf_handle *h, int skip, int xcd);
 // defined in cf_adjoint.hip

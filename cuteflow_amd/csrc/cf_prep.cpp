@@ -224,8 +224,21 @@ int prepare(const cf_mesh *mesh, const cf_params *params, int tile, bool keep_or
         for (int k = 0; k < 3; k++) out.conn[3 * n + k] = conn[3 * (size_t)o + k];
     }
 
+    // cells with an inflow or outflow face, in the caller's order (the order
+    // in which their discharges are summed)
+    out.io_cells.clear();
+    for (size_t o = 0; o < N; o++)
+        for (int k = 0; k < 3; k++) {
+            const int c = nb0[o].c[k];
+            if (c < 0 && (((-c) & 3) == 1 || ((-c) & 3) == 2)) {
+                out.io_cells.push_back(out.inv[o]);
+                break;
+            }
+        }
+
     // tiles of `tile` consecutive internal cells: neighbour tiles and the
-    // tiles that must always run (inflow faces give a flux into dry cells)
+    // tiles that must always run (inflow faces give a flux into dry cells;
+    // inflow and outflow cells report their discharge every step)
     const int ntiles = (int)((N + tile - 1) / tile);
     out.tile_off.assign(ntiles + 1, 0);
     out.tile_adj.clear();
@@ -240,7 +253,7 @@ int prepare(const cf_mesh *mesh, const cf_params *params, int tile, bool keep_or
                 if (c >= 0) {
                     const int tt = c / tile;
                     if (tt != t) tmp.push_back(tt);
-                } else if (((-c) & 3) == 1) {
+                } else if (((-c) & 3) == 1 || ((-c) & 3) == 2) {
                     out.tile_always[t] = 1;
                 }
             }

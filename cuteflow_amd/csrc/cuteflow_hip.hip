@@ -120,10 +120,15 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
     if (i < a.inst_end[b]) {
         const DevLd ld{st_in, a.nbr, a.dfc, a.geo, a.manning, a.ncell};
         const double zml = st_in[i].w;
-        double o[3], xo[3];
-        cell_update<double, IFLUX, FRIC>(ld, p, a.hydro, i, dt, tc, o, a.xs ? xo : nullptr);
+        double o[3], xo[3], qio[3];
+        cell_update<double, IFLUX, FRIC>(ld, p, a.hydro, i, dt, tc, o, a.xs ? xo : nullptr, a.q_io ? qio : nullptr);
         const double h = o[0], hu = o[1], hv = o[2];
         st_out[i] = make_double4(h, hu, hv, zml);
+        if (a.q_io && qio[2] != 0.0) {
+            const int sl = a.io_slot[i];
+            a.q_io[2 * sl] = qio[0];
+            a.q_io[2 * sl + 1] = qio[1];
+        }
         if (replay) {
             if (a.xs) {
                 a.xs[i] = xo[0];
@@ -200,15 +205,40 @@ __device__ double block_min_1024(const double *__restrict__ part, int n, double 
 // End of a step, one workgroup per instance: dt of the new state (the
 // reference's reduction, cfl_condition.cuf:68-104), tc += dt, the dt range
 // check (:129-132) and the loop condition (finite_volumes_method.cuf:123).
+// With discharge bookkeeping (q_io set): the step's inflow and outflow
+// discharges (summed over the instance's slots in the caller's cell order)
+// and check_steady_state (finite_volumes_method.cuf:370-393): more than
+// 1000 consecutive steps with |Qin - Qout| / Qin < tol_reg_perm stop the
+// instance after the step.
 __global__ __launch_bounds__(1024) void finalize_kernel(InstCtrl *ctrl, const KParams *kp, const double *partial,
                                                          const int *inst_tile, double *dt_hist, double *tc_hist,
-                                                         int *ran, int hist_stride, int step) {
+                                                         int *ran, int hist_stride, int step, const double *q_io,
+                                                         const int *io_off, double *qin_hist, double *qout_hist) {
+    __shared__ double s_q[2 * 1024];
     const int b = blockIdx.x;
     InstCtrl *c = ctrl + b;
     if (c->status != 0) return;
     const int t0 = inst_tile[b], nt = inst_tile[b + 1] - t0;
     int nan, nskip;
     const double dtn = block_min_1024(partial + t0, nt, kp[b].pad_dt, nan, nskip);
+    double qo = 0.0, qi = 0.0;
+    if (q_io) {
+        const int s0 = io_off[b], s1 = io_off[b + 1];
+        for (int base = s0; base < s1; base += (int)blockDim.x) {
+            const int n = min((int)blockDim.x, s1 - base);
+            __syncthreads();
+            if ((int)threadIdx.x < n) {
+                s_q[2 * threadIdx.x] = q_io[2 * (size_t)(base + threadIdx.x)];
+                s_q[2 * threadIdx.x + 1] = q_io[2 * (size_t)(base + threadIdx.x) + 1];
+            }
+            __syncthreads();
+            if (threadIdx.x == 0)
+                for (int k = 0; k < n; k++) {
+                    qo = qo + s_q[2 * k];
+                    qi = qi + s_q[2 * k + 1];
+                }
+        }
+    }
     if (threadIdx.x == 0) {
         const double tcn = c->tc + c->dt;
         c->dt = dtn;
@@ -218,13 +248,22 @@ __global__ __launch_bounds__(1024) void finalize_kernel(InstCtrl *ctrl, const KP
         dt_hist[(size_t)b * hist_stride + step + 1] = dtn;
         tc_hist[(size_t)b * hist_stride + step + 1] = tcn;
         ran[(size_t)b * hist_stride + step] = nt - nskip;
+        bool steady = false;
+        if (q_io) {
+            qout_hist[(size_t)b * hist_stride + step] = qo;
+            qin_hist[(size_t)b * hist_stride + step] = qi;
+            if (qi > kQSteady && fabs(qi - qo) / qi < kp[b].tol_reg_perm) steady = ++c->reg > 1000;
+            else c->reg = 0;
+        }
         if (nan || !(dtn <= 9999.0 && dtn >= kDtLow)) c->status = -1;
+        else if (steady) c->status = 2;
         else if (!(((tcn - kp[b].ts) + 2.0 * kp[b].tol) < kp[b].tol)) c->status = 1;
     }
 }
 
 // Start of a run: the buffer parity moves past the previous run's steps,
-// the loop condition is evaluated (a failed instance stays failed).
+// the loop condition is evaluated (a failed instance stays failed, one at
+// steady state stays stopped: a new state restarts both).
 __global__ void run_init_kernel(InstCtrl *ctrl, const KParams *kp, double *dt_hist, double *tc_hist,
                                 int hist_stride, int ninst) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -232,7 +271,7 @@ __global__ void run_init_kernel(InstCtrl *ctrl, const KParams *kp, double *dt_hi
     InstCtrl *c = ctrl + b;
     c->par = (c->par + c->steps) & 1;
     c->steps = 0;
-    if (c->status >= 0) c->status = (((c->tc - kp[b].ts) + 2.0 * kp[b].tol) < kp[b].tol) ? 0 : 1;
+    if (c->status == 0 || c->status == 1) c->status = (((c->tc - kp[b].ts) + 2.0 * kp[b].tol) < kp[b].tol) ? 0 : 1;
     dt_hist[(size_t)b * hist_stride] = c->dt;
     tc_hist[(size_t)b * hist_stride] = c->tc;
 }
@@ -278,6 +317,7 @@ __global__ __launch_bounds__(1024) void reset_inst_kernel(const double *part, in
         ctrl[b].dt = nan ? __builtin_nan("") : m;
         ctrl[b].tc = 0.0;
         ctrl[b].status = 0;
+        ctrl[b].reg = 0;
     }
 }
 
@@ -366,6 +406,7 @@ static KParams make_kparams(const cf_params &p, int nelt, int hydro_off) {
     k.gp = p.gp; k.tolisec = p.tolisec; k.tol = p.tol; k.cfl = p.cfl; k.ts = p.ts;
     k.constant_dt = p.constant_dt; k.override_manning = p.override_manning;
     k.pad_dt = (nelt % 1024 != 0) ? 100000.0 : 1000000.0;
+    k.tol_reg_perm = p.tol_reg_perm;
     k.ilimiteur = p.ilimiteur; k.inlet = p.inlet; k.outlet = p.outlet;
     k.is_dt_constant = p.is_dt_constant; k.is_override_manning = p.is_override_manning;
     k.hydro_off = hydro_off;
@@ -414,10 +455,12 @@ static int ensure_hist(cf_handle *h, int nsteps) {
     if (nsteps <= h->cap) return CF_OK;
     const int cap = std::max(nsteps, 2 * h->cap);
     HIPCHK(hipStreamSynchronize(h->stream));
-    dfree(h->dt_hist); dfree(h->tc_hist); dfree(h->ran);
+    dfree(h->dt_hist); dfree(h->tc_hist); dfree(h->ran); dfree(h->qin_hist); dfree(h->qout_hist);
     drop_graphs(h);
     const size_t n = (size_t)h->ninst * (size_t)(cap + 1);
-    if (!dalloc(h->dt_hist, n) || !dalloc(h->tc_hist, n) || !dalloc(h->ran, n)) return CF_ERR_NOMEM;
+    if (!dalloc(h->dt_hist, n) || !dalloc(h->tc_hist, n) || !dalloc(h->ran, n) || !dalloc(h->qin_hist, n) ||
+        !dalloc(h->qout_hist, n))
+        return CF_ERR_NOMEM;
     h->cap = cap;
     return CF_OK;
 }
@@ -436,7 +479,7 @@ extern "C" const char *cf_error_string(int s) {
 
 static int validate_params(const cf_params *p) {
     if (!(p->gp > 0.0) || !(p->cfl > 0.0) || !(p->tolisec >= 0.0)) return CF_ERR_ARG;
-    if (p->iflux != 1 && p->iflux != 3) return CF_ERR_ARG;
+    if (p->iflux < 1 || p->iflux > 3) return CF_ERR_ARG;  // iflux 4 (MUSCL) not provided
     if (p->inlet != 1 && p->inlet != 2) return CF_ERR_ARG;  // 'phys_inflow' not provided
     if (p->outlet != 1 && p->outlet != 2) return CF_ERR_ARG;
     if (p->friction != 0 && p->friction != 1) return CF_ERR_ARG;
@@ -456,6 +499,7 @@ static void destroy_buffers(cf_handle *h) {
     dfree(h->tile_adj_off); dfree(h->tile_adj); dfree(h->always);
     dfree(h->tile_inst); dfree(h->inst_end); dfree(h->inst_tile);
     dfree(h->dt_hist); dfree(h->tc_hist); dfree(h->ran);
+    dfree(h->io_slot); dfree(h->io_off); dfree(h->q_io); dfree(h->qin_hist); dfree(h->qout_hist);
 }
 
 extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params *params, cf_handle **out) {
@@ -479,8 +523,24 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
         ((params[0].options & CF_OPT_TILE_128) && (params[0].options & CF_OPT_TILE_64)))
         return CF_ERR_ARG;
     const int tile = (params[0].options & CF_OPT_TILE_64) ? 64 : (params[0].options & CF_OPT_TILE_128) ? 128 : kTile;
+    // Instances that pass the same mesh arrays (e.g. scenarios of one reach)
+    // share one preparation: it depends on the mesh alone, apart from the
+    // inflow-length check of inlet 2.
     std::vector<cfprep::Prep> prep(ninst);
     for (int b = 0; b < ninst; b++) {
+        const cf_mesh &m = meshes[b];
+        int same = -1;
+        for (int q = 0; q < b && same < 0; q++) {
+            const cf_mesh &mq = meshes[q];
+            if (mq.nelt == m.nelt && mq.nnt == m.nnt && mq.conn == m.conn && mq.coords == m.coords &&
+                mq.boundary == m.boundary && params[q].inlet == params[b].inlet &&
+                memcmp(params[q].long_entree, params[b].long_entree, sizeof(params[b].long_entree)) == 0)
+                same = q;
+        }
+        if (same >= 0) {
+            prep[b] = prep[same];
+            continue;
+        }
         int st = cfprep::prepare(meshes + b, params + b, tile, (params[b].options & CF_OPT_KEEP_ORDER) != 0, prep[b]);
         if (st) return st;
     }
@@ -497,6 +557,8 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
     h->hctrl.assign(ninst, InstCtrl{});
     h->last_run.assign(ninst, 0);
     h->last_total.assign(ninst, 0);
+    h->last_qin.assign(ninst, {});
+    h->last_qout.assign(ninst, {});
     // skipping dry tiles is exact except with a constant dt (every cell then
     // contributes constant_dt to the minimum)
     h->skip = (params[0].options & CF_OPT_NO_TILE_SKIP) ? 0 : 1;
@@ -526,6 +588,15 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
     std::vector<double> mann(N, 0.0);
     std::vector<int> perm(N, 0), tile_inst(tiles), inst_end(ninst), inst_tile(ninst + 1), adj_off(tiles + 1, 0), adj;
     std::vector<unsigned char> always(tiles, 0);
+    // discharge slots: the instances' inflow/outflow cells, caller order
+    std::vector<int> io_slot(N, -1), io_off(ninst + 1, 0);
+    for (int b = 0; b < ninst; b++) {
+        io_off[b + 1] = io_off[b] + (int)prep[b].io_cells.size();
+        for (size_t q = 0; q < prep[b].io_cells.size(); q++)
+            io_slot[h->cell_off[b] + prep[b].io_cells[q]] = io_off[b] + (int)q;
+        if (params[b].tol_reg_perm > 0.0) h->discharge = 1;
+    }
+    h->nio = io_off[ninst];
     for (int b = 0; b < ninst; b++) {
         const int c0 = h->cell_off[b], t0 = h->tile_off[b];
         for (int i = 0; i < h->nelt[b]; i++) {
@@ -561,7 +632,8 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
             !dalloc(h->stamp, (size_t)tiles) || !dalloc(h->tile_adj_off, adj_off.size()) ||
             !dalloc(h->tile_adj, adj.size()) || !dalloc(h->always, (size_t)tiles) ||
             !dalloc(h->tile_inst, (size_t)tiles) || !dalloc(h->inst_end, (size_t)ninst) ||
-            !dalloc(h->inst_tile, (size_t)ninst + 1)) {
+            !dalloc(h->inst_tile, (size_t)ninst + 1) || !dalloc(h->io_slot, N) || !dalloc(h->io_off, (size_t)ninst + 1) ||
+            !dalloc(h->q_io, 2 * std::max<size_t>((size_t)h->nio, 1))) {
             rc = CF_ERR_NOMEM;
             break;
         }
@@ -574,6 +646,10 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
                   h2d(h->tile_inst, tile_inst.data(), (size_t)tiles, h->stream) &&
                   h2d(h->inst_end, inst_end.data(), (size_t)ninst, h->stream) &&
                   h2d(h->inst_tile, inst_tile.data(), (size_t)ninst + 1, h->stream) &&
+                  h2d(h->io_slot, io_slot.data(), N, h->stream) &&
+                  h2d(h->io_off, io_off.data(), (size_t)ninst + 1, h->stream) &&
+                  hipMemsetAsync(h->q_io, 0, 2 * std::max<size_t>((size_t)h->nio, 1) * sizeof(double), h->stream) ==
+                      hipSuccess &&
                   h2d(h->ctrl, h->hctrl.data(), (size_t)ninst, h->stream) &&
                   hipMemsetAsync(h->stamp, 0, (size_t)tiles * sizeof(unsigned), h->stream) == hipSuccess &&
                   hipMemsetAsync(h->stage, 0, 3 * (size_t)h->max_cells * sizeof(double), h->stream) == hipSuccess &&
@@ -708,10 +784,11 @@ static void launch_step_t(const StepArgs &a, int grid, hipStream_t s) {
 typedef void (*step_launcher)(const StepArgs &, int, hipStream_t);
 
 static step_launcher pick_launcher(int iflux, int fric) {
-    static const step_launcher tab[2][4] = {
+    static const step_launcher tab[3][4] = {
         {launch_step_t<1, 0>, launch_step_t<1, 1>, launch_step_t<1, 2>, launch_step_t<1, 3>},
+        {launch_step_t<2, 0>, launch_step_t<2, 1>, launch_step_t<2, 2>, launch_step_t<2, 3>},
         {launch_step_t<3, 0>, launch_step_t<3, 1>, launch_step_t<3, 2>, launch_step_t<3, 3>}};
-    return tab[iflux == 3 ? 1 : 0][fric];
+    return tab[iflux - 1][fric];
 }
 
 void cf::launch_step(const StepArgs &a, int iflux, int fric, int grid, hipStream_t s) {
@@ -719,9 +796,10 @@ void cf::launch_step(const StepArgs &a, int iflux, int fric, int grid, hipStream
 }
 
 void cf::launch_finalize(cf_handle *h, double *dt_hist, double *tc_hist, int *ran, int stride, int step,
-                         hipStream_t s) {
+                         hipStream_t s, double *qin_hist, double *qout_hist) {
+    const bool q = qin_hist && qout_hist;
     hipLaunchKernelGGL(finalize_kernel, dim3(h->ninst), dim3(1024), 0, s, h->ctrl, h->d_kp, h->partial, h->inst_tile,
-                       dt_hist, tc_hist, ran, stride, step);
+                       dt_hist, tc_hist, ran, stride, step, q ? h->q_io : nullptr, h->io_off, qin_hist, qout_hist);
 }
 
 void cf::launch_run_init(cf_handle *h, double *dt_hist, double *tc_hist, int stride, hipStream_t s) {
@@ -762,6 +840,8 @@ StepArgs cf::base_args(cf_handle *h) {
     a.hist_stride = h->cap + 1;
     a.skip = h->skip;
     a.xcd = h->xcd;
+    a.q_io = h->discharge ? h->q_io : nullptr;
+    a.io_slot = h->io_slot;
     return a;
 }
 
@@ -773,7 +853,8 @@ static void enqueue_run(cf_handle *h, int nsteps, hipStream_t s) {
     for (int k = 0; k < nsteps; k++) {
         a.step = k;
         L(a, h->ntiles, s);
-        launch_finalize(h, h->dt_hist, h->tc_hist, h->ran, h->cap + 1, k, s);
+        launch_finalize(h, h->dt_hist, h->tc_hist, h->ran, h->cap + 1, k, s, h->discharge ? h->qin_hist : nullptr,
+                        h->discharge ? h->qout_hist : nullptr);
     }
 }
 
@@ -834,10 +915,17 @@ extern "C" int cf_batch_step_finish(cf_handle *h, double *dt_hist, int *steps_do
     }
     std::vector<InstCtrl> c(B);
     HIPCHK(hipMemcpyAsync(c.data(), h->ctrl, B * sizeof(InstCtrl), hipMemcpyDeviceToHost, h->stream));
-    std::vector<double> dts((size_t)B * S);
+    std::vector<double> dts((size_t)B * S), qin, qout;
     std::vector<int> ran((size_t)B * S);
     HIPCHK(hipMemcpyAsync(dts.data(), h->dt_hist, dts.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipMemcpyAsync(ran.data(), h->ran, ran.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    if (h->discharge) {
+        qin.resize((size_t)B * S);
+        qout.resize((size_t)B * S);
+        HIPCHK(hipMemcpyAsync(qin.data(), h->qin_hist, qin.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(qout.data(), h->qout_hist, qout.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              h->stream));
+    }
     HIPCHK(hipStreamSynchronize(h->stream));
     int rc = CF_OK;
     for (int b = 0; b < B; b++) {
@@ -850,6 +938,13 @@ extern "C" int cf_batch_step_finish(cf_handle *h, double *dt_hist, int *steps_do
         for (int k = 0; k < done; k++) run += ran[(size_t)b * S + k];
         h->last_run[b] = run;
         h->last_total[b] = (long long)done * (h->tile_off[b + 1] - h->tile_off[b]);
+        if (h->discharge) {
+            h->last_qin[b].assign(qin.begin() + (size_t)b * S, qin.begin() + (size_t)b * S + done);
+            h->last_qout[b].assign(qout.begin() + (size_t)b * S, qout.begin() + (size_t)b * S + done);
+        } else {
+            h->last_qin[b].clear();
+            h->last_qout[b].clear();
+        }
     }
     h->hctrl = c;
     return rc;
@@ -918,6 +1013,31 @@ extern "C" int cf_last_run_stats(cf_handle *h, long long *tiles_run, long long *
     if (tiles_total) *tiles_total = t;
     if (tile_cells) *tile_cells = h->tile;
     return CF_OK;
+}
+
+extern "C" int cf_set_discharge(cf_handle *h, int on) {
+    if (!h || h->pending >= 0) return CF_ERR_ARG;
+    on = on ? 1 : 0;
+    for (int b = 0; b < h->ninst; b++)
+        if (h->prm[b].tol_reg_perm > 0.0) on = 1;  // the steady-state check needs it
+    if (on != h->discharge) {
+        HIPCHK(hipStreamSynchronize(h->stream));
+        drop_graphs(h);
+        h->discharge = on;
+    }
+    return CF_OK;
+}
+
+extern "C" int cf_batch_get_discharge(cf_handle *h, int b, double *q_in, double *q_out, int *steady) {
+    if (bad_inst(h, b)) return CF_ERR_ARG;
+    if (q_in) memcpy(q_in, h->last_qin[b].data(), h->last_qin[b].size() * sizeof(double));
+    if (q_out) memcpy(q_out, h->last_qout[b].data(), h->last_qout[b].size() * sizeof(double));
+    if (steady) *steady = h->hctrl[b].status == 2 ? 1 : 0;
+    return CF_OK;
+}
+
+extern "C" int cf_get_discharge(cf_handle *h, double *q_in, double *q_out, int *steady) {
+    return cf_batch_get_discharge(h, 0, q_in, q_out, steady);
 }
 
 extern "C" int cf_batch_set_inflow(cf_handle *h, int b, const double *debitglob, int n) {

@@ -98,12 +98,13 @@ def run_case(case_dir: str, verbose: bool = True, max_steps: int | None = None):
         _, U0 = cio.read_initial_condition(os.path.join(case_dir, prm.fich_sol_init.strip()), mesh.nelt)
     else:
         U0 = initial_state(prm, mesh, geo["zm"])
+    s.set_discharge(True)  # debit_entree / debit_sortie of the status block
     s.set_state(U0)
     snaps = SnapshotWriter(prm, mesh, geo["zm"], geo["surf"], case_dir)
     snaps.check(0.0, s.dt, s.get_state)
 
     freq = max(1, prm.freqaffich)
-    nt, dts = 0, []
+    nt, dts, q_in, q_out, steady = 0, [], [], [], False
     t0 = time.perf_counter()
     if verbose:
         print(" Loop over time starts")
@@ -121,12 +122,23 @@ def run_case(case_dir: str, verbose: bool = True, max_steps: int | None = None):
                 break
         hist, done, st = s.step(n, raise_on_error=False)
         dts.extend(hist[:done])
+        qi, qo, steady = s.discharge()
+        q_in.extend(qi)
+        q_out.extend(qo)
         nt += done
         if done and snaps.pending:
             snaps.check(s.tc, hist[done - 1], s.get_state)
-        if verbose and done and nt % freq == 0:
+        if verbose and steady:  # check_steady_state (finite_volumes_method.cuf:380-383)
+            print(" -----------------------------------------------")
+            print(" LE REGIME PERMANENT EST ATTEINT")
+            print(" -----------------------------------------------")
+        if verbose and done and nt % freq == 0:  # print_status_in_console (:346-363)
             print(" -----------------------------------------------")
             print(" Iteration number ", nt, ", t = ", s.tc)
+            if prm.nombre_entrees == 1:
+                print(" debit_entree ", 1, " = ", qi[-1])
+            print(" debit_entree_sum =", qi[-1], "M3/S")
+            print(" debit_sortie     =", qo[-1], "M3/S")
             print(" -----------------------------------------------")
         if st != 0:
             print(" Erreur sur le pas de temps dt=", s.dt)
@@ -143,7 +155,8 @@ def run_case(case_dir: str, verbose: bool = True, max_steps: int | None = None):
     if verbose:
         print(" DUREE DU CALCUL :", wall, " Secondes ")
     out = {"U": U, "tc": s.tc, "steps": nt, "dt_hist": np.array(dts), "status": status, "wall_s": wall,
-           "snapshots": snaps.written, "zm": geo["zm"]}
+           "snapshots": snaps.written, "zm": geo["zm"], "q_in": np.array(q_in), "q_out": np.array(q_out),
+           "steady": steady}
     s.close()
     return out
 

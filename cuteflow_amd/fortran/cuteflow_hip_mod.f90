@@ -104,6 +104,19 @@ module cuteflow_hip
        type(c_ptr), value :: handle
        real(c_double) :: n(*)
      end function cf_set_manning
+
+     integer(c_int) function cf_set_discharge(handle, on) bind(C, name="cf_set_discharge")
+       import :: c_int, c_ptr
+       type(c_ptr), value :: handle
+       integer(c_int), value :: on
+     end function cf_set_discharge
+
+     integer(c_int) function cf_get_discharge(handle, q_in, q_out, steady) bind(C, name="cf_get_discharge")
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: handle
+       real(c_double) :: q_in(*), q_out(*)
+       integer(c_int) :: steady
+     end function cf_get_discharge
   end interface
 
 contains

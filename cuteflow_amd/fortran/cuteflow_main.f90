@@ -51,7 +51,8 @@ program cuteflow_main
   integer, allocatable :: conn(:, :), ndinput(:), num_in(:), ndoutput(:), num_out(:), ndwall(:)
   integer(c_int), allocatable, target :: conn_c(:), bnd(:)
   real(c_double), allocatable, target :: coords_c(:), mann_c(:)
-  real(c_double), allocatable :: dt_hist(:)
+  real(c_double), allocatable :: dt_hist(:), q_in(:), q_out(:)
+  integer(c_int) :: steady
   character(len=1) :: tarati
   character(len=260) :: file_bc
   type(cf_params_t) :: prm
@@ -252,12 +253,14 @@ program cuteflow_main
         end if
      end do
   end if
+  st = cf_set_discharge(h, 1_c_int)   ! debit_entree / debit_sortie of the status block
+  call cf_check(st, 'cf_set_discharge')
   st = cf_set_state(h, u)
   call cf_check(st, 'cf_set_state')
 
   ! ---- time loop (finite_volumes_method.cuf:113-242), freqaffich steps per batch ----
   nbatch = max(1, freqaffich)
-  allocate(dt_hist(nbatch + 1))
+  allocate(dt_hist(nbatch + 1), q_in(nbatch), q_out(nbatch))
   nt = 0
   write(*, *) 'Loop over time starts'
   call system_clock(c0, crate)
@@ -271,9 +274,19 @@ program cuteflow_main
      end if
      call cf_check(st, 'cf_step')
      st = cf_get_time(h, tc, dt)
-     if (mod(nt, freqaffich) == 0 .and. done > 0) then
+     st = cf_get_discharge(h, q_in, q_out, steady)
+     call cf_check(st, 'cf_get_discharge')
+     if (steady /= 0) then   ! check_steady_state (finite_volumes_method.cuf:380-383)
+        print *, '-----------------------------------------------'
+        print *, 'LE REGIME PERMANENT EST ATTEINT'
+        print *, '-----------------------------------------------'
+     end if
+     if (mod(nt, freqaffich) == 0 .and. done > 0) then   ! print_status_in_console (:346-363)
         print *, '-----------------------------------------------'
         write(*, *) 'Iteration number ', nt, ", t = ", tc
+        if (nombre_entrees == 1) print *, 'debit_entree ', 1, " = ", q_in(done)
+        print *, 'debit_entree_sum =', q_in(done), 'M3/S'
+        print *, 'debit_sortie     =', q_out(done), 'M3/S'
         print *, '-----------------------------------------------'
      end if
      if (done < nbatch) exit

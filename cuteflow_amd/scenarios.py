@@ -11,6 +11,8 @@
   channel/floodplain);
 - calibration(): the zonal calibration reach: 8 zones, gauge cells in the
   channel, a hidden true n per zone and the start 20 % above it;
+- calibration_batch(): many calibration scenarios (flood events) on one
+  reach, each with its own peak inflow, hidden zonal n and gauges;
 - field_ensemble(): members with a per-cell log-normal Manning field and
   their own inflow hydrographs (the floodplain wets and dries as the flood
   passes).
@@ -107,6 +109,30 @@ def calibration(ncells: int = 100_000, ngauges: int = 20, seed: int = 7, iflux: 
     cand = np.flatnonzero(s.channel & (xc > 0.25 * xc.max()))
     s.gauges = np.sort(rng.choice(cand, size=ngauges, replace=False)).astype(np.int32)
     return s
+
+
+def calibration_batch(nscen: int = 64, ncells: int = 100_000, ngauges: int = 20, seed: int = 64,
+                      iflux: int = 1):
+    """nscen calibration scenarios on the zonal calibration reach (one mesh,
+    8 zones): scenario k has its own peak inflow (uniform in 500-4000 m3/s
+    over the 1 h ramp), hidden true n per zone (uniform in 0.025-0.05) and
+    ngauges seeded channel gauges in the downstream three quarters."""
+    rng = np.random.default_rng(seed)
+    base = reach(ncells, seed=seed, iflux=iflux)
+    zone = zones_of(base.mesh, base.channel)
+    xc, _ = base.mesh.centroids()
+    cand = np.flatnonzero(base.channel & (xc > 0.25 * xc.max()))
+    out = []
+    for k in range(nscen):
+        q = float(rng.uniform(500.0, 4000.0))
+        p = dataclasses.replace(base.params)
+        p.hydrographs = {1: ([0.0, 3600.0], [0.0, q])}
+        nz = rng.uniform(0.025, 0.05, 8)
+        m = dataclasses.replace(base.mesh, manning=nz[zone].copy())
+        g = np.sort(rng.choice(cand, size=ngauges, replace=False)).astype(np.int32)
+        out.append(Scenario(mesh=m, params=p, state=base.state.copy(), channel=base.channel, zone=zone,
+                            n_zones=nz, gauges=g))
+    return out
 
 
 def field_ensemble(nmembers: int = 16, ncells: int = 1_000_000, ngauges: int = 50, seed: int = 99,

@@ -86,9 +86,24 @@ class Solver:
         hist = np.zeros(nsteps + 1)
         done = C.c_int()
         st = self.lib.cf_step(self.h, int(nsteps), _lib.dptr(hist), C.byref(done))
+        self.last_done = done.value
         if st != _lib.CF_OK and (raise_on_error or st != _lib.CF_ERR_DT):
             raise _lib.CuteflowError(st, "cf_step")
         return hist[: done.value + 1], done.value, st
+
+    def set_discharge(self, on: bool = True) -> None:
+        """Per-step inflow/outflow discharge of the following runs (always on
+        when tol_reg_perm > 0: the steady-state stop uses it)."""
+        _lib.check(self.lib.cf_set_discharge(self.h, int(bool(on))), "cf_set_discharge")
+
+    def discharge(self):
+        """(q_in, q_out, steady) of the last step(): per-step inflow and
+        outflow discharges and whether the run stopped at steady state."""
+        n = getattr(self, "last_done", 0)
+        q_in, q_out, steady = np.zeros(n), np.zeros(n), C.c_int()
+        _lib.check(self.lib.cf_get_discharge(self.h, _lib.dptr(q_in), _lib.dptr(q_out), C.byref(steady)),
+                   "cf_get_discharge")
+        return q_in, q_out, bool(steady.value)
 
     def last_run_stats(self) -> dict:
         """Tiles that ran in the last step() (dry tiles with dry neighbours are
@@ -144,11 +159,15 @@ def _batch_structs(meshes, params, ts, options):
     cms = (_lib.CfMesh * n)()
     cps = (_lib.CfParams * n)()
     keep, boundaries = [], []
+    shared = {}  # meshes sharing their arrays (scenarios of one reach) share the tables
     for b, (m, p) in enumerate(zip(meshes, params)):
-        bt = m.boundary_table()
-        le = m.inlet_lengths(bt, p.nombre_entrees)
-        conn = np.ascontiguousarray(m.conn.reshape(-1), np.int32)
-        coords = np.ascontiguousarray(m.coords.reshape(-1), np.float64)
+        key = (id(m.conn), id(m.coords), id(m.ndinput), id(m.numero_ndinput), id(m.ndoutput),
+               id(m.numero_ndoutput), id(m.ndwall), p.nombre_entrees)
+        if key not in shared:
+            bt = m.boundary_table()
+            shared[key] = (bt, m.inlet_lengths(bt, p.nombre_entrees), np.ascontiguousarray(m.conn.reshape(-1), np.int32),
+                           np.ascontiguousarray(m.coords.reshape(-1), np.float64))
+        bt, le, conn, coords = shared[key]
         mann = np.ascontiguousarray(m.manning, np.float64)
         keep += [bt, conn, coords, mann]
         boundaries.append(bt)
@@ -278,9 +297,21 @@ class BatchSolver:
         done = np.zeros(B, np.int32)
         status = np.zeros(B, np.int32)
         st = self.lib.cf_batch_step(self.h, int(nsteps), _lib.dptr(hist), _lib.iptr(done), _lib.iptr(status))
+        self.last_done = done.copy()
         if st != _lib.CF_OK and (raise_on_error or st != _lib.CF_ERR_DT):
             raise _lib.CuteflowError(st, "cf_batch_step")
         return [hist[b, : done[b] + 1] for b in range(B)], done, status
+
+    def set_discharge(self, on: bool = True) -> None:
+        _lib.check(self.lib.cf_set_discharge(self.h, int(bool(on))), "cf_set_discharge")
+
+    def discharge(self, b: int):
+        """(q_in, q_out, steady) of instance b in the last step()."""
+        n = int(self.last_done[b]) if hasattr(self, "last_done") else 0
+        q_in, q_out, steady = np.zeros(n), np.zeros(n), C.c_int()
+        _lib.check(self.lib.cf_batch_get_discharge(self.h, b, _lib.dptr(q_in), _lib.dptr(q_out), C.byref(steady)),
+                   "cf_batch_get_discharge")
+        return q_in, q_out, bool(steady.value)
 
     def last_run_stats(self) -> dict:
         run, tot, cells = C.c_longlong(), C.c_longlong(), C.c_int()

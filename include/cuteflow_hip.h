@@ -54,7 +54,7 @@ typedef struct cf_params {
     double ts;               /* end time; use 1e300 for a fixed step count */
     double constant_dt;      /* used when is_dt_constant == 1 */
     double override_manning; /* used when is_override_manning == 1 */
-    double tol_reg_perm;     /* steady-state tolerance (not used by the step) */
+    double tol_reg_perm;     /* steady-state tolerance: > 0 stops a run at steady state (see cf_set_discharge) */
     int iflux;               /* 1 = HLLC, 3 = WAF (Loukili & Soulaimani) */
     int ilimiteur;           /* WAF limiter 0..4 */
     int friction;            /* 0/1 */
@@ -191,6 +191,22 @@ int cf_batch_set_hydrograph(cf_handle *h, int inst, int inlet, int npts, const d
 int cf_batch_set_manning(cf_handle *h, int inst, const double *n);
 int cf_batch_get_geometry(cf_handle *h, int inst, double *surf, double *zm, double *cotemin_arr,
                           double *cotemin);
+
+/* Discharge bookkeeping and the steady-state stop (flux_riemann.cuf:125-126,
+ * 161 record the inflow and outflow edge discharges; check_steady_state,
+ * finite_volumes_method.cuf:370-393).  With it on, every step records its
+ * flux's inflow discharge (inflow edges, inlet 'inflow') and outflow
+ * discharge (fixed-level outlet edges), each summed over the cells in the
+ * caller's order, and a run stops after a step that makes more than 1000
+ * consecutive steps with Qin > 1e-8f and |Qin - Qout| / Qin < tol_reg_perm
+ * (status CF_OK, steps_done < nsteps).  It is on whenever an instance has
+ * tol_reg_perm > 0; cf_set_discharge(h, 1) turns it on for the discharges
+ * alone.  cf_batch_get_discharge copies the last run's per-step Qin and
+ * Qout (steps_done entries each, may be NULL) and whether the instance
+ * stopped at steady state. */
+int cf_set_discharge(cf_handle *h, int on);
+int cf_batch_get_discharge(cf_handle *h, int inst, double *q_in, double *q_out, int *steady);
+int cf_get_discharge(cf_handle *h, double *q_in, double *q_out, int *steady);
 
 /* Schedule autotuner.  No reference counterpart (the reference's launch
  * shapes are fixed: 24-thread flux blocks, flux_riemann.cuf; 1024-thread

@@ -17,8 +17,8 @@
  *     algorithm is repeated bit-for-bit in the device code.
  *
  * Reference undefined behaviour is given a defined meaning:
- *   - dry/dry faces (flux_riemann.cuf:753,930 `goto 20` skips al,ar,pql,pqr):
- *     the Riemann solvers return a zero flux;
+ *   - dry/dry faces (flux_riemann.cuf:753,930,1055 `goto 20` skips
+ *     al,ar,pql,pqr): the Riemann solvers return a zero flux;
  *   - WAF second-neighbour distance d_kvol_v1/v2 used uninitialised
  *     (flux_riemann.cuf:320-331): computed from centroids like :347,355;
  *   - tjv unset when no face of the neighbour matches (flux_riemann.cuf:262-277):
@@ -42,6 +42,7 @@
 /* single-precision literals of the reference, widened to double */
 #define LIT_1EM16 ((double)1e-16f) /* flux_riemann.cuf:749 */
 #define LIT_1EM8 ((double)1e-8f)   /* flux_riemann.cuf:926 */
+#define LIT_1EM6 ((double)1e-6f)   /* flux_riemann.cuf:1051 */
 #define LIT_1EM12 ((double)1e-12f) /* cfl_condition.cuf:129 */
 #define LIT_1P8 ((double)1.8f)     /* maillage.cuf:157 */
 
@@ -453,6 +454,61 @@ static void or_hllc(double gp, double n1, double n2, double hl, double ul, doubl
     flux[2] = n2 * fh[1] + n1 * fh[2];
 }
 
+/* solver_Riemann_riadh (flux_riemann.cuf:1029-1182), iflux 2: the same
+ * wave speeds as the HLLC solver with eps = 1e-6f, then the flux of the
+ * region holding x/t = 0 among U_L, U*_L, U*_R, U_R, the star fluxes being
+ * F_K + S_K (U*_K - U_K) with U*_K = h_K (S_K - u_K)/(S_K - S*) (1, S*, v_K)
+ * (the weight set to 0 when |S_K - S*| <= eps). */
+static void or_riadh(double gp, double n1, double n2, double hl, double ul, double vl,
+                     double hr, double ur, double vr, double flux[3])
+{
+    const double eps = LIT_1EM6;
+    double al, ar, het, uet, pql, pqr, sl, sr, set, denom, pond;
+    double q1[3], q2[3], f1[3], f2[3], e1[3], e2[3], fx[3];
+    if (hl < eps && hr < eps) {  /* both dry (:1055): zero flux, see header */
+        flux[0] = flux[1] = flux[2] = 0.0;
+        return;
+    }
+    al = sqrt(gp * hl);
+    ar = sqrt(gp * hr);
+    het = (hl + hr) / 2.0 - (((ur - ul) * (hl + hr)) / (al + ar)) / 4.0;
+    uet = (ul + ur) / 2.0 - ((hr - hl) * (al + ar)) / (hl + hr);
+    if (het < hl) pql = 1.0;
+    else pql = (hl > eps) ? sqrt(((0.5 * (het + hl)) * het) / (hl * hl)) : 0.0;
+    if (het < hr) pqr = 1.0;
+    else pqr = (hr > eps) ? sqrt(((0.5 * (het + hr)) * het) / (hr * hr)) : 0.0;
+    if (hl > eps) {
+        sl = ul - al * pql;
+        if (hr > eps) sr = ur + ar * pqr;
+        else { sl = ul - al; sr = ul + 2.0 * al; }
+    } else {
+        sl = ur - 2.0 * ar;
+        sr = ur + ar;
+    }
+    denom = hr * (ur - sr) - hl * (ul - sl);
+    if (fabs(denom) < eps) set = uet;
+    else set = ((sl * hr) * (ur - sr) - (sr * hl) * (ul - sl)) / denom;
+    q1[0] = hl; q1[1] = hl * ul; q1[2] = hl * vl;
+    q2[0] = hr; q2[1] = hr * ur; q2[2] = hr * vr;
+    pond = (fabs(sl - set) > eps) ? (hl * (sl - ul)) / (sl - set) : 0.0;
+    e1[0] = pond; e1[1] = pond * set; e1[2] = pond * vl;
+    pond = (fabs(sr - set) > eps) ? (hr * (sr - ur)) / (sr - set) : 0.0;
+    e2[0] = pond; e2[1] = pond * set; e2[2] = pond * vr;
+    f1[0] = hl * ul; f1[1] = hl * (ul * ul) + ((gp * hl) * hl) / 2.0; f1[2] = (hl * ul) * vl;
+    f2[0] = hr * ur; f2[1] = hr * (ur * ur) + ((gp * hr) * hr) / 2.0; f2[2] = (hr * ur) * vr;
+    for (int k = 0; k < 3; k++) {
+        e1[k] = f1[k] + sl * (e1[k] - q1[k]);
+        e2[k] = f2[k] + sr * (e2[k] - q2[k]);
+    }
+    if (0.0 < sl) { fx[0] = f1[0]; fx[1] = f1[1]; fx[2] = f1[2]; }
+    else if (0.0 < set && 0.0 > sl) { fx[0] = e1[0]; fx[1] = e1[1]; fx[2] = e1[2]; }
+    else if (0.0 > set && 0.0 < sr) { fx[0] = e2[0]; fx[1] = e2[1]; fx[2] = e2[2]; }
+    else { fx[0] = f2[0]; fx[1] = f2[1]; fx[2] = f2[2]; }
+    flux[0] = fx[0];
+    flux[1] = n1 * fx[1] - n2 * fx[2];
+    flux[2] = n2 * fx[1] + n1 * fx[2];
+}
+
 /* limiter (flux_riemann.cuf:465-524) */
 static double or_limiter(int ilim, double r, double c)
 {
@@ -576,12 +632,14 @@ static inline void or_face_geom(const double *vx, const double *vy, int gi, int 
 }
 
 /* compute_flux (flux_riemann.cuf:4-429) for one (cell, face): returns
- * flux*ln components; *qo receives the outflow edge discharge. */
+ * flux*ln components; *qo receives the outflow edge discharge
+ * (debit_sortie_arr, flux_riemann.cuf:161) and *qi the inflow edge
+ * discharge (debit_entree_arr, :125-126). */
 static void or_face(const or_params *p, const int *conn, const double *vx,
                     const double *vy, const double *xc, const double *yc,
                     const double *zm, const int *b1, const int *b2,
                     const int *upw, const double *U0, double dt, double tc,
-                    int gi, int tj, double fl[3], double *qo)
+                    int gi, int tj, double fl[3], double *qo, double *qi)
 {
     const double g = p->gp, tls = p->tolisec;
     double n1, n2, ln, flux[3] = {0.0, 0.0, 0.0};
@@ -592,6 +650,7 @@ static void or_face(const or_params *p, const int *conn, const double *vx,
     zml = zm[gi];
     or_rot(tls, n1, n2, U0 + 3 * gi, &hautl, &vitul, &vitvl);
     *qo = 0.0;
+    *qi = 0.0;
 
     if (kvoisin == -1) {
         if (p->inlet == 1) {
@@ -603,6 +662,7 @@ static void or_face(const or_params *p, const int *conn, const double *vx,
             flux[0] = dn;
             flux[1] = n1 * ((dn * dn) / (hautl + tls) + (g * (hautl * hautl)) / 2.0);
             flux[2] = n2 * ((dn * dn) / (hautl + tls) + (g * (hautl * hautl)) / 2.0);
+            *qi = (-dn) * ln;
         }
     }
     if (kvoisin == -3) {
@@ -637,6 +697,10 @@ static void or_face(const or_params *p, const int *conn, const double *vx,
             if (hijet < tls) { hijet = 0.0; vitul = 0.0; vitvl = 0.0; }
             if (hjiet < tls) { hjiet = 0.0; vitur = 0.0; vitvr = 0.0; }
             or_hllc(g, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr, flux);
+        } else if (p->iflux == 2) {  /* flux_riemann.cuf:213-228 */
+            if (hijet < tls) { hijet = 0.0; vitul = 0.0; vitvl = 0.0; }
+            if (hjiet < tls) { hjiet = 0.0; vitur = 0.0; vitvr = 0.0; }
+            or_riadh(g, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr, flux);
         } else if (p->iflux == 3) {
             int kv1 = upw[6 * gi + 2 * tj] - 1, kv2 = upw[6 * gi + 2 * tj + 1] - 1;
             int A = conn[3 * gi + tj], B = conn[3 * gi + (tj + 1) % 3];
@@ -715,21 +779,23 @@ static void or_face(const or_params *p, const int *conn, const double *vx,
 }
 
 /* compute_flux for all cells: resc(k,i) = (F1 + F2) + F3 (flux_riemann.cuf:411-426).
- * qout_cell[i] (optional) receives the cell's outflow discharge. */
+ * qout_cell[i] / qin_cell[i] (optional) receive the cell's outflow / inflow
+ * discharge, summed over its faces in face order. */
 void or_compute_flux(const or_params *p, int nelt, const int *conn,
                      const double *vx, const double *vy, const double *xc,
                      const double *yc, const double *zm, const int *b1,
                      const int *b2, const int *upw, const double *U0,
-                     double dt, double tc, double *resc, double *qout_cell)
+                     double dt, double tc, double *resc, double *qout_cell, double *qin_cell)
 {
     int gi;
 #pragma omp parallel for schedule(static)
     for (gi = 0; gi < nelt; gi++) {
-        double f[3][3], q[3];
+        double f[3][3], q[3], qi[3];
         for (int tj = 0; tj < 3; tj++)
-            or_face(p, conn, vx, vy, xc, yc, zm, b1, b2, upw, U0, dt, tc, gi, tj, f[tj], &q[tj]);
+            or_face(p, conn, vx, vy, xc, yc, zm, b1, b2, upw, U0, dt, tc, gi, tj, f[tj], &q[tj], &qi[tj]);
         for (int k = 0; k < 3; k++) resc[3 * gi + k] = (f[0][k] + f[1][k]) + f[2][k];
         if (qout_cell) qout_cell[gi] = (q[0] + q[1]) + q[2];
+        if (qin_cell) qin_cell[gi] = (qi[0] + qi[1]) + qi[2];
     }
 }
 
@@ -861,7 +927,7 @@ int or_run(const or_params *p, int nelt, const int *conn, const double *vx,
            const double *surf, const double *zm, const double *cotemin_arr,
            const double *manning, const int *b1, const int *b2, const int *upw,
            double *U, int nsteps, double *dt_hist, double *tc_out,
-           int *steps_done, double *qout_hist, int nthreads)
+           int *steps_done, double *qout_hist, double *qin_hist, int *steady, int nthreads)
 {
     size_t n3 = (size_t)nelt * 3;
     double *U1 = (double *)malloc(n3 * sizeof(double));
@@ -870,8 +936,9 @@ int or_run(const or_params *p, int nelt, const int *conn, const double *vx,
     double *sf = (double *)malloc(n3 * sizeof(double));
     double *afm1 = (double *)malloc((size_t)nelt * 9 * sizeof(double));
     double *qc = (double *)malloc((size_t)nelt * sizeof(double));
+    double *qic = (double *)malloc((size_t)nelt * sizeof(double));
     double tc = 0.0, dt, dtn;
-    int n = 0, status = 0;
+    int n = 0, status = 0, reg = 0, reached = 0;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #else
@@ -880,21 +947,31 @@ int or_run(const or_params *p, int nelt, const int *conn, const double *vx,
     for (size_t i = 0; i < (size_t)nelt * 9; i++) afm1[i] = (i % 9) % 4 == 0 ? 1.0 : 0.0;
     /* the initial dt is not range-checked (finite_volumes_method.cuf:113-116) */
     dt = or_cfl_dt(p, nelt, cotemin_arr, U);
-    while (status == 0 && n < nsteps && ((tc - p->ts) + 2.0 * p->tol) < p->tol) {
-        or_compute_flux(p, nelt, conn, vx, vy, xc, yc, zm, b1, b2, upw, U, dt, tc, resc, qc);
+    while (status == 0 && !reached && n < nsteps && ((tc - p->ts) + 2.0 * p->tol) < p->tol) {
+        double qo = 0.0, qi = 0.0;
+        or_compute_flux(p, nelt, conn, vx, vy, xc, yc, zm, b1, b2, upw, U, dt, tc, resc, qc, qic);
         or_source_terms(p, nelt, vx, vy, surf, zm, manning, b1, U, dt, sb, sf, afm1);
         or_update(p, nelt, surf, U, resc, sb, sf, afm1, dt, U1);
-        if (qout_hist) {
-            double q = 0.0;
-            for (int i = 0; i < nelt; i++) q = q + qc[i];
-            qout_hist[n] = q;
+        for (int i = 0; i < nelt; i++) {
+            qo = qo + qc[i];
+            qi = qi + qic[i];
         }
+        if (qout_hist) qout_hist[n] = qo;
+        if (qin_hist) qin_hist[n] = qi;
         dtn = or_cfl_dt(p, nelt, cotemin_arr, U1);
         memcpy(U, U1, n3 * sizeof(double));
         dt_hist[n] = dt;
         tc = tc + dt;
         n++;
         dt = dtn;
+        /* check_steady_state (finite_volumes_method.cuf:370-393) on this
+           step's discharges: more than 1000 consecutive steps with
+           |Qin - Qout| / Qin < tol_reg_perm end the run after the step */
+        if (qi > LIT_1EM8 && fabs(qi - qo) / qi < p->tol_reg_perm) {
+            if (++reg > 1000) reached = 1;
+        } else {
+            reg = 0;
+        }
         /* dt range check of cfl_condition.cuf:129-132 applied to the new dt;
            NaN states are an error too (the reference does not catch them) */
         if (!(dt <= 9999.0 && dt >= LIT_1EM12)) status = -1;
@@ -904,7 +981,8 @@ int or_run(const or_params *p, int nelt, const int *conn, const double *vx,
     dt_hist[n] = dt;
     if (tc_out) *tc_out = tc;
     if (steps_done) *steps_done = n;
-    free(U1); free(resc); free(sb); free(sf); free(afm1); free(qc);
+    if (steady) *steady = reached;
+    free(U1); free(resc); free(sb); free(sf); free(afm1); free(qc); free(qic);
     return status;
 }
 
@@ -919,6 +997,8 @@ void or_riemann_face(int iflux, int ilimiteur, double gp, double n1, double n2,
 {
     if (iflux == 1) {
         or_hllc(gp, n1, n2, hl, ul, vl, hr, ur, vr, flux);
+    } else if (iflux == 2) {
+        or_riadh(gp, n1, n2, hl, ul, vl, hr, ur, vr, flux);
     } else {
         or_params p;
         memset(&p, 0, sizeof(p));

@@ -84,7 +84,7 @@ void or_compute_flux(const or_params *p, int nelt, const int *conn,
                      const double *vx, const double *vy, const double *xc,
                      const double *yc, const double *zm, const int *b1,
                      const int *b2, const int *upw, const double *U0,
-                     double dt, double tc, double *resc, double *qout_cell);
+                     double dt, double tc, double *resc, double *qout_cell, double *qin_cell);
 void or_source_terms(const or_params *p, int nelt, const double *vx,
                      const double *vy, const double *surf, const double *zm,
                      const double *manning, const int *b1, const double *U0,
@@ -99,7 +99,10 @@ double or_cfl_dt(const or_params *p, int nelt, const double *cotemin_arr,
  * Runs at most nsteps steps (or until tc >= ts - tol, an error, or steady
  * state).  U (3*nelt) is updated in place.  dt_hist[k] receives the dt used
  * by step k (k < steps done), dt_hist[steps_done] the dt for the next step.
- * qout_hist (optional) receives the outflow discharge of each step.
+ * qout_hist / qin_hist (optional) receive the outflow / inflow discharge of
+ * each step's flux, summed over cells in cell order.  *steady (optional) is
+ * set to 1 when the run stopped at steady state (check_steady_state,
+ * finite_volumes_method.cuf:370-393).
  * Returns 0 on success, -1 on a dt sanity failure (dt>9999, dt<1e-12 or NaN).
  */
 int or_run(const or_params *p, int nelt, const int *conn, const double *vx,
@@ -107,7 +110,7 @@ int or_run(const or_params *p, int nelt, const int *conn, const double *vx,
            const double *surf, const double *zm, const double *cotemin_arr,
            const double *manning, const int *b1, const int *b2, const int *upw,
            double *U, int nsteps, double *dt_hist, double *tc_out,
-           int *steps_done, double *qout_hist, int nthreads);
+           int *steps_done, double *qout_hist, double *qin_hist, int *steady, int nthreads);
 
 /* face solvers (flux in the global frame) and limiter, for the pin tests */
 void or_riemann_face(int iflux, int ilimiteur, double gp, double n1, double n2,

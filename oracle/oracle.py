@@ -152,22 +152,28 @@ class Prepared:
 
 def run(prep: Prepared, params, U0: np.ndarray, nsteps: int, ts: float | None = None,
         nthreads: int = 0, qout: bool = False):
-    """The reference loop on the CPU: returns dict(U, dt_hist, steps, tc, status[, qout])."""
+    """The reference loop on the CPU: returns dict(U, dt_hist, steps, tc,
+    status, steady[, qout, qin]) -- qout/qin the outflow/inflow discharge of
+    each step's flux; steady whether the run stopped at steady state."""
     lib = load()
     p = or_params(params, prep.long_entree, ts=ts)
     U = np.ascontiguousarray(U0, np.float64).copy()
     dt_hist = np.zeros(nsteps + 1)
     qo = np.zeros(max(nsteps, 1)) if qout else None
+    qi = np.zeros(max(nsteps, 1)) if qout else None
     tc = C.c_double()
     done = C.c_int()
+    steady = C.c_int()
     st = lib.or_run(C.byref(p), prep.nelt, _i(prep.conn), _d(prep.vx), _d(prep.vy), _d(prep.xc),
                     _d(prep.yc), _d(prep.surf), _d(prep.zm), _d(prep.cot), _d(prep.manning),
                     _i(prep.b1), _i(prep.b2), _i(prep.upw), _d(U), int(nsteps), _d(dt_hist),
-                    C.byref(tc), C.byref(done), _d(qo) if qout else None, int(nthreads))
+                    C.byref(tc), C.byref(done), _d(qo) if qout else None, _d(qi) if qout else None,
+                    C.byref(steady), int(nthreads))
     out = {"U": U.reshape(-1, 3), "dt_hist": dt_hist[: done.value + 1], "steps": done.value,
-           "tc": tc.value, "status": st}
+           "tc": tc.value, "status": st, "steady": bool(steady.value)}
     if qout:
         out["qout"] = qo[: done.value]
+        out["qin"] = qi[: done.value]
     return out
 
 

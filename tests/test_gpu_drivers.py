@@ -48,7 +48,7 @@ def _oracle(mesh, case_dir):
     p = read_donnees(os.path.join(case_dir, "donnees.f"))
     prep = O.Prepared(mesh)
     U0 = initial_state(p, mesh, prep.zm)
-    return p, prep, O.run(prep, p, U0, 100000)
+    return p, prep, O.run(prep, p, U0, 100000, qout=True)
 
 
 def test_python_driver_matches_oracle(tmp_path):
@@ -58,6 +58,9 @@ def test_python_driver_matches_oracle(tmp_path):
     assert out["status"] == 0 and ref["status"] == 0
     assert out["steps"] == ref["steps"] and out["tc"] == ref["tc"]
     assert np.array_equal(out["U"], ref["U"])
+    # per-step discharges of the status block
+    assert np.array_equal(out["q_in"], ref["qin"]) and np.array_equal(out["q_out"], ref["qout"])
+    assert not out["steady"]
     # boundary table file written like the reference
     bnd = M.read_boundary_table(str(tmp_path / "boundary_table_reach.txt"))
     assert np.array_equal(bnd, prep.boundary_fortran())
@@ -87,3 +90,10 @@ def test_fortran_driver_matches_python(tmp_path):
     assert fort_restart[1:] == [cio._fmt(row) for row in ref["U"]]
     assert M.read_boundary_table(str(tmp_path / "boundary_table_reach.txt")).shape == (2, 3, mesh.nelt)
     assert fort_bt.splitlines()[0].split() == [str(v) for v in prep.boundary_fortran()[:, :, 0].reshape(-1)]
+    # the status block prints each freqaffich-th step's discharges (list-directed: ~16 digits)
+    printed = [float(line.split("=")[1].split()[0]) for line in proc.stdout.splitlines() if "debit_sortie" in line]
+    iters = [int(line.split("Iteration number")[1].split(",")[0]) for line in proc.stdout.splitlines()
+             if "Iteration number" in line]
+    assert len(printed) == len(iters) > 0
+    for it, q in zip(iters, printed):
+        assert abs(q - ref["qout"][it - 1]) <= 1e-15 * max(abs(ref["qout"][it - 1]), 1e-300), (it, q)

@@ -139,6 +139,120 @@ def test_waf_limiters(ilim):
     _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
 
 
+@pytest.mark.parametrize("fricimplic", [0, 1, 2])
+def test_riadh_river(fricimplic):
+    """iflux 2 (solver_Riemann_riadh) on the wetting reach, every friction
+    mode (explicit friction on the dam break: it is unstable on the reach's
+    shallow front)."""
+    if fricimplic == 0:
+        m, p, U0 = dam_case(24)
+        p.iflux, p.friction, p.fricimplic = 2, 1, 0
+        m.manning[:] = 0.03
+    else:
+        m, p, U0 = river_case(iflux=2, fricimplic=fricimplic)
+    ref, U, hist, done, st, _, _ = _run_both(m, p, U0, 300)
+    assert st == 0 and ref["status"] == 0 and done == 300
+    _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
+
+
+def test_riadh_reference_canal_runs():
+    """The divergent canal's two reference GPU runs (iflux 2): the GPU gives
+    the oracle's bits over the whole logged length (16000 and 15000 steps),
+    and the oracle reproduces the reference's logged outflow discharges."""
+    import json
+    import os
+
+    from cuteflow_amd import mesh as Mm
+    from cuteflow_amd.params import read_donnees
+    from cuteflow_amd.solver import initial_state
+
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    m = Mm.read_text_mesh(os.path.join(golden, "divergent_canal.mesh"))
+    exp = json.load(open(os.path.join(golden, "divergent_canal.expected.json")))
+    prep = O.Prepared(m)
+    for log_name, n in (("outfile_manning_min.out", 0.01), ("outfile_manning_max.out", 0.05)):
+        log = exp["logs"][log_name]
+        p = read_donnees(os.path.join(golden, "divergent_canal.donnees.f"))
+        p.debitglob[0] = float(log["debitglob_1"])
+        p.h_sortie[0] = float(log["h_sortie_1"])
+        p.override_manning = n
+        p.ts = 1e300
+        U0 = initial_state(p, m, prep.zm)
+        nsteps = log["series"][-1][0]
+        print(f"{log_name}: {nsteps} steps", flush=True)
+        ref = O.run(prep, p, U0, nsteps)
+        with Solver(m, p) as s:
+            s.set_discharge(True)
+            s.set_state(U0)
+            hist, done, st = s.step(nsteps)
+            U = s.get_state()
+            q_in, q_out, steady = s.discharge()
+        assert st == 0 and done == nsteps and not steady
+        _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
+        # the reference's printed outflow discharges, from the GPU run
+        for it, _, qi, _, qo in log["series"]:
+            assert abs(q_out[it - 1] - qo) <= 4 * np.spacing(qo), (log_name, it, q_out[it - 1], qo)
+            assert abs(q_in[it - 1] - qi) <= 4 * np.spacing(qi), (log_name, it, q_in[it - 1], qi)
+
+
+@pytest.mark.parametrize("options", [0, 2, 8 | 16])
+def test_discharge_and_steady_state(options):
+    """Per-step inflow/outflow discharges (summed in the caller's cell
+    order) equal the oracle's bit for bit, over chunked runs; with
+    tol_reg_perm the run stops at the oracle's step with the same tc, and
+    stays stopped until a new state is set."""
+    import json
+    import os
+
+    from cuteflow_amd import mesh as Mm
+    from cuteflow_amd.params import read_donnees
+    from cuteflow_amd.solver import initial_state
+
+    m, p, U0 = river_case(nx=60, ny=12, q_peak=400.0, t_ramp=20.0)
+    ref = O.run(O.Prepared(m), p, U0, 500, qout=True)
+    with Solver(m, p, options=options) as s:
+        s.set_discharge(True)
+        s.set_state(U0)
+        parts = []
+        for k in (120, 380):
+            s.step(k)
+            parts.append(s.discharge())
+        q_in = np.concatenate([x[0] for x in parts])
+        q_out = np.concatenate([x[1] for x in parts])
+    assert np.array_equal(q_in.view(np.int64), ref["qin"].view(np.int64))
+    assert np.array_equal(q_out.view(np.int64), ref["qout"].view(np.int64))
+    assert np.any(q_out > 0) and q_in[0] == 0.0 and np.all(q_in[1:] > 0)  # the hydrograph starts at 0
+
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    mc = Mm.read_text_mesh(os.path.join(golden, "divergent_canal.mesh"))
+    prep = O.Prepared(mc)
+    pc = read_donnees(os.path.join(golden, "divergent_canal.donnees.f"))
+    pc.debitglob[0], pc.h_sortie[0], pc.override_manning, pc.ts = 0.5, 0.02, 0.05, 1e300
+    pc.tol_reg_perm = 1e-9
+    Uc = initial_state(pc, mc, prep.zm)
+    refc = O.run(prep, pc, Uc, 9000, qout=True)
+    assert refc["steady"] and refc["steps"] < 9000
+    with Solver(mc, pc, options=options) as s:
+        s.set_state(Uc)
+        hist, done, st = s.step(9000)
+        q_in, q_out, steady = s.discharge()
+        assert st == 0 and steady and done == refc["steps"] and s.tc == refc["tc"]
+        _compare(s.get_state(), refc["U"], hist, refc["dt_hist"], pc.tolisec)
+        assert np.array_equal(q_out, refc["qout"]) and np.array_equal(q_in, refc["qin"])
+        assert s.step(10)[1] == 0           # stopped until a new state
+        s.set_state(Uc)
+        assert s.step(10)[1] == 10
+
+
+@pytest.mark.parametrize("n", [20, 60])
+def test_dam_break_riadh(n):
+    m, p, U0 = dam_case(n)
+    p.iflux = 2
+    ref, U, hist, done, st, _, tc = _run_both(m, p, U0, 400)
+    assert st == 0 and done == 400
+    _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
+
+
 def test_river_hllc_transmissive_outlet():
     m, p, U0 = river_case(iflux=1, outlet="transm")
     ref, U, hist, done, st, _, _ = _run_both(m, p, U0, 300)
@@ -178,8 +292,9 @@ def test_reordering_invariance():
         _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
 
 
-def test_lake_at_rest():
-    m, p = lake_case()
+@pytest.mark.parametrize("iflux", [1, 2, 3])
+def test_lake_at_rest(iflux):
+    m, p = lake_case(iflux=iflux)
     prep = O.Prepared(m)
     U0 = np.zeros((m.nelt, 3))
     U0[:, 0] = np.where(-prep.zm > p.tolisec, -prep.zm, 0.0)

@@ -133,6 +133,112 @@ def test_hllc_consistency_and_symmetry():
     assert abs(f[0]) < 1e-12 and f[1] > g / 2
 
 
+def test_riadh_pins():
+    """iflux 2 (flux_riemann.cuf:1029-1182) is Toro's state-based HLLC: its
+    S* makes h*_L = h*_R, so its mass flux is the HLL mass flux -- which the
+    iflux 1 solver also returns (up to its eps terms) -- and the tangential
+    flux is the mass flux times the upwind v (the side of S*).  An isolated
+    contact (same h and u, jump in v) is resolved exactly.  Plus
+    consistency, the supersonic branches, mirror symmetry and dry/dry."""
+    g = 9.81
+    rng = np.random.default_rng(4)
+    for _ in range(300):
+        L = (rng.uniform(0.1, 3), rng.uniform(-2, 2), rng.uniform(-1, 1))
+        R = (rng.uniform(0.1, 3), rng.uniform(-2, 2), rng.uniform(-1, 1))
+        fr = O.riemann(2, L, R, n=(1.0, 0.0))
+        fh = O.riemann(1, L, R, n=(1.0, 0.0))
+        assert math.isclose(fr[0], fh[0], rel_tol=1e-7, abs_tol=1e-9), (L, R, fr, fh)
+        # tangential flux = mass flux x upwind v (either side's v)
+        assert min(abs(fr[2] - fr[0] * L[2]), abs(fr[2] - fr[0] * R[2])) <= 1e-12 * (1 + abs(fr[2])), (L, R, fr)
+    for (h, u, vl, vr) in [(1.3, 0.4, 0.2, -0.7), (0.6, -0.9, 0.5, 0.1)]:
+        f = O.riemann(2, (h, u, vl), (h, u, vr))
+        v = vl if u > 0 else vr
+        assert np.allclose(f, [h * u, h * u * u + g * h * h / 2, h * u * v], rtol=1e-12, atol=1e-14)
+    for (h, u, v) in [(1.0, 0.3, -0.2), (2.5, -1.0, 0.7), (0.01, 0.0, 0.0)]:
+        f = O.riemann(2, (h, u, v), (h, u, v), n=(1.0, 0.0))
+        phys = np.array([h * u, h * u * u + g * h * h / 2, h * u * v])
+        assert np.allclose(f, phys, rtol=1e-12, atol=1e-14)
+    # at rest: exactly the hydrostatic pressure (well-balancing needs the bits)
+    assert np.array_equal(O.riemann(2, (0.7, 0.0, 0.0), (0.7, 0.0, 0.0)), [0.0, (g * 0.7 * 0.7) / 2, 0.0])
+    f = O.riemann(2, (1.0, 5.0, 0.1), (0.8, 4.0, 0.0))
+    assert np.array_equal(f, [1.0 * 5.0, 1.0 * 25.0 + (g * 1.0 * 1.0) / 2, (1.0 * 5.0) * 0.1])
+    f = O.riemann(2, (0.8, -4.0, 0.2), (1.0, -5.0, 0.1))
+    assert np.array_equal(f, [1.0 * -5.0, 1.0 * 25.0 + (g * 1.0 * 1.0) / 2, (1.0 * -5.0) * 0.1])
+    L, R = (1.2, 0.4, 0.1), (0.7, -0.3, 0.5)
+    f1 = O.riemann(2, L, R, n=(0.6, 0.8))
+    f2 = O.riemann(2, (R[0], -R[1], -R[2]), (L[0], -L[1], -L[2]), n=(-0.6, -0.8))
+    assert np.allclose(f1, -f2, rtol=1e-12, atol=1e-14)
+    assert np.all(O.riemann(2, (0.0, 0, 0), (0.0, 0, 0), n=(0.6, 0.8)) == 0)
+    assert np.all(O.riemann(2, (5e-7, 0, 0), (9e-7, 0, 0), n=(0.6, 0.8)) == 0)  # both below eps = 1e-6f
+    # a dry side: the wet side drains into the dry bed
+    f = O.riemann(2, (1.0, 0.0, 0.0), (0.0, 0.0, 0.0))
+    assert f[0] > 0.0 and np.isfinite(f).all()
+
+
+def test_reference_gpu_discharge_series_iflux2():
+    """The divergent canal's two reference GPU runs (iflux 2, fricimplic 1,
+    inflow 0.5 m3/s, outlet level 0.02 m, plane IC at 0.016 m, override n
+    0.01 -- the namelist's own value -- and 0.05): the outflow discharge the
+    reference printed every 1000 iterations (up to 16000) is the oracle's
+    discharge of the flux of that iteration to a few ulps (the reference
+    sums its per-edge array in its own order and prints 16 digits).  The
+    printed t is not pinned bit for bit: the reference adds a dt copied back
+    by an unsynchronised cudaMemcpyAsync (cfl_condition.cuf:125,
+    finite_volumes_method.cuf:232), so each iteration adds the new or the
+    old dt; the printed t lies between those two sums at every point."""
+    from cuteflow_amd.params import read_donnees
+    from cuteflow_amd.solver import initial_state
+
+    m, _, exp = _fixture("divergent_canal")
+    prep = O.Prepared(m)
+    for log_name, n in (("outfile_manning_min.out", 0.01), ("outfile_manning_max.out", 0.05)):
+        log = exp["logs"][log_name]
+        p = read_donnees(os.path.join(GOLDEN, "divergent_canal.donnees.f"))
+        assert p.iflux == 2 and p.fricimplic == 1 and p.plan == 1 and p.is_override_manning == 1
+        p.debitglob[0] = float(log["debitglob_1"])
+        p.h_sortie[0] = float(log["h_sortie_1"])
+        p.override_manning = n
+        assert p.tolisec == float(log["tolisec"]) and int(log["iflux"]) == 2
+        U0 = initial_state(p, m, prep.zm)
+        ser = log["series"]
+        r = O.run(prep, p, U0, ser[-1][0] + 1, qout=True)
+        assert r["status"] == 0
+        c = np.concatenate([[0.0], np.cumsum(r["dt_hist"])])
+        for it, t, q_in, q_in_sum, q_out in ser:
+            assert abs(r["qout"][it - 1] - q_out) <= 4 * math.ulp(q_out), (log_name, it, r["qout"][it - 1], q_out)
+            assert c[it + 1] - c[1] <= t <= c[it], (log_name, it, t)
+            assert q_in == q_in_sum == p.debitglob[0]
+
+
+def test_steady_state_stop():
+    """check_steady_state (finite_volumes_method.cuf:370-393): the run ends
+    after the step that makes more than 1000 consecutive steps with
+    Qin > 1e-8f and |Qin - Qout| / Qin < tol_reg_perm.  On the canal's
+    second reference run (outflow converging to the 0.5 m3/s inflow) the
+    stop falls on the first step that completes 1001 qualifying steps of
+    the discharge history of a run without the check."""
+    from cuteflow_amd.params import read_donnees
+    from cuteflow_amd.solver import initial_state
+
+    m, _, _ = _fixture("divergent_canal")
+    prep = O.Prepared(m)
+    p = read_donnees(os.path.join(GOLDEN, "divergent_canal.donnees.f"))
+    p.debitglob[0], p.h_sortie[0], p.override_manning, p.ts = 0.5, 0.02, 0.05, 1e300
+    U0 = initial_state(p, m, prep.zm)
+    free = O.run(prep, p, U0, 9000, qout=True)
+    assert not free["steady"] and free["steps"] == 9000
+    p.tol_reg_perm = 1e-9
+    r = O.run(prep, p, U0, 9000, qout=True)
+    ok = (free["qin"] > f32(1e-8)) & (np.abs(free["qin"] - free["qout"]) / free["qin"] < 1e-9)
+    run = 0
+    for k, v in enumerate(ok):
+        run = run + 1 if v else 0
+        if run > 1000:
+            break
+    assert r["steady"] and r["steps"] == k + 1 < 9000
+    assert np.array_equal(r["qout"], free["qout"][: k + 1]) and r["tc"] == np.cumsum(free["dt_hist"][: k + 1])[-1]
+
+
 def test_waf_reduces_to_hllc():
     """ilimiteur = 0 (A = 1) makes WAF the HLLC flux (flux_riemann.cuf:475-476)."""
     rng = np.random.default_rng(2)
@@ -183,7 +289,7 @@ def test_mass_conservation_walls():
     assert r["status"] == 0 and abs(v1 - v0) <= 1e-12 * v0
 
 
-@pytest.mark.parametrize("iflux", [1, 3])
+@pytest.mark.parametrize("iflux", [1, 2, 3])
 def test_lake_at_rest(iflux):
     m, p = lake_case(iflux=iflux)
     prep = O.Prepared(m)
