@@ -173,17 +173,31 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
 
 // Reduces per-tile minima (NaN marks a NaN state, +inf a skipped tile)
 // starting from `pad`; every thread gets the result, the NaN flag and the
-// number of skipped tiles.
+// number of skipped tiles.  Each thread issues its loads eight at a time
+// before using them: one workgroup reads up to ~16k partials, and a loop
+// that waits for each load in turn is bound by memory latency.
 __device__ double block_min_1024(const double *__restrict__ part, int n, double pad, int &nan, int &nskip) {
     __shared__ double s[1024 / 64];
     __shared__ int sk[1024 / 64];
+    constexpr int kU = 8;
+    const int stride = (int)blockDim.x;
     double m = pad;
     int bad = 0, skipped = 0;
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        const double v = part[k];
-        if (v != v) bad = 1;
-        else m = dmin(m, v);
-        skipped += (v == __builtin_inf());
+    for (int base = threadIdx.x; base < n; base += kU * stride) {
+        double v[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const int k = base + u * stride;
+            v[u] = k < n ? part[k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            if (base + u * stride < n) {
+                if (v[u] != v[u]) bad = 1;
+                else m = dmin(m, v[u]);
+                skipped += (v[u] == __builtin_inf());
+            }
+        }
     }
     m = wave_min(m);
 #pragma unroll
