@@ -138,7 +138,7 @@ struct RevArgs {
 // than at 2 waves on MI355X; the WAF reverse needs 256 (1 wave).
 template <int IFLUX>
 struct RevWaves {
-    static constexpr int value = IFLUX == 1 ? 3 : 1;
+    static constexpr int value = IFLUX == 3 ? 1 : 3;
 };
 
 __device__ __forceinline__ void acc4(double4 &a, const double4 &b) {
@@ -705,7 +705,6 @@ static void enqueue_forward(cf_handle *h, AdjState *A, hipStream_t s) {
 
 extern "C" int cf_adj_forward(cf_handle *h, int nsteps, int sample_every, int checkpoint_every, int want_hmax) {
     if (!h || nsteps <= 0 || sample_every <= 0 || checkpoint_every <= 0 || h->pending >= 0) return CF_ERR_ARG;
-    if (h->iflux == 2) return CF_ERR_ARG;  // no reverse of the iflux 2 solver
     for (int b = 0; b < h->ninst; b++)
         if (h->hctrl[b].status != 0 || h->prm[b].is_override_manning) return CF_ERR_ARG;
     AdjState *A = adj_of(h);
@@ -860,9 +859,10 @@ static void launch_rev_t(const RevArgs &r, int grid, hipStream_t s) {
 
 static void launch_rev(const RevArgs &r, int iflux, int fric, int grid, hipStream_t s) {
     typedef void (*fn)(const RevArgs &, int, hipStream_t);
-    static const fn tab[2][4] = {{launch_rev_t<1, 0>, launch_rev_t<1, 1>, launch_rev_t<1, 2>, launch_rev_t<1, 3>},
+    static const fn tab[3][4] = {{launch_rev_t<1, 0>, launch_rev_t<1, 1>, launch_rev_t<1, 2>, launch_rev_t<1, 3>},
+                                 {launch_rev_t<2, 0>, launch_rev_t<2, 1>, launch_rev_t<2, 2>, launch_rev_t<2, 3>},
                                  {launch_rev_t<3, 0>, launch_rev_t<3, 1>, launch_rev_t<3, 2>, launch_rev_t<3, 3>}};
-    tab[iflux == 3 ? 1 : 0][fric](r, grid, s);
+    tab[iflux - 1][fric](r, grid, s);
 }
 
 // Reverse sweep: segments from the last, each replayed from its checkpoint
@@ -1011,9 +1011,10 @@ static void launch_tlm_t(const TlmArgs &t, int grid, hipStream_t s) {
 static void enqueue_tlm(cf_handle *h, AdjState *A, hipStream_t s) {
     const size_t N = (size_t)h->ncell;
     typedef void (*fn)(const TlmArgs &, int, hipStream_t);
-    static const fn tab[2][4] = {{launch_tlm_t<1, 0>, launch_tlm_t<1, 1>, launch_tlm_t<1, 2>, launch_tlm_t<1, 3>},
+    static const fn tab[3][4] = {{launch_tlm_t<1, 0>, launch_tlm_t<1, 1>, launch_tlm_t<1, 2>, launch_tlm_t<1, 3>},
+                                 {launch_tlm_t<2, 0>, launch_tlm_t<2, 1>, launch_tlm_t<2, 2>, launch_tlm_t<2, 3>},
                                  {launch_tlm_t<3, 0>, launch_tlm_t<3, 1>, launch_tlm_t<3, 2>, launch_tlm_t<3, 3>}};
-    const fn L = tab[h->iflux == 3 ? 1 : 0][h->fric];
+    const fn L = tab[h->iflux - 1][h->fric];
     // start: the recorded initial state and the direction's initial tangent
     hipLaunchKernelGGL(copy_d4_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, A->ckpt, A->tst[0], N);
     hipLaunchKernelGGL(copy_d4_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, A->tdir, A->tdst[0], N);

@@ -639,16 +639,17 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
             const double hijet = dmax(0.0, rawl);
             double hautr, vitur, vitvr;
             rot(tls, n1, n2, ur4.h, ur4.hu, ur4.hv, hautr, vitur, vitvr);
-            const double hjiet = dmax(0.0, (IFLUX == 1 ? (ur4.h + zmr) - zij : (hautr + zmr) - zij));
+            const double hjiet = dmax(0.0, (IFLUX <= 2 ? (ur4.h + zmr) - zij : (hautr + zmr) - zij));
             const bool zl = hijet < tls, zr = hjiet < tls;
             const double fhl = zl ? 0.0 : hijet, ful = zl ? 0.0 : vitul, fvl = zl ? 0.0 : vitvl;
             const double fhr = zr ? 0.0 : hjiet, fur = zr ? 0.0 : vitur, fvr = zr ? 0.0 : vitvr;
             const bool actl = !zl && !(0.0 > rawl);  // dmax(0, raw) returns raw on a tie
-            const double rawr2 = (IFLUX == 1) ? (ur4.h + zmr) - zij : (hautr + zmr) - zij;
+            const double rawr2 = (IFLUX <= 2) ? (ur4.h + zmr) - zij : (hautr + zmr) - zij;
             const bool actr = !zr && !(0.0 > rawr2);
-            if (IFLUX == 1) {
+            if (IFLUX == 1 || IFLUX == 2) {
                 double gg[6] = {0, 0, 0, 0, 0, 0};
-                hllc_flux_rev(g, n1, n2, fhl, ful, fvl, fhr, fur, fvr, bf, gg);
+                if (IFLUX == 1) hllc_flux_rev(g, n1, n2, fhl, ful, fvl, fhr, fur, fvr, bf, gg);
+                else riadh_flux_rev(g, n1, n2, fhl, ful, fvl, fhr, fur, fvr, bf, gg);
                 if (!zl) {
                     if (actl) bhl += gg[0];
                     bul += gg[1];

@@ -552,6 +552,83 @@ CF_HD inline void unrotate_rev(double n1, double n2, const double bflux[3], doub
     b[2] = -n2 * bflux[1] + n1 * bflux[2];
 }
 
+// Reverse of riadh_flux: g[0..5] += cotangents of (hl, ul, vl, hr, ur, vr).
+// The region's flux is reversed here; the wave speeds S_L, S_R, S* (the
+// same estimates as hllc_core with eps = 1e-6f) through hllc_core_rev.
+CF_HD inline void riadh_flux_rev(double gp, double n1, double n2, double hl, double ul, double vl, double hr,
+                                 double ur, double vr, const double bflux[3], double g[6]) {
+    const double eps = kEpsRiadh;
+    if (hl < eps && hr < eps) return;
+    const double al = sqrt(gp * hl), ar = sqrt(gp * hr);
+    const double het = (hl + hr) / 2.0 - (((ur - ul) * (hl + hr)) / (al + ar)) / 4.0;
+    const double uet = (ul + ur) / 2.0 - ((hr - hl) * (al + ar)) / (hl + hr);
+    double pql, pqr, sl, sr, set;
+    if (het < hl) pql = 1.0;
+    else pql = (hl > eps) ? sqrt(((0.5 * (het + hl)) * het) / (hl * hl)) : 0.0;
+    if (het < hr) pqr = 1.0;
+    else pqr = (hr > eps) ? sqrt(((0.5 * (het + hr)) * het) / (hr * hr)) : 0.0;
+    if (hl > eps) {
+        sl = ul - al * pql;
+        if (hr > eps) {
+            sr = ur + ar * pqr;
+        } else {
+            sl = ul - al;
+            sr = ul + 2.0 * al;
+        }
+    } else {
+        sl = ur - 2.0 * ar;
+        sr = ur + ar;
+    }
+    const double denom = hr * (ur - sr) - hl * (ul - sl);
+    set = (fabs(denom) < eps) ? uet : ((sl * hr) * (ur - sr) - (sr * hl) * (ul - sl)) / denom;
+    double b[3];
+    unrotate_rev(n1, n2, bflux, b);
+    double bh = 0.0, bu = 0.0, bv = 0.0, bsk = 0.0, bset = 0.0;  // side K of the selected region
+    int side;  // 0 left, 1 right
+    bool star;
+    if (0.0 < sl) {
+        side = 0;
+        star = false;
+    } else if (0.0 < set && 0.0 > sl) {
+        side = 0;
+        star = true;
+    } else if (0.0 > set && 0.0 < sr) {
+        side = 1;
+        star = true;
+    } else {
+        side = 1;
+        star = false;
+    }
+    const double h = side ? hr : hl, u = side ? ur : ul, v = side ? vr : vl, sk = side ? sr : sl;
+    // physical flux F(U_K) = (h u, h u^2 + g h^2 / 2, h u v)
+    bh += b[0] * u + b[1] * (u * u + gp * h) + b[2] * (u * v);
+    bu += b[0] * h + b[1] * (2.0 * h * u) + b[2] * (h * v);
+    bv += b[2] * (h * u);
+    if (star) {
+        // + S_K (P (1, S*, v) - (h, h u, h v)), P = h (S_K - u) / (S_K - S*)
+        const bool pa = fabs(sk - set) > eps;
+        const double P = pa ? (h * (sk - u)) / (sk - set) : 0.0;
+        bsk += b[0] * (P - h) + b[1] * (P * set - h * u) + b[2] * (P * v - h * v);
+        const double bP = b[0] * sk + b[1] * (sk * set) + b[2] * (sk * v);
+        bset += b[1] * (sk * P);
+        bh -= b[0] * sk + b[1] * (sk * u) + b[2] * (sk * v);
+        bu -= b[1] * (sk * h);
+        bv += b[2] * (sk * (P - h));
+        if (pa) {
+            const double inv = 1.0 / (sk - set);
+            bh += bP * ((sk - u) * inv);
+            bu -= bP * (h * inv);
+            bsk += bP * (h * inv) - bP * (P * inv);
+            bset += bP * (P * inv);
+        }
+    }
+    g[3 * side] += bh;
+    g[3 * side + 1] += bu;
+    g[3 * side + 2] += bv;
+    const double z3[3] = {0.0, 0.0, 0.0}, z2[2] = {0.0, 0.0};
+    hllc_core_rev(gp, eps, hl, ul, vl, hr, ur, vr, z3, z2, z2, side ? 0.0 : bsk, side ? bsk : 0.0, bset, g);
+}
+
 // Reverse of hllc_flux: g[0..5] += cotangents of (hl, ul, vl, hr, ur, vr).
 CF_HD inline void hllc_flux_rev(double gp, double n1, double n2, double hl, double ul, double vl, double hr,
                                 double ur, double vr, const double bflux[3], double g[6]) {

@@ -117,6 +117,38 @@ struct DevLd {
     __device__ __forceinline__ double manning(int c) const { return man_[c]; }
 };
 
+// Loader of the step kernel: the tile's own states and neighbour codes are
+// staged in LDS (s_st, s_nb: cells base .. base + n - 1); a cell of the same
+// tile is read from there, any other from global memory.  Same values, so
+// the same bits; neighbour gathers mostly hit LDS instead of L2.
+struct TileLd {
+    const double4 *st_;
+    const int4 *nbr_;
+    const double4 *dfc_;
+    const double *geo_;
+    const double *man_;
+    int stride;
+    const double4 *s_st;
+    const int4 *s_nb;
+    int base, n;
+    __device__ __forceinline__ St<double> state(int c) const {
+        const unsigned o = (unsigned)(c - base);
+        const double4 v = o < (unsigned)n ? s_st[o] : st_[c];
+        return {v.x, v.y, v.z, v.w};
+    }
+    __device__ __forceinline__ I4 nbr(int c) const {
+        const unsigned o = (unsigned)(c - base);
+        const int4 v = o < (unsigned)n ? s_nb[o] : nbr_[c];
+        return {v.x, v.y, v.z, v.w};
+    }
+    __device__ __forceinline__ D4 dfc(int c) const {
+        const double4 v = dfc_[c];
+        return {v.x, v.y, v.z, v.w};
+    }
+    __device__ __forceinline__ double geo(int comp, int c) const { return geo_[comp * (size_t)stride + c]; }
+    __device__ __forceinline__ double manning(int c) const { return man_[c]; }
+};
+
 struct AdjState;  // reverse-sweep storage (cf_adjoint.hip)
 
 }  // namespace cf

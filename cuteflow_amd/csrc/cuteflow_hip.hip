@@ -54,6 +54,15 @@ __device__ __forceinline__ double wave_min(double v) {
 // Occupancy target of the step kernel (waves per SIMD): 4 waves caps the
 // kernel at 128 VGPRs without spilling (measured best on MI355X; 6 and 8
 // spill); 0 lets the compiler choose the register budget.
+// Neighbour gathers of the step kernel through LDS (TileLd) when 1.  Off:
+// on MI355X staging the tile (one barrier, a range check per gather) made
+// the 2M-cell reach slower (70.8 against 66.4 us/step with 128-cell tiles,
+// 67.9 against 64.0 with 64-cell tiles): the neighbour reads hit L2 and are
+// not what bounds the kernel.
+#ifndef CF_STEP_LDS
+#define CF_STEP_LDS 0
+#endif
+
 #ifndef CF_STEP_WAVES_PER_EU
 #define CF_STEP_WAVES_PER_EU 4
 #endif
@@ -109,6 +118,15 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
     const double4 *__restrict__ st_in = replay ? a.in_x : a.st[(c.par + a.step) & 1];
     double4 *__restrict__ st_out = replay ? a.out_x : a.st[(c.par + a.step + 1) & 1];
     const int i = tile * (int)blockDim.x + threadIdx.x;
+#if CF_STEP_LDS
+    // the tile's states and neighbour codes in LDS (blockDim.x cells each)
+    extern __shared__ double4 s_tile[];
+    double4 *s_st = s_tile;
+    int4 *s_nb = reinterpret_cast<int4 *>(s_tile + blockDim.x);
+    s_st[threadIdx.x] = st_in[i];
+    s_nb[threadIdx.x] = a.nbr[i];
+    __syncthreads();
+#endif
     const size_t hix = (size_t)b * a.hist_stride + a.step;
     const double dt = replay ? a.dt_fix[hix] : c.dt;
     const double tc = replay ? a.tc_fix[hix] : c.tc;
@@ -118,8 +136,14 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
     int isnan_local = 0, wet_local = 0;
 
     if (i < a.inst_end[b]) {
+#if CF_STEP_LDS
+        const TileLd ld{st_in, a.nbr, a.dfc, a.geo, a.manning, a.ncell, s_st, s_nb, tile * (int)blockDim.x,
+                        (int)blockDim.x};
+        const double zml = s_st[threadIdx.x].w;
+#else
         const DevLd ld{st_in, a.nbr, a.dfc, a.geo, a.manning, a.ncell};
         const double zml = st_in[i].w;
+#endif
         double o[3], xo[3], qio[3];
         cell_update<double, IFLUX, FRIC>(ld, p, a.hydro, i, dt, tc, o, a.xs ? xo : nullptr, a.q_io ? qio : nullptr);
         const double h = o[0], hu = o[1], hv = o[2];
@@ -792,7 +816,8 @@ extern "C" int cf_get_time(cf_handle *h, double *tc, double *dt) { return cf_bat
 
 template <int IFLUX, int FRIC>
 static void launch_step_t(const StepArgs &a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((step_kernel<IFLUX, FRIC>), dim3(grid), dim3(a.tile_cells), 0, s, a, a.kp);
+    hipLaunchKernelGGL((step_kernel<IFLUX, FRIC>), dim3(grid), dim3(a.tile_cells),
+                       CF_STEP_LDS ? (size_t)a.tile_cells * (sizeof(double4) + sizeof(int4)) : 0, s, a, a.kp);
 }
 
 typedef void (*step_launcher)(const StepArgs &, int, hipStream_t);

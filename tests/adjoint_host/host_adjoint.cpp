@@ -149,19 +149,22 @@ typedef void (*tlm_fn)(const HS &, const double *, const double *, const double 
 typedef long (*rev_fn)(const HS &, const double *, const double *, const double *, double *, double *, double, double);
 
 step_fn pick_step(int iflux, int fric) {
-    static const step_fn t[2][4] = {{step_t<1, 0>, step_t<1, 1>, step_t<1, 2>, step_t<1, 3>},
-                                    {step_t<3, 0>, step_t<3, 1>, step_t<3, 2>, step_t<3, 3>}};
-    return t[iflux == 3][fric];
+    static const step_fn t[3][4] = {{step_t<1, 0>, step_t<1, 1>, step_t<1, 2>, step_t<1, 3>},
+                                  {step_t<2, 0>, step_t<2, 1>, step_t<2, 2>, step_t<2, 3>},
+                                  {step_t<3, 0>, step_t<3, 1>, step_t<3, 2>, step_t<3, 3>}};
+    return t[iflux - 1][fric];
 }
 tlm_fn pick_tlm(int iflux, int fric) {
-    static const tlm_fn t[2][4] = {{tlm_t<1, 0>, tlm_t<1, 1>, tlm_t<1, 2>, tlm_t<1, 3>},
-                                   {tlm_t<3, 0>, tlm_t<3, 1>, tlm_t<3, 2>, tlm_t<3, 3>}};
-    return t[iflux == 3][fric];
+    static const tlm_fn t[3][4] = {{tlm_t<1, 0>, tlm_t<1, 1>, tlm_t<1, 2>, tlm_t<1, 3>},
+                                  {tlm_t<2, 0>, tlm_t<2, 1>, tlm_t<2, 2>, tlm_t<2, 3>},
+                                  {tlm_t<3, 0>, tlm_t<3, 1>, tlm_t<3, 2>, tlm_t<3, 3>}};
+    return t[iflux - 1][fric];
 }
 rev_fn pick_rev(int iflux, int fric) {
-    static const rev_fn t[2][4] = {{rev_t<1, 0>, rev_t<1, 1>, rev_t<1, 2>, rev_t<1, 3>},
-                                   {rev_t<3, 0>, rev_t<3, 1>, rev_t<3, 2>, rev_t<3, 3>}};
-    return t[iflux == 3][fric];
+    static const rev_fn t[3][4] = {{rev_t<1, 0>, rev_t<1, 1>, rev_t<1, 2>, rev_t<1, 3>},
+                                  {rev_t<2, 0>, rev_t<2, 1>, rev_t<2, 2>, rev_t<2, 3>},
+                                  {rev_t<3, 0>, rev_t<3, 1>, rev_t<3, 2>, rev_t<3, 3>}};
+    return t[iflux - 1][fric];
 }
 
 // cflcond + reduction (cfl_condition.cuf:4-104): dt of a state
@@ -330,8 +333,8 @@ long hs_adjoint(void *h, const double *lam_final, double *g_n, double *g_U0) {
 }
 
 // Face-level check: value of <bflux, J dir> by the dual-number forward and
-// <J^T bflux, dir> by the reverse, for the first-order (iflux 1, inputs
-// 0..5) or WAF (iflux 3, inputs 0..9) face flux in the global frame.
+// <J^T bflux, dir> by the reverse, for the first-order (iflux 1 and 2,
+// inputs 0..5) or WAF (iflux 3, inputs 0..9) face flux in the global frame.
 void hs_face_dot(int iflux, int ilim, double gp, double n1, double n2, const double *in, const double *dir,
                  const double *bflux, double d_mrl, double d_mkvol, double d_mkvoisin, double dt, double *out) {
     Dual v[10];
@@ -341,6 +344,9 @@ void hs_face_dot(int iflux, int ilim, double gp, double n1, double n2, const dou
     if (iflux == 1) {
         hllc_flux(gp, n1, n2, v[0], v[1], v[2], v[3], v[4], v[5], fl);
         hllc_flux_rev(gp, n1, n2, in[0], in[1], in[2], in[3], in[4], in[5], bflux, g);
+    } else if (iflux == 2) {
+        riadh_flux(gp, n1, n2, v[0], v[1], v[2], v[3], v[4], v[5], fl);
+        riadh_flux_rev(gp, n1, n2, in[0], in[1], in[2], in[3], in[4], in[5], bflux, g);
     } else {
         waf_flux(gp, ilim, n1, n2, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], v[9], d_mrl, d_mkvol,
                  d_mkvoisin, dt, fl);

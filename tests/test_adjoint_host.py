@@ -29,12 +29,14 @@ def hs(tmp_path_factory):
 # ---------------------------------------------------------------------------
 # forward: host build of the templated cell body == oracle, bit for bit
 
-@pytest.mark.parametrize("case", ["dam", "river_waf", "river_hllc_f2"])
+@pytest.mark.parametrize("case", ["dam", "river_waf", "river_hllc_f2", "river_riadh"])
 def test_host_forward_matches_oracle_bits(hs, case):
     if case == "dam":
         m, p, U0 = dam_case(12)
     elif case == "river_waf":
         m, p, U0 = river_case(24, 6)
+    elif case == "river_riadh":
+        m, p, U0 = river_case(24, 6, iflux=2, fricimplic=1)
     else:
         m, p, U0 = river_case(24, 6, iflux=1, fricimplic=2)
     ref = O.run(O.Prepared(m), p, U0, 60)
@@ -65,12 +67,12 @@ def _face_cases(rng, iflux, n):
         hru = max(0.0, hr + rng.normal(0.0, 0.6))
         vlu, vru = vl + rng.normal(0.0, 0.5), vr + rng.normal(0.0, 0.5)
         x = np.array([hl, ul, vl, hr, ur, vr, hlu, vlu, hru, vru])
-        if iflux == 1:
+        if iflux in (1, 2):
             x[6:] = 0.0
         yield x
 
 
-@pytest.mark.parametrize("iflux,ilim", [(1, 0), (3, 0), (3, 1), (3, 2), (3, 3), (3, 4)])
+@pytest.mark.parametrize("iflux,ilim", [(1, 0), (2, 0), (3, 0), (3, 1), (3, 2), (3, 3), (3, 4)])
 def test_face_reverse_matches_forward_mode(hs, iflux, ilim):
     rng = np.random.default_rng(100 + 10 * iflux + ilim)
     out = np.zeros(5)
@@ -79,7 +81,7 @@ def test_face_reverse_matches_forward_mode(hs, iflux, ilim):
         th = rng.uniform(0, 2 * np.pi)
         n1, n2 = np.cos(th), np.sin(th)
         d = rng.normal(size=10)
-        if iflux == 1:
+        if iflux in (1, 2):
             d[6:] = 0.0
         # directions only along variables that are not frozen zeros
         d[:3] *= x[0] > 0
@@ -125,6 +127,14 @@ def _mesh_case(name):
         m, p, U0 = river_case(20, 6, iflux=1, outlet="transm")
     elif name == "river_waf_f2":
         m, p, U0 = river_case(20, 6, fricimplic=2)
+    elif name == "river_riadh_f1":
+        m, p, U0 = river_case(20, 6, iflux=2, fricimplic=1)
+    elif name == "dam_riadh_f2":
+        m, p, U0 = dam_case(10, seed=6)
+        p.iflux, p.friction, p.fricimplic = 2, 1, 2
+        m.manning[:] = 0.03
+    elif name == "river_riadh_transm":
+        m, p, U0 = river_case(20, 6, iflux=2, outlet="transm")
     elif name == "lake_hllc":
         # lake at rest over a rough bed with dry islands (walls).  With WAF
         # this state is ill-conditioned for derivatives: the two face depths
@@ -142,7 +152,7 @@ def _mesh_case(name):
 
 MESH_CASES = ["dam_hllc", "dam_hllc_transm_f1", "dam_hllc_f0", "river_hllc_f1", "river_hllc_f2",
               "river_waf_l1", "river_waf_l2", "river_waf_l3", "river_waf_l4", "river_hllc_transm",
-              "river_waf_f2", "lake_hllc"]
+              "river_waf_f2", "lake_hllc", "river_riadh_f1", "dam_riadh_f2", "river_riadh_transm"]
 
 
 @pytest.mark.parametrize("name", MESH_CASES)
@@ -204,7 +214,7 @@ def test_adjoint_dot_tangent_linear_frozen_limiters(hs, name):
     assert abs(lhs - rhs) <= 1e-11 * scale, (lhs, rhs, scale)
 
 
-@pytest.mark.parametrize("name", ["river_hllc_f1", "river_waf_l1", "river_waf_f2"])
+@pytest.mark.parametrize("name", ["river_hllc_f1", "river_waf_l1", "river_waf_f2", "river_riadh_f1"])
 def test_tangent_linear_matches_central_differences(hs, name):
     """Directional derivative of a smooth functional of the final state
     along a Manning perturbation: TLM vs central differences (step 1e-6
