@@ -136,9 +136,12 @@ struct RevArgs {
 // Occupancy target of the reverse kernel (waves per SIMD): the first-order
 // reverse fits 168 VGPRs (3 waves) with a few spills and runs 8 % faster
 // than at 2 waves on MI355X; the WAF reverse needs 256 (1 wave).
+#ifndef CF_REV_WAVES_FIRST_ORDER
+#define CF_REV_WAVES_FIRST_ORDER 3
+#endif
 template <int IFLUX>
 struct RevWaves {
-    static constexpr int value = IFLUX == 3 ? 1 : 3;
+    static constexpr int value = IFLUX == 3 ? 1 : CF_REV_WAVES_FIRST_ORDER;
 };
 
 __device__ __forceinline__ void acc4(double4 &a, const double4 &b) {
@@ -678,7 +681,6 @@ static void enqueue_forward(cf_handle *h, AdjState *A, hipStream_t s) {
     hipLaunchKernelGGL(ckpt_kernel, dim3(h->ntiles), dim3(h->tile), 0, s, sp, h->ctrl, h->tile_inst, h->inst_end,
                        A->ckpt);
     StepArgs a = base_args(h);
-    a.q_io = nullptr;  // no discharge bookkeeping in recorded runs
     a.dt_hist = A->dt;
     a.tc_hist = A->tc;
     a.ran = A->ran;
@@ -883,7 +885,6 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
     fill0(A->gn, N, s);
 
     StepArgs a = base_args(h);
-    a.q_io = nullptr;
     a.dt_fix = A->dt;
     a.tc_fix = A->tc;
     a.hist_stride = S;

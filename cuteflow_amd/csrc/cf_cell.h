@@ -130,15 +130,44 @@ CF_HD inline void zero_flux(double n1, double n2, T flux[3]) {
 }
 
 // New state of cell i after one step of length dt starting at time tc.
+// Inflow and outflow discharges of cell i at the start of a step (state u0,
+// time tc): the mass flux x length of its inflow faces (debit_entree_arr,
+// flux_riemann.cuf:125-126, inlet 'inflow') and of its fixed-level outflow
+// faces (debit_sortie_arr, :147-161), each summed over the faces in face
+// order.  The same expressions as the boundary fluxes of cell_update.
+template <class L>
+CF_HD inline void cell_discharge(const L &ld, const KParams &p, const double *hydro, int i, double tc, double &qo,
+                                 double &qi) {
+    const double g = p.gp, tls = p.tolisec;
+    const St<double> u0 = ld.state(i);
+    const I4 nb = ld.nbr(i);
+    double qof[3] = {0.0, 0.0, 0.0}, qif[3] = {0.0, 0.0, 0.0};
+    for (int j = 0; j < 3; j++) {
+        const int code = nb_at(nb, j);
+        if (code >= 0) continue;
+        const int cls = (-code) & 3, id = (-code) >> 2;
+        const double n1 = ld.geo(G_N1 + j, i), n2 = ld.geo(G_N2 + j, i), ln = ld.geo(G_LN + j, i);
+        if (cls == 1 && p.inlet == 2) {
+            const double dn = -(inflow_q(p, hydro, id - 1, tc) / p.long_entree[id - 1]);
+            qif[j] = (-dn) * ln;
+        } else if (cls == 2 && p.outlet == 1) {
+            double hautl, vitul, vitvl;
+            rot(tls, n1, n2, u0.h, u0.hu, u0.hv, hautl, vitul, vitvl);
+            double f0 = 0.0;
+            if (vitul >= 0.0 && vitul / sqrt(g * hautl) < 1.0) f0 = dmax(0.0, p.h_sortie[id - 1] - u0.zm) * vitul;
+            qof[j] = f0 * ln;
+        }
+    }
+    qo = (qof[0] + qof[1]) + qof[2];
+    qi = (qif[0] + qif[1]) + qif[2];
+}
+
 // out = (h, hu, hv) after the wet/dry clamp; xout (may be null) = the
 // residual before the implicit friction matrix: res = afm1 * x
-// (finite_volumes_method.cuf:279-294).  qio (may be null) = the cell's
-// outflow discharge (debit_sortie_arr, flux_riemann.cuf:161), inflow
-// discharge (debit_entree_arr, :125-126), each summed over its faces in
-// face order, and 1 if the cell has an inflow or outflow face, else 0.
+// (finite_volumes_method.cuf:279-294).
 template <class T, int IFLUX, int FRIC, class L>
 CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro, int i, double dt, double tc,
-                              T out[3], T *xout, double *qio = nullptr) {
+                              T out[3], T *xout) {
     const double g = p.gp, tls = p.tolisec;
     const St<T> u0 = ld.state(i);
     const I4 nb = ld.nbr(i);
@@ -154,8 +183,6 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
     if (IFLUX == 3) dfi = ld.dfc(i);
     T fl[3][3];
     double fn1[3], fn2[3], fln[3];
-    double qof[3] = {0.0, 0.0, 0.0}, qif[3] = {0.0, 0.0, 0.0};
-    bool io = false;
 #pragma unroll
     for (int j = 0; j < 3; j++) {
         const double n1 = ld.geo(G_N1 + j, i);
@@ -181,14 +208,11 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
                     flux[0] = dn;
                     flux[1] = n1 * ((dn * dn) / (hautl + tls) + (g * (hautl * hautl)) / 2.0);
                     flux[2] = n2 * ((dn * dn) / (hautl + tls) + (g * (hautl * hautl)) / 2.0);
-                    qif[j] = (-dn) * ln;
                 }
-                io = true;
             } else if (cls == 3) {  // wall (:141-145)
                 flux[1] = ((n1 * g) * (hautl * hautl)) / 2.0;
                 flux[2] = ((n2 * g) * (hautl * hautl)) / 2.0;
             } else if (cls == 2) {
-                io = true;
                 if (p.outlet == 1) {  // fixed level (:147-163)
                     if (vitul >= 0.0 && vitul / vsqrt(g * hautl) < 1.0) {
                         hautl = dmax(0.0, p.h_sortie[id - 1] - zml);
@@ -199,7 +223,6 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
                         flux[1] = ((n1 * g) * (hautl * hautl)) / 2.0;
                         flux[2] = ((n2 * g) * (hautl * hautl)) / 2.0;
                     }
-                    qof[j] = val(flux[0]) * ln;
                 } else {  // transmissive: Riemann problem with itself (:166-171)
                     kv = i;
                 }
@@ -300,11 +323,6 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
         fl[j][2] = flux[2] * ln;
     }
     // face sum (flux_riemann.cuf:411-426)
-    if (qio) {
-        qio[0] = (qof[0] + qof[1]) + qof[2];
-        qio[1] = (qif[0] + qif[1]) + qif[2];
-        qio[2] = io ? 1.0 : 0.0;
-    }
     const T resc0 = (fl[0][0] + fl[1][0]) + fl[2][0];
     const T resc1 = (fl[0][1] + fl[1][1]) + fl[2][1];
     const T resc2 = (fl[0][2] + fl[1][2]) + fl[2][2];

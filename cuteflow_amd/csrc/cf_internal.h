@@ -68,11 +68,6 @@ struct StepArgs {
     // rewritten to (+0, +0, +0, zm).
     unsigned char *tile_ran;
     const unsigned char *prev_use;
-    // Discharge bookkeeping (null when off): each cell with an inflow or
-    // outflow face writes its (outflow, inflow) discharge to slot
-    // io_slot[cell] of q_io.
-    double *q_io;
-    const int *io_slot;
     int ncell;        // padded cell count (geo stride)
     int tile_cells;   // cells per tile = workgroup size
     int hist_stride;
@@ -190,7 +185,7 @@ struct cf_handle {
     // for or when an instance has tol_reg_perm > 0
     int discharge = 0;
     int nio = 0;                       // cells with an inflow or outflow face
-    int *io_slot = nullptr;            // per cell: its slot (cells with an I/O face)
+    int *io_cell = nullptr;            // per slot: the cell (cells with an I/O face, caller order)
     int *io_off = nullptr;             // per instance: first slot (ninst + 1)
     double *q_io = nullptr;            // 2 per slot: outflow, inflow discharge of the step
     double *qin_hist = nullptr, *qout_hist = nullptr;  // like dt_hist, when discharge is on

@@ -237,8 +237,7 @@ int prepare(const cf_mesh *mesh, const cf_params *params, int tile, bool keep_or
         }
 
     // tiles of `tile` consecutive internal cells: neighbour tiles and the
-    // tiles that must always run (inflow faces give a flux into dry cells;
-    // inflow and outflow cells report their discharge every step)
+    // tiles that must always run (inflow faces give a flux into dry cells)
     const int ntiles = (int)((N + tile - 1) / tile);
     out.tile_off.assign(ntiles + 1, 0);
     out.tile_adj.clear();
@@ -253,7 +252,7 @@ int prepare(const cf_mesh *mesh, const cf_params *params, int tile, bool keep_or
                 if (c >= 0) {
                     const int tt = c / tile;
                     if (tt != t) tmp.push_back(tt);
-                } else if (((-c) & 3) == 1 || ((-c) & 3) == 2) {
+                } else if (((-c) & 3) == 1) {
                     out.tile_always[t] = 1;
                 }
             }

@@ -26,7 +26,7 @@ struct Prep {
     std::vector<int> conn;          // 3*nelt node ids (1-based), internal order
     std::vector<int> tile_off;      // CSR offsets, ntiles + 1
     std::vector<int> tile_adj;      // tiles holding a neighbour of the tile's cells
-    std::vector<unsigned char> tile_always;  // tiles that always run (inflow and outflow faces)
+    std::vector<unsigned char> tile_always;  // tiles that always run (inflow faces)
     std::vector<int> io_cells;      // internal indices of cells with an inflow or outflow face, caller order
 };
 

@@ -144,15 +144,10 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
         const DevLd ld{st_in, a.nbr, a.dfc, a.geo, a.manning, a.ncell};
         const double zml = st_in[i].w;
 #endif
-        double o[3], xo[3], qio[3];
-        cell_update<double, IFLUX, FRIC>(ld, p, a.hydro, i, dt, tc, o, a.xs ? xo : nullptr, a.q_io ? qio : nullptr);
+        double o[3], xo[3];
+        cell_update<double, IFLUX, FRIC>(ld, p, a.hydro, i, dt, tc, o, a.xs ? xo : nullptr);
         const double h = o[0], hu = o[1], hv = o[2];
         st_out[i] = make_double4(h, hu, hv, zml);
-        if (a.q_io && qio[2] != 0.0) {
-            const int sl = a.io_slot[i];
-            a.q_io[2 * sl] = qio[0];
-            a.q_io[2 * sl + 1] = qio[1];
-        }
         if (replay) {
             if (a.xs) {
                 a.xs[i] = xo[0];
@@ -238,6 +233,25 @@ __device__ double block_min_1024(const double *__restrict__ part, int n, double 
         nskip += sk[w];
     }
     return r;
+}
+
+// Discharges of a step, one thread per inflow/outflow cell (slot s, cell
+// io_cell[s]): cell_discharge() on the step's input state, which the step
+// kernel leaves intact (it writes the other state buffer).  Runs between
+// the step kernel and finalize_kernel, only with discharge bookkeeping on.
+__global__ void discharge_kernel(StepArgs a, const KParams *__restrict__ kp, const int *__restrict__ io_cell, int nio,
+                                 double *__restrict__ q_io) {
+    const int sl = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sl >= nio) return;
+    const int i = io_cell[sl];
+    const int b = a.tile_inst[i / a.tile_cells];
+    const InstCtrl c = a.ctrl[b];
+    if (c.status != 0) return;
+    const DevLd ld{a.st[(c.par + a.step) & 1], a.nbr, a.dfc, a.geo, a.manning, a.ncell};
+    double qo, qi;
+    cell_discharge(ld, kp[b], a.hydro, i, c.tc, qo, qi);
+    q_io[2 * (size_t)sl] = qo;
+    q_io[2 * (size_t)sl + 1] = qi;
 }
 
 // End of a step, one workgroup per instance: dt of the new state (the
@@ -537,7 +551,7 @@ static void destroy_buffers(cf_handle *h) {
     dfree(h->tile_adj_off); dfree(h->tile_adj); dfree(h->always);
     dfree(h->tile_inst); dfree(h->inst_end); dfree(h->inst_tile);
     dfree(h->dt_hist); dfree(h->tc_hist); dfree(h->ran);
-    dfree(h->io_slot); dfree(h->io_off); dfree(h->q_io); dfree(h->qin_hist); dfree(h->qout_hist);
+    dfree(h->io_cell); dfree(h->io_off); dfree(h->q_io); dfree(h->qin_hist); dfree(h->qout_hist);
 }
 
 extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params *params, cf_handle **out) {
@@ -627,11 +641,10 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
     std::vector<int> perm(N, 0), tile_inst(tiles), inst_end(ninst), inst_tile(ninst + 1), adj_off(tiles + 1, 0), adj;
     std::vector<unsigned char> always(tiles, 0);
     // discharge slots: the instances' inflow/outflow cells, caller order
-    std::vector<int> io_slot(N, -1), io_off(ninst + 1, 0);
+    std::vector<int> io_cell, io_off(ninst + 1, 0);
     for (int b = 0; b < ninst; b++) {
         io_off[b + 1] = io_off[b] + (int)prep[b].io_cells.size();
-        for (size_t q = 0; q < prep[b].io_cells.size(); q++)
-            io_slot[h->cell_off[b] + prep[b].io_cells[q]] = io_off[b] + (int)q;
+        for (size_t q = 0; q < prep[b].io_cells.size(); q++) io_cell.push_back(h->cell_off[b] + prep[b].io_cells[q]);
         if (params[b].tol_reg_perm > 0.0) h->discharge = 1;
     }
     h->nio = io_off[ninst];
@@ -670,7 +683,7 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
             !dalloc(h->stamp, (size_t)tiles) || !dalloc(h->tile_adj_off, adj_off.size()) ||
             !dalloc(h->tile_adj, adj.size()) || !dalloc(h->always, (size_t)tiles) ||
             !dalloc(h->tile_inst, (size_t)tiles) || !dalloc(h->inst_end, (size_t)ninst) ||
-            !dalloc(h->inst_tile, (size_t)ninst + 1) || !dalloc(h->io_slot, N) || !dalloc(h->io_off, (size_t)ninst + 1) ||
+            !dalloc(h->inst_tile, (size_t)ninst + 1) || !dalloc(h->io_cell, io_cell.size()) || !dalloc(h->io_off, (size_t)ninst + 1) ||
             !dalloc(h->q_io, 2 * std::max<size_t>((size_t)h->nio, 1))) {
             rc = CF_ERR_NOMEM;
             break;
@@ -684,7 +697,7 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
                   h2d(h->tile_inst, tile_inst.data(), (size_t)tiles, h->stream) &&
                   h2d(h->inst_end, inst_end.data(), (size_t)ninst, h->stream) &&
                   h2d(h->inst_tile, inst_tile.data(), (size_t)ninst + 1, h->stream) &&
-                  h2d(h->io_slot, io_slot.data(), N, h->stream) &&
+                  h2d(h->io_cell, io_cell.data(), io_cell.size(), h->stream) &&
                   h2d(h->io_off, io_off.data(), (size_t)ninst + 1, h->stream) &&
                   hipMemsetAsync(h->q_io, 0, 2 * std::max<size_t>((size_t)h->nio, 1) * sizeof(double), h->stream) ==
                       hipSuccess &&
@@ -879,8 +892,6 @@ StepArgs cf::base_args(cf_handle *h) {
     a.hist_stride = h->cap + 1;
     a.skip = h->skip;
     a.xcd = h->xcd;
-    a.q_io = h->discharge ? h->q_io : nullptr;
-    a.io_slot = h->io_slot;
     return a;
 }
 
@@ -892,6 +903,9 @@ static void enqueue_run(cf_handle *h, int nsteps, hipStream_t s) {
     for (int k = 0; k < nsteps; k++) {
         a.step = k;
         L(a, h->ntiles, s);
+        if (h->discharge && h->nio)
+            hipLaunchKernelGGL(discharge_kernel, dim3(grid_of(h->nio, 256)), dim3(256), 0, s, a, h->d_kp, h->io_cell,
+                               h->nio, h->q_io);
         launch_finalize(h, h->dt_hist, h->tc_hist, h->ran, h->cap + 1, k, s, h->discharge ? h->qin_hist : nullptr,
                         h->discharge ? h->qout_hist : nullptr);
     }
