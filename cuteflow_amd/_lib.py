@@ -11,7 +11,7 @@ import os
 
 import numpy as np
 
-from .build import LIB_PATH
+from .build import CGNS_LIB_PATH, LIB_PATH
 
 CF_MAX_IO = 20
 CF_MAX_HYDRO = 64
@@ -31,6 +31,7 @@ CF_ERR_HIP = -2
 CF_ERR_DT = -3
 CF_ERR_MESH = -4
 CF_ERR_NOMEM = -5
+CF_ERR_IO = -6
 
 
 class CfParams(C.Structure):
@@ -169,6 +170,52 @@ def load(path: str | None = None):
         fn.argtypes = args
     _lib = lib
     return lib
+
+
+class CfCgnsInfo(C.Structure):
+    """Mirror of struct cf_cgns_info (include/cuteflow_cgns.h)."""
+
+    _fields_ = [(n, C.c_int) for n in ("nnt", "nelt", "ndi", "ndo", "nzones", "zone", "has_manning",
+                                        "partitioned")]
+
+
+_CP = C.c_char_p
+CGNS_SIGNATURES = {
+    "cf_cgns_mesh_info": (C.c_int, [_CP, C.POINTER(CfCgnsInfo)]),
+    "cf_cgns_read_mesh": (C.c_int, [_CP, C.c_int, _CP, C.c_int, _CP, C.c_int, C.c_int, C.c_int,
+                                    _DP, _IP, _DP, _IP, _IP, _IP, _IP]),
+    "cf_cgns_write_mesh": (C.c_int, [_CP, C.c_int, C.c_int, _DP, _IP, C.c_int, C.c_int, _IP, _IP,
+                                     C.c_int, C.c_int, _IP, _IP, C.c_int, _IP, _DP]),
+    "cf_cgns_export_init": (C.c_int, [_CP, C.c_int, _DP]),
+    "cf_cgns_export": (C.c_int, [_CP, C.c_int, C.c_int, _DP, _DP, _DP, C.c_double, C.c_double]),
+    "cf_cgns_error": (C.c_char_p, []),
+}
+
+_cgns = None
+
+
+def load_cgns(path: str | None = None):
+    """Load libcuteflow_cgns.so (CGNS/HDF5 files, host only); raises if it
+    has not been built (it needs an HDF5 installation at build time)."""
+    global _cgns
+    if _cgns is not None:
+        return _cgns
+    path = path or CGNS_LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is missing: build it first (cuteflow_amd.build.build_cgns(), "
+                           "which needs HDF5 headers and library)")
+    lib = C.CDLL(path)
+    for name, (res, args) in CGNS_SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _cgns = lib
+    return lib
+
+
+def check_cgns(status: int, what: str) -> None:
+    if status != CF_OK:
+        raise CuteflowError(status, f"{what}: {load_cgns().cf_cgns_error().decode()}")
 
 
 def check(status: int, what: str) -> None:

@@ -56,6 +56,9 @@ def build_hip(verbose: bool = False) -> str:
     return LIB_PATH
 
 
+CGNS_SOURCE = os.path.join(CSRC, "cf_cgns.cpp")
+CGNS_LIB_PATH = os.path.join(LIB_DIR, "libcuteflow_cgns.so")
+
 FORTRAN_DIR = os.path.join(PKG_DIR, "fortran")
 BIN_DIR = os.path.join(PKG_DIR, "bin")
 FORTRAN_EXE = os.path.join(BIN_DIR, "cuteflow_hip")
@@ -63,19 +66,29 @@ AMDFLANG = os.environ.get("AMDFLANG", "/opt/rocm/bin/amdflang")
 
 
 def build_fortran(verbose: bool = False) -> str:
-    """Compile the Fortran driver (amdflang) against libcuteflow_hip.so."""
+    """Compile the Fortran driver (amdflang) against libcuteflow_hip.so, and
+    against libcuteflow_cgns.so (CGNS meshes, CF_WITH_CGNS) when that library
+    has been built."""
     os.makedirs(BIN_DIR, exist_ok=True)
     obj = os.path.join(PKG_DIR, "_build", "fortran")
     os.makedirs(obj, exist_ok=True)
+    with_cgns = os.path.exists(CGNS_LIB_PATH)
     mod_o = os.path.join(obj, "cuteflow_hip_mod.o")
+    cgns_o = os.path.join(obj, "cuteflow_cgns_mod.o")
     main_o = os.path.join(obj, "cuteflow_main.o")
-    steps = [
-        [AMDFLANG, "-O2", "-module-dir", obj, "-c", os.path.join(FORTRAN_DIR, "cuteflow_hip_mod.f90"), "-o", mod_o],
-        [AMDFLANG, "-O2", "-module-dir", obj, "-I", obj, "-c", os.path.join(FORTRAN_DIR, "cuteflow_main.f90"),
-         "-o", main_o],
-        [AMDFLANG, "-o", FORTRAN_EXE + ".tmp", mod_o, main_o, "-L" + LIB_DIR, "-lcuteflow_hip",
-         "-Wl,-rpath,$ORIGIN/../lib"],
-    ]
+    steps = [[AMDFLANG, "-O2", "-module-dir", obj, "-c", os.path.join(FORTRAN_DIR, "cuteflow_hip_mod.f90"),
+              "-o", mod_o]]
+    objs = [mod_o]
+    libs = ["-lcuteflow_hip"]
+    if with_cgns:
+        steps.append([AMDFLANG, "-O2", "-module-dir", obj, "-c", os.path.join(FORTRAN_DIR, "cuteflow_cgns_mod.f90"),
+                      "-o", cgns_o])
+        objs.append(cgns_o)
+        libs.insert(0, "-lcuteflow_cgns")
+    steps.append([AMDFLANG, "-O2", "-cpp", *(["-DCF_WITH_CGNS"] if with_cgns else []), "-module-dir", obj, "-I",
+                  obj, "-c", os.path.join(FORTRAN_DIR, "cuteflow_main.f90"), "-o", main_o])
+    steps.append([AMDFLANG, "-o", FORTRAN_EXE + ".tmp", *objs, main_o, "-L" + LIB_DIR, *libs,
+                  "-Wl,-rpath,$ORIGIN/../lib"])
     for cmd in steps:
         if verbose:
             print(" ".join(cmd))
@@ -84,6 +97,34 @@ def build_fortran(verbose: bool = False) -> str:
     return FORTRAN_EXE
 
 
+HDF5_DIR = os.environ.get("CF_HDF5_DIR", "/opt/conda")
+
+
+def hdf5_available() -> bool:
+    return os.path.exists(os.path.join(HDF5_DIR, "include", "hdf5.h")) and any(
+        f.startswith("libhdf5.so") for f in os.listdir(os.path.join(HDF5_DIR, "lib")))
+
+
+def build_cgns(verbose: bool = False) -> str | None:
+    """Compile libcuteflow_cgns.so (host C++ over the HDF5 C API; the HDF5
+    library is found at run time through an rpath to HDF5_DIR/lib).  Returns
+    None, with a message, when no HDF5 installation is found."""
+    if not hdf5_available():
+        print(f"cuteflow_amd: no HDF5 under {HDF5_DIR}; the CGNS reader/writer is not built")
+        return None
+    os.makedirs(LIB_DIR, exist_ok=True)
+    tmp = CGNS_LIB_PATH + ".tmp"
+    inc, lib = os.path.join(HDF5_DIR, "include"), os.path.join(HDF5_DIR, "lib")
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", "-I" + inc, "-o", tmp,
+           CGNS_SOURCE, "-L" + lib, "-lhdf5", "-Wl,-rpath," + lib, "-static-libstdc++", "-static-libgcc"]
+    if verbose:
+        print(" ".join(cmd))
+    _run(cmd)
+    os.replace(tmp, CGNS_LIB_PATH)
+    return CGNS_LIB_PATH
+
+
 if __name__ == "__main__":
     print(build_hip(verbose=True))
+    print(build_cgns(verbose=True))
     print(build_fortran(verbose=True))

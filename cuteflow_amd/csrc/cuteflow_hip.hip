@@ -525,6 +525,7 @@ extern "C" const char *cf_error_string(int s) {
     case CF_ERR_DT: return "time step out of range (dt > 9999, dt < 1e-12) or NaN state";
     case CF_ERR_MESH: return "inconsistent mesh";
     case CF_ERR_NOMEM: return "out of memory";
+    case CF_ERR_IO: return "file missing, unreadable or not in the expected layout";
     default: return "unknown status";
     }
 }

@@ -3,15 +3,17 @@ finite_volumes_method.cuf:4-258) for one subdomain, with the step on the GPU.
 
     python -m cuteflow_amd <case_dir>
 
-reads <case_dir>/donnees.f, the text mesh, the boundary table (built and
-written when elt_bound = 0, read otherwise), the initial condition (file,
-dam line or plane), runs to ts, prints the status block every freqaffich
-steps and writes the snapshots (legacy VTK, element/node text) and the
-restart file like the reference.
+reads <case_dir>/donnees.f, the mesh (text layout, or CGNS when is_cgns =
+1), the boundary table (built and written when elt_bound = 0, read
+otherwise), the initial condition (file, dam line or plane), runs to ts,
+prints the status block every freqaffich steps and writes the snapshots
+(legacy VTK, element/node text, CGNS FlowSolutions in out_<meshfile>) and
+the restart file like the reference.
 """
 from __future__ import annotations
 
 import os
+import shutil
 import sys
 import time
 
@@ -28,12 +30,17 @@ class SnapshotWriter:
     written when tc is within +-tolaffiche of k*ts/visu_snapshots, with
     tolaffiche = 1.5*dt after the first one."""
 
-    def __init__(self, prm, mesh, zm, surf, outdir):
+    def __init__(self, prm, mesh, zm, surf, outdir, cgns_out=None):
         self.prm, self.mesh, self.zm, self.surf, self.outdir = prm, mesh, zm, surf, outdir
         self.times = cio.snapshot_times(prm.ts, prm.visu_snapshots) if prm.solvisu > 0 else np.zeros(0)
         self.cpt = 0
         self.tolaffiche = prm.tolaffiche
         self.written = []
+        # CGNS export (pre_post_traitement.cuf:27-31) into the mesh copy
+        self.cgns = None
+        if cgns_out is not None and prm.is_cgns == 1 and prm.solvisu in (1, 3):
+            from .cgns import CgnsExporter
+            self.cgns = CgnsExporter(cgns_out)
 
     @property
     def pending(self):
@@ -65,6 +72,12 @@ class SnapshotWriter:
             nodes = cio.solution_on_nodes(m.conn, m.nnt, U, self.surf)
             cio.write_simple_nodes(os.path.join(d, f"solution_noeuds_simple_{k}.txt"), tc, nodes,
                                    m.coords[:, 2], p.sol_z_offset)
+        if self.cgns is not None:
+            if k == 0:
+                self.cgns.init(self.times)
+            # the Manning field is written when the per-cell n is in use
+            self.cgns.export(k, U, self.zm, m.manning if p.is_override_manning == 0 else None, p.tolisec,
+                             p.sol_z_offset)
         if p.solvisu in (1, 5):
             cio.write_simple_elements(os.path.join(d, f"solution_elements_simple_{k}.txt"), tc, U, self.zm,
                                       p.sol_z_offset)
@@ -74,10 +87,17 @@ class SnapshotWriter:
 def run_case(case_dir: str, verbose: bool = True, max_steps: int | None = None):
     """Runs a case directory; returns a dict with the final state and times."""
     prm = read_donnees(os.path.join(case_dir, "donnees.f"))
-    if prm.is_cgns:
-        raise NotImplementedError("CGNS meshes are not read by this build; use the text mesh format")
-    mesh = read_text_mesh(os.path.join(case_dir, prm.meshfile_path, prm.meshfile), prm.nombre_entrees,
-                          prm.nombre_sorties)
+    mesh_path = os.path.join(case_dir, prm.meshfile_path, prm.meshfile.strip())
+    cgns_out = None
+    if prm.is_cgns:  # lec_mail_cgns (maillage.cuf:398-778)
+        from .cgns import read_cgns_mesh
+        mesh = read_cgns_mesh(mesh_path, prm.inlet_name, prm.outlet_name, prm.nombre_entrees,
+                              prm.nombre_sorties, prm.is_override_manning, verbose)
+        if prm.solvisu in (1, 3):  # the solutions go into a copy of the mesh file (maillage.cuf:427-437)
+            cgns_out = os.path.join(case_dir, "out_" + prm.meshfile.strip())
+            shutil.copyfile(mesh_path, cgns_out)
+    else:
+        mesh = read_text_mesh(mesh_path, prm.nombre_entrees, prm.nombre_sorties)
     prm.nombre_entrees, prm.nombre_sorties = mesh.nombre_entrees, mesh.nombre_sorties
     file_bc = os.path.join(case_dir, "boundary_table_" + prm.meshfile.strip())
     if prm.elt_bound == 0:
@@ -85,8 +105,6 @@ def run_case(case_dir: str, verbose: bool = True, max_steps: int | None = None):
         write_boundary_table(file_bc, bnd)
     else:
         bnd = read_boundary_table(file_bc, mesh.nelt)
-    if prm.is_override_manning == 0:
-        pass  # per-cell n from the mesh file
     s = Solver(mesh, prm, boundary=bnd)
     geo = s.geometry()
     if verbose:
@@ -100,7 +118,7 @@ def run_case(case_dir: str, verbose: bool = True, max_steps: int | None = None):
         U0 = initial_state(prm, mesh, geo["zm"])
     s.set_discharge(True)  # debit_entree / debit_sortie of the status block
     s.set_state(U0)
-    snaps = SnapshotWriter(prm, mesh, geo["zm"], geo["surf"], case_dir)
+    snaps = SnapshotWriter(prm, mesh, geo["zm"], geo["surf"], case_dir, cgns_out)
     snaps.check(0.0, s.dt, s.get_state)
 
     freq = max(1, prm.freqaffich)

@@ -3,7 +3,10 @@
 ! the per-step kernels replaced by the HIP library through cuteflow_hip.
 !
 ! Keeps the reference's inputs and outputs: the `donnees.f` namelist
-! (global_data.cuf:65-94), the text mesh (maillage.cuf:196-396), the
+! (global_data.cuf:65-94), the text mesh (maillage.cuf:196-396) or, when
+! built with the CGNS library (CF_WITH_CGNS), the CGNS mesh
+! (maillage.cuf:398-778) with the initial and final FlowSolutions in
+! out_<meshfile> (pre_post_traitement.cuf:209-405), the
 ! boundary_table_<meshfile> file (maillage.cuf:55-79), the initial condition
 ! file (initial_condition.cuf:82-102) or the dam-line / plane initial
 ! condition (initial_condition.cuf:4-80), the status block every freqaffich
@@ -12,6 +15,9 @@
 program cuteflow_main
   use, intrinsic :: iso_c_binding
   use cuteflow_hip
+#ifdef CF_WITH_CGNS
+  use cuteflow_cgns
+#endif
   implicit none
 
   integer, parameter :: dp = c_double
@@ -61,63 +67,101 @@ program cuteflow_main
   real(dp) :: cotemin, tc, dt, long_entree(20), x1, x2, x3, y1, y2, y3, v1, v2, hh, alpha, beta
   integer(8) :: c0, c1, crate
   logical :: ex
+  character(len=300) :: mesh_path, out_path
+  real(dp), allocatable :: tsolvisu(:)
+  real(c_double), allocatable, target :: cg_coords(:)
+  integer(c_int), allocatable :: cg_conn(:)
+  integer :: nsnap
+#ifdef CF_WITH_CGNS
+  type(cf_cgns_info_t) :: cinfo
+#endif
 
   open(10, file='donnees.f', form='formatted', status='old')
   read(10, nml=donnees_namelist)
   close(10)
   inlet = trim(adjustl(inlet))
 
-  ! ---- text mesh (maillage.cuf:196-396) ----
-  open(11, file=trim(adjustl(meshfile_path))//trim(adjustl(meshfile)), form='formatted', status='old')
-  read(11, *) tarati
-  read(11, *) nnt
-  allocate(coords(nnt, 3), manning_nd(nnt))
-  do i = 1, nnt
-     read(11, *) n, coords(i, 1), coords(i, 2), coords(i, 3), manning_nd(i)
-  end do
-  read(11, *) tarati
-  read(11, *) nelt
-  allocate(conn(nelt, 3), manning(nelt))
-  do i = 1, nelt
-     read(11, *) n, conn(i, 1), conn(i, 2), conn(i, 3), manning(i)
-  end do
-  read(11, *) tarati
-  if (nombre_entrees == 1) then
-     read(11, *) ndi
-     allocate(ndinput(ndi), num_in(ndi))
-     do i = 1, ndi
-        read(11, *) ndinput(i)
-        num_in(i) = 1
+  ! ---- mesh: CGNS (maillage.cuf:398-778) or text (maillage.cuf:196-396) ----
+  mesh_path = trim(adjustl(meshfile_path))//trim(adjustl(meshfile))
+  if (is_cgns == 1) then
+#ifdef CF_WITH_CGNS
+     st = cf_cgns_mesh_info(trim(mesh_path)//c_null_char, cinfo)
+     call cf_cgns_check(st, 'cf_cgns_mesh_info')
+     nnt = cinfo%nnt; nelt = cinfo%nelt; ndi = cinfo%ndi; ndo = cinfo%ndo; ndw = 0
+     allocate(coords(nnt, 3), manning_nd(nnt), conn(nelt, 3), manning(nelt), ndwall(0))
+     allocate(ndinput(max(ndi, 1)), num_in(max(ndi, 1)), ndoutput(max(ndo, 1)), num_out(max(ndo, 1)))
+     allocate(cg_coords(3 * nnt), cg_conn(3 * nelt))
+     st = cf_cgns_read_mesh(trim(mesh_path)//c_null_char, nombre_entrees, inlet_name, nombre_sorties, &
+          outlet_name, len(inlet_name), is_override_manning, 1_c_int, cg_coords, cg_conn, manning, ndinput, &
+          num_in, ndoutput, num_out)
+     call cf_cgns_check(st, 'cf_cgns_read_mesh')
+     do i = 1, nnt
+        coords(i, :) = cg_coords(3 * i - 2:3 * i)
      end do
+     do i = 1, nelt
+        conn(i, :) = cg_conn(3 * i - 2:3 * i)
+     end do
+     manning_nd = 0
+     if (solvisu == 1 .or. solvisu == 3) then
+        out_path = 'out_'//trim(adjustl(meshfile))
+        call copy_file(trim(mesh_path), trim(out_path))
+     end if
+#else
+     write(*, *) 'is_cgns = 1: this driver was built without the CGNS library (no HDF5 found)'
+     call exit(1)
+#endif
   else
-     read(11, *) ndi, nombre_entrees
-     allocate(ndinput(ndi), num_in(ndi))
-     do i = 1, ndi
-        read(11, *) ndinput(i), num_in(i)
+     open(11, file=trim(adjustl(meshfile_path))//trim(adjustl(meshfile)), form='formatted', status='old')
+     read(11, *) tarati
+     read(11, *) nnt
+     allocate(coords(nnt, 3), manning_nd(nnt))
+     do i = 1, nnt
+        read(11, *) n, coords(i, 1), coords(i, 2), coords(i, 3), manning_nd(i)
      end do
+     read(11, *) tarati
+     read(11, *) nelt
+     allocate(conn(nelt, 3), manning(nelt))
+     do i = 1, nelt
+        read(11, *) n, conn(i, 1), conn(i, 2), conn(i, 3), manning(i)
+     end do
+     read(11, *) tarati
+     if (nombre_entrees == 1) then
+        read(11, *) ndi
+        allocate(ndinput(ndi), num_in(ndi))
+        do i = 1, ndi
+           read(11, *) ndinput(i)
+           num_in(i) = 1
+        end do
+     else
+        read(11, *) ndi, nombre_entrees
+        allocate(ndinput(ndi), num_in(ndi))
+        do i = 1, ndi
+           read(11, *) ndinput(i), num_in(i)
+        end do
+     end if
+     read(11, *) tarati
+     if (nombre_sorties == 1) then
+        read(11, *) ndo
+        allocate(ndoutput(ndo), num_out(ndo))
+        do i = 1, ndo
+           read(11, *) ndoutput(i)
+           num_out(i) = 1
+        end do
+     else
+        read(11, *) ndo, nombre_sorties
+        allocate(ndoutput(ndo), num_out(ndo))
+        do i = 1, ndo
+           read(11, *) ndoutput(i), num_out(i)
+        end do
+     end if
+     read(11, *) tarati
+     read(11, *) ndw
+     allocate(ndwall(ndw))
+     do i = 1, ndw
+        read(11, *) ndwall(i)
+     end do
+     close(11)
   end if
-  read(11, *) tarati
-  if (nombre_sorties == 1) then
-     read(11, *) ndo
-     allocate(ndoutput(ndo), num_out(ndo))
-     do i = 1, ndo
-        read(11, *) ndoutput(i)
-        num_out(i) = 1
-     end do
-  else
-     read(11, *) ndo, nombre_sorties
-     allocate(ndoutput(ndo), num_out(ndo))
-     do i = 1, ndo
-        read(11, *) ndoutput(i), num_out(i)
-     end do
-  end if
-  read(11, *) tarati
-  read(11, *) ndw
-  allocate(ndwall(ndw))
-  do i = 1, ndw
-     read(11, *) ndwall(i)
-  end do
-  close(11)
   write(*, *) nnt, ' nodes, ', nelt, ' elems, ', ndi, ' inlet nodes, ', ndo, ' outlet nodes, ', &
        ndw, ' wall nodes'
 
@@ -139,8 +183,9 @@ program cuteflow_main
      call cf_check(st, 'cf_build_boundary')
      open(12, file=trim(file_bc), status='unknown')
      do i = 1, nelt
-        write(12, *) bnd(i), bnd(nelt + i), bnd(2 * nelt + i), bnd(3 * nelt + i), bnd(4 * nelt + i), &
-             bnd(5 * nelt + i)
+        ! the reference compiler's list-directed layout of six integers
+        write(12, '(6(1x,i12))') bnd(i), bnd(nelt + i), bnd(2 * nelt + i), bnd(3 * nelt + i), &
+             bnd(4 * nelt + i), bnd(5 * nelt + i)
      end do
      close(12)
   else
@@ -257,6 +302,20 @@ program cuteflow_main
   call cf_check(st, 'cf_set_discharge')
   st = cf_set_state(h, u)
   call cf_check(st, 'cf_set_state')
+#ifdef CF_WITH_CGNS
+  ! CGNS snapshots (main.cuf:113-122, pre_post_traitement.cuf:27-31): the
+  ! initial state as Solution_0 here, the final one as the last snapshot
+  if (is_cgns == 1 .and. (solvisu == 1 .or. solvisu == 3)) then
+     nsnap = max(visu_snapshots, 1)
+     allocate(tsolvisu(0:nsnap))
+     do i = 0, nsnap
+        tsolvisu(i) = i * ts / real(nsnap)
+     end do
+     st = cf_cgns_export_init(trim(out_path)//c_null_char, nsnap + 1, tsolvisu)
+     call cf_cgns_check(st, 'cf_cgns_export_init')
+     call cgns_snapshot(0)
+  end if
+#endif
 
   ! ---- time loop (finite_volumes_method.cuf:113-242), freqaffich steps per batch ----
   nbatch = max(1, freqaffich)
@@ -315,6 +374,9 @@ program cuteflow_main
      close(15)
   end if
   if (solvisu == 1 .or. solvisu == 2) call vtk_export('solution-Paraview_final.vtk')
+#ifdef CF_WITH_CGNS
+  if (is_cgns == 1 .and. (solvisu == 1 .or. solvisu == 3)) call cgns_snapshot(nsnap)
+#endif
 
   print *, '==========================================================================='
   print *, '=========================   FIN DE LA SIMULATION   ========================'
@@ -323,6 +385,19 @@ program cuteflow_main
   call cf_destroy(h)
 
 contains
+
+#ifdef CF_WITH_CGNS
+  ! cgns_export (pre_post_traitement.cuf:287-399) of the current state u; the
+  ! Manning field is written when the per-cell n is in use
+  subroutine cgns_snapshot(k)
+    integer, intent(in) :: k
+    type(c_ptr) :: mp
+    mp = c_null_ptr
+    if (is_override_manning == 0) mp = c_loc(mann_c)
+    st = cf_cgns_export(trim(out_path)//c_null_char, k, nelt, u, zm, mp, tolisec, sol_z_offset)
+    call cf_cgns_check(st, 'cf_cgns_export')
+  end subroutine cgns_snapshot
+#endif
 
   ! paraview_export (pre_post_traitement.cuf:407-496)
   subroutine vtk_export(fname)

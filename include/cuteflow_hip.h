@@ -40,7 +40,8 @@ enum {
     CF_ERR_HIP = -2,        /* HIP runtime error */
     CF_ERR_DT = -3,         /* dt sanity failure (cfl_condition.cuf:129-132) or NaN state */
     CF_ERR_MESH = -4,       /* inconsistent mesh */
-    CF_ERR_NOMEM = -5
+    CF_ERR_NOMEM = -5,
+    CF_ERR_IO = -6          /* file missing, unreadable, or not in the expected layout */
 };
 
 /* Every constant the reference kernels read from `constant` memory

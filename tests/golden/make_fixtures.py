@@ -13,10 +13,13 @@ Sources (relative to the reference root), data only:
       GPU run logs: known answers (cotemin, tol, counts, inlet length, series)
   Calibration_manning/docs_calibration/<case>/Froude/donnees.f, Solution/Manning.txt,
       Solution/Entree_gen_manning.txt, src/cuteflow/donnees.f: input files
+  Calibration_manning/Exemple2/base_files/mesh_6072.cgns: a CGNS mesh without BCs
+      or Manning data (copied as is, like the two manning_<mesh>.cgns files)
 
 Outputs: <case>.mesh (the text mesh layout of maillage.cuf:196-396),
-<case>.boundary_table (copied), <case>.expected.json, <case>.donnees.f,
-<case>.Manning.txt, <case>.Entree_gen_manning.txt, default.donnees.f.
+<case>.cgns (the HDF5 original, copied), <case>.boundary_table (copied),
+<case>.expected.json, <case>.donnees.f, <case>.Manning.txt,
+<case>.Entree_gen_manning.txt, default.donnees.f, mesh_6072.cgns.
 """
 import json
 import os
@@ -87,6 +90,7 @@ def main():
             fh.write(f"inflow nodes\n{len(inflow)}\n" + "".join(f"{v}\n" for v in inflow))
             fh.write(f"outflow nodes\n{len(outflow)}\n" + "".join(f"{v}\n" for v in outflow))
             fh.write("wall nodes\n0\n")
+        shutil.copy(cg, os.path.join(OUT, f"{name}.cgns"))
         shutil.copy(os.path.join(froude, f"boundary_table_manning_{mesh}.cgns"),
                     os.path.join(OUT, f"{name}.boundary_table"))
         shutil.copy(os.path.join(froude, "donnees.f"), os.path.join(OUT, f"{name}.donnees.f"))
@@ -102,6 +106,8 @@ def main():
             json.dump(expected, fh, indent=1)
         print(name, nnt, nelt, len(inflow), len(outflow))
     shutil.copy(os.path.join(REF, "src/cuteflow/donnees.f"), os.path.join(OUT, "default.donnees.f"))
+    shutil.copy(os.path.join(REF, "Calibration_manning/Exemple2/base_files/mesh_6072.cgns"),
+                os.path.join(OUT, "mesh_6072.cgns"))
 
 
 if __name__ == "__main__":

@@ -1,7 +1,10 @@
 """Case drivers on the GPU: the Fortran host (amdflang, iso_c_binding) and the
 Python driver run the same case directory (donnees.f, text mesh, boundary
-table, dam-line initial condition) and agree with the oracle."""
+table, dam-line initial condition) and agree with the oracle; the Python
+driver also runs a reference case directory as is (CGNS mesh, solvisu=3)."""
 import os
+import re
+import shutil
 import subprocess
 
 import numpy as np
@@ -97,3 +100,89 @@ def test_fortran_driver_matches_python(tmp_path):
     assert len(printed) == len(iters) > 0
     for it, q in zip(iters, printed):
         assert abs(q - ref["qout"][it - 1]) <= 1e-15 * max(abs(ref["qout"][it - 1]), 1e-300), (it, q)
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_python_driver_reference_cgns_case(tmp_path):
+    """The divergent canal case directory as the reference ran it (its
+    donnees.f, the CGNS mesh file it names, solvisu=3), shortened to
+    ts = 5 s with 4 snapshots: the state equals the oracle's on the text
+    copy of the mesh, the boundary table file written is the reference's
+    own, byte for byte, and out_<meshfile> receives the FlowSolutions."""
+    from cuteflow_amd.build import CGNS_LIB_PATH
+    if not os.path.exists(CGNS_LIB_PATH):
+        pytest.skip("libcuteflow_cgns.so not built")
+    src = open(os.path.join(GOLDEN, "divergent_canal.donnees.f"), encoding="latin-1").read()
+    src = src.replace("TS=280.0", "TS=5.0").replace("visu_snapshots=100", "visu_snapshots=4")
+    src = src.replace("freqaffich=1000", "freqaffich=50")
+    (tmp_path / "donnees.f").write_text(src, encoding="latin-1")
+    shutil.copyfile(os.path.join(GOLDEN, "divergent_canal.cgns"), tmp_path / "manning_Canal_divergent.cgns")
+    print("cgns case: divergent canal, ts=5", flush=True)
+    out = run_case(str(tmp_path), verbose=False)
+    mesh = M.read_text_mesh(os.path.join(GOLDEN, "divergent_canal.mesh"))
+    p, prep, ref = _oracle(mesh, str(tmp_path))
+    assert out["status"] == 0 and ref["status"] == 0
+    assert out["steps"] == ref["steps"] > 100 and out["tc"] == ref["tc"]
+    assert np.array_equal(out["U"], ref["U"])
+    bt = open(tmp_path / "boundary_table_manning_Canal_divergent.cgns").read()
+    assert bt == open(os.path.join(GOLDEN, "divergent_canal.boundary_table")).read()
+    written = [k for k, _ in out["snapshots"]]
+    assert written == [0, 1, 2, 3, 4]
+    h5dump = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+    if not os.path.exists(h5dump):
+        return
+    listing = subprocess.run([h5dump, "-n", str(tmp_path / "out_manning_Canal_divergent.cgns")],
+                             capture_output=True, text=True, check=True).stdout
+    for k in written:
+        assert f"/Base/Zone/Solution_{k}/Water Depth" in listing
+    assert "Manning" not in listing.split("/Base/Zone/Solution_0", 1)[1]  # is_override_manning = 1
+    k, tc = out["snapshots"][-1]
+    if tc == out["tc"]:
+        path = tmp_path / "out_manning_Canal_divergent.cgns"
+        assert np.array_equal(_h5_field(path, f"/Base/Zone/Solution_{k}/Water Depth/ data"), out["U"][:, 0])
+
+
+def _h5_field(path, ds):
+    h5dump = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+    dump = subprocess.run([h5dump, "-y", "-w", "0", "-m", "%.17g", "-d", ds, str(path)], capture_output=True,
+                          text=True, check=True).stdout
+    body = dump[dump.index("DATA {") + 6:]
+    return np.array([float(v) for v in re.split(r"[,\s]+", body[: body.index("}")]) if v])
+
+
+def test_fortran_driver_reference_cgns_case(tmp_path):
+    """The Fortran host on the same reference case directory (CGNS mesh,
+    solvisu=3): step count and restart file as the oracle, the reference's
+    boundary table file byte for byte, and the first and last FlowSolutions
+    in out_<meshfile> equal to the oracle's initial and final states."""
+    from cuteflow_amd.build import CGNS_LIB_PATH
+    from cuteflow_amd.solver import initial_state
+    if not (os.path.exists(FORTRAN_EXE) and os.path.exists(CGNS_LIB_PATH)):
+        pytest.skip("Fortran driver or libcuteflow_cgns.so not built")
+    src = open(os.path.join(GOLDEN, "divergent_canal.donnees.f"), encoding="latin-1").read()
+    src = src.replace("TS=280.0", "TS=5.0").replace("visu_snapshots=100", "visu_snapshots=4")
+    src = src.replace("freqaffich=1000", "freqaffich=50").replace("solrestart=0", "solrestart=1")
+    (tmp_path / "donnees.f").write_text(src, encoding="latin-1")
+    shutil.copyfile(os.path.join(GOLDEN, "divergent_canal.cgns"), tmp_path / "manning_Canal_divergent.cgns")
+    print("fortran cgns case: divergent canal, ts=5", flush=True)
+    proc = subprocess.run([FORTRAN_EXE], cwd=str(tmp_path), capture_output=True, text=True, timeout=120)
+    assert proc.returncode == 0, proc.stdout + proc.stderr
+    assert "MANNING DETECTED IN MESH FILE BUT WILL NOT BE USED" in proc.stdout
+    mesh = M.read_text_mesh(os.path.join(GOLDEN, "divergent_canal.mesh"))
+    p, prep, ref = _oracle(mesh, str(tmp_path))
+    nt = int(proc.stdout.split("Loop on time ends at iteration :")[1].split()[0])
+    assert nt == ref["steps"] > 100
+    lines = open(tmp_path / "solution_elements_restart.txt").read().splitlines()
+    assert lines[1:] == [cio._fmt(row) for row in ref["U"]]
+    bt = open(tmp_path / "boundary_table_manning_Canal_divergent.cgns").read()
+    assert bt == open(os.path.join(GOLDEN, "divergent_canal.boundary_table")).read()
+    h5dump = shutil.which("h5dump") or "/opt/conda/bin/h5dump"
+    if not os.path.exists(h5dump):
+        return
+    out = tmp_path / "out_manning_Canal_divergent.cgns"
+    U0 = initial_state(p, mesh, prep.zm)
+    assert np.array_equal(_h5_field(out, "/Base/Zone/Solution_0/Water Depth/ data"), U0[:, 0])
+    assert np.array_equal(_h5_field(out, "/Base/Zone/Solution_4/Water Depth/ data"), ref["U"][:, 0])
+    assert np.array_equal(_h5_field(out, "/Base/Zone/Solution_4/Bathymetry/ data"), prep.zm)
