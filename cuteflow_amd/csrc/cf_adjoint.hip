@@ -954,6 +954,9 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
 
 extern "C" int cf_adj_backward(cf_handle *h) {
     if (!h || !h->adj || !h->adj->recorded || h->pending >= 0) return CF_ERR_ARG;
+    // the reverse sweep covers iflux 1-3; MUSCL (iflux 4) has the
+    // tangent-linear model only
+    if (h->iflux == 4) return CF_ERR_ARG;
     AdjState *A = h->adj;
     int rc = run_captured(h, A, 1, enqueue_backward);
     if (rc) return rc;
@@ -1012,9 +1015,10 @@ static void launch_tlm_t(const TlmArgs &t, int grid, hipStream_t s) {
 static void enqueue_tlm(cf_handle *h, AdjState *A, hipStream_t s) {
     const size_t N = (size_t)h->ncell;
     typedef void (*fn)(const TlmArgs &, int, hipStream_t);
-    static const fn tab[3][4] = {{launch_tlm_t<1, 0>, launch_tlm_t<1, 1>, launch_tlm_t<1, 2>, launch_tlm_t<1, 3>},
+    static const fn tab[4][4] = {{launch_tlm_t<1, 0>, launch_tlm_t<1, 1>, launch_tlm_t<1, 2>, launch_tlm_t<1, 3>},
                                  {launch_tlm_t<2, 0>, launch_tlm_t<2, 1>, launch_tlm_t<2, 2>, launch_tlm_t<2, 3>},
-                                 {launch_tlm_t<3, 0>, launch_tlm_t<3, 1>, launch_tlm_t<3, 2>, launch_tlm_t<3, 3>}};
+                                 {launch_tlm_t<3, 0>, launch_tlm_t<3, 1>, launch_tlm_t<3, 2>, launch_tlm_t<3, 3>},
+                                 {launch_tlm_t<4, 0>, launch_tlm_t<4, 1>, launch_tlm_t<4, 2>, launch_tlm_t<4, 3>}};
     const fn L = tab[h->iflux - 1][h->fric];
     // start: the recorded initial state and the direction's initial tangent
     hipLaunchKernelGGL(copy_d4_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, A->ckpt, A->tst[0], N);

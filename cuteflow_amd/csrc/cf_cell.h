@@ -180,7 +180,7 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
         unb[j] = cc >= 0 ? ld.state(cc) : u0;
     }
     D4 dfi = {0.0, 0.0, 0.0, 0.0};
-    if (IFLUX == 3) dfi = ld.dfc(i);
+    if (IFLUX >= 3) dfi = ld.dfc(i);
     T fl[3][3];
     double fn1[3], fn2[3], fln[3];
 #pragma unroll
@@ -243,12 +243,16 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
                 else riadh_flux(g, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr, flux);
             } else {
                 // reconstructed depths of flux_riemann.cuf:371-394 (the
-                // values solver_Riemann_waf receives)
+                // values solver_Riemann_waf / _muscl receive)
                 T hjiet = dmax(T(0.0), (hautr + zmr) - zij);
                 if (hijet < tls) { hijet = 0.0; vitul = 0.0; vitvl = 0.0; }
                 if (hjiet < tls) { hjiet = 0.0; vitur = 0.0; vitvr = 0.0; }
-                if (hijet < kEpsWaf && hjiet < kEpsWaf) {
-                    zero_flux(n1, n2, flux);  // dry/dry: no upwind data needed
+                // dry/dry: no upwind data needed.  WAF: the solver's own
+                // test; MUSCL: both depths zero reconstruct and evolve to zero
+                const bool dry = IFLUX == 3 ? (hijet < kEpsWaf && hjiet < kEpsWaf)
+                                            : (hijet <= 0.0 && hjiet <= 0.0);
+                if (dry) {
+                    zero_flux(n1, n2, flux);
                 } else {
                     const double d_mrl = d4_at(dfi, j);
                     const int ja = (j + 1) % 3, jb = (j + 2) % 3;
@@ -267,41 +271,43 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
                     T g1h, g1u, g1w, g2h, g2u, g2w;
                     rot(tls, n1, n2, g1.h, g1.hu, g1.hv, g1h, g1u, g1w);
                     rot(tls, n1, n2, g2.h, g2.hu, g2.hv, g2h, g2u, g2w);
-                    T hl_upw, vl_upw, hr_upw, vr_upw;
+                    T hl_upw, vl_upw, hr_upw, vr_upw, ul_upw, ur_upw;  // u_upw: MUSCL only
                     double zml_upw, zmr_upw, d_mkvol, d_mkvoisin;
                     if ((own1 && own2) || (kw1 == kv && kw2 == kv)) {
-                        hl_upw = hautl; vl_upw = vitvl;
-                        hr_upw = hautr; vr_upw = vitvr;
+                        hl_upw = hautl; ul_upw = vitul; vl_upw = vitvl;
+                        hr_upw = hautr; ur_upw = vitur; vr_upw = vitvr;
                         d_mkvol = d_mrl; d_mkvoisin = d_mrl;
                         zml_upw = zml; zmr_upw = zmr;
                     } else {
                         if (own1) {
-                            hl_upw = h2; vl_upw = w2;
+                            hl_upw = h2; ul_upw = u2; vl_upw = w2;
                             d_mkvol = d4_at(dfi, jb);
                             zml_upw = s2.zm;
                         } else if (own2) {
-                            hl_upw = h1; vl_upw = w1;
+                            hl_upw = h1; ul_upw = u1; vl_upw = w1;
                             d_mkvol = d4_at(dfi, ja);
                             zml_upw = s1.zm;
                         } else {
                             const double d1 = d4_at(dfi, ja), d2 = d4_at(dfi, jb);
                             hl_upw = (d1 * h1 + d2 * h2) / (d1 + d2);
+                            ul_upw = (d1 * u1 + d2 * u2) / (d1 + d2);
                             vl_upw = (d1 * w1 + d2 * w2) / (d1 + d2);
                             zml_upw = (d1 * s1.zm + d2 * s2.zm) / (d1 + d2);
                             d_mkvol = (d1 + d2) / 2.0;
                         }
                         const D4 dfk = (kv == i) ? dfi : ld.dfc(kv);
                         if (kw1 == kv) {
-                            hr_upw = g2h; vr_upw = g2w;
+                            hr_upw = g2h; ur_upw = g2u; vr_upw = g2w;
                             d_mkvoisin = d4_at(dfk, kb);
                             zmr_upw = g2.zm;
                         } else if (kw2 == kv) {
-                            hr_upw = g1h; vr_upw = g1w;
+                            hr_upw = g1h; ur_upw = g1u; vr_upw = g1w;
                             d_mkvoisin = d4_at(dfk, ka);
                             zmr_upw = g1.zm;
                         } else {
                             const double d1 = d4_at(dfk, ka), d2 = d4_at(dfk, kb);
                             hr_upw = (d1 * g1h + d2 * g2h) / (d1 + d2);
+                            ur_upw = (d1 * g1u + d2 * g2u) / (d1 + d2);
                             vr_upw = (d1 * g1w + d2 * g2w) / (d1 + d2);
                             zmr_upw = (d1 * g1.zm + d2 * g2.zm) / (d1 + d2);
                             d_mkvoisin = (d1 + d2) / 2.0;
@@ -310,11 +316,16 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
                     // upwind hydrostatic reconstruction (flux_riemann.cuf:373,376,384-400)
                     hl_upw = dmax(T(0.0), (hl_upw + zml_upw) - zij);
                     hr_upw = dmax(T(0.0), (hr_upw + zmr_upw) - zij);
-                    if (hl_upw < tls) { hl_upw = 0.0; vl_upw = 0.0; }
-                    if (hr_upw < tls) { hr_upw = 0.0; vr_upw = 0.0; }
-                    waf_flux(g, p.ilimiteur, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr,
-                             hl_upw, vl_upw, hr_upw, vr_upw, d_mrl, d_mkvol, d_mkvoisin, dt, flux,
-                             (p.ad_flags & 1) != 0);
+                    if (hl_upw < tls) { hl_upw = 0.0; ul_upw = 0.0; vl_upw = 0.0; }
+                    if (hr_upw < tls) { hr_upw = 0.0; ur_upw = 0.0; vr_upw = 0.0; }
+                    if (IFLUX == 3)
+                        waf_flux(g, p.ilimiteur, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr,
+                                 hl_upw, vl_upw, hr_upw, vr_upw, d_mrl, d_mkvol, d_mkvoisin, dt, flux,
+                                 (p.ad_flags & 1) != 0);
+                    else
+                        muscl_flux(g, p.ilimiteur, tls, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr,
+                                   hl_upw, ul_upw, vl_upw, hr_upw, ur_upw, vr_upw, d_mrl, d_mkvol, d_mkvoisin,
+                                   dt, flux);
                 }
             }
         }

@@ -25,6 +25,7 @@ constexpr double kEpsHllc = (double)1e-8f;   // flux_riemann.cuf:926
 constexpr double kEpsRiadh = (double)1e-6f;  // flux_riemann.cuf:1051
 constexpr double kQSteady = (double)1e-8f;   // finite_volumes_method.cuf:379
 constexpr double kDtLow = (double)1e-12f;    // cfl_condition.cuf:129
+constexpr double kEpsLeer = (double)1e-12f;  // flux_riemann.cuf:453
 constexpr double kCote = (double)1.8f;       // maillage.cuf:157
 
 template <class T>
@@ -343,6 +344,90 @@ CF_HD inline void waf_flux(double gp, int ilim, double n1, double n2, T hl, T ul
     flux[0] = fw0;
     flux[1] = n1 * fw1 - n2 * fw2;
     flux[2] = n2 * fw1 + n1 * fw2;
+}
+
+// simple_limiter (flux_riemann.cuf:430-462): the limited slope of the MUSCL
+// reconstruction from the one-sided slopes r1 (upwind) and r2 (across the
+// face).  ilimiteur 0: none (first order), 1: superbee, 2: van Leer,
+// 3: minmod; the reference leaves the slope unset for other values, taken
+// as 0 (first order) here.
+template <class T>
+CF_HD inline T slope_limiter(int ilim, T r1, T r2) {
+    switch (ilim) {
+    case 1:
+        if (r2 > 0.0) return dmax(dmax(T(0.0), dmin(2.0 * r1, r2)), dmin(r1, 2.0 * r2));
+        return dmin(dmin(T(0.0), dmax(2.0 * r1, r2)), dmax(r1, 2.0 * r2));
+    case 2: return (r1 * r2 + vabs(r1 * r2)) / ((r1 + r2) + kEpsLeer);
+    case 3:
+        if (r2 > 0.0) return dmax(T(0.0), dmin(r1, r2));
+        return dmin(T(0.0), dmax(r1, r2));
+    default: return 0.0;
+    }
+}
+
+// solver_Riemann_muscl (flux_riemann.cuf:526-718), iflux 4: MUSCL-Hancock.
+// Each side's cell is reconstructed to its two face values along the face
+// normal (limited slope from its upwind value and the other side; zero
+// velocities when either reconstructed depth is below tolisec), evolved
+// half a step with the difference of its face fluxes, and the evolved
+// inner values go to the HLLC solver with eps = 1e-16f.  The evolution of
+// the velocity components with momentum-flux differences is the
+// reference's.  Both sides dry after the evolution: zero flux.
+template <class T>
+CF_HD inline void muscl_flux(double gp, int ilim, double tolisec, double n1, double n2, T hl, T ul, T vl, T hr,
+                             T ur, T vr, T hl_upw, T ul_upw, T vl_upw, T hr_upw, T ur_upw, T vr_upw,
+                             double d_mrl, double d_mkvol, double d_mkvoisin, double dt, T flux[3]) {
+    const double dtoverdx = dt / d_mrl;
+    // data reconstruction I, left cell (:560-584)
+    T hll, hlr, ull, ulr, vll, vlr;
+    {
+        const T dh = slope_limiter(ilim, (hl - hl_upw) / d_mkvol, (hr - hl) / d_mrl);
+        hll = dmax(T(0.0), hl - (0.5 * dh) * d_mrl);
+        hlr = dmax(T(0.0), hl + (0.5 * dh) * d_mrl);
+        if (hlr < tolisec || hll < tolisec) {
+            ull = ulr = vll = vlr = 0.0;
+        } else {
+            const T du = slope_limiter(ilim, (ul - ul_upw) / d_mkvol, (ur - ul) / d_mrl);
+            const T dv = slope_limiter(ilim, (vl - vl_upw) / d_mkvol, (vr - vl) / d_mrl);
+            ull = ul - (0.5 * du) * d_mrl;
+            vll = vl - (0.5 * dv) * d_mrl;
+            ulr = ul + (0.5 * du) * d_mrl;
+            vlr = vl + (0.5 * dv) * d_mrl;
+        }
+    }
+    // right cell (:586-610)
+    T hrl, hrr, url, urr, vrl, vrr;
+    {
+        const T dh = slope_limiter(ilim, (hr - hl) / d_mrl, (hr_upw - hr) / d_mkvoisin);
+        hrr = dmax(T(0.0), hr + (0.5 * dh) * d_mrl);
+        hrl = dmax(T(0.0), hr - (0.5 * dh) * d_mrl);
+        if (hrl < tolisec || hrr < tolisec) {
+            url = urr = vrl = vrr = 0.0;
+        } else {
+            const T du = slope_limiter(ilim, (ur - ul) / d_mrl, (ur_upw - ur) / d_mkvoisin);
+            const T dv = slope_limiter(ilim, (vr - vl) / d_mrl, (vr_upw - vr) / d_mkvoisin);
+            urr = ur + (0.5 * du) * d_mrl;
+            vrr = vr + (0.5 * dv) * d_mrl;
+            url = ur - (0.5 * du) * d_mrl;
+            vrl = vr - (0.5 * dv) * d_mrl;
+        }
+    }
+    // data reconstruction II: half-step evolution (:613-620)
+    const double hdt = 0.5 * dtoverdx, hg = 0.5 * gp;
+    const T eh = hlr + hdt * (hll * ull - hlr * ulr);
+    const T eu = ulr + hdt * ((hll * (ull * ull) + hg * (hll * hll)) - (hlr * (ulr * ulr) + hg * (hlr * hlr)));
+    const T ev = vlr + hdt * ((hll * ull) * vll - (hlr * ulr) * vlr);
+    const T fh = hrl + hdt * (hrl * url - hrr * urr);
+    const T fu = url + hdt * ((hrl * (url * url) + hg * (hrl * hrl)) - (hrr * (urr * urr) + hg * (hrr * hrr)));
+    const T fv = vrl + hdt * ((hrl * url) * vrl - (hrr * urr) * vrr);
+    HllcT<T> o;
+    T fh0 = 0.0, fh1 = 0.0, fh2 = 0.0;
+    if (hllc_core(gp, kEpsWaf, eh, eu, ev, fh, fu, fv, o)) {
+        fh0 = o.fh[0]; fh1 = o.fh[1]; fh2 = o.fh[2];
+    }
+    flux[0] = fh0;
+    flux[1] = n1 * fh1 - n2 * fh2;
+    flux[2] = n2 * fh1 + n1 * fh2;
 }
 
 // Rotation to the face frame with the cell dry threshold

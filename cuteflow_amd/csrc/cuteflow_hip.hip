@@ -532,7 +532,7 @@ extern "C" const char *cf_error_string(int s) {
 
 static int validate_params(const cf_params *p) {
     if (!(p->gp > 0.0) || !(p->cfl > 0.0) || !(p->tolisec >= 0.0)) return CF_ERR_ARG;
-    if (p->iflux < 1 || p->iflux > 3) return CF_ERR_ARG;  // iflux 4 (MUSCL) not provided
+    if (p->iflux < 1 || p->iflux > 4) return CF_ERR_ARG;
     if (p->inlet != 1 && p->inlet != 2) return CF_ERR_ARG;  // 'phys_inflow' not provided
     if (p->outlet != 1 && p->outlet != 2) return CF_ERR_ARG;
     if (p->friction != 0 && p->friction != 1) return CF_ERR_ARG;
@@ -837,10 +837,11 @@ static void launch_step_t(const StepArgs &a, int grid, hipStream_t s) {
 typedef void (*step_launcher)(const StepArgs &, int, hipStream_t);
 
 static step_launcher pick_launcher(int iflux, int fric) {
-    static const step_launcher tab[3][4] = {
+    static const step_launcher tab[4][4] = {
         {launch_step_t<1, 0>, launch_step_t<1, 1>, launch_step_t<1, 2>, launch_step_t<1, 3>},
         {launch_step_t<2, 0>, launch_step_t<2, 1>, launch_step_t<2, 2>, launch_step_t<2, 3>},
-        {launch_step_t<3, 0>, launch_step_t<3, 1>, launch_step_t<3, 2>, launch_step_t<3, 3>}};
+        {launch_step_t<3, 0>, launch_step_t<3, 1>, launch_step_t<3, 2>, launch_step_t<3, 3>},
+        {launch_step_t<4, 0>, launch_step_t<4, 1>, launch_step_t<4, 2>, launch_step_t<4, 3>}};
     return tab[iflux - 1][fric];
 }
 

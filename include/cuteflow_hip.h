@@ -56,8 +56,8 @@ typedef struct cf_params {
     double constant_dt;      /* used when is_dt_constant == 1 */
     double override_manning; /* used when is_override_manning == 1 */
     double tol_reg_perm;     /* steady-state tolerance: > 0 stops a run at steady state (see cf_set_discharge) */
-    int iflux;               /* 1 = HLLC, 3 = WAF (Loukili & Soulaimani) */
-    int ilimiteur;           /* WAF limiter 0..4 */
+    int iflux;               /* 1 = HLLC, 2 = Riadh's state HLLC, 3 = WAF (Loukili & Soulaimani), 4 = MUSCL-Hancock */
+    int ilimiteur;           /* WAF limiter 0..4; MUSCL slope limiter 0 none, 1 superbee, 2 van Leer, 3 minmod */
     int friction;            /* 0/1 */
     int fricimplic;          /* 0 explicit, 1 (I-dt/2 J)^-1, 2 (I-dt B)^-1 */
     int inlet;               /* 1 'transm', 2 'inflow' */
@@ -264,7 +264,8 @@ int cf_adj_get_samples(cf_handle *h, int inst, double *eta, double *hmax);
  * hmax (nelt) and the final state (3*nelt); NULL = 0. */
 int cf_adj_set_seed(cf_handle *h, int inst, const double *g_eta, const double *g_hmax, const double *g_state);
 /* Reverse sweep over the recorded run: segments of checkpoint_every steps
- * replayed from their checkpoints, then reversed step by step. */
+ * replayed from their checkpoints, then reversed step by step.  iflux 1-3;
+ * CF_ERR_ARG for MUSCL (iflux 4), which has the tangent-linear model only. */
 int cf_adj_backward(cf_handle *h);
 /* Gradient of instance inst with respect to per-cell Manning n (nelt) and
  * to the initial state (3*nelt); either may be NULL. */

@@ -43,7 +43,7 @@
 #define LIT_1EM16 ((double)1e-16f) /* flux_riemann.cuf:749 */
 #define LIT_1EM8 ((double)1e-8f)   /* flux_riemann.cuf:926 */
 #define LIT_1EM6 ((double)1e-6f)   /* flux_riemann.cuf:1051 */
-#define LIT_1EM12 ((double)1e-12f) /* cfl_condition.cuf:129 */
+#define LIT_1EM12 ((double)1e-12f) /* cfl_condition.cuf:129, flux_riemann.cuf:453 */
 #define LIT_1P8 ((double)1.8f)     /* maillage.cuf:157 */
 
 static inline double dmax(double a, double b) { return a > b ? a : b; }
@@ -579,6 +579,82 @@ static void or_waf(const or_params *p, double n1, double n2,
     flux[2] = n2 * fw[1] + n1 * fw[2];
 }
 
+/* simple_limiter (flux_riemann.cuf:430-462): the MUSCL slope from the
+ * upwind slope r1 and the slope across the face r2.  The reference leaves
+ * the slope unset for ilimiteur outside 0..3: 0 here. */
+static double or_slope_limiter(int ilim, double r1, double r2)
+{
+    switch (ilim) {
+    case 1:  /* superbee */
+        if (r2 > 0.0) return dmax(dmax(0.0, dmin(2.0 * r1, r2)), dmin(r1, 2.0 * r2));
+        return dmin(dmin(0.0, dmax(2.0 * r1, r2)), dmax(r1, 2.0 * r2));
+    case 2:  /* van Leer */
+        return (r1 * r2 + fabs(r1 * r2)) / ((r1 + r2) + LIT_1EM12);
+    case 3:  /* minmod */
+        if (r2 > 0.0) return dmax(0.0, dmin(r1, r2));
+        return dmin(0.0, dmax(r1, r2));
+    default:
+        return 0.0;
+    }
+}
+
+/* solver_Riemann_muscl (flux_riemann.cuf:526-718), iflux 4: limited linear
+ * reconstruction of each side to its two face values (:560-610), half-step
+ * evolution with the difference of the face fluxes (:613-620, velocities
+ * evolved with momentum-flux differences as written there), then the HLLC
+ * solver with eps = 1e-16f on the evolved inner values (:622-717). */
+static void or_muscl(const or_params *p, double n1, double n2,
+                     double hl, double ul, double vl, double hr, double ur, double vr,
+                     double hl_upw, double ul_upw, double vl_upw,
+                     double hr_upw, double ur_upw, double vr_upw,
+                     double d_mrl, double d_mkvol, double d_mkvoisin, double dt, double flux[3])
+{
+    const double tls = p->tolisec, gp = p->gp;
+    const int il = p->ilimiteur;
+    double dtoverdx = dt / d_mrl;
+    double d, hll, hlr, ull, ulr, vll, vlr, hrl, hrr, url, urr, vrl, vrr;
+    double eh, eu, ev, fhh, fu, fv, f1[3], f2[3], fh[3] = {0.0, 0.0, 0.0}, sl, sr, set;
+
+    d = or_slope_limiter(il, (hl - hl_upw) / d_mkvol, (hr - hl) / d_mrl);
+    hll = dmax(0.0, hl - (0.5 * d) * d_mrl);
+    hlr = dmax(0.0, hl + (0.5 * d) * d_mrl);
+    if (hlr < tls || hll < tls) {
+        ulr = 0.0; vlr = 0.0; ull = 0.0; vll = 0.0;
+    } else {
+        double du = or_slope_limiter(il, (ul - ul_upw) / d_mkvol, (ur - ul) / d_mrl);
+        double dv = or_slope_limiter(il, (vl - vl_upw) / d_mkvol, (vr - vl) / d_mrl);
+        ull = ul - (0.5 * du) * d_mrl;
+        vll = vl - (0.5 * dv) * d_mrl;
+        ulr = ul + (0.5 * du) * d_mrl;
+        vlr = vl + (0.5 * dv) * d_mrl;
+    }
+    d = or_slope_limiter(il, (hr - hl) / d_mrl, (hr_upw - hr) / d_mkvoisin);
+    hrr = dmax(0.0, hr + (0.5 * d) * d_mrl);
+    hrl = dmax(0.0, hr - (0.5 * d) * d_mrl);
+    if (hrl < tls || hrr < tls) {
+        url = 0.0; vrl = 0.0; urr = 0.0; vrr = 0.0;
+    } else {
+        double du = or_slope_limiter(il, (ur - ul) / d_mrl, (ur_upw - ur) / d_mkvoisin);
+        double dv = or_slope_limiter(il, (vr - vl) / d_mrl, (vr_upw - vr) / d_mkvoisin);
+        urr = ur + (0.5 * du) * d_mrl;
+        vrr = vr + (0.5 * dv) * d_mrl;
+        url = ur - (0.5 * du) * d_mrl;
+        vrl = vr - (0.5 * dv) * d_mrl;
+    }
+    eh = hlr + (0.5 * dtoverdx) * (hll * ull - hlr * ulr);
+    eu = ulr + (0.5 * dtoverdx) * ((hll * (ull * ull) + (0.5 * gp) * (hll * hll)) -
+                                   (hlr * (ulr * ulr) + (0.5 * gp) * (hlr * hlr)));
+    ev = vlr + (0.5 * dtoverdx) * ((hll * ull) * vll - (hlr * ulr) * vlr);
+    fhh = hrl + (0.5 * dtoverdx) * (hrl * url - hrr * urr);
+    fu = url + (0.5 * dtoverdx) * ((hrl * (url * url) + (0.5 * gp) * (hrl * hrl)) -
+                                   (hrr * (urr * urr) + (0.5 * gp) * (hrr * hrr)));
+    fv = vrl + (0.5 * dtoverdx) * ((hrl * url) * vrl - (hrr * urr) * vrr);
+    or_hllc_core(gp, LIT_1EM16, eh, eu, ev, fhh, fu, fv, f1, f2, fh, &sl, &sr, &set);
+    flux[0] = fh[0];
+    flux[1] = n1 * fh[1] - n2 * fh[2];
+    flux[2] = n2 * fh[1] + n1 * fh[2];
+}
+
 /* ------------------------------------------------------------------------ */
 /* Per-step kernels                                                          */
 
@@ -701,7 +777,7 @@ static void or_face(const or_params *p, const int *conn, const double *vx,
             if (hijet < tls) { hijet = 0.0; vitul = 0.0; vitvl = 0.0; }
             if (hjiet < tls) { hjiet = 0.0; vitur = 0.0; vitvr = 0.0; }
             or_riadh(g, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr, flux);
-        } else if (p->iflux == 3) {
+        } else if (p->iflux == 3 || p->iflux == 4) {  /* shared stencil (:229-394) */
             int kv1 = upw[6 * gi + 2 * tj] - 1, kv2 = upw[6 * gi + 2 * tj + 1] - 1;
             int A = conn[3 * gi + tj], B = conn[3 * gi + (tj + 1) % 3];
             int c1 = conn[3 * kv], c2 = conn[3 * kv + 1], c3 = conn[3 * kv + 2];
@@ -768,9 +844,12 @@ static void or_face(const or_params *p, const int *conn, const double *vx,
             if (hl_upw < tls) { hl_upw = 0.0; ul_upw = 0.0; vl_upw = 0.0; }
             if (hjiet < tls) { hjiet = 0.0; vitur = 0.0; vitvr = 0.0; }
             if (hr_upw < tls) { hr_upw = 0.0; ur_upw = 0.0; vr_upw = 0.0; }
-            (void)ul_upw; (void)ur_upw;
-            or_waf(p, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr,
-                   hl_upw, vl_upw, hr_upw, vr_upw, d_mrl, d_mkvol, d_mkvoisin, dt, flux);
+            if (p->iflux == 3)
+                or_waf(p, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr,
+                       hl_upw, vl_upw, hr_upw, vr_upw, d_mrl, d_mkvol, d_mkvoisin, dt, flux);
+            else
+                or_muscl(p, n1, n2, hijet, vitul, vitvl, hjiet, vitur, vitvr, hl_upw, ul_upw, vl_upw,
+                         hr_upw, ur_upw, vr_upw, d_mrl, d_mkvol, d_mkvoisin, dt, flux);
         }
     }
     fl[0] = flux[0] * ln;
@@ -1012,4 +1091,24 @@ void or_riemann_face(int iflux, int ilimiteur, double gp, double n1, double n2,
 double or_limiter_value(int ilimiteur, double r, double c)
 {
     return or_limiter(ilimiteur, r, c);
+}
+
+double or_slope_limiter_value(int ilimiteur, double r1, double r2)
+{
+    return or_slope_limiter(ilimiteur, r1, r2);
+}
+
+void or_muscl_face(int ilimiteur, double gp, double tolisec, double n1, double n2,
+                   double hl, double ul, double vl, double hr, double ur, double vr,
+                   double hl_upw, double ul_upw, double vl_upw,
+                   double hr_upw, double ur_upw, double vr_upw,
+                   double d_mrl, double d_mkvol, double d_mkvoisin, double dt, double flux[3])
+{
+    or_params p;
+    memset(&p, 0, sizeof(p));
+    p.gp = gp;
+    p.tolisec = tolisec;
+    p.ilimiteur = ilimiteur;
+    or_muscl(&p, n1, n2, hl, ul, vl, hr, ur, vr, hl_upw, ul_upw, vl_upw, hr_upw, ur_upw, vr_upw,
+             d_mrl, d_mkvol, d_mkvoisin, dt, flux);
 }

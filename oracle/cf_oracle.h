@@ -119,6 +119,13 @@ void or_riemann_face(int iflux, int ilimiteur, double gp, double n1, double n2,
                      double d_mrl, double d_mkvol, double d_mkvoisin, double dt,
                      double flux[3]);
 double or_limiter_value(int ilimiteur, double r, double c);
+double or_slope_limiter_value(int ilimiteur, double r1, double r2);
+/* solver_Riemann_muscl (iflux 4) with its upwind values, global frame */
+void or_muscl_face(int ilimiteur, double gp, double tolisec, double n1, double n2,
+                   double hl, double ul, double vl, double hr, double ur, double vr,
+                   double hl_upw, double ul_upw, double vl_upw,
+                   double hr_upw, double ur_upw, double vr_upw,
+                   double d_mrl, double d_mkvol, double d_mkvoisin, double dt, double flux[3]);
 
 #ifdef __cplusplus
 }

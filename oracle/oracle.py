@@ -65,6 +65,10 @@ def load():
         lib.or_riemann_face.argtypes = [C.c_int, C.c_int] + [C.c_double] * 17 + [_D]
         lib.or_limiter_value.restype = C.c_double
         lib.or_limiter_value.argtypes = [C.c_int, C.c_double, C.c_double]
+        lib.or_slope_limiter_value.restype = C.c_double
+        lib.or_slope_limiter_value.argtypes = [C.c_int, C.c_double, C.c_double]
+        lib.or_muscl_face.restype = None
+        lib.or_muscl_face.argtypes = [C.c_int] + [C.c_double] * 20 + [_D]
         _lib = lib
     return _lib
 
@@ -187,6 +191,21 @@ def riemann(iflux, left, right, n=(1.0, 0.0), gp=9.81, ilimiteur=1, upw=(0.0, 0.
 
 def limiter(ilimiteur, r, c):
     return load().or_limiter_value(int(ilimiteur), float(r), float(c))
+
+
+def slope_limiter(ilimiteur, r1, r2):
+    """simple_limiter: MUSCL slope from the upwind slope r1 and the slope r2 across the face."""
+    return load().or_slope_limiter_value(int(ilimiteur), float(r1), float(r2))
+
+
+def muscl(left, right, upw_left, upw_right, n=(1.0, 0.0), gp=9.81, ilimiteur=1, tolisec=1e-5,
+          d=(1.0, 1.0, 1.0), dt=0.0):
+    """MUSCL-Hancock face flux (global frame): face-frame states left/right=(h,un,ut), their
+    upwind values upw_left/upw_right=(h,un,ut), d=(d_mrl, d_mkvol, d_mkvoisin)."""
+    out = np.zeros(3)
+    load().or_muscl_face(int(ilimiteur), gp, tolisec, n[0], n[1], *left, *right, *upw_left, *upw_right, *d,
+                         dt, _d(out))
+    return out
 
 
 def inflow_q(params, inlet, tc):

@@ -165,6 +165,54 @@ def test_gpu_zonal_gradient_matches_tlm_and_central_differences(iflux):
     assert np.all(best <= tol), (grad, fd)
 
 
+def test_gpu_muscl_tlm_matches_central_differences():
+    """MUSCL (iflux 4) has the tangent-linear model (the forward body run on
+    dual numbers) but no reverse sweep: the zonal directional derivatives of
+    a gauge misfit equal central differences of the forward with the
+    recorded dt (best of the steps 1e-6, 1e-7, 1e-8: the slope limiters are
+    piecewise), and cf_adj_backward is refused."""
+    from cuteflow_amd import _lib
+
+    m, p, U0 = river_case(40, 10, seed=41, iflux=4, ilimiteur=3)
+    nz, nsteps = 4, 80
+    xc = m.centroids()[0]
+    zone = np.minimum((xc / xc.max() * nz).astype(np.int64), nz - 1)
+    Zn = np.zeros((m.nelt, nz))
+    Zn[np.arange(m.nelt), zone] = 1.0
+    g = _gauges(m, 10, 2)
+    nz0 = np.array([0.03, 0.045, 0.06, 0.04])
+    with Solver(m, p) as s:
+        run = AdjointRun(s, g, sample_every=10, checkpoint_every=32)
+        obs = run.forward(nsteps, U0=[U0], manning=[Zn @ (nz0 * 0.8)])[0][0].reshape(-1)
+        run.forward(nsteps, U0=[U0], manning=[Zn @ nz0])
+        with pytest.raises(_lib.CuteflowError):
+            run.backward([np.ones((run.nsamp, len(g)))])
+
+        def loss_at(nzp):
+            run.set_manning([Zn @ nzp])
+            e = run.tlm()[0]["eta"].reshape(-1)
+            return float(np.sum((e - obs) ** 2))
+
+        tlm = np.zeros(nz)
+        run.set_manning([Zn @ nz0])
+        for z in range(nz):
+            t = run.tlm([Zn[:, z].copy()])[0]
+            tlm[z] = np.sum(2.0 * (t["eta"].reshape(-1) - obs) * t["d_eta"].reshape(-1))
+        steps = (1e-6, 1e-7, 1e-8)
+        fd = np.zeros((len(steps), nz))
+        for k, eps in enumerate(steps):
+            for z in range(nz):
+                up, dn = nz0.copy(), nz0.copy()
+                up[z] *= 1.0 + eps
+                dn[z] *= 1.0 - eps
+                fd[k, z] = (loss_at(up) - loss_at(dn)) / (2.0 * eps * nz0[z])
+    print("tlm", tlm, "fd", fd, flush=True)
+    assert np.any(tlm != 0.0)
+    tol = 1e-6 * np.abs(tlm) + 1e-9 * np.max(np.abs(tlm))
+    best = np.min(np.abs(fd - tlm[None, :]), axis=0)
+    assert np.all(best <= tol), (tlm, fd)
+
+
 def test_gpu_batch_gradients_equal_single_runs():
     cases = [river_case(30, 8, seed=51), river_case(20, 6, seed=52, q_peak=300.0), river_case(36, 8, seed=53)]
     gauges = [_gauges(c[0], 6, 60 + k) for k, c in enumerate(cases)]

@@ -244,6 +244,52 @@ def test_discharge_and_steady_state(options):
         assert s.step(10)[1] == 10
 
 
+@pytest.mark.parametrize("ilim", [0, 1, 2, 3, 4])
+def test_muscl_limiters(ilim):
+    """iflux 4 (solver_Riemann_muscl) on the wetting reach with semi-implicit
+    friction, every slope limiter (4 is unset in the reference: first order)."""
+    m, p, U0 = river_case(iflux=4, ilimiteur=ilim, nx=30, ny=9)
+    ref, U, hist, done, st, _, _ = _run_both(m, p, U0, 150)
+    assert st == 0 and ref["status"] == 0 and done == 150
+    _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
+
+
+@pytest.mark.parametrize("fricimplic", [0, 2])
+def test_muscl_friction_modes(fricimplic):
+    if fricimplic == 0:
+        m, p, U0 = dam_case(24)
+        p.iflux, p.ilimiteur, p.friction, p.fricimplic = 4, 3, 1, 0
+        m.manning[:] = 0.03
+    else:
+        m, p, U0 = river_case(iflux=4, ilimiteur=1, fricimplic=fricimplic)
+    ref, U, hist, done, st, _, _ = _run_both(m, p, U0, 300)
+    assert st == 0 and ref["status"] == 0 and done == 300
+    _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
+
+
+@pytest.mark.parametrize("n", [20, 60])
+def test_dam_break_muscl(n):
+    m, p, U0 = dam_case(n)
+    p.iflux, p.ilimiteur = 4, 1
+    ref, U, hist, done, st, _, tc = _run_both(m, p, U0, 400)
+    assert st == 0 and done == 400
+    _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
+
+
+@pytest.mark.parametrize("options", [2, 8 | 16, 4 | 1 | 16])
+def test_muscl_layout_options_identical(options):
+    """MUSCL under the tile sizes, dry-tile skipping and tile orders."""
+    m, p, U0 = river_case(nx=120, ny=24, q_peak=2000.0, t_ramp=30.0, iflux=4, ilimiteur=3)
+    prep = O.Prepared(m)
+    ref = O.run(prep, p, U0, 600)
+    with Solver(m, p, options=options) as s:
+        s.set_state(U0)
+        hist, done, st = s.step(600)
+        U = s.get_state()
+    assert st == 0 and done == 600
+    _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
+
+
 @pytest.mark.parametrize("n", [20, 60])
 def test_dam_break_riadh(n):
     m, p, U0 = dam_case(n)
@@ -292,7 +338,7 @@ def test_reordering_invariance():
         _compare(U, ref["U"], hist, ref["dt_hist"], p.tolisec)
 
 
-@pytest.mark.parametrize("iflux", [1, 2, 3])
+@pytest.mark.parametrize("iflux", [1, 2, 3, 4])
 def test_lake_at_rest(iflux):
     m, p = lake_case(iflux=iflux)
     prep = O.Prepared(m)

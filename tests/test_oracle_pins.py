@@ -267,6 +267,131 @@ def test_limiter_closed_forms():
     assert O.limiter(0, 0.5, c) == 1.0 and O.limiter(7, 0.5, c) == 1.0
 
 
+def test_muscl_reduces_to_waf():
+    """ilimiteur = 0 gives a zero MUSCL slope: the face values are the cell
+    values, the half-step evolution adds exactly 0, and the flux is the HLLC
+    solver with eps = 1e-16f, i.e. WAF with ilimiteur = 0, bit for bit."""
+    rng = np.random.default_rng(4)
+    for _ in range(300):
+        L, R, UL, UR = ((rng.uniform(0.1, 3), rng.uniform(-2, 2), rng.uniform(-1, 1)) for _ in range(4))
+        n = rng.normal(size=2)
+        n /= np.linalg.norm(n)
+        fm = O.muscl(L, R, UL, UR, n=tuple(n), ilimiteur=0, d=(2.0, 1.5, 2.5), dt=0.05)
+        fw = O.riemann(3, L, R, n=tuple(n), ilimiteur=0, upw=(UL[0], UL[2], UR[0], UR[2]), d=(2.0, 1.5, 2.5),
+                       dt=0.05)
+        assert np.array_equal(fm, fw)
+        assert np.allclose(fm, O.riemann(1, L, R, n=tuple(n)), rtol=1e-7, atol=1e-9)
+
+
+def test_slope_limiter_closed_forms():
+    """simple_limiter against the textbook slope limiters (Toro, Riemann
+    Solvers, ch. 13-14): superbee max(minmod(2a, b), minmod(a, 2b)) in sign,
+    van Leer's harmonic mean 2ab/(a+b), minmod; zero for slopes of opposite
+    sign or for ilimiteur 0 and unknown values."""
+    rng = np.random.default_rng(6)
+    for _ in range(500):
+        a, b = rng.normal(size=2) * rng.choice([1e-3, 1.0, 1e3])
+        same = a * b > 0
+        s = math.copysign(1.0, b)
+        sb = s * max(min(2 * abs(a), abs(b)), min(abs(a), 2 * abs(b))) if same else 0.0
+        mm = s * min(abs(a), abs(b)) if same else 0.0
+        vl = 2 * a * b / (a + b) if same else 0.0
+        assert O.slope_limiter(1, a, b) == sb
+        assert O.slope_limiter(3, a, b) == mm
+        # the reference's 1e-12f in the denominator shifts it by a relative 1e-12 / |a + b|
+        assert O.slope_limiter(2, a, b) == pytest.approx(vl, rel=2e-12 / abs(a + b) + 1e-15, abs=1e-300)
+        assert O.slope_limiter(0, a, b) == 0.0 and O.slope_limiter(4, a, b) == 0.0
+    # van Leer's 1e-12f guard: equal and opposite slopes give 0, not 0/0
+    assert O.slope_limiter(2, 1.0, -1.0) == 0.0
+
+
+def _phys(h, u, v, n, g=9.81):
+    e, t = h * u * u + g * h * h / 2.0, h * u * v
+    return np.array([h * u, n[0] * e - n[1] * t, n[1] * e + n[0] * t])
+
+
+@pytest.mark.parametrize("ilim", [1, 3])
+def test_muscl_linear_data_reconstructs_face_state(ilim):
+    """Data linear along the normal through the upwind, left, right and
+    downwind points: the limited slope is the data's slope, both sides
+    reconstruct the state at the face midpoint, and with dt = 0 (no
+    evolution) the flux is the physical flux of that state.  A wrong upwind
+    value, distance or side fails this."""
+    rng = np.random.default_rng(8)
+    for _ in range(50):
+        d_mrl, d_kvol, d_kvois = rng.uniform(0.5, 2.0, 3)
+        x = np.array([-d_kvol, 0.0, d_mrl, d_mrl + d_kvois])      # upwind-left, left, right, upwind-right
+        h0, hs = rng.uniform(1.0, 2.0), rng.uniform(-0.2, 0.2)
+        u0, us = rng.uniform(-1, 1), rng.uniform(-0.3, 0.3)
+        v0, vs = rng.uniform(-1, 1), rng.uniform(-0.3, 0.3)
+        st = [(h0 + hs * xi, u0 + us * xi, v0 + vs * xi) for xi in x]
+        n = rng.normal(size=2)
+        n /= np.linalg.norm(n)
+        f = O.muscl(st[1], st[2], st[0], st[3], n=tuple(n), ilimiteur=ilim, d=(d_mrl, d_kvol, d_kvois), dt=0.0)
+        xm = d_mrl / 2
+        assert np.allclose(f, _phys(h0 + hs * xm, u0 + us * xm, v0 + vs * xm, n), rtol=1e-12, atol=1e-12)
+
+
+def test_muscl_uniform_state_is_physical_flux():
+    """A uniform state has zero slopes and zero flux differences: the flux is
+    the physical flux for any dt."""
+    for U in ((1.0, 0.5, -0.2), (3.0, -1.2, 0.4), (0.2, 0.0, 0.0)):
+        for ilim in (1, 2, 3):
+            f = O.muscl(U, U, U, U, n=(0.6, 0.8), ilimiteur=ilim, d=(1.0, 1.3, 0.7), dt=0.1)
+            assert np.allclose(f, _phys(*U, (0.6, 0.8)), rtol=1e-14, atol=1e-14)
+
+
+def _strip_mesh(nx, ny=2, length=200.0, width=2.0):
+    x, y = M._grid_nodes(nx, ny, length, width, 0.0, np.random.default_rng(0))
+    coords = np.stack([x, y, np.zeros_like(x)], 1)
+    _, walls = M._boundary_nodes(nx, ny)
+    conn = M._grid_triangles(nx, ny)
+    return M.Mesh(coords=coords, conn=conn, manning=np.zeros(len(conn)), ndwall=walls)
+
+
+def _stoker(x, t, hl, hr, g=9.81):
+    """Exact wet-bed dam break (Stoker; Toro, Shock-Capturing Methods for
+    Free-Surface Shallow Flows, 2001, ch. 4): left rarefaction, right shock."""
+    from scipy.optimize import brentq
+    cl = math.sqrt(g * hl)
+
+    def gap(hs):
+        return 2 * (cl - math.sqrt(g * hs)) - (hs - hr) * math.sqrt(0.5 * g * (hs + hr) / (hs * hr))
+
+    hs = brentq(gap, hr, hl, xtol=1e-15)
+    us, cs = 2 * (cl - math.sqrt(g * hs)), math.sqrt(g * hs)
+    s = hs * us / (hs - hr)
+    return np.where(x < -cl * t, hl, np.where(x < (us - cs) * t, (2 * cl - x / t) ** 2 / (9 * g),
+                                              np.where(x < s * t, hs, hr)))
+
+
+def test_muscl_dam_break_against_exact_solution():
+    """Stoker's dam break (h 2 | 0.5 m, 8 s) on a strip: MUSCL's L1 depth
+    error is well below first-order HLLC's for every limiter and halves with
+    the mesh size; ilimiteur 0 is first order (HLLC's error)."""
+    from cuteflow_amd.params import Params
+    errs = {}
+    for nx in (200, 400):
+        m = _strip_mesh(nx)
+        xc, _ = m.centroids()
+        U0 = np.zeros((m.nelt, 3))
+        U0[:, 0] = np.where(xc < 100.0, 2.0, 0.5)
+        prep = O.Prepared(m)
+        exact = _stoker(xc - 100.0, 8.0, 2.0, 0.5)
+        for iflux, ilim in ((1, 1), (4, 0), (4, 1), (4, 2), (4, 3)):
+            p = Params(iflux=iflux, ilimiteur=ilim, tolisec=1e-5, cfl=0.9, ts=8.0, inlet="transm",
+                       outlet="transm", friction=0)
+            r = O.run(prep, p, U0, 100000)
+            assert r["status"] == 0 and r["tc"] >= 8.0 - 1e-9
+            errs[nx, iflux, ilim] = np.mean(np.abs(r["U"][:, 0] - exact))
+    for nx in (200, 400):
+        assert errs[nx, 4, 0] == pytest.approx(errs[nx, 1, 1], rel=1e-6)
+        for ilim in (1, 2, 3):
+            assert errs[nx, 4, ilim] < 0.8 * errs[nx, 1, 1], (nx, ilim, errs)
+    for ilim in (1, 2, 3):
+        assert errs[400, 4, ilim] < 0.6 * errs[200, 4, ilim], (ilim, errs)
+
+
 def test_dt_closed_form():
     m, p, U0 = dam_case(16)
     prep = O.Prepared(m)
@@ -280,8 +405,10 @@ def test_dt_closed_form():
     assert O.cfl_dt(prep, p, U) == 1e5        # all dry: the 1e5 padding wins
 
 
-def test_mass_conservation_walls():
+@pytest.mark.parametrize("iflux", [1, 4])
+def test_mass_conservation_walls(iflux):
     m, p, U0 = dam_case(24)
+    p.iflux = iflux
     prep = O.Prepared(m)
     r = O.run(prep, p, U0, 300)
     v0 = np.sum(U0[:, 0] * prep.surf)
@@ -289,7 +416,7 @@ def test_mass_conservation_walls():
     assert r["status"] == 0 and abs(v1 - v0) <= 1e-12 * v0
 
 
-@pytest.mark.parametrize("iflux", [1, 2, 3])
+@pytest.mark.parametrize("iflux", [1, 2, 3, 4])
 def test_lake_at_rest(iflux):
     m, p = lake_case(iflux=iflux)
     prep = O.Prepared(m)
