@@ -85,6 +85,8 @@ _H = C.c_void_p
 SIGNATURES = {
     "cf_build_boundary": (C.c_int, [C.c_int, C.c_int, _IP, C.c_int, _IP, _IP, C.c_int, _IP, _IP,
                                     C.c_int, _IP, _IP]),
+    "cf_build_boundary_device": (C.c_int, [C.c_int, C.c_int, _IP, C.c_int, _IP, _IP, C.c_int, _IP, _IP,
+                                           C.c_int, _IP, _IP]),
     "cf_inlet_lengths": (C.c_int, [C.c_int, C.c_int, _IP, _DP, _IP, C.c_int, _DP]),
     "cf_create": (C.c_int, [C.POINTER(CfMesh), C.POINTER(CfParams), C.POINTER(_H)]),
     "cf_destroy": (None, [_H]),
@@ -135,6 +137,8 @@ SIGNATURES = {
     "cf_tlm_run": (C.c_int, [_H]),
     "cf_tlm_get": (C.c_int, [_H, C.c_int, _DP, _DP, C.c_void_p, C.c_void_p, C.c_void_p]),
     "cf_debug_pow": (C.c_int, [C.c_int, _DP, C.c_double, _DP]),
+    "cf_debug_prepare": (C.c_int, [C.POINTER(CfMesh), C.POINTER(CfParams), C.c_int, C.c_int, C.c_int, _IP, _IP, _IP,
+                                   _IP, _IP, C.c_int, _IP, C.c_void_p, _IP, _IP]),
 }
 
 _lib = None

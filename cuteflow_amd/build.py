@@ -33,7 +33,8 @@ HIP_FLAGS = [
 ]
 
 SOURCES = [os.path.join(CSRC, "cuteflow_hip.hip"), os.path.join(CSRC, "cf_adjoint.hip"),
-           os.path.join(CSRC, "cf_autotune.hip"), os.path.join(CSRC, "cf_prep.cpp")]
+           os.path.join(CSRC, "cf_autotune.hip"), os.path.join(CSRC, "cf_prep.cpp"),
+           os.path.join(CSRC, "cf_prep_gpu.hip")]
 
 
 def _run(cmd: list[str], cwd: str | None = None) -> None:

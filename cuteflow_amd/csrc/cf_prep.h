@@ -35,4 +35,8 @@ struct Prep {
 // a tile of `tile` consecutive cells is a compact patch of the mesh.
 int prepare(const cf_mesh *mesh, const cf_params *params, int tile, bool keep_order, Prep &out);
 
+// The same on the current GPU (sorts, per-cell kernels), identical results;
+// CF_ERR_HIP / CF_ERR_NOMEM on device failures.
+int prepare_device(const cf_mesh *mesh, const cf_params *params, int tile, bool keep_order, Prep &out);
+
 }  // namespace cfprep

@@ -594,7 +594,8 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
             prep[b] = prep[same];
             continue;
         }
-        int st = cfprep::prepare(meshes + b, params + b, tile, (params[b].options & CF_OPT_KEEP_ORDER) != 0, prep[b]);
+        int st = cfprep::prepare_device(meshes + b, params + b, tile, (params[b].options & CF_OPT_KEEP_ORDER) != 0,
+                                        prep[b]);
         if (st) return st;
     }
     cf_handle *h = new (std::nothrow) cf_handle();
@@ -1170,6 +1171,34 @@ extern "C" int cf_set_stream(cf_handle *h, void *stream) {
     if (h->own_stream) (void)hipStreamDestroy(h->stream);
     h->stream = (hipStream_t)stream;
     h->own_stream = false;
+    return CF_OK;
+}
+
+extern "C" int cf_debug_prepare(const cf_mesh *mesh, const cf_params *params, int tile, int keep_order, int device,
+                                int *perm, int *nbr, int *conn, int *tile_off, int *tile_adj, int adj_cap, int *n_adj,
+                                unsigned char *tile_always, int *io_cells, int *n_io) {
+    if (!mesh || !params || (tile != 64 && tile != 128 && tile != 256) || !perm || !nbr || !conn || !tile_off ||
+        !tile_adj || !n_adj || !tile_always || !io_cells || !n_io || mesh->nelt <= 0 || !mesh->conn ||
+        !mesh->coords || !mesh->boundary)
+        return CF_ERR_ARG;
+    cfprep::Prep P;
+    const int st = device ? cfprep::prepare_device(mesh, params, tile, keep_order != 0, P)
+                          : cfprep::prepare(mesh, params, tile, keep_order != 0, P);
+    if (st) return st;
+    if ((int)P.tile_adj.size() > adj_cap) return CF_ERR_ARG;
+    const size_t N = (size_t)mesh->nelt;
+    std::copy(P.perm.begin(), P.perm.end(), perm);
+    for (size_t i = 0; i < N; i++) {
+        for (int k = 0; k < 3; k++) nbr[4 * i + k] = P.nbr[i].c[k];
+        nbr[4 * i + 3] = P.nbr[i].w;
+    }
+    std::copy(P.conn.begin(), P.conn.end(), conn);
+    std::copy(P.tile_off.begin(), P.tile_off.end(), tile_off);
+    std::copy(P.tile_adj.begin(), P.tile_adj.end(), tile_adj);
+    *n_adj = (int)P.tile_adj.size();
+    std::copy(P.tile_always.begin(), P.tile_always.end(), tile_always);
+    std::copy(P.io_cells.begin(), P.io_cells.end(), io_cells);
+    *n_io = (int)P.io_cells.size();
     return CF_OK;
 }
 

@@ -101,7 +101,7 @@ def run_case(case_dir: str, verbose: bool = True, max_steps: int | None = None):
     prm.nombre_entrees, prm.nombre_sorties = mesh.nombre_entrees, mesh.nombre_sorties
     file_bc = os.path.join(case_dir, "boundary_table_" + prm.meshfile.strip())
     if prm.elt_bound == 0:
-        bnd = mesh.boundary_table()
+        bnd = mesh.boundary_table(device=True)
         write_boundary_table(file_bc, bnd)
     else:
         bnd = read_boundary_table(file_bc, mesh.nelt)

@@ -52,18 +52,19 @@ class Mesh:
         y = self.coords[:, 1][self.conn - 1]
         return ((x[:, 0] + x[:, 1]) + x[:, 2]) / 3.0, ((y[:, 0] + y[:, 1]) + y[:, 2]) / 3.0
 
-    def boundary_table(self) -> np.ndarray:
+    def boundary_table(self, device: bool = False) -> np.ndarray:
         """boundary(nelt,3,2) as a (2, 3, nelt) int32 array (the Fortran
-        column-major layout), built by the library (cf_build_boundary)."""
+        column-major layout), built by the library: cf_build_boundary on the
+        host, or cf_build_boundary_device on the GPU (same table)."""
         lib = _lib.load()
         bnd = np.zeros((2, 3, self.nelt), np.int32)
         conn = np.ascontiguousarray(self.conn.reshape(-1))
-        st = lib.cf_build_boundary(
-            self.nelt, self.nnt, _lib.iptr(conn),
-            len(self.ndinput), _lib.iptr(self.ndinput), _lib.iptr(self.numero_ndinput),
-            len(self.ndoutput), _lib.iptr(self.ndoutput), _lib.iptr(self.numero_ndoutput),
-            len(self.ndwall), _lib.iptr(self.ndwall), _lib.iptr(bnd))
-        _lib.check(st, "cf_build_boundary")
+        fn = lib.cf_build_boundary_device if device else lib.cf_build_boundary
+        st = fn(self.nelt, self.nnt, _lib.iptr(conn),
+                len(self.ndinput), _lib.iptr(self.ndinput), _lib.iptr(self.numero_ndinput),
+                len(self.ndoutput), _lib.iptr(self.ndoutput), _lib.iptr(self.numero_ndoutput),
+                len(self.ndwall), _lib.iptr(self.ndwall), _lib.iptr(bnd))
+        _lib.check(st, "cf_build_boundary_device" if device else "cf_build_boundary")
         return bnd
 
     def inlet_lengths(self, bnd: np.ndarray, nentrees: int | None = None) -> np.ndarray:

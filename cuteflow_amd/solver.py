@@ -24,7 +24,7 @@ class Solver:
         self.lib = lib
         self.mesh = mesh
         self.params = params
-        self.boundary = mesh.boundary_table() if boundary is None else np.ascontiguousarray(boundary, np.int32)
+        self.boundary = mesh.boundary_table(device=True) if boundary is None else np.ascontiguousarray(boundary, np.int32)
         self.long_entree = mesh.inlet_lengths(self.boundary, params.nombre_entrees)
         self.cfp = params.to_cf(self.long_entree, ts=ts)
         self.cfp.options = int(options)
@@ -164,7 +164,7 @@ def _batch_structs(meshes, params, ts, options):
         key = (id(m.conn), id(m.coords), id(m.ndinput), id(m.numero_ndinput), id(m.ndoutput),
                id(m.numero_ndoutput), id(m.ndwall), p.nombre_entrees)
         if key not in shared:
-            bt = m.boundary_table()
+            bt = m.boundary_table(device=True)
             shared[key] = (bt, m.inlet_lengths(bt, p.nombre_entrees), np.ascontiguousarray(m.conn.reshape(-1), np.int32),
                            np.ascontiguousarray(m.coords.reshape(-1), np.float64))
         bt, le, conn, coords = shared[key]

@@ -102,6 +102,16 @@ int cf_build_boundary(int nelt, int nnt, const int *conn,
                       int ndo, const int *ndoutput, const int *num_out,
                       int ndw, const int *ndwall, int *boundary);
 
+/* cf_build_boundary computed on the current GPU: the same table bit for bit
+ * (host arrays in and out).  The directed edges are matched by a radix sort
+ * and one binary search per face, the boundary classes by one thread per
+ * face; for large meshes it replaces the host sort (and the reference's
+ * all-pairs GPU search, maillage.cuf:881-952). */
+int cf_build_boundary_device(int nelt, int nnt, const int *conn,
+                             int ndi, const int *ndinput, const int *num_in,
+                             int ndo, const int *ndoutput, const int *num_out,
+                             int ndw, const int *ndwall, int *boundary);
+
 /* Inlet lengths long_entree (maillage.cuf:1423-1452) from a boundary table. */
 int cf_inlet_lengths(int nelt, int nnt, const int *conn, const double *coords,
                      const int *boundary, int nentrees, double *long_entree);
@@ -283,6 +293,16 @@ int cf_tlm_get(cf_handle *h, int inst, double *eta, double *d_eta, double *d_hma
 /* Elementwise pow used by the friction kernel, evaluated on the device for
  * n values (x > 0): lets tests compare the device pow bit-for-bit. */
 int cf_debug_pow(int n, const double *x, double y, double *out);
+
+/* The internal layout cf_create builds for a mesh (cell order, neighbour
+ * codes, tile adjacency, inflow tiles, inflow/outflow cells), from the
+ * GPU preparation (device = 1, what cf_create uses) or the host one
+ * (device = 0): lets tests compare the two.  Arrays: perm (nelt), nbr
+ * (4*nelt), conn (3*nelt), tile_off (ntiles+1), tile_adj (adj_cap, count in
+ * *n_adj), tile_always (ntiles), io_cells (nelt, count in *n_io). */
+int cf_debug_prepare(const cf_mesh *mesh, const cf_params *params, int tile, int keep_order, int device,
+                     int *perm, int *nbr, int *conn, int *tile_off, int *tile_adj, int adj_cap, int *n_adj,
+                     unsigned char *tile_always, int *io_cells, int *n_io);
 
 #ifdef __cplusplus
 }
