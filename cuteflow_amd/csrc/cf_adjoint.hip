@@ -135,13 +135,13 @@ struct RevArgs {
 
 // Occupancy target of the reverse kernel (waves per SIMD): the first-order
 // reverse fits 168 VGPRs (3 waves) with a few spills and runs 8 % faster
-// than at 2 waves on MI355X; the WAF reverse needs 256 (1 wave).
+// than at 2 waves on MI355X; the WAF and MUSCL reverses need 256 (1 wave).
 #ifndef CF_REV_WAVES_FIRST_ORDER
 #define CF_REV_WAVES_FIRST_ORDER 3
 #endif
 template <int IFLUX>
 struct RevWaves {
-    static constexpr int value = IFLUX == 3 ? 1 : CF_REV_WAVES_FIRST_ORDER;
+    static constexpr int value = IFLUX >= 3 ? 1 : CF_REV_WAVES_FIRST_ORDER;
 };
 
 __device__ __forceinline__ void acc4(double4 &a, const double4 &b) {
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(RevWaves<
     const int b = r.tile_inst[tile];
     const int i = tile * (int)blockDim.x + threadIdx.x;
     if (i >= r.inst_end[b] || !r.ran[tile]) return;
-    const double4 l4 = r.ran_next ? gather_cell<IFLUX == 3>(i, (int)blockDim.x, r.ran_next, r.nbr, r.R1in, r.R2in,
+    const double4 l4 = r.ran_next ? gather_cell<IFLUX >= 3>(i, (int)blockDim.x, r.ran_next, r.nbr, r.R1in, r.R2in,
                                                             r.r2_off, r.r2_idx, r.argmax, r.ghmax, r.sidx,
                                                             r.geta_row, r.gfirst, r.gnext)
                                   : r.lam[i];
@@ -219,7 +219,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(RevWaves<
 #pragma unroll
     for (int k = 0; k < 3; k++) r.R1[4 * (size_t)i + k] = make_double4(r1[k][0], r1[k][1], r1[k][2], 0.0);
     r.R1[4 * (size_t)i + 3] = make_double4(own[0], own[1], own[2], 0.0);
-    if (IFLUX == 3) {
+    if (IFLUX >= 3) {
         const int mask = r.nbr[i].w >> kR2Shift;
 #pragma unroll
         for (int k = 0; k < 3; k++)
@@ -722,7 +722,7 @@ extern "C" int cf_adj_forward(cf_handle *h, int nsteps, int sample_every, int ch
     int rc = upload_gauges(h, A);
     if (rc) return rc;
     if ((rc = adj_alloc(h, A)) != CF_OK) return rc;
-    if (h->iflux == 3 && (rc = build_r2(h, A)) != CF_OK) return rc;
+    if (h->iflux >= 3 && (rc = build_r2(h, A)) != CF_OK) return rc;
     // seeds default to zero for instances the caller does not seed
     fill0(A->geta, (size_t)std::max(A->nsamp, 1) * (size_t)std::max(A->gtotal, 1), h->stream);
     fill0(A->gstate, (size_t)h->ncell, h->stream);
@@ -861,9 +861,10 @@ static void launch_rev_t(const RevArgs &r, int grid, hipStream_t s) {
 
 static void launch_rev(const RevArgs &r, int iflux, int fric, int grid, hipStream_t s) {
     typedef void (*fn)(const RevArgs &, int, hipStream_t);
-    static const fn tab[3][4] = {{launch_rev_t<1, 0>, launch_rev_t<1, 1>, launch_rev_t<1, 2>, launch_rev_t<1, 3>},
+    static const fn tab[4][4] = {{launch_rev_t<1, 0>, launch_rev_t<1, 1>, launch_rev_t<1, 2>, launch_rev_t<1, 3>},
                                  {launch_rev_t<2, 0>, launch_rev_t<2, 1>, launch_rev_t<2, 2>, launch_rev_t<2, 3>},
-                                 {launch_rev_t<3, 0>, launch_rev_t<3, 1>, launch_rev_t<3, 2>, launch_rev_t<3, 3>}};
+                                 {launch_rev_t<3, 0>, launch_rev_t<3, 1>, launch_rev_t<3, 2>, launch_rev_t<3, 3>},
+                                 {launch_rev_t<4, 0>, launch_rev_t<4, 1>, launch_rev_t<4, 2>, launch_rev_t<4, 3>}};
     tab[iflux - 1][fric](r, grid, s);
 }
 
@@ -944,7 +945,7 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
             launch_rev(r, h->iflux, h->fric, h->ntiles, s);
         }
     }
-    if (h->iflux == 3)
+    if (h->iflux >= 3)
         hipLaunchKernelGGL(gather_kernel<true>, dim3(h->ntiles), dim3(h->tile), 0, s, h->nbr, A->R1[0], A->R2[0],
                            A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, A->tile_ran, A->lam[0]);
     else
@@ -954,9 +955,6 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
 
 extern "C" int cf_adj_backward(cf_handle *h) {
     if (!h || !h->adj || !h->adj->recorded || h->pending >= 0) return CF_ERR_ARG;
-    // the reverse sweep covers iflux 1-3; MUSCL (iflux 4) has the
-    // tangent-linear model only
-    if (h->iflux == 4) return CF_ERR_ARG;
     AdjState *A = h->adj;
     int rc = run_captured(h, A, 1, enqueue_backward);
     if (rc) return rc;

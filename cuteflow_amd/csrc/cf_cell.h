@@ -619,7 +619,7 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
 
     // faces: resc = sum of flux * ln
     D4 dfi = {0.0, 0.0, 0.0, 0.0};
-    if (IFLUX == 3) dfi = ld.dfc(i);
+    if (IFLUX >= 3) dfi = ld.dfc(i);
     for (int j = 0; j < 3; j++) {
         const double n1 = ld.geo(G_N1 + j, i);
         const double n2 = ld.geo(G_N2 + j, i);
@@ -689,7 +689,8 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
                     bur += gg[4];
                     bvr += gg[5];
                 }
-            } else if (!(fhl < kEpsWaf && fhr < kEpsWaf)) {
+            } else if (IFLUX == 3 ? !(fhl < kEpsWaf && fhr < kEpsWaf) : !(fhl <= 0.0 && fhr <= 0.0)) {
+                // WAF (iflux 3) or MUSCL (iflux 4) on the two-ring stencil
                 const double d_mrl = d4_at(dfi, j);
                 const int ja = (j + 1) % 3, jb = (j + 2) % 3;
                 const bool own1 = nb_at(nb, ja) < 0, own2 = nb_at(nb, jb) < 0;
@@ -708,21 +709,22 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
                 rot(tls, n1, n2, g1.h, g1.hu, g1.hv, g1h, g1u, g1w);
                 rot(tls, n1, n2, g2.h, g2.hu, g2.hv, g2h, g2u, g2w);
                 const bool special = (own1 && own2) || (kw1 == kv && kw2 == kv);
-                double hl0, vl0, hr0, vr0, zml_upw, zmr_upw, d_mkvol, d_mkvoisin;
+                double hl0, ul0, vl0, hr0, ur0, vr0, zml_upw, zmr_upw, d_mkvol, d_mkvoisin;
                 double wl1 = 0.0, wl2 = 0.0, wr1 = 0.0, wr2 = 0.0;  // weights of s1, s2, g1, g2
                 if (special) {
-                    hl0 = hautl; vl0 = fvl;
-                    hr0 = hautr; vr0 = fvr;
+                    hl0 = hautl; ul0 = ful; vl0 = fvl;
+                    hr0 = hautr; ur0 = fur; vr0 = fvr;
                     d_mkvol = d_mrl; d_mkvoisin = d_mrl;
                     zml_upw = zml; zmr_upw = zmr;
                 } else {
                     if (own1) {
-                        hl0 = h2; vl0 = w2; d_mkvol = d4_at(dfi, jb); zml_upw = s2.zm; wl2 = 1.0;
+                        hl0 = h2; ul0 = u2; vl0 = w2; d_mkvol = d4_at(dfi, jb); zml_upw = s2.zm; wl2 = 1.0;
                     } else if (own2) {
-                        hl0 = h1; vl0 = w1; d_mkvol = d4_at(dfi, ja); zml_upw = s1.zm; wl1 = 1.0;
+                        hl0 = h1; ul0 = u1; vl0 = w1; d_mkvol = d4_at(dfi, ja); zml_upw = s1.zm; wl1 = 1.0;
                     } else {
                         const double d1 = d4_at(dfi, ja), d2 = d4_at(dfi, jb);
                         hl0 = (d1 * h1 + d2 * h2) / (d1 + d2);
+                        ul0 = (d1 * u1 + d2 * u2) / (d1 + d2);
                         vl0 = (d1 * w1 + d2 * w2) / (d1 + d2);
                         zml_upw = (d1 * s1.zm + d2 * s2.zm) / (d1 + d2);
                         d_mkvol = (d1 + d2) / 2.0;
@@ -731,12 +733,13 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
                     }
                     const D4 dfk = (kv == i) ? dfi : ld.dfc(kv);
                     if (kw1 == kv) {
-                        hr0 = g2h; vr0 = g2w; d_mkvoisin = d4_at(dfk, kb); zmr_upw = g2.zm; wr2 = 1.0;
+                        hr0 = g2h; ur0 = g2u; vr0 = g2w; d_mkvoisin = d4_at(dfk, kb); zmr_upw = g2.zm; wr2 = 1.0;
                     } else if (kw2 == kv) {
-                        hr0 = g1h; vr0 = g1w; d_mkvoisin = d4_at(dfk, ka); zmr_upw = g1.zm; wr1 = 1.0;
+                        hr0 = g1h; ur0 = g1u; vr0 = g1w; d_mkvoisin = d4_at(dfk, ka); zmr_upw = g1.zm; wr1 = 1.0;
                     } else {
                         const double d1 = d4_at(dfk, ka), d2 = d4_at(dfk, kb);
                         hr0 = (d1 * g1h + d2 * g2h) / (d1 + d2);
+                        ur0 = (d1 * g1u + d2 * g2u) / (d1 + d2);
                         vr0 = (d1 * g1w + d2 * g2w) / (d1 + d2);
                         zmr_upw = (d1 * g1.zm + d2 * g2.zm) / (d1 + d2);
                         d_mkvoisin = (d1 + d2) / 2.0;
@@ -747,13 +750,25 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
                 const double rawlu = (hl0 + zml_upw) - zij, rawru = (hr0 + zmr_upw) - zij;
                 double hlu = dmax(0.0, rawlu), hru = dmax(0.0, rawru);
                 const bool zlu = hlu < tls, zru = hru < tls;
-                const double fhlu = zlu ? 0.0 : hlu, fvlu = zlu ? 0.0 : vl0;
-                const double fhru = zru ? 0.0 : hru, fvru = zru ? 0.0 : vr0;
-                double gg[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-                waf_flux_rev(g, p.ilimiteur, n1, n2, fhl, ful, fvl, fhr, fur, fvr, fhlu, fvlu, fhru, fvru, d_mrl,
-                             d_mkvol, d_mkvoisin, dt, bf, gg, (p.ad_flags & 1) != 0);
+                const double fhlu = zlu ? 0.0 : hlu, fulu = zlu ? 0.0 : ul0, fvlu = zlu ? 0.0 : vl0;
+                const double fhru = zru ? 0.0 : hru, furu = zru ? 0.0 : ur0, fvru = zru ? 0.0 : vr0;
+                // gg: face states (0..5), then the upwind (h, u, v) left and right
+                double gg[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                if (IFLUX == 3) {
+                    double gw[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+                    waf_flux_rev(g, p.ilimiteur, n1, n2, fhl, ful, fvl, fhr, fur, fvr, fhlu, fvlu, fhru, fvru, d_mrl,
+                                 d_mkvol, d_mkvoisin, dt, bf, gw, (p.ad_flags & 1) != 0);
+                    for (int q = 0; q < 6; q++) gg[q] = gw[q];
+                    gg[6] = gw[6];
+                    gg[8] = gw[7];
+                    gg[9] = gw[8];
+                    gg[11] = gw[9];
+                } else {
+                    muscl_flux_rev(g, p.ilimiteur, tls, n1, n2, fhl, ful, fvl, fhr, fur, fvr, fhlu, fulu, fvlu, fhru,
+                                   furu, fvru, d_mrl, d_mkvol, d_mkvoisin, dt, bf, gg);
+                }
                 // face states
-                double bvl_post = 0.0, bvr_post = 0.0;
+                double bul_post = 0.0, bvl_post = 0.0, bur_post = 0.0, bvr_post = 0.0;
                 if (!zl) {
                     if (actl) bhl += gg[0];
                     bul += gg[1];
@@ -765,39 +780,49 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
                     bvr_post += gg[5];
                 }
                 // upwind states
-                const double bhl0 = (!zlu && !(0.0 > rawlu)) ? gg[6] : 0.0, bvl0 = zlu ? 0.0 : gg[7];
-                const double bhr0 = (!zru && !(0.0 > rawru)) ? gg[8] : 0.0, bvr0 = zru ? 0.0 : gg[9];
+                const double bhl0 = (!zlu && !(0.0 > rawlu)) ? gg[6] : 0.0, bul0 = zlu ? 0.0 : gg[7],
+                             bvl0 = zlu ? 0.0 : gg[8];
+                const double bhr0 = (!zru && !(0.0 > rawru)) ? gg[9] : 0.0, bur0 = zru ? 0.0 : gg[10],
+                             bvr0 = zru ? 0.0 : gg[11];
                 if (special) {
                     bhl += bhl0;
+                    bul_post += bul0;
                     bvl_post += bvl0;
                     bhr += bhr0;
+                    bur_post += bur0;
                     bvr_post += bvr0;
                 } else {
                     // left upwind: s1 = neighbour ja, s2 = neighbour jb (never own here)
                     if (wl1 != 0.0) {
                         double bh = 0.0, bhu = 0.0, bhv = 0.0;
-                        rot_rev(tls, n1, n2, s1.h, s1.hu, s1.hv, bhl0 * wl1, 0.0, bvl0 * wl1, bh, bhu, bhv);
+                        rot_rev(tls, n1, n2, s1.h, s1.hu, s1.hv, bhl0 * wl1, bul0 * wl1, bvl0 * wl1, bh, bhu, bhv);
                         add3(r1[ja], bh, bhu, bhv);
                     }
                     if (wl2 != 0.0) {
                         double bh = 0.0, bhu = 0.0, bhv = 0.0;
-                        rot_rev(tls, n1, n2, s2.h, s2.hu, s2.hv, bhl0 * wl2, 0.0, bvl0 * wl2, bh, bhu, bhv);
+                        rot_rev(tls, n1, n2, s2.h, s2.hu, s2.hv, bhl0 * wl2, bul0 * wl2, bvl0 * wl2, bh, bhu, bhv);
                         add3(r1[jb], bh, bhu, bhv);
                     }
                     // right upwind: second-ring slots 0 (kw1) and 1 (kw2)
                     if (wr1 != 0.0) {
                         double bh = 0.0, bhu = 0.0, bhv = 0.0;
-                        rot_rev(tls, n1, n2, g1.h, g1.hu, g1.hv, bhr0 * wr1, 0.0, bvr0 * wr1, bh, bhu, bhv);
+                        rot_rev(tls, n1, n2, g1.h, g1.hu, g1.hv, bhr0 * wr1, bur0 * wr1, bvr0 * wr1, bh, bhu, bhv);
                         add3(r2[j][0], bh, bhu, bhv);
                     }
                     if (wr2 != 0.0) {
                         double bh = 0.0, bhu = 0.0, bhv = 0.0;
-                        rot_rev(tls, n1, n2, g2.h, g2.hu, g2.hv, bhr0 * wr2, 0.0, bvr0 * wr2, bh, bhu, bhv);
+                        rot_rev(tls, n1, n2, g2.h, g2.hu, g2.hv, bhr0 * wr2, bur0 * wr2, bvr0 * wr2, bh, bhu, bhv);
                         add3(r2[j][1], bh, bhu, bhv);
                     }
                 }
-                if (!zl) bvl += bvl_post;
-                if (!zr) bvr += bvr_post;
+                if (!zl) {
+                    bul += bul_post;
+                    bvl += bvl_post;
+                }
+                if (!zr) {
+                    bur += bur_post;
+                    bvr += bvr_post;
+                }
             }
             // the right state's cotangent goes to cell kv
             double bh = bhr_raw, bhu = 0.0, bhv = 0.0;

@@ -861,4 +861,202 @@ CF_HD inline void waf_flux_rev(double gp, int ilim, double n1, double n2, double
     g[9] += bvru;
 }
 
+// slope_limiter with its partial derivatives, the branch and the argument
+// of each max/min taken as the forward takes them (dmax(a, b) keeps b on a
+// tie, as the dual type does); d|p|/dp = +1 at p = 0 (the dual type's vabs).
+CF_HD inline double slope_limiter_d(int ilim, double r1, double r2, double &d1, double &d2) {
+    d1 = 0.0;
+    d2 = 0.0;
+    switch (ilim) {
+    case 1: {
+        double a, da1, da2, b, db1, db2;
+        if (r2 > 0.0) {
+            if (2.0 * r1 < r2) { a = 2.0 * r1; da1 = 2.0; da2 = 0.0; } else { a = r2; da1 = 0.0; da2 = 1.0; }
+            if (r1 < 2.0 * r2) { b = r1; db1 = 1.0; db2 = 0.0; } else { b = 2.0 * r2; db1 = 0.0; db2 = 2.0; }
+            double m, dm1, dm2;
+            if (0.0 > a) { m = 0.0; dm1 = 0.0; dm2 = 0.0; } else { m = a; dm1 = da1; dm2 = da2; }
+            if (m > b) { d1 = dm1; d2 = dm2; return m; }
+            d1 = db1;
+            d2 = db2;
+            return b;
+        }
+        if (2.0 * r1 > r2) { a = 2.0 * r1; da1 = 2.0; da2 = 0.0; } else { a = r2; da1 = 0.0; da2 = 1.0; }
+        if (r1 > 2.0 * r2) { b = r1; db1 = 1.0; db2 = 0.0; } else { b = 2.0 * r2; db1 = 0.0; db2 = 2.0; }
+        double m, dm1, dm2;
+        if (0.0 < a) { m = 0.0; dm1 = 0.0; dm2 = 0.0; } else { m = a; dm1 = da1; dm2 = da2; }
+        if (m < b) { d1 = dm1; d2 = dm2; return m; }
+        d1 = db1;
+        d2 = db2;
+        return b;
+    }
+    case 2: {
+        const double p = r1 * r2, sp = p < 0.0 ? -1.0 : 1.0;
+        const double num = p + fabs(p), den = (r1 + r2) + kEpsLeer;
+        const double v = num / den;
+        d1 = (r2 * (1.0 + sp) - v) / den;
+        d2 = (r1 * (1.0 + sp) - v) / den;
+        return v;
+    }
+    case 3:
+        if (r2 > 0.0) {
+            const double in = r1 < r2 ? r1 : r2;
+            if (0.0 > in) return 0.0;
+            if (r1 < r2) d1 = 1.0;
+            else d2 = 1.0;
+            return in;
+        } else {
+            const double in = r1 > r2 ? r1 : r2;
+            if (0.0 < in) return 0.0;
+            if (r1 > r2) d1 = 1.0;
+            else d2 = 1.0;
+            return in;
+        }
+    default: return 0.0;
+    }
+}
+
+// Reverse of muscl_flux (dt, the limiter branches, the dry-velocity
+// switches and the clamps at zero as taken in the forward).  g[0..11] +=
+// cotangents of (hl, ul, vl, hr, ur, vr, hl_upw, ul_upw, vl_upw, hr_upw,
+// ur_upw, vr_upw).
+CF_HD inline void muscl_flux_rev(double gp, int ilim, double tolisec, double n1, double n2, double hl, double ul,
+                                 double vl, double hr, double ur, double vr, double hlu, double ulu, double vlu,
+                                 double hru, double uru, double vru, double d_mrl, double d_mkvol,
+                                 double d_mkvoisin, double dt, const double bflux[3], double g[12]) {
+    const double d = d_mrl, dk = d_mkvol, dv = d_mkvoisin;
+    const double dtoverdx = dt / d_mrl, hdt = 0.5 * dtoverdx, hg = 0.5 * gp;
+    // forward: reconstruction I
+    double sLh1, sLh2, sLu1 = 0.0, sLu2 = 0.0, sLv1 = 0.0, sLv2 = 0.0;
+    const double dhL = slope_limiter_d(ilim, (hl - hlu) / dk, (hr - hl) / d, sLh1, sLh2);
+    const double rLl = hl - (0.5 * dhL) * d, rLr = hl + (0.5 * dhL) * d;
+    const double hll = dmax(0.0, rLl), hlr = dmax(0.0, rLr);
+    const bool velL = !(hlr < tolisec || hll < tolisec);
+    double ull = 0.0, ulr = 0.0, vll = 0.0, vlr = 0.0;
+    if (velL) {
+        const double du = slope_limiter_d(ilim, (ul - ulu) / dk, (ur - ul) / d, sLu1, sLu2);
+        const double dw = slope_limiter_d(ilim, (vl - vlu) / dk, (vr - vl) / d, sLv1, sLv2);
+        ull = ul - (0.5 * du) * d;
+        vll = vl - (0.5 * dw) * d;
+        ulr = ul + (0.5 * du) * d;
+        vlr = vl + (0.5 * dw) * d;
+    }
+    double sRh1, sRh2, sRu1 = 0.0, sRu2 = 0.0, sRv1 = 0.0, sRv2 = 0.0;
+    const double dhR = slope_limiter_d(ilim, (hr - hl) / d, (hru - hr) / dv, sRh1, sRh2);
+    const double rRr = hr + (0.5 * dhR) * d, rRl = hr - (0.5 * dhR) * d;
+    const double hrr = dmax(0.0, rRr), hrl = dmax(0.0, rRl);
+    const bool velR = !(hrl < tolisec || hrr < tolisec);
+    double url = 0.0, urr = 0.0, vrl = 0.0, vrr = 0.0;
+    if (velR) {
+        const double du = slope_limiter_d(ilim, (ur - ul) / d, (uru - ur) / dv, sRu1, sRu2);
+        const double dw = slope_limiter_d(ilim, (vr - vl) / d, (vru - vr) / dv, sRv1, sRv2);
+        urr = ur + (0.5 * du) * d;
+        vrr = vr + (0.5 * dw) * d;
+        url = ur - (0.5 * du) * d;
+        vrl = vr - (0.5 * dw) * d;
+    }
+    // forward: reconstruction II
+    const double eh = hlr + hdt * (hll * ull - hlr * ulr);
+    const double eu = ulr + hdt * ((hll * (ull * ull) + hg * (hll * hll)) - (hlr * (ulr * ulr) + hg * (hlr * hlr)));
+    const double ev = vlr + hdt * ((hll * ull) * vll - (hlr * ulr) * vlr);
+    const double fh = hrl + hdt * (hrl * url - hrr * urr);
+    const double fu = url + hdt * ((hrl * (url * url) + hg * (hrl * hrl)) - (hrr * (urr * urr) + hg * (hrr * hrr)));
+    const double fv = vrl + hdt * ((hrl * url) * vrl - (hrr * urr) * vrr);
+    if (eh < kEpsWaf && fh < kEpsWaf) return;  // dry/dry: zero flux
+    // reverse: the HLLC solver on the evolved values
+    double bfh[3];
+    unrotate_rev(n1, n2, bflux, bfh);
+    const double z2[2] = {0.0, 0.0};
+    double ge[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    hllc_core_rev(gp, kEpsWaf, eh, eu, ev, fh, fu, fv, bfh, z2, z2, 0.0, 0.0, 0.0, ge);
+    const double beh = ge[0], beu = ge[1], bev = ge[2], bfhh = ge[3], bfu = ge[4], bfv = ge[5];
+    // reverse: reconstruction II
+    double bhll = 0.0, bhlr = 0.0, bull = 0.0, bulr = 0.0, bvll = 0.0, bvlr = 0.0;
+    bhlr += beh * (1.0 - hdt * ulr);
+    bhll += beh * hdt * ull;
+    bull += beh * hdt * hll;
+    bulr -= beh * hdt * hlr;
+    bulr += beu * (1.0 - hdt * (2.0 * hlr * ulr));
+    bhll += beu * hdt * (ull * ull + 2.0 * hg * hll);
+    bull += beu * hdt * (2.0 * hll * ull);
+    bhlr -= beu * hdt * (ulr * ulr + 2.0 * hg * hlr);
+    bvlr += bev * (1.0 - hdt * (hlr * ulr));
+    bhll += bev * hdt * (ull * vll);
+    bull += bev * hdt * (hll * vll);
+    bvll += bev * hdt * (hll * ull);
+    bhlr -= bev * hdt * (ulr * vlr);
+    bulr -= bev * hdt * (hlr * vlr);
+    double bhrl = 0.0, bhrr = 0.0, burl = 0.0, burr = 0.0, bvrl = 0.0, bvrr = 0.0;
+    bhrl += bfhh * (1.0 + hdt * url);
+    burl += bfhh * hdt * hrl;
+    bhrr -= bfhh * hdt * urr;
+    burr -= bfhh * hdt * hrr;
+    burl += bfu * (1.0 + hdt * (2.0 * hrl * url));
+    bhrl += bfu * hdt * (url * url + 2.0 * hg * hrl);
+    bhrr -= bfu * hdt * (urr * urr + 2.0 * hg * hrr);
+    burr -= bfu * hdt * (2.0 * hrr * urr);
+    bvrl += bfv * (1.0 + hdt * (hrl * url));
+    bhrl += bfv * hdt * (url * vrl);
+    burl += bfv * hdt * (hrl * vrl);
+    bhrr -= bfv * hdt * (urr * vrr);
+    burr -= bfv * hdt * (hrr * vrr);
+    bvrr -= bfv * hdt * (hrr * urr);
+    // reverse: reconstruction I
+    double bhl = 0.0, bul = 0.0, bvl = 0.0, bhr = 0.0, bur = 0.0, bvr = 0.0;
+    double bhlu = 0.0, bulu = 0.0, bvlu = 0.0, bhru = 0.0, buru = 0.0, bvru = 0.0;
+    if (velL) {
+        bul += bull + bulr;
+        bvl += bvll + bvlr;
+        const double bdu = 0.5 * d * (bulr - bull), bdw = 0.5 * d * (bvlr - bvll);
+        const double a1 = bdu * sLu1, a2 = bdu * sLu2, c1 = bdw * sLv1, c2 = bdw * sLv2;
+        bul += a1 / dk - a2 / d;
+        bulu -= a1 / dk;
+        bur += a2 / d;
+        bvl += c1 / dk - c2 / d;
+        bvlu -= c1 / dk;
+        bvr += c2 / d;
+    }
+    {
+        const double bl = !(0.0 > rLl) ? bhll : 0.0, br = !(0.0 > rLr) ? bhlr : 0.0;
+        bhl += bl + br;
+        const double bdh = 0.5 * d * (br - bl);
+        const double a1 = bdh * sLh1, a2 = bdh * sLh2;
+        bhl += a1 / dk - a2 / d;
+        bhlu -= a1 / dk;
+        bhr += a2 / d;
+    }
+    if (velR) {
+        bur += burr + burl;
+        bvr += bvrr + bvrl;
+        const double bdu = 0.5 * d * (burr - burl), bdw = 0.5 * d * (bvrr - bvrl);
+        const double a1 = bdu * sRu1, a2 = bdu * sRu2, c1 = bdw * sRv1, c2 = bdw * sRv2;
+        bur += a1 / d - a2 / dv;
+        bul -= a1 / d;
+        buru += a2 / dv;
+        bvr += c1 / d - c2 / dv;
+        bvl -= c1 / d;
+        bvru += c2 / dv;
+    }
+    {
+        const double brr = !(0.0 > rRr) ? bhrr : 0.0, brl = !(0.0 > rRl) ? bhrl : 0.0;
+        bhr += brr + brl;
+        const double bdh = 0.5 * d * (brr - brl);
+        const double a1 = bdh * sRh1, a2 = bdh * sRh2;
+        bhr += a1 / d - a2 / dv;
+        bhl -= a1 / d;
+        bhru += a2 / dv;
+    }
+    g[0] += bhl;
+    g[1] += bul;
+    g[2] += bvl;
+    g[3] += bhr;
+    g[4] += bur;
+    g[5] += bvr;
+    g[6] += bhlu;
+    g[7] += bulu;
+    g[8] += bvlu;
+    g[9] += bhru;
+    g[10] += buru;
+    g[11] += bvru;
+}
+
 }  // namespace cf

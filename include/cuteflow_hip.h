@@ -274,8 +274,7 @@ int cf_adj_get_samples(cf_handle *h, int inst, double *eta, double *hmax);
  * hmax (nelt) and the final state (3*nelt); NULL = 0. */
 int cf_adj_set_seed(cf_handle *h, int inst, const double *g_eta, const double *g_hmax, const double *g_state);
 /* Reverse sweep over the recorded run: segments of checkpoint_every steps
- * replayed from their checkpoints, then reversed step by step.  iflux 1-3;
- * CF_ERR_ARG for MUSCL (iflux 4), which has the tangent-linear model only. */
+ * replayed from their checkpoints, then reversed step by step. */
 int cf_adj_backward(cf_handle *h);
 /* Gradient of instance inst with respect to per-cell Manning n (nelt) and
  * to the initial state (3*nelt); either may be NULL. */

@@ -34,6 +34,8 @@ def build(outdir):
     lib.hs_adjoint.argtypes = [C.c_void_p, _DP, _DP, _DP]
     lib.hs_face_dot.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, _DP, _DP, _DP,
                                 C.c_double, C.c_double, C.c_double, C.c_double, _DP]
+    lib.hs_muscl_face_dot.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, _DP, _DP, _DP,
+                                      C.c_double, C.c_double, C.c_double, C.c_double, _DP]
     return lib
 
 

@@ -149,21 +149,24 @@ typedef void (*tlm_fn)(const HS &, const double *, const double *, const double 
 typedef long (*rev_fn)(const HS &, const double *, const double *, const double *, double *, double *, double, double);
 
 step_fn pick_step(int iflux, int fric) {
-    static const step_fn t[3][4] = {{step_t<1, 0>, step_t<1, 1>, step_t<1, 2>, step_t<1, 3>},
+    static const step_fn t[4][4] = {{step_t<1, 0>, step_t<1, 1>, step_t<1, 2>, step_t<1, 3>},
                                   {step_t<2, 0>, step_t<2, 1>, step_t<2, 2>, step_t<2, 3>},
-                                  {step_t<3, 0>, step_t<3, 1>, step_t<3, 2>, step_t<3, 3>}};
+                                  {step_t<3, 0>, step_t<3, 1>, step_t<3, 2>, step_t<3, 3>},
+                                  {step_t<4, 0>, step_t<4, 1>, step_t<4, 2>, step_t<4, 3>}};
     return t[iflux - 1][fric];
 }
 tlm_fn pick_tlm(int iflux, int fric) {
-    static const tlm_fn t[3][4] = {{tlm_t<1, 0>, tlm_t<1, 1>, tlm_t<1, 2>, tlm_t<1, 3>},
+    static const tlm_fn t[4][4] = {{tlm_t<1, 0>, tlm_t<1, 1>, tlm_t<1, 2>, tlm_t<1, 3>},
                                   {tlm_t<2, 0>, tlm_t<2, 1>, tlm_t<2, 2>, tlm_t<2, 3>},
-                                  {tlm_t<3, 0>, tlm_t<3, 1>, tlm_t<3, 2>, tlm_t<3, 3>}};
+                                  {tlm_t<3, 0>, tlm_t<3, 1>, tlm_t<3, 2>, tlm_t<3, 3>},
+                                  {tlm_t<4, 0>, tlm_t<4, 1>, tlm_t<4, 2>, tlm_t<4, 3>}};
     return t[iflux - 1][fric];
 }
 rev_fn pick_rev(int iflux, int fric) {
-    static const rev_fn t[3][4] = {{rev_t<1, 0>, rev_t<1, 1>, rev_t<1, 2>, rev_t<1, 3>},
+    static const rev_fn t[4][4] = {{rev_t<1, 0>, rev_t<1, 1>, rev_t<1, 2>, rev_t<1, 3>},
                                   {rev_t<2, 0>, rev_t<2, 1>, rev_t<2, 2>, rev_t<2, 3>},
-                                  {rev_t<3, 0>, rev_t<3, 1>, rev_t<3, 2>, rev_t<3, 3>}};
+                                  {rev_t<3, 0>, rev_t<3, 1>, rev_t<3, 2>, rev_t<3, 3>},
+                                  {rev_t<4, 0>, rev_t<4, 1>, rev_t<4, 2>, rev_t<4, 3>}};
     return t[iflux - 1][fric];
 }
 
@@ -356,6 +359,28 @@ void hs_face_dot(int iflux, int ilim, double gp, double n1, double n2, const dou
     double fwd = 0.0, rev = 0.0;
     for (int k = 0; k < 3; k++) fwd += bflux[k] * fl[k].d;
     for (int k = 0; k < 10; k++) rev += g[k] * dir[k];
+    out[0] = fwd;
+    out[1] = rev;
+    out[2] = fl[0].v;
+    out[3] = fl[1].v;
+    out[4] = fl[2].v;
+}
+
+// The same check for the MUSCL face flux (iflux 4): inputs (hl, ul, vl, hr,
+// ur, vr, hl_upw, ul_upw, vl_upw, hr_upw, ur_upw, vr_upw).
+void hs_muscl_face_dot(int ilim, double gp, double tolisec, double n1, double n2, const double *in, const double *dir,
+                       const double *bflux, double d_mrl, double d_mkvol, double d_mkvoisin, double dt, double *out) {
+    Dual v[12];
+    for (int k = 0; k < 12; k++) v[k] = Dual(in[k], dir[k]);
+    Dual fl[3];
+    double g[12] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    muscl_flux(gp, ilim, tolisec, n1, n2, v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8], v[9], v[10], v[11],
+               d_mrl, d_mkvol, d_mkvoisin, dt, fl);
+    muscl_flux_rev(gp, ilim, tolisec, n1, n2, in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], in[8], in[9],
+                   in[10], in[11], d_mrl, d_mkvol, d_mkvoisin, dt, bflux, g);
+    double fwd = 0.0, rev = 0.0;
+    for (int k = 0; k < 3; k++) fwd += bflux[k] * fl[k].d;
+    for (int k = 0; k < 12; k++) rev += g[k] * dir[k];
     out[0] = fwd;
     out[1] = rev;
     out[2] = fl[0].v;
