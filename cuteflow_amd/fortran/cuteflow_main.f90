@@ -68,7 +68,9 @@ program cuteflow_main
   integer(8) :: c0, c1, crate
   logical :: ex
   character(len=300) :: mesh_path, out_path
-  real(dp), allocatable :: tsolvisu(:)
+  real(dp), allocatable :: tsolvisu(:), unodes(:, :), anodes(:)
+  integer :: cptsolvisu
+  real(dp) :: tolaff, room
   real(c_double), allocatable, target :: cg_coords(:)
   integer(c_int), allocatable :: cg_conn(:)
   integer :: nsnap
@@ -302,39 +304,43 @@ program cuteflow_main
   call cf_check(st, 'cf_set_discharge')
   st = cf_set_state(h, u)
   call cf_check(st, 'cf_set_state')
-#ifdef CF_WITH_CGNS
-  ! CGNS snapshots (main.cuf:113-122, pre_post_traitement.cuf:27-31): the
-  ! initial state as Solution_0 here, the final one as the last snapshot
-  if (is_cgns == 1 .and. (solvisu == 1 .or. solvisu == 3)) then
-     nsnap = max(visu_snapshots, 1)
-     allocate(tsolvisu(0:nsnap))
-     do i = 0, nsnap
-        tsolvisu(i) = i * ts / real(nsnap)
-     end do
-     st = cf_cgns_export_init(trim(out_path)//c_null_char, nsnap + 1, tsolvisu)
-     call cf_cgns_check(st, 'cf_cgns_export_init')
-     call cgns_snapshot(0)
-  end if
-#endif
 
-  ! ---- time loop (finite_volumes_method.cuf:113-242), freqaffich steps per batch ----
+  ! ---- snapshot times (main.cuf:113-122) and the initial state's snapshot (main.cuf:179) ----
+  nsnap = max(visu_snapshots, 1)
+  allocate(tsolvisu(0:nsnap), unodes(3, nnt), anodes(nnt))
+  do i = 0, nsnap
+     tsolvisu(i) = i * ts / real(nsnap)
+  end do
+  cptsolvisu = 0
+  tolaff = tolaffiche
+  st = cf_get_time(h, tc, dt)
+  call save_sol_visu(tc, dt)
+
+  ! ---- time loop (finite_volumes_method.cuf:113-242): batches up to the next
+  ! status print, one step at a time close to a snapshot time ----
   nbatch = max(1, freqaffich)
   allocate(dt_hist(nbatch + 1), q_in(nbatch), q_out(nbatch))
   nt = 0
   write(*, *) 'Loop over time starts'
   call system_clock(c0, crate)
   do
-     st = cf_step(h, nbatch, dt_hist, done)
-     nt = nt + done
-     if (st == CF_ERR_DT) then
-        st = cf_get_time(h, tc, dt)
-        print *, "Erreur sur le pas de temps dt=", dt
-        exit
+     n = nbatch - mod(nt, nbatch)
+     if (solvisu > 0 .and. cptsolvisu <= nsnap) then
+        room = tsolvisu(cptsolvisu) - 1.5d0 * dt - tc
+        if (room > 0) then
+           n = min(n, max(1, int(room / (4.d0 * dt))))
+        else
+           n = 1
+        end if
      end if
-     call cf_check(st, 'cf_step')
-     st = cf_get_time(h, tc, dt)
-     st = cf_get_discharge(h, q_in, q_out, steady)
-     call cf_check(st, 'cf_get_discharge')
+     st = cf_step(h, n, dt_hist, done)
+     if (st /= 0 .and. st /= CF_ERR_DT) call cf_check(st, 'cf_step')
+     k = cf_get_time(h, tc, dt)
+     k = cf_get_discharge(h, q_in, q_out, steady)
+     call cf_check(k, 'cf_get_discharge')
+     nt = nt + done
+     ! global_save_sol_visu after each step (finite_volumes_method.cuf:241)
+     if (done > 0) call save_sol_visu(tc, dt_hist(done))
      if (steady /= 0) then   ! check_steady_state (finite_volumes_method.cuf:380-383)
         print *, '-----------------------------------------------'
         print *, 'LE REGIME PERMANENT EST ATTEINT'
@@ -348,7 +354,11 @@ program cuteflow_main
         print *, 'debit_sortie     =', q_out(done), 'M3/S'
         print *, '-----------------------------------------------'
      end if
-     if (done < nbatch) exit
+     if (st == CF_ERR_DT) then
+        print *, "Erreur sur le pas de temps dt=", dt
+        exit
+     end if
+     if (done < n) exit
   end do
   call system_clock(c1)
   write(*, *) 'Loop on time ends at iteration : ', nt
@@ -365,18 +375,6 @@ program cuteflow_main
      end do
      close(14)
   end if
-  if (solvisu == 1 .or. solvisu == 5) then   ! simple_export, elements
-     open(15, file='solution_elements_simple_final.txt', status='unknown')
-     write(15, *) tc
-     do i = 1, nelt
-        write(15, '(4f16.6)') u(1, i), u(1, i) + zm(i) - sol_z_offset, u(2, i), u(3, i)
-     end do
-     close(15)
-  end if
-  if (solvisu == 1 .or. solvisu == 2) call vtk_export('solution-Paraview_final.vtk')
-#ifdef CF_WITH_CGNS
-  if (is_cgns == 1 .and. (solvisu == 1 .or. solvisu == 3)) call cgns_snapshot(nsnap)
-#endif
 
   print *, '==========================================================================='
   print *, '=========================   FIN DE LA SIMULATION   ========================'
@@ -385,6 +383,79 @@ program cuteflow_main
   call cf_destroy(h)
 
 contains
+
+  ! global_save_sol_visu (pre_post_traitement.cuf:5-49): snapshot cptsolvisu
+  ! is written when tcv is within +-tolaffiche of its time, tolaffiche
+  ! becoming 1.5 dt (the last step's) after the first snapshot; a snapshot
+  ! time stepped over is skipped with the reference's message.
+  subroutine save_sol_visu(tcv, dtv)
+    real(dp), intent(in) :: tcv, dtv
+    integer :: stv
+    if (solvisu <= 0 .or. cptsolvisu > nsnap) return
+    if (cptsolvisu > 0) tolaff = 1.5d0 * dtv
+    if (tcv <= tsolvisu(cptsolvisu) + tolaff .and. tcv >= tsolvisu(cptsolvisu) - tolaff) then
+       stv = cf_get_state(h, u)
+       call cf_check(stv, 'cf_get_state')
+       call write_snapshot(cptsolvisu, tcv)
+       cptsolvisu = cptsolvisu + 1
+    else if (tcv > tsolvisu(cptsolvisu) + tolaff) then
+       print *, "Skipped save sol ! : The timestep is too large for the number of snapshots to save!", &
+            cptsolvisu
+       cptsolvisu = cptsolvisu + 1
+    end if
+  end subroutine save_sol_visu
+
+  ! The files of snapshot ksnap of the state u at time tcv: paraview_export
+  ! (solvisu 1, 2), cgns_export (is_cgns, solvisu 1, 3), simple_export of
+  ! the nodes (solvisu 1, 4: compute_sol_on_nodes of the current state) and
+  ! of the elements (solvisu 1, 5) (pre_post_traitement.cuf:22-41, 167-207).
+  subroutine write_snapshot(ksnap, tcv)
+    integer, intent(in) :: ksnap
+    real(dp), intent(in) :: tcv
+    character(len=20) :: kstr
+    integer :: ie, jn, nd
+    write(kstr, '(I20)') ksnap
+    kstr = adjustl(kstr)
+    if (solvisu == 1 .or. solvisu == 2) call vtk_export('_solution-Paraview_'//trim(kstr)//'.vtk')
+#ifdef CF_WITH_CGNS
+    if (is_cgns == 1 .and. (solvisu == 1 .or. solvisu == 3)) then
+       if (ksnap == 0) then
+          st = cf_cgns_export_init(trim(out_path)//c_null_char, nsnap + 1, tsolvisu)
+          call cf_cgns_check(st, 'cf_cgns_export_init')
+       end if
+       call cgns_snapshot(ksnap)
+    end if
+#endif
+    if (solvisu == 1 .or. solvisu == 4) then
+       ! compute_sol_on_nodes (pre_post_traitement.cuf:51-71): area-weighted
+       ! average of the cells around each node, accumulated in cell order
+       unodes = 0.d0
+       anodes = 0.d0
+       do ie = 1, nelt
+          do jn = 1, 3
+             nd = conn(ie, jn)
+             unodes(:, nd) = unodes(:, nd) + u(:, ie) * surf(ie)
+             anodes(nd) = anodes(nd) + surf(ie)
+          end do
+       end do
+       open(15, file='solution_noeuds_simple_'//trim(kstr)//'.txt', status='unknown')
+       write(15, *) tcv
+       do nd = 1, nnt
+          unodes(:, nd) = unodes(:, nd) / anodes(nd)
+          write(15, '(4f16.6)') unodes(1, nd), unodes(1, nd) + coords(nd, 3) - sol_z_offset, unodes(2, nd), &
+               unodes(3, nd)
+       end do
+       close(15)
+    end if
+    if (solvisu == 1 .or. solvisu == 5) then
+       open(15, file='solution_elements_simple_'//trim(kstr)//'.txt', status='unknown')
+       write(15, *) tcv
+       do ie = 1, nelt
+          write(15, '(4f16.6)') u(1, ie), u(1, ie) + zm(ie) - sol_z_offset, u(2, ie), u(3, ie)
+       end do
+       close(15)
+    end if
+  end subroutine write_snapshot
 
 #ifdef CF_WITH_CGNS
   ! cgns_export (pre_post_traitement.cuf:287-399) of the current state u; the

@@ -102,6 +102,37 @@ def test_fortran_driver_matches_python(tmp_path):
         assert abs(q - ref["qout"][it - 1]) <= 1e-15 * max(abs(ref["qout"][it - 1]), 1e-300), (it, q)
 
 
+def test_fortran_and_python_drivers_write_the_same_snapshots(tmp_path):
+    """Both hosts on one case with every output on (solvisu=1, 3 snapshots):
+    the snapshot scheduler (global_save_sol_visu) picks the same steps, and
+    the VTK, node and element files are identical line for line (the
+    list-directed time line compared as a number)."""
+    if not os.path.exists(FORTRAN_EXE):
+        pytest.skip("Fortran driver not built")
+    dirs = []
+    for who in ("python", "fortran"):
+        d = tmp_path / who
+        d.mkdir()
+        _make_case(d, ts=12.0)
+        src = (d / "donnees.f").read_text().replace("solvisu=5, visu_snapshots=2", "solvisu=1, visu_snapshots=3")
+        (d / "donnees.f").write_text(src)
+        dirs.append(d)
+    out = run_case(str(dirs[0]), verbose=False)
+    proc = subprocess.run([FORTRAN_EXE], cwd=str(dirs[1]), capture_output=True, text=True, timeout=120)
+    assert proc.returncode == 0, proc.stdout + proc.stderr
+    assert [k for k, _ in out["snapshots"]] == [0, 1, 2, 3]
+    for k in range(4):
+        for name in (f"_solution-Paraview_{k}.vtk", f"solution_noeuds_simple_{k}.txt",
+                     f"solution_elements_simple_{k}.txt"):
+            a = (dirs[0] / name).read_text().splitlines()
+            b = (dirs[1] / name).read_text().splitlines()
+            if name.endswith(".txt"):
+                assert float(a[0]) == float(b[0]), name
+                a, b = a[1:], b[1:]
+            assert a == b, name
+    assert not list(dirs[1].glob("*_final*"))
+
+
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
@@ -155,8 +186,9 @@ def _h5_field(path, ds):
 def test_fortran_driver_reference_cgns_case(tmp_path):
     """The Fortran host on the same reference case directory (CGNS mesh,
     solvisu=3): step count and restart file as the oracle, the reference's
-    boundary table file byte for byte, and the first and last FlowSolutions
-    in out_<meshfile> equal to the oracle's initial and final states."""
+    boundary table file byte for byte, the first FlowSolution in
+    out_<meshfile> equal to the oracle's initial state, and every
+    FlowSolution equal to the Python driver's."""
     from cuteflow_amd.build import CGNS_LIB_PATH
     from cuteflow_amd.solver import initial_state
     if not (os.path.exists(FORTRAN_EXE) and os.path.exists(CGNS_LIB_PATH)):
@@ -184,5 +216,15 @@ def test_fortran_driver_reference_cgns_case(tmp_path):
     out = tmp_path / "out_manning_Canal_divergent.cgns"
     U0 = initial_state(p, mesh, prep.zm)
     assert np.array_equal(_h5_field(out, "/Base/Zone/Solution_0/Water Depth/ data"), U0[:, 0])
-    assert np.array_equal(_h5_field(out, "/Base/Zone/Solution_4/Water Depth/ data"), ref["U"][:, 0])
     assert np.array_equal(_h5_field(out, "/Base/Zone/Solution_4/Bathymetry/ data"), prep.zm)
+    # the Python driver on a copy of the case writes the same FlowSolutions
+    py = tmp_path / "python"
+    py.mkdir()
+    shutil.copyfile(tmp_path / "donnees.f", py / "donnees.f")
+    shutil.copyfile(os.path.join(GOLDEN, "divergent_canal.cgns"), py / "manning_Canal_divergent.cgns")
+    pout = run_case(str(py), verbose=False)
+    assert [k for k, _ in pout["snapshots"]] == [0, 1, 2, 3, 4]
+    for k in range(5):
+        for field in ("Water Depth", "VelocityX", "VelocityY"):
+            ds = f"/Base/Zone/Solution_{k}/{field}/ data"
+            assert np.array_equal(_h5_field(out, ds), _h5_field(py / "out_manning_Canal_divergent.cgns", ds)), ds
