@@ -79,6 +79,15 @@ extern "C" int cf_batch_autotune(int ninst, const cf_mesh *meshes, const cf_para
     int kb = -1;
     for (int k = 0; k < CF_TUNE_NCAND; k++)
         if (t[k] >= 0.0 && (kb < 0 || t[k] < t[kb])) kb = k;
+    // Near-ties (within 3 %, the spread of repeated short timings) go to the
+    // fastest candidate that skips dry tiles: the skip also saves the replay
+    // and the reverse sweep of derivative runs, which are not timed here.
+    if (kb >= 0 && (kb & 2)) {
+        int ks = -1;
+        for (int k = 0; k < CF_TUNE_NCAND; k++)
+            if (t[k] >= 0.0 && !(k & 2) && t[k] <= 1.03 * t[kb] && (ks < 0 || t[k] < t[ks])) ks = k;
+        if (ks >= 0) kb = ks;
+    }
     if (times_ms)
         for (int k = 0; k < CF_TUNE_NCAND; k++) times_ms[k] = t[k];
     if (kb < 0) return first_err != CF_OK ? first_err : CF_ERR_HIP;

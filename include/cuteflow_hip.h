@@ -226,7 +226,10 @@ int cf_get_discharge(cf_handle *h, double *q_in, double *q_out, int *steady);
  * (3, nelt) initial states) under each of CF_TUNE_NCAND schedules and
  * returns in *best_options params[0].options with the tuned bits
  * (CF_OPT_TILE_128, CF_OPT_TILE_64, CF_OPT_NO_TILE_SKIP, CF_OPT_XCD_CONTIG)
- * set to the fastest; pass it as options to cf_batch_create.  Candidate k
+ * set to the fastest (a candidate that skips dry tiles is taken over a
+ * faster one that does not if it is within 3 %: the skip also shortens the
+ * replay and reverse sweep of derivative runs, which are not timed); pass it
+ * as options to cf_batch_create.  Candidate k
  * has cells per tile 256 >> (k / 4) (256, 128, 64), dry-tile skipping off
  * if k & 2, XCD-contiguous tiles if k & 1.  times_ms (CF_TUNE_NCAND
  * entries, may be NULL) receives each candidate's best time of the timed

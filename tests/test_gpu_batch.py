@@ -142,6 +142,10 @@ def test_autotune_picks_a_candidate_with_identical_results():
     assert [o for o, _ in table] == [tune_candidate_options(k) for k in range(_lib.CF_TUNE_NCAND)]
     assert all(t is not None and t > 0.0 for _, t in table)
     k_best = min(range(len(table)), key=lambda k: table[k][1])
+    if k_best & 2:  # skipping off: a skipping candidate within 3 % is preferred
+        near = [k for k in range(len(table)) if not k & 2 and table[k][1] <= 1.03 * table[k_best][1]]
+        if near:
+            k_best = min(near, key=lambda k: table[k][1])
     assert best == table[k_best][0]
     nsteps = 200
     refs = [O.run(O.Prepared(m), p, U, nsteps) for m, p, U in cases]
