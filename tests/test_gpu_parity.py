@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 from cuteflow_amd import mesh as M
+from cuteflow_amd.params import Params
 from cuteflow_amd.solver import Solver
 from oracle import oracle as O
 
@@ -407,3 +408,41 @@ def test_chunked_steps_equal_single_run():
         U2 = s.get_state()
     assert np.array_equal(np.concatenate(parts), h1[:-1])
     assert np.array_equal(U1, U2)
+
+
+def _tiny_case(nx, ny, iflux, seed):
+    """nx x ny quads (2 nx ny triangles) on a sloped, jittered bed with walls
+    all around, per-cell n, and a dam state: a wet block of two depths next
+    to dry cells."""
+    rng = np.random.default_rng(seed)
+    x, y = M._grid_nodes(nx, ny, 10.0 * nx, 10.0 * ny, 0.2, rng)
+    z = -0.01 * x + 0.05 * rng.standard_normal(x.shape)
+    conn = M._grid_triangles(nx, ny)
+    _, walls = M._boundary_nodes(nx, ny)
+    m = M.Mesh(coords=np.stack([x, y, z], 1), conn=conn, manning=rng.uniform(0.02, 0.06, len(conn)), ndwall=walls)
+    p = Params(iflux=iflux, ilimiteur=1, tolisec=1e-3, cfl=0.9, ts=1e300, inlet="transm", outlet="transm",
+                 friction=1, fricimplic=1)
+    xc, _ = m.centroids()
+    U0 = np.zeros((m.nelt, 3))
+    U0[:, 0] = np.where(xc < 3.0 * nx, 2.0, np.where(xc < 7.0 * nx, 0.5, 0.0))
+    U0[:, 1] = np.where(U0[:, 0] > 0, 0.1 * U0[:, 0], 0.0)
+    return m, p, U0
+
+
+@pytest.mark.parametrize("nx,ny", [(1, 1), (2, 2), (4, 8), (33, 1), (129, 1)])
+@pytest.mark.parametrize("iflux", [1, 2, 3, 4])
+def test_tiny_meshes_at_tile_edges(nx, ny, iflux):
+    """Meshes of 2, 8, 64, 66 and 258 triangles (one partial tile, exactly one
+    64-cell tile, one tile and two cells, one 256-cell tile and two cells),
+    every scheme, 256- and 64-cell tiles: the same bits as the oracle."""
+    from cuteflow_amd import _lib
+    m, p, U0 = _tiny_case(nx, ny, iflux, seed=nx * 7 + ny)
+    nsteps = 30
+    ref = O.run(O.Prepared(m), p, U0, nsteps)
+    for options in (0, _lib.CF_OPT_TILE_64 | _lib.CF_OPT_XCD_CONTIG):
+        with Solver(m, p, options=options) as s:
+            s.set_state(U0)
+            hist, done, st = s.step(nsteps, raise_on_error=False)
+            U = s.get_state()
+        assert st == ref["status"] and done == ref["steps"], (options, st, done)
+        _compare(U, ref["U"], np.asarray(hist), ref["dt_hist"], p.tolisec)
