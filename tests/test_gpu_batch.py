@@ -167,3 +167,25 @@ def test_autotune_rejects_bad_arguments():
         autotune([m], [p], [U[:-1]], nsteps=5)
     with pytest.raises(RuntimeError):
         autotune([m], [p], [U], nsteps=0)
+
+
+@pytest.mark.parametrize("iflux", [1, 3])
+def test_batch_with_tiny_instances(iflux):
+    """Instances of 2 and 66 triangles (smaller than a tile, one tile and two
+    cells) batched with a river reach, 256- and 64-cell tiles: each instance
+    gives the oracle's bits for that instance alone."""
+    from cuteflow_amd import _lib
+    from test_gpu_parity import _tiny_case
+    cases = [_tiny_case(1, 1, iflux, seed=3), _tiny_case(33, 1, iflux, seed=4),
+             river_case(30, 8, seed=25, iflux=iflux), _tiny_case(2, 2, iflux, seed=5)]
+    nsteps = 40
+    refs = [O.run(O.Prepared(m), p, U, nsteps) for m, p, U in cases]
+    for options in (0, _lib.CF_OPT_TILE_64 | _lib.CF_OPT_XCD_CONTIG):
+        with BatchSolver([c[0] for c in cases], [c[1] for c in cases], options=options) as bs:
+            for b, (_, _, U) in enumerate(cases):
+                bs.set_state(b, U)
+            hist, done, status = bs.step(nsteps)
+            for b, ref in enumerate(refs):
+                assert done[b] == ref["steps"], (options, b)
+                assert _same(hist[b], ref["dt_hist"]), (options, b)
+                assert _same(bs.get_state(b), ref["U"]), (options, b)
