@@ -44,7 +44,10 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-BYTES_WAF = 156  # algorithmic HBM bytes per cell-update
+# algorithmic HBM bytes per cell-update of the WAF step with per-cell n:
+# state in 24 + out 24, neighbour ids 12, vertex x, y 48, zm/surf/cotemin 24,
+# n 8, centroid x, y 16
+BYTES_WAF = 156
 BYTES_HLLC = 132
 PEAK_HBM_GBS = 8000.0
 METRIC = "FP64 cell-updates/s + %HBM roofline; adjoint/forward cost ratio; batched grads/s"
@@ -236,6 +239,18 @@ def bench_headline(args):
             "sample": f"first {nsamp} steps of the same workload from the same initial state "
                       f"(C restatement of the reference kernels, OpenMP, {threads} threads, 1 MPI rank)",
         }
+        # the same oracle on one host thread, over a shorter sample: the
+        # single-core rate beside the all-core one
+        t0 = time.perf_counter()
+        O.run(prep, p, U0, 1, nthreads=1)
+        per_step1 = time.perf_counter() - t0
+        n1 = max(1, min(nsamp, int(args.cpu_seconds / 3.0 / max(per_step1, 1e-6))))
+        t0 = time.perf_counter()
+        O.run(prep, p, U0, n1, nthreads=1)
+        cpu_t1 = time.perf_counter() - t0
+        log(f"bench: CPU oracle {n1} steps in {cpu_t1:.1f}s on 1 thread")
+        result["cpu_baseline"]["single_core"] = {"value": m.nelt * n1 / cpu_t1, "unit": "cell-updates/s",
+                                                 "cores": 1, "sample": f"first {n1} steps, 1 thread"}
         result["parity_check"] = {"steps": nsamp, "bit_identical": same}
     s.close()
     return result

@@ -2,7 +2,7 @@
 
 The bar: bit-identical states, dt sequences and wet/dry masks on the same
 seeded inputs (the kernels and the oracle evaluate the same FP64 operations
-in the same order, without FMA contraction).  The north-star tolerances
+in the same order, without FMA contraction).  The parity tolerances
 (1e-10 relative L-inf on h, hu, hv; 1e-12 on dt) are asserted as well, so a
 failure message shows which bar broke.
 """
