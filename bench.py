@@ -14,6 +14,9 @@ over the XCDs) is picked by the autotuner before the run (--options auto),
 outside the timed region.
 
 Further sections of the same JSON line (each skippable):
+- "c1": the 20,000-cell circular dam break (HLLC, walls) for its full 1000
+  steps: GPU time per step, and the oracle's whole run on all host threads
+  and on one, with the final states compared bit for bit;
 - "calibration": adjoint/forward cost ratio of a gauge-misfit gradient on a
   100k-cell reach with 8 Manning zones, 20 gauges sampled every 10 steps
   over 2000 steps, checkpoints every 100 steps; the gradient checked
@@ -257,6 +260,62 @@ def bench_headline(args):
 
 
 # ---------------------------------------------------------------------------
+def bench_c1(args):
+    """The small configuration in full: circular dam break on a 200 m x 200 m
+    square of 100 x 100 jittered quads (20,000 triangles), HLLC, walls, no
+    friction, 1000 steps; GPU time per step, the oracle's whole run on all
+    host threads and on one, and the bit comparison of the final states."""
+    import ctypes as C
+
+    import numpy as np
+    import torch
+
+    from cuteflow_amd import _lib
+    from cuteflow_amd import mesh as M
+    from cuteflow_amd.params import Params
+    from cuteflow_amd.solver import Solver
+    from oracle import oracle as O
+
+    nsteps = 1000
+    m = M.dam_break_mesh(100, size=200.0, jitter=0.2, seed=0)
+    p = Params(iflux=1, tolisec=1e-5, cfl=0.9, ts=1e300, inlet="transm", outlet="transm", friction=0)
+    U0 = M.circular_dam_state(m)
+    out = {"workload": "circular dam break, 100 x 100 jittered quads = 20,000 triangles, HLLC, walls, no friction, "
+                       "1000 steps (h 10 m inside r = 30 m, 1 m outside)", "nelt": m.nelt, "steps": nsteps}
+    options, out["schedule"] = section_options(args, [m], [p], [U0], "c1")
+    with Solver(m, p, options=options) as s:
+        stream = torch.cuda.ExternalStream(s.stream)
+        hist = np.zeros(nsteps + 1)
+        done = C.c_int()
+        times = []
+        for _ in range(2):  # the first run captures the step graph
+            s.set_state(U0)
+            sync()
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record(stream)
+            _lib.check(s.lib.cf_step_async(s.h, nsteps), "cf_step_async")
+            ev1.record(stream)
+            _lib.check(s.lib.cf_step_finish(s.h, _lib.dptr(hist), C.byref(done)), "cf_step_finish")
+            sync()
+            times.append(ev0.elapsed_time(ev1) / 1000.0)
+        Ug = s.get_state()
+    t_gpu = times[-1]
+    out.update({"gpu_us_per_step": 1e6 * t_gpu / nsteps, "gpu_cell_updates_per_s": m.nelt * nsteps / t_gpu})
+    log(f"bench: c1 GPU {1e6 * t_gpu / nsteps:.2f} us/step, {m.nelt * nsteps / t_gpu:.3e} cell-updates/s")
+    if args.cpu_seconds > 0:
+        prep = O.Prepared(m)
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        for nt in (threads, 1):
+            t = time.perf_counter()
+            ref = O.run(prep, p, U0, nsteps, nthreads=nt)
+            dt = time.perf_counter() - t
+            same = bool(np.array_equal(Ug, ref["U"]) and np.array_equal(hist[: nsteps + 1], ref["dt_hist"]))
+            out[f"cpu_{'all' if nt == threads else 'one'}_core"] = {
+                "threads": nt, "seconds": dt, "cell_updates_per_s": m.nelt * nsteps / dt, "gpu_bit_identical": same}
+            log(f"bench: c1 CPU oracle {nsteps} steps in {dt:.2f}s on {nt} threads; GPU bit-identical: {same}")
+    return out
+
+
 def bench_calibration(args):
     """Zonal calibration reach: gradient cost, checks, L-BFGS."""
     import numpy as np
@@ -639,8 +698,9 @@ def main():
                     help="before the timed region, run a device copy of this many MiB (3 times): a "
                          "known-size kernel that calibrates HBM byte counters in a profile")
     ap.add_argument("--tune-steps", type=int, default=100, help="steps per autotuner candidate")
-    ap.add_argument("--sections", default="calibration,ensemble,field,scenarios",
-                    help="comma-separated extra sections (calibration, ensemble, field, scenarios); empty for none")
+    ap.add_argument("--sections", default="c1,calibration,ensemble,field,scenarios",
+                    help="comma-separated extra sections (c1, calibration, ensemble, field, scenarios); empty "
+                         "for none")
     ap.add_argument("--nscen", type=int, default=64, help="scenarios of the scenarios section")
     ap.add_argument("--lbfgs-iters", type=int, default=20)
     args = ap.parse_args()
@@ -655,6 +715,8 @@ def main():
     torch.cuda.init()
     result = bench_headline(args)
     sections = [x for x in args.sections.split(",") if x]
+    if "c1" in sections:
+        result["c1"] = bench_c1(args)
     if "calibration" in sections:
         result["calibration"] = bench_calibration(args)
     if "ensemble" in sections:
