@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import numpy as np
 
-from .adjoint import AdjointRun, GaugeLevels, zone_map
+from .adjoint import AdjointRun, GaugeLevels, zone_gather, zone_map
 
 
 def gauge_misfit(eta, obs):
@@ -60,17 +60,18 @@ def zonal_lbfgs(run: AdjointRun, zone_of_cell, n_start, obs, nsteps: int, U0, it
     dev = device or torch.device("cuda")
     nz = len(n_start)
     Z = zone_map(zone_of_cell, nz, device=dev)
+    zidx = torch.as_tensor(np.asarray(zone_of_cell, np.int64), device=dev)
     obs_t = torch.as_tensor(np.asarray(obs).reshape(-1), device=dev)
     n_z = torch.tensor(np.asarray(n_start, dtype=np.float64), device=dev, requires_grad=True)
     with torch.no_grad():
-        l0 = ((GaugeLevels.apply(Z @ n_z, run, nsteps, [U0]) - obs_t) ** 2).sum().item()
+        l0 = ((GaugeLevels.apply(zone_gather(n_z, zidx, Z), run, nsteps, [U0]) - obs_t) ** 2).sum().item()
     opt = torch.optim.LBFGS([n_z], lr=1.0, max_iter=iters, history_size=10, tolerance_grad=1e-14,
                             tolerance_change=1e-16, line_search_fn="strong_wolfe")
     hist = []
 
     def closure():
         opt.zero_grad()
-        eta = GaugeLevels.apply(Z @ n_z, run, nsteps, [U0])
+        eta = GaugeLevels.apply(zone_gather(n_z, zidx, Z), run, nsteps, [U0])
         loss = ((eta - obs_t) ** 2).sum() / l0
         loss.backward()
         hist.append(loss.item())
