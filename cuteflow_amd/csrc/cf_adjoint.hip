@@ -707,8 +707,10 @@ static void enqueue_forward(cf_handle *h, AdjState *A, hipStream_t s) {
 
 extern "C" int cf_adj_forward(cf_handle *h, int nsteps, int sample_every, int checkpoint_every, int want_hmax) {
     if (!h || nsteps <= 0 || sample_every <= 0 || checkpoint_every <= 0 || h->pending >= 0) return CF_ERR_ARG;
+    // phys_inflow couples every inlet face to the inlet's summed depth: its
+    // derivative is not provided
     for (int b = 0; b < h->ninst; b++)
-        if (h->hctrl[b].status != 0 || h->prm[b].is_override_manning) return CF_ERR_ARG;
+        if (h->hctrl[b].status != 0 || h->prm[b].is_override_manning || h->prm[b].inlet == 3) return CF_ERR_ARG;
     AdjState *A = adj_of(h);
     if (!A) return CF_ERR_NOMEM;
     A->recorded = false;

@@ -129,15 +129,33 @@ CF_HD inline void zero_flux(double n1, double n2, T flux[3]) {
     flux[2] = n2 * z + n1 * z;
 }
 
+// phys_inflow: below this wetted inlet area (sum of length x depth) the
+// inlet gives no discharge (flux_riemann.cuf:128, single-precision literal)
+constexpr double kSurfEntreeMin = (double)1e-8f;
+
+// Normal inflow discharge per unit length of an inlet face for 'inflow'
+// (flux_riemann.cuf:121: the inlet's discharge over its length) and
+// 'phys_inflow' (:128-132: shared in proportion to the face's depth hautl;
+// surf_in = the inlet areas of this instance's current state).
+template <class T>
+CF_HD inline T inlet_normal_q(const KParams &p, const double *hydro, const double *surf_in, int id, double tc,
+                              const T &hautl) {
+    if (p.inlet == 2) return T(-(inflow_q(p, hydro, id - 1, tc) / p.long_entree[id - 1]));
+    const double s = surf_in ? surf_in[id - 1] : 0.0;
+    if (s > kSurfEntreeMin) return -((inflow_q(p, hydro, id - 1, tc) * hautl) / s);
+    return T(0.0);
+}
+
 // New state of cell i after one step of length dt starting at time tc.
 // Inflow and outflow discharges of cell i at the start of a step (state u0,
 // time tc): the mass flux x length of its inflow faces (debit_entree_arr,
-// flux_riemann.cuf:125-126, inlet 'inflow') and of its fixed-level outflow
+// flux_riemann.cuf:125-126 and 135-136, inlets 'inflow' and 'phys_inflow';
+// surf_in: the instance's inlet areas) and of its fixed-level outflow
 // faces (debit_sortie_arr, :147-161), each summed over the faces in face
 // order.  The same expressions as the boundary fluxes of cell_update.
 template <class L>
 CF_HD inline void cell_discharge(const L &ld, const KParams &p, const double *hydro, int i, double tc, double &qo,
-                                 double &qi) {
+                                 double &qi, const double *surf_in = nullptr) {
     const double g = p.gp, tls = p.tolisec;
     const St<double> u0 = ld.state(i);
     const I4 nb = ld.nbr(i);
@@ -147,8 +165,15 @@ CF_HD inline void cell_discharge(const L &ld, const KParams &p, const double *hy
         if (code >= 0) continue;
         const int cls = (-code) & 3, id = (-code) >> 2;
         const double n1 = ld.geo(G_N1 + j, i), n2 = ld.geo(G_N2 + j, i), ln = ld.geo(G_LN + j, i);
-        if (cls == 1 && p.inlet == 2) {
-            const double dn = -(inflow_q(p, hydro, id - 1, tc) / p.long_entree[id - 1]);
+        if (cls == 1 && (p.inlet == 2 || p.inlet == 3)) {
+            double dn;
+            if (p.inlet == 2) {
+                dn = inlet_normal_q<double>(p, hydro, surf_in, id, tc, 0.0);
+            } else {
+                double hautl, vitul, vitvl;
+                rot(tls, n1, n2, u0.h, u0.hu, u0.hv, hautl, vitul, vitvl);
+                dn = inlet_normal_q<double>(p, hydro, surf_in, id, tc, hautl);
+            }
             qif[j] = (-dn) * ln;
         } else if (cls == 2 && p.outlet == 1) {
             double hautl, vitul, vitvl;
@@ -167,7 +192,7 @@ CF_HD inline void cell_discharge(const L &ld, const KParams &p, const double *hy
 // (finite_volumes_method.cuf:279-294).
 template <class T, int IFLUX, int FRIC, class L>
 CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro, int i, double dt, double tc,
-                              T out[3], T *xout) {
+                              T out[3], T *xout, const double *surf_in = nullptr) {
     const double g = p.gp, tls = p.tolisec;
     const St<T> u0 = ld.state(i);
     const I4 nb = ld.nbr(i);
@@ -203,8 +228,8 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
                     flux[0] = hautl * vitul;
                     flux[1] = n1 * (hautl * (vitul * vitul) + (g * (hautl * hautl)) / 2.0) - ((n2 * hautl) * vitul) * vitvl;
                     flux[2] = n2 * (hautl * (vitul * vitul) + (g * (hautl * hautl)) / 2.0) + ((n1 * hautl) * vitul) * vitvl;
-                } else if (p.inlet == 2) {
-                    const double dn = -(inflow_q(p, hydro, id - 1, tc) / p.long_entree[id - 1]);
+                } else if (p.inlet == 2 || p.inlet == 3) {
+                    const T dn = inlet_normal_q<T>(p, hydro, surf_in, id, tc, hautl);
                     flux[0] = dn;
                     flux[1] = n1 * ((dn * dn) / (hautl + tls) + (g * (hautl * hautl)) / 2.0);
                     flux[2] = n2 * ((dn * dn) / (hautl + tls) + (g * (hautl * hautl)) / 2.0);

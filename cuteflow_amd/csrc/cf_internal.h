@@ -68,6 +68,9 @@ struct StepArgs {
     // rewritten to (+0, +0, +0, zm).
     unsigned char *tile_ran;
     const unsigned char *prev_use;
+    // phys_inflow: per instance, CF_MAX_IO wetted inlet areas of the step's
+    // input state (null when no instance uses phys_inflow)
+    const double *surf_in;
     int ncell;        // padded cell count (geo stride)
     int tile_cells;   // cells per tile = workgroup size
     int hist_stride;
@@ -178,6 +181,8 @@ struct cf_handle {
     int *tile_adj_off = nullptr, *tile_adj = nullptr;
     unsigned char *always = nullptr;
     int *tile_inst = nullptr, *inst_end = nullptr, *inst_tile = nullptr;
+    double *surf_in = nullptr;         // ninst x CF_MAX_IO inlet areas (phys_inflow)
+    int phys_in = 0;                   // an instance uses inlet = phys_inflow
     double *dt_hist = nullptr, *tc_hist = nullptr;
     int *ran = nullptr;
     int cap = 0;                       // steps per run the history arrays hold

@@ -145,7 +145,8 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
         const double zml = st_in[i].w;
 #endif
         double o[3], xo[3];
-        cell_update<double, IFLUX, FRIC>(ld, p, a.hydro, i, dt, tc, o, a.xs ? xo : nullptr);
+        cell_update<double, IFLUX, FRIC>(ld, p, a.hydro, i, dt, tc, o, a.xs ? xo : nullptr,
+                                         a.surf_in ? a.surf_in + (size_t)b * CF_MAX_IO : nullptr);
         const double h = o[0], hu = o[1], hv = o[2];
         st_out[i] = make_double4(h, hu, hv, zml);
         if (replay) {
@@ -249,9 +250,40 @@ __global__ void discharge_kernel(StepArgs a, const KParams *__restrict__ kp, con
     if (c.status != 0) return;
     const DevLd ld{a.st[(c.par + a.step) & 1], a.nbr, a.dfc, a.geo, a.manning, a.ncell};
     double qo, qi;
-    cell_discharge(ld, kp[b], a.hydro, i, c.tc, qo, qi);
+    cell_discharge(ld, kp[b], a.hydro, i, c.tc, qo, qi, a.surf_in ? a.surf_in + (size_t)b * CF_MAX_IO : nullptr);
     q_io[2 * (size_t)sl] = qo;
     q_io[2 * (size_t)sl + 1] = qi;
+}
+
+// phys_inflow (calcul_surf_entree, maillage.cuf:1368-1421, run before each
+// step by finite_volumes_method.cuf:133-139): thread id-1 of workgroup b
+// sums, over instance b's cells with an inflow or outflow face in the
+// caller's cell order, the length of the cell's last face on inlet id times
+// the cell's depth in the step's input state (the reference assigns per
+// face, so a cell's last face on the inlet counts; it adds block sums with
+// atomics, in no fixed order).
+__global__ void surf_in_kernel(StepArgs a, const int *__restrict__ io_cell, const int *__restrict__ io_off,
+                               double *__restrict__ surf_in) {
+    const int b = blockIdx.x, id = (int)threadIdx.x + 1;
+    if (id > CF_MAX_IO) return;
+    const InstCtrl c = a.ctrl[b];
+    const double4 *__restrict__ st = a.st[(c.par + a.step) & 1];
+    double s = 0.0;
+    for (int q = io_off[b]; q < io_off[b + 1]; q++) {
+        const int i = io_cell[q];
+        const int4 nb = a.nbr[i];
+        double v = 0.0;
+        bool hit = false;
+        for (int j = 0; j < 3; j++) {
+            const int code = nb4_at(nb, j);
+            if (code < 0 && ((-code) & 3) == 1 && ((-code) >> 2) == id) {
+                v = a.geo[(G_LN + j) * (size_t)a.ncell + i] * st[i].x;
+                hit = true;
+            }
+        }
+        if (hit) s = s + v;
+    }
+    surf_in[(size_t)b * CF_MAX_IO + id - 1] = s;
 }
 
 // End of a step, one workgroup per instance: dt of the new state (the
@@ -533,7 +565,7 @@ extern "C" const char *cf_error_string(int s) {
 static int validate_params(const cf_params *p) {
     if (!(p->gp > 0.0) || !(p->cfl > 0.0) || !(p->tolisec >= 0.0)) return CF_ERR_ARG;
     if (p->iflux < 1 || p->iflux > 4) return CF_ERR_ARG;
-    if (p->inlet != 1 && p->inlet != 2) return CF_ERR_ARG;  // 'phys_inflow' not provided
+    if (p->inlet < 1 || p->inlet > 3) return CF_ERR_ARG;
     if (p->outlet != 1 && p->outlet != 2) return CF_ERR_ARG;
     if (p->friction != 0 && p->friction != 1) return CF_ERR_ARG;
     if (p->fricimplic < 0 || p->fricimplic > 2) return CF_ERR_ARG;
@@ -553,6 +585,7 @@ static void destroy_buffers(cf_handle *h) {
     dfree(h->tile_inst); dfree(h->inst_end); dfree(h->inst_tile);
     dfree(h->dt_hist); dfree(h->tc_hist); dfree(h->ran);
     dfree(h->io_cell); dfree(h->io_off); dfree(h->q_io); dfree(h->qin_hist); dfree(h->qout_hist);
+    dfree(h->surf_in);
 }
 
 extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params *params, cf_handle **out) {
@@ -619,6 +652,8 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
     for (int b = 0; b < ninst; b++)
         if (params[b].is_dt_constant == 1) h->skip = 0;
     h->xcd = (params[0].options & CF_OPT_XCD_CONTIG) ? 1 : 0;
+    for (int b = 0; b < ninst; b++)
+        if (params[b].inlet == 3) h->phys_in = 1;
     int cells = 0, tiles = 0;
     for (int b = 0; b < ninst; b++) {
         h->nelt[b] = meshes[b].nelt;
@@ -686,7 +721,8 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
             !dalloc(h->tile_adj, adj.size()) || !dalloc(h->always, (size_t)tiles) ||
             !dalloc(h->tile_inst, (size_t)tiles) || !dalloc(h->inst_end, (size_t)ninst) ||
             !dalloc(h->inst_tile, (size_t)ninst + 1) || !dalloc(h->io_cell, io_cell.size()) || !dalloc(h->io_off, (size_t)ninst + 1) ||
-            !dalloc(h->q_io, 2 * std::max<size_t>((size_t)h->nio, 1))) {
+            !dalloc(h->q_io, 2 * std::max<size_t>((size_t)h->nio, 1)) ||
+            !dalloc(h->surf_in, (size_t)ninst * CF_MAX_IO)) {
             rc = CF_ERR_NOMEM;
             break;
         }
@@ -890,6 +926,7 @@ StepArgs cf::base_args(cf_handle *h) {
     a.out_x = nullptr;
     a.xs = nullptr;
     a.tile_ran = nullptr;
+    a.surf_in = h->phys_in ? h->surf_in : nullptr;
     a.ncell = h->ncell;
     a.tile_cells = h->tile;
     a.hist_stride = h->cap + 1;
@@ -905,6 +942,8 @@ static void enqueue_run(cf_handle *h, int nsteps, hipStream_t s) {
     step_launcher L = pick_launcher(h->iflux, h->fric);
     for (int k = 0; k < nsteps; k++) {
         a.step = k;
+        if (h->phys_in)
+            hipLaunchKernelGGL(surf_in_kernel, dim3(h->ninst), dim3(32), 0, s, a, h->io_cell, h->io_off, h->surf_in);
         L(a, h->ntiles, s);
         if (h->discharge && h->nio)
             hipLaunchKernelGGL(discharge_kernel, dim3(grid_of(h->nio, 256)), dim3(256), 0, s, a, h->d_kp, h->io_cell,

@@ -214,6 +214,8 @@ program cuteflow_main
      prm%inlet = 1
   else if (inlet == 'inflow') then
      prm%inlet = 2
+  else if (inlet == 'phys_inflow') then
+     prm%inlet = 3
   else
      print *, "Bad inlet !"
      error stop

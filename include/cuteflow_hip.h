@@ -60,7 +60,8 @@ typedef struct cf_params {
     int ilimiteur;           /* WAF limiter 0..4; MUSCL slope limiter 0 none, 1 superbee, 2 van Leer, 3 minmod */
     int friction;            /* 0/1 */
     int fricimplic;          /* 0 explicit, 1 (I-dt/2 J)^-1, 2 (I-dt B)^-1 */
-    int inlet;               /* 1 'transm', 2 'inflow' */
+    int inlet;               /* 1 'transm', 2 'inflow', 3 'phys_inflow' (the inlet's discharge shared
+                                over its faces in proportion to the depth; not with cf_adj_*) */
     int outlet;              /* 1 'fixedheight', 2 'transm' */
     int is_dt_constant;
     int is_override_manning;
@@ -206,7 +207,7 @@ int cf_batch_get_geometry(cf_handle *h, int inst, double *surf, double *zm, doub
 /* Discharge bookkeeping and the steady-state stop (flux_riemann.cuf:125-126,
  * 161 record the inflow and outflow edge discharges; check_steady_state,
  * finite_volumes_method.cuf:370-393).  With it on, every step records its
- * flux's inflow discharge (inflow edges, inlet 'inflow') and outflow
+ * flux's inflow discharge (inflow edges, inlets 'inflow' and 'phys_inflow') and outflow
  * discharge (fixed-level outlet edges), each summed over the cells in the
  * caller's order, and a run stops after a step that makes more than 1000
  * consecutive steps with Qin > 1e-8f and |Qin - Qout| / Qin < tol_reg_perm

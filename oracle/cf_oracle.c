@@ -739,6 +739,16 @@ static void or_face(const or_params *p, const int *conn, const double *vx,
             flux[1] = n1 * ((dn * dn) / (hautl + tls) + (g * (hautl * hautl)) / 2.0);
             flux[2] = n2 * ((dn * dn) / (hautl + tls) + (g * (hautl * hautl)) / 2.0);
             *qi = (-dn) * ln;
+        } else if (p->inlet == 3) {
+            /* phys_inflow (flux_riemann.cuf:127-137): the inlet discharge
+               shared out in proportion to the depth */
+            double dn = 0.0;
+            if (p->surf_entree[id - 1] > LIT_1EM8)
+                dn = -((or_inflow_q(p, id - 1, tc) * hautl) / p->surf_entree[id - 1]);
+            flux[0] = dn;
+            flux[1] = n1 * ((dn * dn) / (hautl + tls) + (g * (hautl * hautl)) / 2.0);
+            flux[2] = n2 * ((dn * dn) / (hautl + tls) + (g * (hautl * hautl)) / 2.0);
+            *qi = (-dn) * ln;
         }
     }
     if (kvoisin == -3) {
@@ -1000,8 +1010,35 @@ double or_cfl_dt(const or_params *p, int nelt, const double *cotemin_arr, const 
     return dt;
 }
 
+/* calcul_surf_entree (maillage.cuf:1368-1421): for each inlet id, the sum
+ * over the cells with a face on it of that face's length times the cell's
+ * depth.  A cell with several faces on the inlet counts its last one (the
+ * reference assigns, not adds, per face).  Summed over the cells in cell
+ * order: the reference adds block sums with atomics, in no fixed order. */
+void or_surf_entree(const or_params *p, int nelt, const double *vx, const double *vy, const int *b1,
+                    const int *b2, const double *U, double *surf_entree)
+{
+    for (int id = 1; id <= OR_MAX_IO; id++) {
+        double s = 0.0;
+        for (int gi = 0; gi < nelt; gi++) {
+            double c = 0.0;
+            int hit = 0;
+            for (int tj = 0; tj < 3; tj++)
+                if (b1[3 * gi + tj] == -1 && b2[3 * gi + tj] == id) {
+                    double n1, n2, ln;
+                    or_face_geom(vx, vy, gi, tj, &n1, &n2, &ln);
+                    c = ln * U[3 * gi];
+                    hit = 1;
+                }
+            if (hit) s = s + c;
+        }
+        surf_entree[id - 1] = s;
+    }
+    (void)p;
+}
+
 /* solve_finite_volumes time loop (finite_volumes_method.cuf:113-242) */
-int or_run(const or_params *p, int nelt, const int *conn, const double *vx,
+int or_run(const or_params *p_in, int nelt, const int *conn, const double *vx,
            const double *vy, const double *xc, const double *yc,
            const double *surf, const double *zm, const double *cotemin_arr,
            const double *manning, const int *b1, const int *b2, const int *upw,
@@ -1018,6 +1055,8 @@ int or_run(const or_params *p, int nelt, const int *conn, const double *vx,
     double *qic = (double *)malloc((size_t)nelt * sizeof(double));
     double tc = 0.0, dt, dtn;
     int n = 0, status = 0, reg = 0, reached = 0;
+    or_params pl = *p_in;  /* the per-step inlet areas live in this copy */
+    const or_params *p = &pl;
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #else
@@ -1028,6 +1067,8 @@ int or_run(const or_params *p, int nelt, const int *conn, const double *vx,
     dt = or_cfl_dt(p, nelt, cotemin_arr, U);
     while (status == 0 && !reached && n < nsteps && ((tc - p->ts) + 2.0 * p->tol) < p->tol) {
         double qo = 0.0, qi = 0.0;
+        if (p->inlet == 3)  /* finite_volumes_method.cuf:133-139 */
+            or_surf_entree(p, nelt, vx, vy, b1, b2, U, pl.surf_entree);
         or_compute_flux(p, nelt, conn, vx, vy, xc, yc, zm, b1, b2, upw, U, dt, tc, resc, qc, qic);
         or_source_terms(p, nelt, vx, vy, surf, zm, manning, b1, U, dt, sb, sf, afm1);
         or_update(p, nelt, surf, U, resc, sb, sf, afm1, dt, U1);

@@ -37,7 +37,7 @@ typedef struct or_params {
     int ilimiteur;        /* 0..4 */
     int friction;         /* 0/1 */
     int fricimplic;       /* 0,1,2 */
-    int inlet;            /* 1 transm, 2 inflow, 3 phys_inflow (unsupported) */
+    int inlet;            /* 1 transm, 2 inflow, 3 phys_inflow */
     int outlet;           /* 1 fixedheight, 2 transm */
     int is_dt_constant;
     int is_override_manning;
@@ -51,6 +51,9 @@ typedef struct or_params {
     int hydro_n[OR_MAX_IO];
     double hydro_t[OR_MAX_IO * OR_MAX_HYDRO];
     double hydro_q[OR_MAX_IO * OR_MAX_HYDRO];
+    /* phys_inflow: wetted area sum(length x depth) of each inlet for the
+       current state; or_run recomputes it before every step */
+    double surf_entree[OR_MAX_IO];
 } or_params;
 
 /* project pow for x > 0 (exp(y*log(x)) in double-double), shared algorithm
@@ -105,6 +108,11 @@ double or_cfl_dt(const or_params *p, int nelt, const double *cotemin_arr,
  * finite_volumes_method.cuf:370-393).
  * Returns 0 on success, -1 on a dt sanity failure (dt>9999, dt<1e-12 or NaN).
  */
+/* calcul_surf_entree: wetted area of each inlet (length x depth summed over
+ * its cells in cell order, a cell's last face on the inlet), for phys_inflow */
+void or_surf_entree(const or_params *p, int nelt, const double *vx, const double *vy, const int *b1,
+                    const int *b2, const double *U, double *surf_entree);
+
 int or_run(const or_params *p, int nelt, const int *conn, const double *vx,
            const double *vy, const double *xc, const double *yc,
            const double *surf, const double *zm, const double *cotemin_arr,

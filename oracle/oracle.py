@@ -36,6 +36,7 @@ class OrParams(C.Structure):
         ("hydro_n", C.c_int * MAX_IO),
         ("hydro_t", C.c_double * (MAX_IO * MAX_HYDRO)),
         ("hydro_q", C.c_double * (MAX_IO * MAX_HYDRO)),
+        ("surf_entree", C.c_double * MAX_IO),
     ]
 
 

@@ -446,3 +446,45 @@ def test_tiny_meshes_at_tile_edges(nx, ny, iflux):
             U = s.get_state()
         assert st == ref["status"] and done == ref["steps"], (options, st, done)
         _compare(U, ref["U"], np.asarray(hist), ref["dt_hist"], p.tolisec)
+
+
+@pytest.mark.parametrize("iflux", [1, 3])
+@pytest.mark.parametrize("options", [0, 24])
+def test_phys_inflow_matches_oracle(iflux, options):
+    """inlet = 'phys_inflow' (the inlet discharge shared in proportion to the
+    depth, flux_riemann.cuf:127-137, the wetted inlet area recomputed before
+    every step): states, dt and the per-step discharges equal the oracle's
+    bit for bit, alone and next to an 'inflow' instance in a batch; the
+    adjoint refuses it."""
+    from cuteflow_amd import _lib
+    from cuteflow_amd.adjoint import AdjointRun
+    from cuteflow_amd.solver import BatchSolver
+    m, p, U0 = river_case(nx=60, ny=12, seed=32, iflux=iflux, q_peak=400.0, t_ramp=20.0)
+    p.inlet = "phys_inflow"
+    nsteps = 300
+    ref = O.run(O.Prepared(m), p, U0, nsteps, qout=True)
+    assert ref["status"] == 0 and np.all(ref["qin"][1:] > 0)
+    with Solver(m, p, options=options) as s:
+        s.set_discharge(True)
+        s.set_state(U0)
+        hist, done, st = s.step(nsteps)
+        q_in, q_out, _ = s.discharge()
+        assert st == 0 and done == nsteps
+        _compare(s.get_state(), ref["U"], np.asarray(hist), ref["dt_hist"], p.tolisec)
+        assert np.array_equal(q_in.view(np.int64), ref["qin"].view(np.int64))
+        assert np.array_equal(q_out.view(np.int64), ref["qout"].view(np.int64))
+        if options == 0:
+            run = AdjointRun(s, [np.array([0], np.int32)], sample_every=10, checkpoint_every=10)
+            with pytest.raises(_lib.CuteflowError) as err:
+                run.forward(20, U0=[U0])
+            assert err.value.status == _lib.CF_ERR_ARG
+    m2, p2, U2 = river_case(nx=40, ny=10, seed=33, iflux=iflux)
+    ref2 = O.run(O.Prepared(m2), p2, U2, nsteps)
+    with BatchSolver([m, m2], [p, p2], options=options) as bs:
+        bs.set_state(0, U0)
+        bs.set_state(1, U2)
+        hist, done, status = bs.step(nsteps)
+        for b, r in enumerate((ref, ref2)):
+            assert done[b] == r["steps"]
+            assert np.array_equal(np.asarray(hist[b]).view(np.int64), r["dt_hist"].view(np.int64))
+            assert np.array_equal(bs.get_state(b).view(np.int64), r["U"].view(np.int64))

@@ -491,6 +491,46 @@ def test_inflow_discharge_enters():
     assert abs(vol - np.sum(q_in * dts)) <= 1e-9 * np.sum(q_in * dts)
 
 
+def test_phys_inflow_discharge_and_uniform_depth():
+    """phys_inflow shares the inlet discharge over the inlet faces in
+    proportion to depth (flux_riemann.cuf:127-137, maillage.cuf:1368-1421):
+    - at every step the inflow discharge summed over the faces equals the
+      hydrograph's Q (the weights h L / sum(h L) add up to one), with the
+      depth varying along the inlet;
+    - from the same depth on every inlet cell, a step is the 'inflow'
+      condition's step (Q over the inlet length on each face) to rounding;
+      from unequal depths the steps differ."""
+    m, p, U0 = river_case(30, 8, seed=31, iflux=1)
+    p.inlet = "phys_inflow"
+    prep = O.Prepared(m)
+    r = O.run(prep, p, U0, 200, qout=True)
+    assert r["status"] == 0
+    tcs = np.concatenate([[0.0], np.cumsum(r["dt_hist"][: r["steps"]])[:-1]])
+    q = np.array([O.inflow_q(p, 1, t) for t in tcs])
+    wet = q > 0
+    assert wet.sum() > 150
+    assert np.max(np.abs(r["qin"][wet] - q[wet]) / q[wet]) <= 1e-14
+    # the inlet cells: uniform depth -> the inflow run to rounding
+    inlet_cells = np.nonzero((prep.b1.reshape(-1, 3) == -1).any(axis=1))[0]
+    assert len(inlet_cells) >= 2
+    for uniform in (True, False):
+        U = U0.copy()
+        U[:, 0] = np.maximum(U[:, 0], 0.5)
+        U[inlet_cells, 0] = 1.5 if uniform else 1.0 + 0.1 * np.arange(len(inlet_cells))
+        pi = river_case(30, 8, seed=31, iflux=1)[1]
+        pp = river_case(30, 8, seed=31, iflux=1)[1]
+        pp.inlet = "phys_inflow"
+        for q in (pi, pp):  # a constant 300 m3/s from t = 0
+            q.hydrographs = {1: ([0.0, 1.0], [300.0, 300.0])}
+        a, b = O.run(prep, pi, U, 1), O.run(prep, pp, U, 1)
+        assert a["status"] == b["status"] == 0
+        d = np.max(np.abs(a["U"] - b["U"])) / np.max(np.abs(a["U"]))
+        if uniform:
+            assert d <= 1e-12, d
+        else:
+            assert d > 1e-6, d
+
+
 def test_nan_state_is_an_error():
     m, p, U0 = dam_case(10)
     U0[5, 1] = np.nan
