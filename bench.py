@@ -76,11 +76,13 @@ def make_workload(nx, ny, order, seed=1):
         perm = None
     if perm is not None:
         m, U0 = m.permuted(perm), U0[perm]
+    else:
+        perm = np.arange(m.nelt)
     p = Params(iflux=3, ilimiteur=1, tolisec=1e-3, cfl=0.9, ts=1e300, inlet="inflow",
                outlet="fixedheight", friction=1, fricimplic=1, nombre_entrees=1, nombre_sorties=1)
     p.h_sortie[0] = r.h_out
     p.hydrographs = {1: ([0.0, r.t_ramp], [0.0, r.q_peak])}
-    return m, p, np.ascontiguousarray(U0)
+    return m, p, np.ascontiguousarray(U0), perm
 
 
 def traffic_from_profile(workload_key):
@@ -97,18 +99,71 @@ def sync():
     torch.cuda.synchronize()
 
 
-# ---------------------------------------------------------------------------
-def bench_headline(args):
+def cpu_info():
+    """Host CPU model and the cores this process may use (affinity mask and
+    the cgroup CPU quota, when one is set)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    quota = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            q, per = fh.read().split()[:2]
+            if q != "max":
+                quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    usable = len(os.sched_getaffinity(0))
+    if quota is not None:
+        usable = max(1, min(usable, int(quota)))
+    return {"model": model, "logical_cpus": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "cgroup_cpu_quota": quota, "usable_cores": usable}
+
+
+def timed_run(s, nsteps):
+    """nsteps steps of solver s, already prepared for that step count:
+    returns (wall seconds between two device synchronisations, seconds on
+    the solver's stream by HIP events, run statistics)."""
     import ctypes as C
 
     import numpy as np
     import torch
 
     from cuteflow_amd import _lib
+
+    stream = torch.cuda.ExternalStream(s.stream)
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    hist = np.zeros(nsteps + 1)
+    done = C.c_int()
+    sync()
+    w0 = time.perf_counter()
+    ev0.record(stream)
+    _lib.check(s.lib.cf_step_async(s.h, nsteps), "cf_step_async")
+    ev1.record(stream)
+    _lib.check(s.lib.cf_step_finish(s.h, _lib.dptr(hist), C.byref(done)), "cf_step_finish")
+    sync()
+    w1 = time.perf_counter()
+    if done.value != nsteps:
+        raise RuntimeError(f"run stopped after {done.value} of {nsteps} steps")
+    return w1 - w0, ev0.elapsed_time(ev1) / 1000.0, s.last_run_stats()
+
+
+# ---------------------------------------------------------------------------
+def bench_headline(args):
+    import numpy as np
+    import torch
+
     from cuteflow_amd.solver import Solver, autotune
 
     t0 = time.perf_counter()
-    m, p, U0 = make_workload(args.nx, args.ny, args.order)
+    m, p, U0, perm = make_workload(args.nx, args.ny, args.order)
     log(f"bench: workload nelt={m.nelt} order={args.order} built in {time.perf_counter() - t0:.1f}s")
     schedule = {"autotuned": args.options == "auto"}
     if args.options == "auto":
@@ -122,9 +177,11 @@ def bench_headline(args):
     schedule.update(describe_options(options), options=options)
     s = Solver(m, p, options=options)
     s.set_state(U0)
-    lib = s.lib
     if args.warmup:
         s.step(args.warmup)
+    # one-time work of a run of args.steps steps (history, graph capture)
+    # stays out of the timed region
+    s.prepare(args.steps)
     log(f"bench: warmup {args.warmup} steps done, tc={s.tc:.3f}s")
 
     if args.copy_probe_mb:
@@ -137,32 +194,16 @@ def bench_headline(args):
         log(f"bench: copy probe {args.copy_probe_mb} MiB read + {args.copy_probe_mb} MiB written, 3 times")
         del src, dst
 
-    stream = torch.cuda.ExternalStream(s.stream)
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    hist = np.zeros(args.steps + 1)
-    done = C.c_int()
-    sync()
-    w0 = time.perf_counter()
-    ev0.record(stream)
-    _lib.check(lib.cf_step_async(s.h, args.steps), "cf_step_async")
-    ev1.record(stream)
-    _lib.check(lib.cf_step_finish(s.h, _lib.dptr(hist), C.byref(done)), "cf_step_finish")
-    sync()
-    w1 = time.perf_counter()
-    if done.value != args.steps:
-        raise RuntimeError(f"run stopped after {done.value} of {args.steps} steps")
-    wall = w1 - w0
-    kern_ms = ev0.elapsed_time(ev1)
-    per_launch_s = kern_ms / 1000.0 / args.steps
-    stats = s.last_run_stats()
+    wall, t_stream, stats = timed_run(s, args.steps)
+    per_launch_s = t_stream / args.steps
     value = m.nelt * args.steps / wall
     # cells the step kernel actually processed per launch: dry tiles with dry
     # neighbours are skipped (their update is the identity)
     active_cells = stats["active_fraction"] * m.nelt
     achieved = BYTES_WAF * active_cells / per_launch_s / 1e9
-    wet = int(np.sum(s.get_state()[:, 0] > p.tolisec))
-    log(f"bench: {args.steps} steps in {wall:.3f}s wall, {kern_ms:.2f} ms on the stream, "
+    U_end = s.get_state()
+    wet = int(np.sum(U_end[:, 0] > p.tolisec))
+    log(f"bench: {args.steps} steps in {wall * 1e3:.3f} ms wall, {t_stream * 1e3:.3f} ms on the stream, "
         f"{value:.3e} cell-updates/s, active tiles {stats['active_fraction']:.3f}, "
         f"{achieved:.0f} GB/s algorithmic over processed cells, wet cells {wet}")
 
@@ -210,7 +251,52 @@ def bench_headline(args):
             "timing": "HIP events on the solver stream over the timed steps (step kernel + "
                       "one-workgroup finalize kernel per step)",
         },
+        "processed_cell_updates_per_s": active_cells * args.steps / wall,
+        "processed_cell_updates_definition": "cells of the tiles the step kernel ran (wet tiles and their "
+                                             "neighbours) x steps / wall time; value counts every cell",
     }
+    workload_desc = result["config"]["workload"]
+
+    # The same run with every tile computed every step (no dry-tile
+    # skipping): the cost of the whole mesh, and the bits of the skipping run.
+    from cuteflow_amd import _lib
+    with Solver(m, p, options=options | _lib.CF_OPT_NO_TILE_SKIP) as sa:
+        sa.set_state(U0)
+        if args.warmup:
+            sa.step(args.warmup)
+        sa.prepare(args.steps)
+        wall_a, ts_a, _ = timed_run(sa, args.steps)
+        same_a = bool(np.array_equal(sa.get_state(), U_end))
+    ach_a = BYTES_WAF * m.nelt / (ts_a / args.steps) / 1e9
+    log(f"bench: every tile: {wall_a * 1e6 / args.steps:.1f} us/step, {ach_a:.0f} GB/s algorithmic, "
+        f"bit-identical to the skipping run: {same_a}")
+    result["all_tiles"] = {
+        "workload": workload_desc + ", every tile computed every step",
+        "ms_per_step": wall_a * 1000.0 / args.steps, "cell_updates_per_s": m.nelt * args.steps / wall_a,
+        "avg_launch_us": ts_a / args.steps * 1e6, "achieved_gbs": ach_a, "frac": ach_a / PEAK_HBM_GBS,
+        "bit_identical_to_skipping_run": same_a}
+
+    # The same reach stored in random cell order (the cells are laid out on
+    # the device along a Hilbert curve whatever the input order).
+    if args.order != "random":
+        mr, pr, U0r, perm_r = make_workload(args.nx, args.ny, "random")
+        with Solver(mr, pr, options=options) as sr:
+            sr.set_state(U0r)
+            if args.warmup:
+                sr.step(args.warmup)
+            sr.prepare(args.steps)
+            wall_r, ts_r, st_r = timed_run(sr, args.steps)
+            Ur = sr.get_state()
+        nat_a, nat_r = np.empty_like(U_end), np.empty_like(Ur)
+        nat_a[perm], nat_r[perm_r] = U_end, Ur
+        same_r = bool(np.array_equal(nat_a, nat_r))
+        log(f"bench: random order: {wall_r * 1e6 / args.steps:.1f} us/step, bit-identical to the "
+            f"{args.order}-order run: {same_r}")
+        result["random_order"] = {
+            "workload": workload_desc.replace(f"{args.order} order", "random order"),
+            "ms_per_step": wall_r * 1000.0 / args.steps, "cell_updates_per_s": m.nelt * args.steps / wall_r,
+            "avg_launch_us": ts_r / args.steps * 1e6, "active_tile_fraction": st_r["active_fraction"],
+            "bit_identical_to_headline": same_r}
 
     # CPU baseline: the oracle on a bounded sample of the same workload, on
     # the host cores, started from the same initial state; the GPU result
@@ -218,14 +304,15 @@ def bench_headline(args):
     if args.cpu_seconds > 0:
         from oracle import oracle as O
 
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        host = cpu_info()
+        threads = args.cpu_threads or host["usable_cores"]
         t0 = time.perf_counter()
         prep = O.Prepared(m)
-        log(f"bench: oracle preprocessing {time.perf_counter() - t0:.1f}s")
+        log(f"bench: oracle preprocessing {time.perf_counter() - t0:.1f}s; host {host}")
         t0 = time.perf_counter()
         O.run(prep, p, U0, 2, nthreads=threads)
         per_step = (time.perf_counter() - t0) / 2
-        nsamp = max(2, min(args.steps, int(args.cpu_seconds / max(per_step, 1e-6))))
+        nsamp = max(2, min(2000, int(args.cpu_seconds / max(per_step, 1e-6))))
         t0 = time.perf_counter()
         ref = O.run(prep, p, U0, nsamp, nthreads=threads)
         cpu_t = time.perf_counter() - t0
@@ -240,14 +327,16 @@ def bench_headline(args):
             "cores": threads,
             "kind": "port",
             "sample": f"first {nsamp} steps of the same workload from the same initial state "
-                      f"(C restatement of the reference kernels, OpenMP, {threads} threads, 1 MPI rank)",
+                      f"(C restatement of the reference kernels, gcc -O3, OpenMP, {threads} threads = every "
+                      f"core this process may use, 1 MPI rank)",
+            "cpu": host,
         }
         # the same oracle on one host thread, over a shorter sample: the
         # single-core rate beside the all-core one
         t0 = time.perf_counter()
         O.run(prep, p, U0, 1, nthreads=1)
         per_step1 = time.perf_counter() - t0
-        n1 = max(1, min(nsamp, int(args.cpu_seconds / 3.0 / max(per_step1, 1e-6))))
+        n1 = max(1, min(nsamp, int(args.cpu_seconds / 4.0 / max(per_step1, 1e-6))))
         t0 = time.perf_counter()
         O.run(prep, p, U0, n1, nthreads=1)
         cpu_t1 = time.perf_counter() - t0

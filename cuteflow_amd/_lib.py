@@ -100,6 +100,7 @@ SIGNATURES = {
     "cf_step": (C.c_int, [_H, C.c_int, _DP, _IP]),
     "cf_step_async": (C.c_int, [_H, C.c_int]),
     "cf_step_finish": (C.c_int, [_H, _DP, _IP]),
+    "cf_step_prepare": (C.c_int, [_H, C.c_int]),
     "cf_set_inflow": (C.c_int, [_H, _DP, C.c_int]),
     "cf_set_hydrograph": (C.c_int, [_H, C.c_int, C.c_int, _DP, _DP]),
     "cf_set_manning": (C.c_int, [_H, C.c_void_p]),

@@ -955,10 +955,11 @@ static void enqueue_run(cf_handle *h, int nsteps, hipStream_t s) {
 
 // A run's launches are captured once per step count and replayed: every
 // argument is fixed (per-run state lives in the device control block).
-static int run_graph(cf_handle *h, int nsteps) {
+// Finds or captures the graph of a run of nsteps steps (nothing is launched).
+static int graph_of(cf_handle *h, int nsteps, hipGraphExec_t *out) {
     for (auto &g : h->graphs)
         if (g.nsteps == nsteps) {
-            HIPCHK(hipGraphLaunch(g.exec, h->stream));
+            *out = g.exec;
             return CF_OK;
         }
     hipGraph_t graph;
@@ -974,8 +975,21 @@ static int run_graph(cf_handle *h, int nsteps) {
         h->graphs.erase(h->graphs.begin());
     }
     h->graphs.push_back({nsteps, exec});
-    HIPCHK(hipGraphLaunch(exec, h->stream));
+    *out = exec;
     return CF_OK;
+}
+
+// the null stream cannot be captured; very long runs are enqueued directly
+static bool graphed(const cf_handle *h, int nsteps) { return h->use_graphs && h->stream && nsteps <= 4096; }
+
+extern "C" int cf_step_prepare(cf_handle *h, int nsteps) {
+    if (!h || nsteps < 0) return CF_ERR_ARG;
+    if (h->pending >= 0) return CF_ERR_ARG;
+    int rc = ensure_hist(h, std::max(nsteps, 1));
+    if (rc) return rc;
+    if (nsteps == 0 || !graphed(h, nsteps)) return CF_OK;
+    hipGraphExec_t exec;
+    return graph_of(h, nsteps, &exec);
 }
 
 extern "C" int cf_step_async(cf_handle *h, int nsteps) {
@@ -986,10 +1000,16 @@ extern "C" int cf_step_async(cf_handle *h, int nsteps) {
     if (!any) return CF_ERR_DT;
     int rc = ensure_hist(h, std::max(nsteps, 1));
     if (rc) return rc;
+    if (nsteps > 0 && graphed(h, nsteps)) {
+        hipGraphExec_t exec;
+        rc = graph_of(h, nsteps, &exec);
+        if (rc) return rc;
+        h->pending = nsteps;
+        HIPCHK(hipGraphLaunch(exec, h->stream));
+        return CF_OK;
+    }
     h->pending = nsteps;
     if (nsteps == 0) return CF_OK;
-    // the null stream cannot be captured
-    if (h->use_graphs && h->stream && nsteps <= 4096) return run_graph(h, nsteps);
     enqueue_run(h, nsteps, h->stream);
     HIPCHK(hipGetLastError());
     return CF_OK;

@@ -91,6 +91,11 @@ class Solver:
             raise _lib.CuteflowError(st, "cf_step")
         return hist[: done.value + 1], done.value, st
 
+    def prepare(self, nsteps: int) -> None:
+        """One-time work of runs of nsteps steps done now (history sized, the
+        run's graph captured): the next step(nsteps) launches only."""
+        _lib.check(self.lib.cf_step_prepare(self.h, int(nsteps)), "cf_step_prepare")
+
     def set_discharge(self, on: bool = True) -> None:
         """Per-step inflow/outflow discharge of the following runs (always on
         when tol_reg_perm > 0: the steady-state stop uses it)."""
@@ -304,6 +309,10 @@ class BatchSolver:
         if st != _lib.CF_OK and (raise_on_error or st != _lib.CF_ERR_DT):
             raise _lib.CuteflowError(st, "cf_batch_step")
         return [hist[b, : done[b] + 1] for b in range(B)], done, status
+
+    def prepare(self, nsteps: int) -> None:
+        """One-time work of runs of nsteps steps done now (see Solver.prepare)."""
+        _lib.check(self.lib.cf_step_prepare(self.h, int(nsteps)), "cf_step_prepare")
 
     def set_discharge(self, on: bool = True) -> None:
         _lib.check(self.lib.cf_set_discharge(self.h, int(bool(on))), "cf_set_discharge")

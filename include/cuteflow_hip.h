@@ -149,6 +149,12 @@ int cf_step(cf_handle *h, int nsteps, double *dt_hist, int *steps_done);
  * one cf_step_finish before any other call on the handle. */
 int cf_step_async(cf_handle *h, int nsteps);
 int cf_step_finish(cf_handle *h, double *dt_hist, int *steps_done);
+/* Readies the handle for runs of nsteps steps without running any: sizes the
+ * per-run history and captures the run's launches in a graph, the one-time
+ * work a first cf_step/cf_step_async of that length would do (no reference
+ * counterpart: the reference launches each step's kernels from the host).
+ * Call it outside a timed region or before a latency-bound loop. */
+int cf_step_prepare(cf_handle *h, int nsteps);
 
 /* Work done by the last cf_step/cf_step_finish: tiles that ran (the others
  * were dry with dry neighbours and skipped), tiles x steps in total, and the
