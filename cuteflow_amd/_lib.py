@@ -32,6 +32,7 @@ CF_ERR_DT = -3
 CF_ERR_MESH = -4
 CF_ERR_NOMEM = -5
 CF_ERR_IO = -6
+CF_ERR_SIZE = -7
 
 
 class CfParams(C.Structure):
@@ -87,6 +88,9 @@ SIGNATURES = {
                                     C.c_int, _IP, _IP]),
     "cf_build_boundary_device": (C.c_int, [C.c_int, C.c_int, _IP, C.c_int, _IP, _IP, C.c_int, _IP, _IP,
                                            C.c_int, _IP, _IP]),
+    "cf_build_boundary_device_noalloc": (C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                                   C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                                   C.c_void_p]),
     "cf_inlet_lengths": (C.c_int, [C.c_int, C.c_int, _IP, _DP, _IP, C.c_int, _DP]),
     "cf_create": (C.c_int, [C.POINTER(CfMesh), C.POINTER(CfParams), C.POINTER(_H)]),
     "cf_destroy": (None, [_H]),

@@ -18,6 +18,8 @@
 #include <algorithm>
 #include <climits>
 #include <cstdint>
+#include <cstdio>
+#include <mutex>
 #include <vector>
 
 #include "cf_prep.h"
@@ -385,6 +387,119 @@ extern "C" int cf_build_boundary_device(int nelt, int nnt, const int *conn, int 
                        d_pw, (const int *)d_nin, (const int *)d_nout, d_bnd);
     GCHK(hipGetLastError());
     if (!down(boundary, d_bnd, 6 * N, s) || hipStreamSynchronize(s) != hipSuccess) return CF_ERR_HIP;
+    return CF_OK;
+}
+
+namespace {
+
+// ---- boundary table without device allocations ----------------------------
+//
+// cf_build_boundary_device_noalloc works in `scratch`, a fixed device array
+// (each GPU the library runs on holds its own copy until the process ends):
+// edge keys and cells twice over (the radix sort ping-pongs between them),
+// the first positions of the nodes in the three node lists, an error flag
+// and the sort's temporary storage.  Host code reaches it only through the
+// address hipGetSymbolAddress returns for the current device, taken once per
+// device on the first call.
+
+constexpr size_t kNaMaxCells = size_t(1) << 17;
+constexpr size_t kNaMaxNodes = size_t(1) << 17;
+constexpr size_t kNaMaxEdges = 3 * kNaMaxCells;
+constexpr size_t kNaSortTemp = size_t(4) << 20;
+
+constexpr size_t na_round(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t kNaKeys0 = 0;
+constexpr size_t kNaKeys1 = kNaKeys0 + na_round(kNaMaxEdges * sizeof(u64));
+constexpr size_t kNaVals0 = kNaKeys1 + na_round(kNaMaxEdges * sizeof(u64));
+constexpr size_t kNaVals1 = kNaVals0 + na_round(kNaMaxEdges * sizeof(int));
+constexpr size_t kNaPos = kNaVals1 + na_round(kNaMaxEdges * sizeof(int));
+constexpr size_t kNaErr = kNaPos + na_round(3 * (kNaMaxNodes + 1) * sizeof(int));
+constexpr size_t kNaTemp = kNaErr + 256;
+constexpr size_t kNaBytes = kNaTemp + kNaSortTemp;
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+constexpr int kNaMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[kNaMaxDevices];
+
+// Device address of the current device's scratch array.
+static int scratch_of_current_device(unsigned char **out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return CF_ERR_HIP;
+    if (dev < 0 || dev >= kNaMaxDevices) return CF_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "cuteflow_hip: hipGetSymbolAddress(scratch) on device %d failed: %s\n", dev,
+                    hipGetErrorString(e));
+            return CF_ERR_HIP;
+        }
+        g_scratch_ptr[dev] = (unsigned char *)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return CF_OK;
+}
+
+}  // namespace
+
+extern "C" int cf_build_boundary_device_noalloc(int nelt, int nnt, const int *conn, int ndi, const int *ndinput,
+                                                const int *num_in, int ndo, const int *ndoutput, const int *num_out,
+                                                int ndw, const int *ndwall, int *boundary, void *stream) {
+    if (nelt <= 0 || nnt <= 0 || !conn || !boundary || ndi < 0 || ndo < 0 || ndw < 0 || (ndi > 0 && !ndinput) ||
+        (ndo > 0 && !ndoutput) || (ndw > 0 && !ndwall))
+        return CF_ERR_ARG;
+    if ((size_t)nelt > kNaMaxCells || (size_t)nnt > kNaMaxNodes) {
+        fprintf(stderr,
+                "cuteflow_hip: cf_build_boundary_device_noalloc: %d cells, %d nodes exceed its fixed scratch "
+                "(at most %zu cells and %zu nodes); use cf_build_boundary_device\n",
+                nelt, nnt, kNaMaxCells, kNaMaxNodes);
+        return CF_ERR_SIZE;
+    }
+    unsigned char *base = nullptr;
+    int rc = scratch_of_current_device(&base);
+    if (rc) return rc;
+    const hipStream_t s = (hipStream_t)stream;
+    const size_t N = (size_t)nelt, M = 3 * N, P = (size_t)nnt + 1;
+    u64 *keys0 = (u64 *)(base + kNaKeys0), *keys1 = (u64 *)(base + kNaKeys1);
+    int *vals0 = (int *)(base + kNaVals0), *vals1 = (int *)(base + kNaVals1);
+    int *pos[3] = {(int *)(base + kNaPos), (int *)(base + kNaPos) + P, (int *)(base + kNaPos) + 2 * P};
+    int *d_err = (int *)(base + kNaErr);
+    void *temp = base + kNaTemp;
+    // the scratch keeps the previous call's contents: clear the error flag
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, d_err, (size_t)1, 0);
+    hipLaunchKernelGGL(check_conn_kernel, dim3(blocks(M)), dim3(256), 0, s, conn, M, nnt, d_err);
+    int err = 0;
+    GCHK(hipGetLastError());
+    if (!down(&err, d_err, 1, s) || hipStreamSynchronize(s) != hipSuccess) return CF_ERR_HIP;
+    if (err) return CF_ERR_MESH;
+    hipLaunchKernelGGL(edge_keys_kernel, dim3(blocks(M)), dim3(256), 0, s, conn, nelt, keys0, vals0);
+    hipcub::DoubleBuffer<u64> kbuf(keys0, keys1);
+    hipcub::DoubleBuffer<int> vbuf(vals0, vals1);
+    size_t bytes = 0;
+    GCHK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kbuf, vbuf, (int)M, 0, 64, s));
+    if (bytes > kNaSortTemp) {
+        fprintf(stderr, "cuteflow_hip: cf_build_boundary_device_noalloc: the sort needs %zu bytes of temporary "
+                        "storage, the scratch holds %zu\n", bytes, kNaSortTemp);
+        return CF_ERR_SIZE;
+    }
+    GCHK(hipcub::DeviceRadixSort::SortPairs(temp, bytes, kbuf, vbuf, (int)M, 0, 64, s));
+    hipLaunchKernelGGL(match_kernel, dim3(blocks(M)), dim3(256), 0, s, conn, nelt, (const u64 *)kbuf.Current(),
+                       (const int *)vbuf.Current(), boundary);
+    const int *lists[3] = {ndinput, ndoutput, ndwall};
+    const int nds[3] = {ndi, ndo, ndw};
+    for (int l = 0; l < 3; l++) {
+        hipLaunchKernelGGL(fill_int_kernel, dim3(blocks(P)), dim3(256), 0, s, pos[l], P, INT_MAX);
+        if (nds[l] > 0)
+            hipLaunchKernelGGL(first_pos_kernel, dim3(blocks((size_t)nds[l])), dim3(256), 0, s, lists[l], nds[l], nnt,
+                               pos[l]);
+    }
+    hipLaunchKernelGGL(classify_kernel, dim3(blocks(M)), dim3(256), 0, s, conn, nelt, ndi, ndo, ndw,
+                       (const int *)pos[0], (const int *)pos[1], (const int *)pos[2], num_in, num_out, boundary);
+    GCHK(hipGetLastError());
+    GCHK(hipStreamSynchronize(s));
     return CF_OK;
 }
 

@@ -558,6 +558,7 @@ extern "C" const char *cf_error_string(int s) {
     case CF_ERR_MESH: return "inconsistent mesh";
     case CF_ERR_NOMEM: return "out of memory";
     case CF_ERR_IO: return "file missing, unreadable or not in the expected layout";
+    case CF_ERR_SIZE: return "input larger than the fixed scratch of a no-allocation entry point";
     default: return "unknown status";
     }
 }

@@ -41,7 +41,8 @@ enum {
     CF_ERR_DT = -3,         /* dt sanity failure (cfl_condition.cuf:129-132) or NaN state */
     CF_ERR_MESH = -4,       /* inconsistent mesh */
     CF_ERR_NOMEM = -5,
-    CF_ERR_IO = -6          /* file missing, unreadable, or not in the expected layout */
+    CF_ERR_IO = -6,         /* file missing, unreadable, or not in the expected layout */
+    CF_ERR_SIZE = -7        /* input larger than a no-allocation entry point's fixed scratch */
 };
 
 /* Every constant the reference kernels read from `constant` memory
@@ -112,6 +113,22 @@ int cf_build_boundary_device(int nelt, int nnt, const int *conn,
                              int ndi, const int *ndinput, const int *num_in,
                              int ndo, const int *ndoutput, const int *num_out,
                              int ndw, const int *ndwall, int *boundary);
+
+/* cf_build_boundary_device with no device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  conn, the node lists and boundary are DEVICE pointers (same
+ * layouts as above, allocated by the caller); the work is ordered on
+ * `stream` (a hipStream_t, NULL for the null stream) and the call returns
+ * when the table is complete.  Its working memory is one fixed array per
+ * GPU, held until the process ends: at most 131072 cells and 131072 nodes
+ * (larger meshes return CF_ERR_SIZE).  Every call on a GPU uses that GPU's
+ * one array, so calls that overlap on one GPU (other host threads, other
+ * streams) race; the calls themselves call no hipMalloc or hipFree. */
+int cf_build_boundary_device_noalloc(int nelt, int nnt, const int *conn,
+                                     int ndi, const int *ndinput, const int *num_in,
+                                     int ndo, const int *ndoutput, const int *num_out,
+                                     int ndw, const int *ndwall, int *boundary, void *stream);
 
 /* Inlet lengths long_entree (maillage.cuf:1423-1452) from a boundary table. */
 int cf_inlet_lengths(int nelt, int nnt, const int *conn, const double *coords,

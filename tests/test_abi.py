@@ -46,7 +46,7 @@ def test_cgns_library_exports_all_declared_symbols():
 
 def test_error_strings():
     lib = _lib.load()
-    for s in (0, -1, -2, -3, -4, -5, -6):
+    for s in (0, -1, -2, -3, -4, -5, -6, -7):
         assert lib.cf_error_string(s)
     assert lib.cf_error_string(-99) == b"unknown status"
 
