@@ -257,6 +257,12 @@ def bench_headline(args):
     }
     workload_desc = result["config"]["workload"]
 
+    if not args.extra_runs:
+        if args.cpu_seconds > 0:
+            raise SystemExit("--extra-runs 0 is for profiling the timed steps: use it with --cpu-seconds 0")
+        s.close()
+        return result
+
     # The same run with every tile computed every step (no dry-tile
     # skipping): the cost of the whole mesh, and the bits of the skipping run.
     from cuteflow_amd import _lib
@@ -787,6 +793,9 @@ def main():
                     help="before the timed region, run a device copy of this many MiB (3 times): a "
                          "known-size kernel that calibrates HBM byte counters in a profile")
     ap.add_argument("--tune-steps", type=int, default=100, help="steps per autotuner candidate")
+    ap.add_argument("--extra-runs", type=int, default=1,
+                    help="0: leave out the every-tile and random-order runs of the headline (for profiles of the "
+                         "timed steps alone; needs --cpu-seconds 0)")
     ap.add_argument("--sections", default="c1,calibration,ensemble,field,scenarios",
                     help="comma-separated extra sections (c1, calibration, ensemble, field, scenarios); empty "
                          "for none")

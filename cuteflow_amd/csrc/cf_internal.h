@@ -20,6 +20,14 @@ namespace cf {
 constexpr int kTile = 256;
 constexpr int kWaves = kTile / 64;
 
+// dt slots per instance (spread the atomics of a step over 8 addresses),
+// each on its own 128-byte line
+constexpr int kDtSlots = 8;
+constexpr int kSlotStride64 = 16;
+constexpr int kSlotStride32 = 32;
+// bit pattern of +inf: no tile has folded its minimum in yet
+constexpr unsigned long long kDtSlotEmpty = 0x7ff0000000000000ull;
+
 // Per-instance control block, resident on the device across runs.
 struct InstCtrl {
     double dt;        // dt of the current state (used by the next step)
@@ -43,7 +51,12 @@ struct StepArgs {
     double *dt_hist;                     // [inst*hist_stride + k]
     double *tc_hist;
     int *ran;                            // tiles that ran, [inst*hist_stride + k]
-    double *partial;                     // per tile
+    // dt reduction of a step: every tile that runs folds its minimum into
+    // one of kDtSlots slots of its instance (atomic min on the bits, exact)
+    // and counts itself; a tile with a NaN state raises the instance's flag
+    unsigned long long *dtslot;          // [inst * kDtSlots * kSlotStride64 + slot * kSlotStride64]
+    unsigned *cntslot;                   // [inst * kDtSlots * kSlotStride32 + slot * kSlotStride32]
+    unsigned *nanflag;                   // [inst * kSlotStride32]
     unsigned *stamp;                     // per tile
     const int *__restrict__ tile_adj_off;
     const int *__restrict__ tile_adj;
@@ -176,13 +189,24 @@ struct cf_handle {
     int *perm = nullptr;               // per cell: caller's local index
     cf::KParams *d_kp = nullptr;
     cf::InstCtrl *ctrl = nullptr;
-    double *partial = nullptr;
+    double *partial = nullptr;         // per tile: cfl_kernel minima (dt of a new state)
+    unsigned long long *dtslot = nullptr;  // per instance: the step's dt slots (StepArgs::dtslot)
+    unsigned *cntslot = nullptr;
+    unsigned *nanflag = nullptr;
     unsigned *stamp = nullptr;
     int *tile_adj_off = nullptr, *tile_adj = nullptr;
     unsigned char *always = nullptr;
     int *tile_inst = nullptr, *inst_end = nullptr, *inst_tile = nullptr;
     double *surf_in = nullptr;         // ninst x CF_MAX_IO inlet areas (phys_inflow)
     int phys_in = 0;                   // an instance uses inlet = phys_inflow
+    // Run record: one device block holding, in this order, a mirror of the
+    // control blocks (kept current by run_init_kernel and finalize_kernel),
+    // dt_hist, ran, qin_hist, qout_hist and tc_hist; a run's results come
+    // back in one copy of its first part into the pinned host block hrec.
+    char *rec = nullptr;
+    char *hrec = nullptr;
+    size_t rec_dt = 0, rec_ran = 0, rec_qin = 0, rec_qout = 0, rec_tc = 0, rec_bytes = 0;
+    cf::InstCtrl *ctrl_mirror = nullptr;
     double *dt_hist = nullptr, *tc_hist = nullptr;
     int *ran = nullptr;
     int cap = 0;                       // steps per run the history arrays hold
@@ -193,7 +217,7 @@ struct cf_handle {
     int *io_cell = nullptr;            // per slot: the cell (cells with an I/O face, caller order)
     int *io_off = nullptr;             // per instance: first slot (ninst + 1)
     double *q_io = nullptr;            // 2 per slot: outflow, inflow discharge of the step
-    double *qin_hist = nullptr, *qout_hist = nullptr;  // like dt_hist, when discharge is on
+    double *qin_hist = nullptr, *qout_hist = nullptr;  // like dt_hist (run record), when discharge is on
     std::vector<std::vector<double>> last_qin, last_qout;  // per instance, last run
     // host mirror of the control block after the last finished run
     std::vector<cf::InstCtrl> hctrl;
@@ -246,8 +270,9 @@ inline bool h2d(T *dst, const T *src, size_t count, hipStream_t s) {
 StepArgs base_args(cf_handle *h);
 void launch_step(const StepArgs &a, int iflux, int fric, int grid, hipStream_t s);
 void launch_finalize(cf_handle *h, double *dt_hist, double *tc_hist, int *ran, int stride, int step, hipStream_t s,
-                     double *qin_hist = nullptr, double *qout_hist = nullptr);
-void launch_run_init(cf_handle *h, double *dt_hist, double *tc_hist, int stride, hipStream_t s);
+                     double *qin_hist = nullptr, double *qout_hist = nullptr, InstCtrl *mirror = nullptr);
+void launch_run_init(cf_handle *h, double *dt_hist, double *tc_hist, int stride, hipStream_t s,
+                     InstCtrl *mirror = nullptr);
 int set_schedule(cf_handle *h, int skip, int xcd);
 // defined in cf_adjoint.hip
 void adj_free(cf_handle *h);

@@ -87,10 +87,7 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
     unsigned char *ran_row = a.tile_ran ? a.tile_ran + (size_t)a.step * gridDim.x : nullptr;
     if (!replay) {
         if (c.status != 0) {  // this instance has stopped (ts reached) or failed
-            if (threadIdx.x == 0) {
-                a.partial[tile] = __builtin_inf();
-                if (ran_row) ran_row[tile] = 0;
-            }
+            if (threadIdx.x == 0 && ran_row) ran_row[tile] = 0;
             return;
         }
         // A tile whose cells and neighbour tiles' cells were all dry after
@@ -99,10 +96,7 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
         // hold that: it is skipped.  Reading a stamp raised concurrently only
         // makes the tile run, which is always correct.
         if (a.skip && !a.always[tile] && a.stamp[tile] + 1u < c.gstep) {
-            if (threadIdx.x == 0) {
-                a.partial[tile] = __builtin_inf();  // no dt contribution
-                if (ran_row) ran_row[tile] = 0;
-            }
+            if (threadIdx.x == 0 && ran_row) ran_row[tile] = 0;  // no dt contribution
             return;
         }
     } else if (ran_row && !ran_row[tile]) {
@@ -170,7 +164,10 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
     }
     if (replay) return;
     // per-workgroup dt minimum: wavefront shuffles, then LDS across waves;
-    // one plain store per workgroup (reduced by finalize_kernel)
+    // then one device-scope atomic min into one of the instance's slots (the
+    // minima are positive doubles, whose bit patterns order like their
+    // values, so the unsigned minimum of the bits is the exact minimum) and
+    // one count; finalize_kernel reads the slots after the step
     const double m = wave_min(dti);
     if ((threadIdx.x & 63) == 0) s_min[threadIdx.x >> 6] = m;
     const int anynan = __syncthreads_or(isnan_local);
@@ -178,7 +175,11 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
         double bm = s_min[0];
 #pragma unroll
         for (int w = 1; w < (int)(blockDim.x >> 6); w++) bm = dmin(bm, s_min[w]);
-        a.partial[tile] = anynan ? __builtin_nan("") : bm;
+        const int slot = (int)(blockIdx.x & (kDtSlots - 1));
+        const size_t so = (size_t)b * kDtSlots + slot;
+        if (anynan) atomicOr(a.nanflag + (size_t)b * kSlotStride32, 1u);
+        else atomicMin(a.dtslot + so * kSlotStride64, (unsigned long long)__double_as_longlong(bm));
+        atomicAdd(a.cntslot + so * kSlotStride32, 1u);
         // stored last: an early store would keep the uniform loads above
         // from being scalar loads (they could alias it)
         if (ran_row) ran_row[tile] = 1;
@@ -287,24 +288,42 @@ __global__ void surf_in_kernel(StepArgs a, const int *__restrict__ io_cell, cons
 }
 
 // End of a step, one workgroup per instance: dt of the new state (the
-// reference's reduction, cfl_condition.cuf:68-104), tc += dt, the dt range
-// check (:129-132) and the loop condition (finite_volumes_method.cuf:123).
-// With discharge bookkeeping (q_io set): the step's inflow and outflow
-// discharges (summed over the instance's slots in the caller's cell order)
-// and check_steady_state (finite_volumes_method.cuf:370-393): more than
-// 1000 consecutive steps with |Qin - Qout| / Qin < tol_reg_perm stop the
-// instance after the step.
-__global__ __launch_bounds__(1024) void finalize_kernel(InstCtrl *ctrl, const KParams *kp, const double *partial,
-                                                         const int *inst_tile, double *dt_hist, double *tc_hist,
-                                                         int *ran, int hist_stride, int step, const double *q_io,
-                                                         const int *io_off, double *qin_hist, double *qout_hist) {
-    __shared__ double s_q[2 * 1024];
+// reference's reduction, cfl_condition.cuf:68-104: the step kernel's tiles
+// have folded their minima into the instance's slots, which are read here
+// and emptied for the next step), tc += dt, the dt range check (:129-132)
+// and the loop condition (finite_volumes_method.cuf:123).  With discharge
+// bookkeeping (q_io set): the step's inflow and outflow discharges (summed
+// over the instance's slots in the caller's cell order) and
+// check_steady_state (finite_volumes_method.cuf:370-393): more than 1000
+// consecutive steps with |Qin - Qout| / Qin < tol_reg_perm stop the
+// instance after the step.  mirror (may be null): the run record's copy of
+// the control blocks, kept equal to ctrl.
+__global__ __launch_bounds__(256) void finalize_kernel(InstCtrl *ctrl, const KParams *kp,
+                                                        unsigned long long *dtslot, unsigned *cntslot,
+                                                        unsigned *nanflag, double *dt_hist, double *tc_hist, int *ran,
+                                                        int hist_stride, int step, const double *q_io,
+                                                        const int *io_off, double *qin_hist, double *qout_hist,
+                                                        InstCtrl *mirror) {
+    __shared__ double s_q[2 * 256];
     const int b = blockIdx.x;
-    InstCtrl *c = ctrl + b;
-    if (c->status != 0) return;
-    const int t0 = inst_tile[b], nt = inst_tile[b + 1] - t0;
-    int nan, nskip;
-    const double dtn = block_min_1024(partial + t0, nt, kp[b].pad_dt, nan, nskip);
+    // every load up front (one memory round trip): the control block and the
+    // constants (uniform), the slots (lanes 0..kDtSlots-1)
+    InstCtrl cc = ctrl[b];
+    const double pad = kp[b].pad_dt, ts = kp[b].ts, tol = kp[b].tol, trp = kp[b].tol_reg_perm;
+    const unsigned nanv = nanflag[(size_t)b * kSlotStride32];
+    unsigned long long bits = kDtSlotEmpty;
+    unsigned cnt = 0;
+    const size_t so = (size_t)b * kDtSlots + threadIdx.x;
+    if (threadIdx.x < kDtSlots) {
+        bits = dtslot[so * kSlotStride64];
+        cnt = cntslot[so * kSlotStride32];
+    }
+    // a stopped instance's tiles did not run: its slots are still empty
+    if (cc.status != 0) return;
+    if (threadIdx.x < kDtSlots) {
+        dtslot[so * kSlotStride64] = kDtSlotEmpty;
+        cntslot[so * kSlotStride32] = 0u;
+    }
     double qo = 0.0, qi = 0.0;
     if (q_io) {
         const int s0 = io_off[b], s1 = io_off[b + 1];
@@ -323,25 +342,38 @@ __global__ __launch_bounds__(1024) void finalize_kernel(InstCtrl *ctrl, const KP
                 }
         }
     }
+    if (threadIdx.x < 64) {
+#pragma unroll
+        for (int off = kDtSlots / 2; off >= 1; off >>= 1) {
+            const unsigned long long o = __shfl_xor(bits, off, 64);
+            bits = o < bits ? o : bits;
+            cnt += __shfl_xor(cnt, off, 64);
+        }
+    }
     if (threadIdx.x == 0) {
-        const double tcn = c->tc + c->dt;
-        c->dt = dtn;
-        c->tc = tcn;
-        c->steps = step + 1;
-        c->gstep += 1u;
+        const bool nan = nanv != 0u;
+        if (nan) nanflag[(size_t)b * kSlotStride32] = 0u;
+        const double dtn = dmin(pad, __longlong_as_double((long long)bits));
+        const double tcn = cc.tc + cc.dt;
+        cc.dt = dtn;
+        cc.tc = tcn;
+        cc.steps = step + 1;
+        cc.gstep += 1u;
         dt_hist[(size_t)b * hist_stride + step + 1] = dtn;
         tc_hist[(size_t)b * hist_stride + step + 1] = tcn;
-        ran[(size_t)b * hist_stride + step] = nt - nskip;
+        ran[(size_t)b * hist_stride + step] = (int)cnt;
         bool steady = false;
         if (q_io) {
             qout_hist[(size_t)b * hist_stride + step] = qo;
             qin_hist[(size_t)b * hist_stride + step] = qi;
-            if (qi > kQSteady && fabs(qi - qo) / qi < kp[b].tol_reg_perm) steady = ++c->reg > 1000;
-            else c->reg = 0;
+            if (qi > kQSteady && fabs(qi - qo) / qi < trp) steady = ++cc.reg > 1000;
+            else cc.reg = 0;
         }
-        if (nan || !(dtn <= 9999.0 && dtn >= kDtLow)) c->status = -1;
-        else if (steady) c->status = 2;
-        else if (!(((tcn - kp[b].ts) + 2.0 * kp[b].tol) < kp[b].tol)) c->status = 1;
+        if (nan || !(dtn <= 9999.0 && dtn >= kDtLow)) cc.status = -1;
+        else if (steady) cc.status = 2;
+        else if (!(((tcn - ts) + 2.0 * tol) < tol)) cc.status = 1;
+        ctrl[b] = cc;
+        if (mirror) mirror[b] = cc;
     }
 }
 
@@ -349,7 +381,7 @@ __global__ __launch_bounds__(1024) void finalize_kernel(InstCtrl *ctrl, const KP
 // the loop condition is evaluated (a failed instance stays failed, one at
 // steady state stays stopped: a new state restarts both).
 __global__ void run_init_kernel(InstCtrl *ctrl, const KParams *kp, double *dt_hist, double *tc_hist,
-                                int hist_stride, int ninst) {
+                                int hist_stride, int ninst, InstCtrl *mirror) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= ninst) return;
     InstCtrl *c = ctrl + b;
@@ -358,6 +390,7 @@ __global__ void run_init_kernel(InstCtrl *ctrl, const KParams *kp, double *dt_hi
     if (c->status == 0 || c->status == 1) c->status = (((c->tc - kp[b].ts) + 2.0 * kp[b].tol) < kp[b].tol) ? 0 : 1;
     dt_hist[(size_t)b * hist_stride] = c->dt;
     tc_hist[(size_t)b * hist_stride] = c->tc;
+    if (mirror) mirror[b] = *c;
 }
 
 // cflcond for the current state of one instance, one minimum per tile
@@ -535,18 +568,50 @@ static int compute_dt(cf_handle *h, int b) {
     return CF_OK;
 }
 
+static size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+static void free_record(cf_handle *h) {
+    dfree(h->rec);
+    if (h->hrec) (void)hipHostFree(h->hrec);
+    h->hrec = nullptr;
+    h->ctrl_mirror = nullptr;
+    h->dt_hist = h->tc_hist = h->qin_hist = h->qout_hist = nullptr;
+    h->ran = nullptr;
+}
+
+// Sizes the run record for runs of up to nsteps steps (history rows of
+// cap + 1 entries per instance).
 static int ensure_hist(cf_handle *h, int nsteps) {
     if (nsteps <= h->cap) return CF_OK;
     const int cap = std::max(nsteps, 2 * h->cap);
     HIPCHK(hipStreamSynchronize(h->stream));
-    dfree(h->dt_hist); dfree(h->tc_hist); dfree(h->ran); dfree(h->qin_hist); dfree(h->qout_hist);
+    free_record(h);
     drop_graphs(h);
-    const size_t n = (size_t)h->ninst * (size_t)(cap + 1);
-    if (!dalloc(h->dt_hist, n) || !dalloc(h->tc_hist, n) || !dalloc(h->ran, n) || !dalloc(h->qin_hist, n) ||
-        !dalloc(h->qout_hist, n))
+    const size_t n = (size_t)h->ninst * (size_t)(cap + 1), B = (size_t)h->ninst;
+    h->rec_dt = round256(B * sizeof(InstCtrl));
+    h->rec_ran = h->rec_dt + round256(n * sizeof(double));
+    h->rec_qin = h->rec_ran + round256(n * sizeof(int));
+    h->rec_qout = h->rec_qin + round256(n * sizeof(double));
+    h->rec_tc = h->rec_qout + round256(n * sizeof(double));
+    h->rec_bytes = h->rec_tc + round256(n * sizeof(double));
+    if (!dalloc(h->rec, h->rec_bytes) || hipHostMalloc((void **)&h->hrec, h->rec_tc, hipHostMallocDefault) != hipSuccess) {
+        h->hrec = nullptr;
+        free_record(h);
         return CF_ERR_NOMEM;
+    }
+    h->ctrl_mirror = reinterpret_cast<InstCtrl *>(h->rec);
+    h->dt_hist = reinterpret_cast<double *>(h->rec + h->rec_dt);
+    h->ran = reinterpret_cast<int *>(h->rec + h->rec_ran);
+    h->qin_hist = reinterpret_cast<double *>(h->rec + h->rec_qin);
+    h->qout_hist = reinterpret_cast<double *>(h->rec + h->rec_qout);
+    h->tc_hist = reinterpret_cast<double *>(h->rec + h->rec_tc);
     h->cap = cap;
     return CF_OK;
+}
+
+__global__ void fill_u64_kernel(unsigned long long *p, size_t n, unsigned long long v) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 extern "C" const char *cf_error_string(int s) {
@@ -584,8 +649,9 @@ static void destroy_buffers(cf_handle *h) {
     dfree(h->d_kp); dfree(h->ctrl); dfree(h->partial); dfree(h->stamp);
     dfree(h->tile_adj_off); dfree(h->tile_adj); dfree(h->always);
     dfree(h->tile_inst); dfree(h->inst_end); dfree(h->inst_tile);
-    dfree(h->dt_hist); dfree(h->tc_hist); dfree(h->ran);
-    dfree(h->io_cell); dfree(h->io_off); dfree(h->q_io); dfree(h->qin_hist); dfree(h->qout_hist);
+    free_record(h);
+    dfree(h->dtslot); dfree(h->cntslot); dfree(h->nanflag);
+    dfree(h->io_cell); dfree(h->io_off); dfree(h->q_io);
     dfree(h->surf_in);
 }
 
@@ -723,7 +789,10 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
             !dalloc(h->tile_inst, (size_t)tiles) || !dalloc(h->inst_end, (size_t)ninst) ||
             !dalloc(h->inst_tile, (size_t)ninst + 1) || !dalloc(h->io_cell, io_cell.size()) || !dalloc(h->io_off, (size_t)ninst + 1) ||
             !dalloc(h->q_io, 2 * std::max<size_t>((size_t)h->nio, 1)) ||
-            !dalloc(h->surf_in, (size_t)ninst * CF_MAX_IO)) {
+            !dalloc(h->surf_in, (size_t)ninst * CF_MAX_IO) ||
+            !dalloc(h->dtslot, (size_t)ninst * kDtSlots * kSlotStride64) ||
+            !dalloc(h->cntslot, (size_t)ninst * kDtSlots * kSlotStride32) ||
+            !dalloc(h->nanflag, (size_t)ninst * kSlotStride32)) {
             rc = CF_ERR_NOMEM;
             break;
         }
@@ -750,6 +819,15 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
             ok = h2d(dconn[b], prep[b].conn.data(), 3 * (size_t)h->nelt[b], h->stream) &&
                  h2d(dcoords[b], meshes[b].coords, 3 * (size_t)h->nnt[b], h->stream);
         if (!ok) { rc = CF_ERR_HIP; break; }
+        {
+            const size_t n64 = (size_t)ninst * kDtSlots * kSlotStride64, n32 = (size_t)ninst * kDtSlots * kSlotStride32;
+            hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_of((int)n64, 256)), dim3(256), 0, h->stream, h->dtslot, n64,
+                               kDtSlotEmpty);
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_of((int)n32, 256)), dim3(256), 0, h->stream, h->cntslot,
+                               (int)n32, 0u);
+            hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_of(ninst * kSlotStride32, 256)), dim3(256), 0, h->stream,
+                               h->nanflag, ninst * kSlotStride32, 0u);
+        }
         for (int b = 0; b < ninst; b++) {
             const int c0 = h->cell_off[b], n = h->nelt[b];
             hipLaunchKernelGGL(geometry_kernel, dim3(grid_of(n, 256)), dim3(256), 0, h->stream, dconn[b], dcoords[b], n,
@@ -888,15 +966,19 @@ void cf::launch_step(const StepArgs &a, int iflux, int fric, int grid, hipStream
 }
 
 void cf::launch_finalize(cf_handle *h, double *dt_hist, double *tc_hist, int *ran, int stride, int step,
-                         hipStream_t s, double *qin_hist, double *qout_hist) {
+                         hipStream_t s, double *qin_hist, double *qout_hist, InstCtrl *mirror) {
     const bool q = qin_hist && qout_hist;
-    hipLaunchKernelGGL(finalize_kernel, dim3(h->ninst), dim3(1024), 0, s, h->ctrl, h->d_kp, h->partial, h->inst_tile,
-                       dt_hist, tc_hist, ran, stride, step, q ? h->q_io : nullptr, h->io_off, qin_hist, qout_hist);
+    // one wave suffices without discharges (a few loads and one thread's
+    // bookkeeping); the discharge sums stage slots through LDS
+    hipLaunchKernelGGL(finalize_kernel, dim3(h->ninst), dim3(q ? 256 : 64), 0, s, h->ctrl, h->d_kp, h->dtslot,
+                       h->cntslot, h->nanflag, dt_hist, tc_hist, ran, stride, step, q ? h->q_io : nullptr, h->io_off,
+                       qin_hist, qout_hist, mirror);
 }
 
-void cf::launch_run_init(cf_handle *h, double *dt_hist, double *tc_hist, int stride, hipStream_t s) {
+void cf::launch_run_init(cf_handle *h, double *dt_hist, double *tc_hist, int stride, hipStream_t s,
+                         InstCtrl *mirror) {
     hipLaunchKernelGGL(run_init_kernel, dim3(grid_of(h->ninst, 256)), dim3(256), 0, s, h->ctrl, h->d_kp, dt_hist,
-                       tc_hist, stride, h->ninst);
+                       tc_hist, stride, h->ninst, mirror);
 }
 
 StepArgs cf::base_args(cf_handle *h) {
@@ -913,7 +995,9 @@ StepArgs cf::base_args(cf_handle *h) {
     a.dt_hist = h->dt_hist;
     a.tc_hist = h->tc_hist;
     a.ran = h->ran;
-    a.partial = h->partial;
+    a.dtslot = h->dtslot;
+    a.cntslot = h->cntslot;
+    a.nanflag = h->nanflag;
     a.stamp = h->stamp;
     a.tile_adj_off = h->tile_adj_off;
     a.tile_adj = h->tile_adj;
@@ -938,7 +1022,7 @@ StepArgs cf::base_args(cf_handle *h) {
 
 // The launches of a run of nsteps steps, on stream s.
 static void enqueue_run(cf_handle *h, int nsteps, hipStream_t s) {
-    launch_run_init(h, h->dt_hist, h->tc_hist, h->cap + 1, s);
+    launch_run_init(h, h->dt_hist, h->tc_hist, h->cap + 1, s, h->ctrl_mirror);
     StepArgs a = base_args(h);
     step_launcher L = pick_launcher(h->iflux, h->fric);
     for (int k = 0; k < nsteps; k++) {
@@ -950,7 +1034,7 @@ static void enqueue_run(cf_handle *h, int nsteps, hipStream_t s) {
             hipLaunchKernelGGL(discharge_kernel, dim3(grid_of(h->nio, 256)), dim3(256), 0, s, a, h->d_kp, h->io_cell,
                                h->nio, h->q_io);
         launch_finalize(h, h->dt_hist, h->tc_hist, h->ran, h->cap + 1, k, s, h->discharge ? h->qin_hist : nullptr,
-                        h->discharge ? h->qout_hist : nullptr);
+                        h->discharge ? h->qout_hist : nullptr, h->ctrl_mirror);
     }
 }
 
@@ -1029,24 +1113,20 @@ extern "C" int cf_batch_step_finish(cf_handle *h, double *dt_hist, int *steps_do
         }
         return CF_OK;
     }
-    std::vector<InstCtrl> c(B);
-    HIPCHK(hipMemcpyAsync(c.data(), h->ctrl, B * sizeof(InstCtrl), hipMemcpyDeviceToHost, h->stream));
-    std::vector<double> dts((size_t)B * S), qin, qout;
-    std::vector<int> ran((size_t)B * S);
-    HIPCHK(hipMemcpyAsync(dts.data(), h->dt_hist, dts.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipMemcpyAsync(ran.data(), h->ran, ran.size() * sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    if (h->discharge) {
-        qin.resize((size_t)B * S);
-        qout.resize((size_t)B * S);
-        HIPCHK(hipMemcpyAsync(qin.data(), h->qin_hist, qin.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipMemcpyAsync(qout.data(), h->qout_hist, qout.size() * sizeof(double), hipMemcpyDeviceToHost,
-                              h->stream));
-    }
+    // one copy of the run record: control blocks, dt and tile counts (and
+    // the discharge histories when they are kept)
+    HIPCHK(hipMemcpyAsync(h->hrec, h->rec, h->discharge ? h->rec_tc : h->rec_qin, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    const InstCtrl *cm = reinterpret_cast<const InstCtrl *>(h->hrec);
+    std::vector<InstCtrl> c(cm, cm + B);
+    const double *dts = reinterpret_cast<const double *>(h->hrec + h->rec_dt);
+    const int *ran = reinterpret_cast<const int *>(h->hrec + h->rec_ran);
+    const double *qin = reinterpret_cast<const double *>(h->hrec + h->rec_qin);
+    const double *qout = reinterpret_cast<const double *>(h->hrec + h->rec_qout);
     int rc = CF_OK;
     for (int b = 0; b < B; b++) {
         const int done = c[b].steps;
-        if (dt_hist) memcpy(dt_hist + (size_t)b * (nsteps + 1), dts.data() + (size_t)b * S, sizeof(double) * ((size_t)done + 1));
+        if (dt_hist) memcpy(dt_hist + (size_t)b * (nsteps + 1), dts + (size_t)b * S, sizeof(double) * ((size_t)done + 1));
         if (steps_done) steps_done[b] = done;
         if (status) status[b] = c[b].status < 0 ? CF_ERR_DT : CF_OK;
         if (c[b].status < 0) rc = CF_ERR_DT;
@@ -1055,8 +1135,8 @@ extern "C" int cf_batch_step_finish(cf_handle *h, double *dt_hist, int *steps_do
         h->last_run[b] = run;
         h->last_total[b] = (long long)done * (h->tile_off[b + 1] - h->tile_off[b]);
         if (h->discharge) {
-            h->last_qin[b].assign(qin.begin() + (size_t)b * S, qin.begin() + (size_t)b * S + done);
-            h->last_qout[b].assign(qout.begin() + (size_t)b * S, qout.begin() + (size_t)b * S + done);
+            h->last_qin[b].assign(qin + (size_t)b * S, qin + (size_t)b * S + done);
+            h->last_qout[b].assign(qout + (size_t)b * S, qout + (size_t)b * S + done);
         } else {
             h->last_qin[b].clear();
             h->last_qout[b].clear();
