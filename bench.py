@@ -61,28 +61,9 @@ def log(*a):
 
 
 def make_workload(nx, ny, order, seed=1):
-    import numpy as np
+    from cuteflow_amd import scenarios as SC
 
-    from cuteflow_amd import mesh as M
-    from cuteflow_amd.params import Params
-
-    r = M.river_mesh(nx=nx, ny=ny, length=20000.0 * nx / 3200, width=2000.0 * ny / 320, seed=seed)
-    m, U0 = r.mesh, r.state
-    if order == "rcm":
-        perm = M.rcm_order(m)
-    elif order == "random":
-        perm = M.random_order(m, seed)
-    else:
-        perm = None
-    if perm is not None:
-        m, U0 = m.permuted(perm), U0[perm]
-    else:
-        perm = np.arange(m.nelt)
-    p = Params(iflux=3, ilimiteur=1, tolisec=1e-3, cfl=0.9, ts=1e300, inlet="inflow",
-               outlet="fixedheight", friction=1, fricimplic=1, nombre_entrees=1, nombre_sorties=1)
-    p.h_sortie[0] = r.h_out
-    p.hydrographs = {1: ([0.0, r.t_ramp], [0.0, r.q_peak])}
-    return m, p, np.ascontiguousarray(U0), perm
+    return SC.river_reach(nx, ny, order, seed)
 
 
 def traffic_from_profile(workload_key):

@@ -140,6 +140,8 @@ SIGNATURES = {
     "cf_adj_get_gradient": (C.c_int, [_H, C.c_int, C.c_void_p, C.c_void_p]),
     "cf_tlm_set_direction": (C.c_int, [_H, C.c_int, C.c_void_p, C.c_void_p]),
     "cf_tlm_run": (C.c_int, [_H]),
+    "cf_tlm_set_mask_mode": (C.c_int, [_H, C.c_int]),
+    "cf_tlm_get_flips": (C.c_int, [_H, C.c_int, C.c_int, _IP, C.c_int]),
     "cf_tlm_get": (C.c_int, [_H, C.c_int, _DP, _DP, C.c_void_p, C.c_void_p, C.c_void_p]),
     "cf_debug_pow": (C.c_int, [C.c_int, _DP, C.c_double, _DP]),
     "cf_debug_prepare": (C.c_int, [C.POINTER(CfMesh), C.POINTER(CfParams), C.c_int, C.c_int, C.c_int, _IP, _IP, _IP,

@@ -73,6 +73,7 @@ class AdjointRun:
         self.want_hmax = bool(want_hmax)
         self.nsteps = 0
         self.nsamp = 0
+        self.records = 0          # recorded runs so far (the autograd Function checks it)
 
     # -- forward ---------------------------------------------------------------
     def set_manning(self, n_list) -> None:
@@ -97,6 +98,7 @@ class AdjointRun:
         _lib.check(self.lib.cf_adj_forward(self.h, int(nsteps), self.sample_every, self.checkpoint_every,
                                            int(self.want_hmax)), "cf_adj_forward")
         self.nsteps = int(nsteps)
+        self.records += 1
         self.nsamp = self.lib.cf_adj_num_samples(self.h)
         etas, hmaxs = [], []
         for b in range(self.ninst):
@@ -132,6 +134,25 @@ class AdjointRun:
         return gns, gus
 
     # -- tangent-linear --------------------------------------------------------
+    def set_mask_mode(self, mode: int) -> None:
+        """Branch masks of the next tangent-linear runs: 1 records the wet/dry
+        clamp decision of each step and cell, 2 counts per cell the steps that
+        differ from the recorded ones; 3 and 4 the same plus a signature of
+        every branch of the cell's update; 0 off."""
+        _lib.check(self.lib.cf_tlm_set_mask_mode(self.h, int(mode)), "cf_tlm_set_mask_mode")
+
+    def flips(self, kind: int = 0, reset: bool = True):
+        """Per instance, per cell: steps whose wet/dry decision (kind 0) or
+        any branch (kind 1) differed from the recorded run in the counting
+        runs (zeroed when reset)."""
+        out = []
+        for b in range(self.ninst):
+            f = np.zeros(self.nelt[b], np.int32)
+            _lib.check(self.lib.cf_tlm_get_flips(self.h, b, int(kind), _lib.iptr(f), int(bool(reset))),
+                       "cf_tlm_get_flips")
+            out.append(f)
+        return out
+
     def tlm(self, dn=None, dU0=None):
         """Tangent-linear run over the recorded run along (dn, dU0) (per
         instance lists; None = 0).  Returns per instance a dict with eta and
@@ -191,6 +212,7 @@ def _fn():
                 ns = _split(n_cells.detach(), run.nelt)
                 etas, hmaxs = run.forward(nsteps, U0=U0, manning=ns)
                 ctx.run = run
+                ctx.record = run.records
                 ctx.device = n_cells.device
                 eta = torch.as_tensor(np.concatenate([e.reshape(-1) for e in etas]), device=n_cells.device)
                 if run.want_hmax:
@@ -201,6 +223,10 @@ def _fn():
             @staticmethod
             def backward(ctx, g_eta, g_hmax=None):
                 run = ctx.run
+                if run.records != ctx.record:
+                    raise RuntimeError("GaugeLevels.backward: the run was recorded again by a later forward "
+                                       "(one AdjointRun holds one recorded run): call backward before the next "
+                                       "forward on the same AdjointRun")
                 sizes = [run.nsamp * len(g) for g in run.gauges]
                 ge = [g.detach().cpu().numpy().reshape(run.nsamp, -1) if g.numel() else None
                       for g in _split(g_eta, sizes)]

@@ -19,13 +19,29 @@ def gauge_misfit(eta, obs):
     return float(np.sum((np.asarray(eta) - np.asarray(obs)) ** 2))
 
 
-def zonal_fd_gradient(run: AdjointRun, Z: np.ndarray, n_zones: np.ndarray, obs, eps: float = 1e-6):
+def zonal_fd_gradient(run: AdjointRun, Z: np.ndarray, n_zones: np.ndarray, obs, eps: float = 1e-6,
+                      count_flips: bool = False):
     """Central differences (relative step eps) of the gauge misfit with
     respect to each zone's n, at the recorded dt: the misfit at a perturbed
     n is taken from a tangent-linear run's values (the recorded run replayed
-    with its dt and tc).  Single-instance runs.  Restores n afterwards."""
+    with its dt and tc).  Single-instance runs.  Restores n afterwards.
+
+    count_flips: also returns, per zone, the numbers of (step, cell) wet/dry
+    clamp decisions and of (step, cell) updates with any branch taken
+    differently by the two perturbed runs than by the run at n_zones (a
+    finite difference across such a kink of the frozen-branch map differs
+    from the derivative): (g, clamp_flips, branch_flips)."""
     obs = np.asarray(obs).reshape(-1)
     g = np.zeros(len(n_zones))
+    flips = np.zeros(len(n_zones), np.int64)
+    bflips = np.zeros(len(n_zones), np.int64)
+    if count_flips:
+        run.set_manning([Z @ np.asarray(n_zones, dtype=np.float64)])
+        run.set_mask_mode(3)
+        run.tlm()
+        run.set_mask_mode(4)
+        run.flips(0, reset=True)
+        run.flips(1, reset=True)
     for z in range(len(n_zones)):
         vals = []
         for sgn in (1.0, -1.0):
@@ -34,8 +50,13 @@ def zonal_fd_gradient(run: AdjointRun, Z: np.ndarray, n_zones: np.ndarray, obs, 
             run.set_manning([Z @ nz])
             vals.append(gauge_misfit(run.tlm()[0]["eta"].reshape(-1), obs))
         g[z] = (vals[0] - vals[1]) / (2.0 * eps * n_zones[z])
+        if count_flips:
+            flips[z] = int(np.sum(run.flips(0, reset=True)[0], dtype=np.int64))
+            bflips[z] = int(np.sum(run.flips(1, reset=True)[0], dtype=np.int64))
+    if count_flips:
+        run.set_mask_mode(0)
     run.set_manning([Z @ np.asarray(n_zones, dtype=np.float64)])
-    return g
+    return (g, flips, bflips) if count_flips else g
 
 
 def zonal_tlm_gradient(run: AdjointRun, Z: np.ndarray, obs):

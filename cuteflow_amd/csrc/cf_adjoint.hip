@@ -35,7 +35,10 @@
 #include <new>
 #include <vector>
 
+#include <type_traits>
+
 #include "cf_dual.h"
+#include "cf_dual_rec.h"
 #include "cf_internal.h"
 
 #pragma clang fp contract(off)
@@ -79,6 +82,23 @@ struct AdjState {
     double4 *tdir = nullptr;  // tangent of the initial state
     double *dman = nullptr;   // tangent of Manning n
     int tlm_cur = 0;
+    // branch masks of tangent-linear runs: mode 1 records, per step and
+    // cell, whether the update left the cell wet (one bit, 64 cells a word);
+    // mode 2 counts per cell the steps whose mask differs from the recorded
+    // one (clamp decisions that a perturbed run takes differently); modes 3
+    // and 4 also record / compare a signature of every branch of the update
+    int mask_mode = 0, mask_steps = 0, sig_steps = 0;
+    bool mask_rec = false, sig_rec = false;  // masks / signatures of the current record exist
+    unsigned *bsig = nullptr;   // nsteps x ncell branch signatures
+    int *bflips = nullptr;
+    // Manning n of the recorded run (the replay and the reverse sweep use
+    // it, whatever n the handle holds later) and the parameter version it
+    // was recorded with (an inflow or hydrograph change since refuses the
+    // reverse sweep)
+    double *man_rec = nullptr;
+    unsigned params_ver = 0;
+    unsigned long long *wmask = nullptr;
+    int *flips = nullptr;
     bool r2_built = false;
     bool gauges_dirty = true;
     int alloc_steps = 0, alloc_K = 0, alloc_nsamp = 0, alloc_gtotal = 0;
@@ -316,15 +336,16 @@ __global__ void ckpt_kernel(StPair st, const InstCtrl *ctrl, const int *__restri
 
 // ---- tangent-linear model ------------------------------------------------
 
+template <class D>
 struct TlmLd {
     const double4 *st_, *dst_;
     const int4 *nbr_;
     const double4 *dfc_;
     const double *geo_, *man_, *dman_;
     int stride;
-    __device__ __forceinline__ St<Dual> state(int c) const {
+    __device__ __forceinline__ St<D> state(int c) const {
         const double4 v = st_[c], d = dst_[c];
-        return {Dual(v.x, d.x), Dual(v.y, d.y), Dual(v.z, d.z), v.w};
+        return {D(v.x, d.x), D(v.y, d.y), D(v.z, d.z), v.w};
     }
     __device__ __forceinline__ I4 nbr(int c) const {
         const int4 v = nbr_[c];
@@ -335,7 +356,7 @@ struct TlmLd {
         return {v.x, v.y, v.z, v.w};
     }
     __device__ __forceinline__ double geo(int comp, int c) const { return geo_[comp * (size_t)stride + c]; }
-    __device__ __forceinline__ Dual manning(int c) const { return Dual(man_[c], dman_[c]); }
+    __device__ __forceinline__ D manning(int c) const { return D(man_[c], dman_[c]); }
 };
 
 struct TlmArgs {
@@ -350,21 +371,68 @@ struct TlmArgs {
     const int *tile_inst, *inst_end;
     const double *dt, *tc;
     int hstride, step, ncell, tile_cells, xcd;
+    // Diagnostic masks (0 none): 1 / 3 record, 2 / 4 count the differences
+    // from the recorded run.  Every mode > 0 keeps the wet bit of each (step,
+    // cell); modes 3 and 4 also a signature of every branch the cell's update
+    // took (a REC kernel).
+    int mask_mode;
+    unsigned long long *wmask;     // nsteps x (ncell / 64) words
+    int *flips;                    // per cell: steps whose wet bit differed
+    unsigned *bsig;                // nsteps x ncell branch signatures
+    int *bflips;                   // per cell: steps whose signature differed
 };
 
-template <int IFLUX, int FRIC>
+template <int IFLUX, int FRIC, bool REC>
 __global__ __launch_bounds__(kTile) void tlm_kernel(TlmArgs a) {
+    using D = typename std::conditional<REC, DualR, Dual>::type;
     const int tile = tile_of_block(a.xcd);
     const int b = a.tile_inst[tile];
     const int i = tile * (int)blockDim.x + threadIdx.x;
-    if (i >= a.inst_end[b]) return;
-    const TlmLd ld{a.st_in, a.dst_in, a.nbr, a.dfc, a.geo, a.manning, a.dman, a.ncell};
-    const size_t hix = (size_t)b * a.hstride + a.step;
-    Dual o[3];
-    cell_update<Dual, IFLUX, FRIC>(ld, a.kp[b], a.hydro, i, a.dt[hix], a.tc[hix], o, (Dual *)nullptr);
-    const double zm = a.st_in[i].w;
-    a.st_out[i] = make_double4(o[0].v, o[1].v, o[2].v, zm);
-    a.dst_out[i] = make_double4(o[0].d, o[1].d, o[2].d, 0.0);
+    const bool valid = i < a.inst_end[b];
+    double h_new = 0.0;
+    unsigned sig = 0u;
+    if (valid) {
+        if (REC) s_branch_sig[threadIdx.x] = 0x811c9dc5u;
+        const TlmLd<D> ld{a.st_in, a.dst_in, a.nbr, a.dfc, a.geo, a.manning, a.dman, a.ncell};
+        const size_t hix = (size_t)b * a.hstride + a.step;
+        D o[3];
+        cell_update<D, IFLUX, FRIC>(ld, a.kp[b], a.hydro, i, a.dt[hix], a.tc[hix], o, (D *)nullptr);
+        const double zm = a.st_in[i].w;
+        a.st_out[i] = make_double4(o[0].v, o[1].v, o[2].v, zm);
+        a.dst_out[i] = make_double4(o[0].d, o[1].d, o[2].d, 0.0);
+        h_new = o[0].v;
+        if (REC) sig = s_branch_sig[threadIdx.x];
+    }
+    if (a.mask_mode) {
+        const bool record = a.mask_mode == 1 || a.mask_mode == 3;
+        // the clamp sets h to +0 exactly and an unclamped depth exceeds the
+        // dry threshold: h > 0 is "the clamp was not taken"
+        const bool wet = valid && h_new > 0.0;
+        const unsigned long long bal = __ballot(wet);
+        const size_t w = (size_t)a.step * (size_t)(a.ncell >> 6) + (size_t)(i >> 6);
+        if (record) {
+            if ((threadIdx.x & 63) == 0) a.wmask[w] = bal;
+        } else if (valid) {
+            const bool was = ((a.wmask[w] >> (i & 63)) & 1ull) != 0ull;
+            if (was != wet) a.flips[i] += 1;
+        }
+        if (REC && valid) {
+            const size_t ws = (size_t)a.step * (size_t)a.ncell + (size_t)i;
+            if (record) a.bsig[ws] = sig;
+            else if (a.bsig[ws] != sig) a.bflips[i] += 1;
+        }
+    }
+}
+
+__global__ void int_out_kernel(const int *__restrict__ src, const int *__restrict__ perm, int c0, int n,
+                               int *__restrict__ dst) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l < n) dst[perm[c0 + l]] = src[c0 + l];
+}
+
+__global__ void fill_i_kernel(int *p, size_t n) {
+    const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 __global__ void tlm_sample_kernel(const double4 *__restrict__ st, const double4 *__restrict__ dst,
@@ -472,6 +540,7 @@ void cf::adj_free(cf_handle *h) {
     dfree(A->lam[0]); dfree(A->lam[1]); dfree(A->gstate);
     dfree(A->R1[0]); dfree(A->R1[1]); dfree(A->R2[0]); dfree(A->R2[1]); dfree(A->r2_off); dfree(A->r2_idx); dfree(A->gn); dfree(A->tile_ran);
     dfree(A->tst[0]); dfree(A->tst[1]); dfree(A->tdst[0]); dfree(A->tdst[1]); dfree(A->tdir); dfree(A->dman);
+    dfree(A->wmask); dfree(A->flips); dfree(A->man_rec); dfree(A->bsig); dfree(A->bflips);
     delete A;
     h->adj = nullptr;
 }
@@ -633,7 +702,7 @@ static int adj_alloc(cf_handle *h, AdjState *A) {
     }
     if (!A->lam[0]) {
         drop_adj_graphs(A);
-        if (!dalloc(A->lam[0], N) || !dalloc(A->lam[1], N) || !dalloc(A->gstate, N) || !dalloc(A->R1[0], 4 * N) ||
+        if (!dalloc(A->lam[0], N) || !dalloc(A->man_rec, N) || !dalloc(A->lam[1], N) || !dalloc(A->gstate, N) || !dalloc(A->R1[0], 4 * N) ||
             !dalloc(A->R1[1], 4 * N) || !dalloc(A->gn, N) || !dalloc(A->hmax, N) || !dalloc(A->ghmax, N) ||
             !dalloc(A->dhmax, N) || !dalloc(A->argmax, N))
             return CF_ERR_NOMEM;
@@ -715,6 +784,9 @@ extern "C" int cf_adj_forward(cf_handle *h, int nsteps, int sample_every, int ch
     if (!A) return CF_ERR_NOMEM;
     A->recorded = false;
     A->tlm_ready = false;
+    if (A->mask_mode) drop_adj_graphs(A);
+    A->mask_mode = 0;
+    A->mask_rec = A->sig_rec = false;
     A->nsteps = nsteps;
     A->every = sample_every;
     A->K = std::min(checkpoint_every, nsteps);
@@ -746,6 +818,9 @@ extern "C" int cf_adj_forward(cf_handle *h, int nsteps, int sample_every, int ch
             return c[b].status < 0 ? CF_ERR_DT : CF_ERR_ARG;
         }
     }
+    HIPCHK(hipMemcpyAsync(A->man_rec, h->manning, (size_t)h->ncell * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    A->params_ver = h->params_ver;
     A->recorded = true;
     return CF_OK;
 }
@@ -888,6 +963,7 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
     fill0(A->gn, N, s);
 
     StepArgs a = base_args(h);
+    a.manning = A->man_rec;
     a.dt_fix = A->dt;
     a.tc_fix = A->tc;
     a.hist_stride = S;
@@ -897,7 +973,7 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
     r.nbr = h->nbr;
     r.dfc = h->dfc;
     r.geo = h->geo;
-    r.manning = h->manning;
+    r.manning = A->man_rec;
     r.hydro = h->hydro;
     r.kp = h->d_kp;
     r.tile_inst = h->tile_inst;
@@ -958,6 +1034,7 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
 extern "C" int cf_adj_backward(cf_handle *h) {
     if (!h || !h->adj || !h->adj->recorded || h->pending >= 0) return CF_ERR_ARG;
     AdjState *A = h->adj;
+    if (A->params_ver != h->params_ver) return CF_ERR_ARG;  // inflow or hydrograph changed since the record
     int rc = run_captured(h, A, 1, enqueue_backward);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -1005,9 +1082,9 @@ extern "C" int cf_tlm_set_direction(cf_handle *h, int inst, const double *dn, co
     return CF_OK;
 }
 
-template <int IFLUX, int FRIC>
+template <int IFLUX, int FRIC, bool REC>
 static void launch_tlm_t(const TlmArgs &t, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((tlm_kernel<IFLUX, FRIC>), dim3(grid), dim3(t.tile_cells), 0, s, t);
+    hipLaunchKernelGGL((tlm_kernel<IFLUX, FRIC, REC>), dim3(grid), dim3(t.tile_cells), 0, s, t);
 }
 
 // Tangent-linear run over the recorded dt and tc from the recorded initial
@@ -1015,11 +1092,19 @@ static void launch_tlm_t(const TlmArgs &t, int grid, hipStream_t s) {
 static void enqueue_tlm(cf_handle *h, AdjState *A, hipStream_t s) {
     const size_t N = (size_t)h->ncell;
     typedef void (*fn)(const TlmArgs &, int, hipStream_t);
-    static const fn tab[4][4] = {{launch_tlm_t<1, 0>, launch_tlm_t<1, 1>, launch_tlm_t<1, 2>, launch_tlm_t<1, 3>},
-                                 {launch_tlm_t<2, 0>, launch_tlm_t<2, 1>, launch_tlm_t<2, 2>, launch_tlm_t<2, 3>},
-                                 {launch_tlm_t<3, 0>, launch_tlm_t<3, 1>, launch_tlm_t<3, 2>, launch_tlm_t<3, 3>},
-                                 {launch_tlm_t<4, 0>, launch_tlm_t<4, 1>, launch_tlm_t<4, 2>, launch_tlm_t<4, 3>}};
-    const fn L = tab[h->iflux - 1][h->fric];
+    static const fn tab[2][4][4] = {
+        {{launch_tlm_t<1, 0, false>, launch_tlm_t<1, 1, false>, launch_tlm_t<1, 2, false>, launch_tlm_t<1, 3, false>},
+         {launch_tlm_t<2, 0, false>, launch_tlm_t<2, 1, false>, launch_tlm_t<2, 2, false>, launch_tlm_t<2, 3, false>},
+         {launch_tlm_t<3, 0, false>, launch_tlm_t<3, 1, false>, launch_tlm_t<3, 2, false>, launch_tlm_t<3, 3, false>},
+         {launch_tlm_t<4, 0, false>, launch_tlm_t<4, 1, false>, launch_tlm_t<4, 2, false>, launch_tlm_t<4, 3, false>}},
+        {{launch_tlm_t<1, 0, true>, launch_tlm_t<1, 1, true>, launch_tlm_t<1, 2, true>, launch_tlm_t<1, 3, true>},
+         {launch_tlm_t<2, 0, true>, launch_tlm_t<2, 1, true>, launch_tlm_t<2, 2, true>, launch_tlm_t<2, 3, true>},
+         {launch_tlm_t<3, 0, true>, launch_tlm_t<3, 1, true>, launch_tlm_t<3, 2, true>, launch_tlm_t<3, 3, true>},
+         {launch_tlm_t<4, 0, true>, launch_tlm_t<4, 1, true>, launch_tlm_t<4, 2, true>, launch_tlm_t<4, 3, true>}}};
+    // masks only for a run of the step count they were sized for
+    int mode = A->mask_steps == A->nsteps ? A->mask_mode : 0;
+    if (mode >= 3 && A->sig_steps != A->nsteps) mode = 0;
+    const fn L = tab[mode >= 3 ? 1 : 0][h->iflux - 1][h->fric];
     // start: the recorded initial state and the direction's initial tangent
     hipLaunchKernelGGL(copy_d4_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, A->ckpt, A->tst[0], N);
     hipLaunchKernelGGL(copy_d4_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, A->tdir, A->tdst[0], N);
@@ -1039,6 +1124,11 @@ static void enqueue_tlm(cf_handle *h, AdjState *A, hipStream_t s) {
     t.ncell = h->ncell;
     t.tile_cells = h->tile;
     t.xcd = h->xcd;
+    t.mask_mode = mode;
+    t.wmask = A->wmask;
+    t.flips = A->flips;
+    t.bsig = A->bsig;
+    t.bflips = A->bflips;
     double4 *vb[2] = {A->tst[0], A->tst[1]};
     double4 *db[2] = {A->tdst[0], A->tdst[1]};
     if (A->want_hmax) fill0(A->dhmax, N, s);
@@ -1061,6 +1151,58 @@ static void enqueue_tlm(cf_handle *h, AdjState *A, hipStream_t s) {
     }
 }
 
+extern "C" int cf_tlm_set_mask_mode(cf_handle *h, int mode) {
+    if (!h || !h->adj || !h->adj->recorded || h->pending >= 0 || mode < 0 || mode > 4) return CF_ERR_ARG;
+    AdjState *A = h->adj;
+    // counting needs the masks (and for mode 4 the signatures) of a run of
+    // this step count
+    if ((mode == 2 || mode == 4) && (!A->mask_rec || A->mask_steps != A->nsteps)) return CF_ERR_ARG;
+    if (mode == 4 && (!A->sig_rec || A->sig_steps != A->nsteps)) return CF_ERR_ARG;
+    const size_t N = (size_t)h->ncell;
+    if ((mode == 1 || mode == 3) && A->mask_steps != A->nsteps) {
+        dfree(A->wmask);
+        A->mask_steps = 0;
+        if (!dalloc(A->wmask, (size_t)A->nsteps * (N >> 6))) return CF_ERR_NOMEM;
+        A->mask_steps = A->nsteps;
+    }
+    if (mode == 3 && A->sig_steps != A->nsteps) {
+        dfree(A->bsig);
+        A->sig_steps = 0;
+        if (!dalloc(A->bsig, (size_t)A->nsteps * N)) return CF_ERR_NOMEM;
+        A->sig_steps = A->nsteps;
+    }
+    if (mode && !A->flips) {
+        if (!dalloc(A->flips, N) || !dalloc(A->bflips, N)) {
+            dfree(A->flips);
+            dfree(A->bflips);
+            return CF_ERR_NOMEM;
+        }
+        hipLaunchKernelGGL(fill_i_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->stream, A->flips, N);
+        hipLaunchKernelGGL(fill_i_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, h->stream, A->bflips, N);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    if (mode != A->mask_mode) drop_adj_graphs(A);
+    A->mask_mode = mode;
+    return CF_OK;
+}
+
+extern "C" int cf_tlm_get_flips(cf_handle *h, int inst, int kind, int *flips, int reset) {
+    if (bad_adj(h, inst) || !h->adj->flips || !flips || kind < 0 || kind > 1) return CF_ERR_ARG;
+    AdjState *A = h->adj;
+    int *src = kind ? A->bflips : A->flips;
+    const int c0 = h->cell_off[inst], n = h->nelt[inst];
+    int *stage = reinterpret_cast<int *>(h->stage);  // 3 * max cells doubles
+    hipLaunchKernelGGL(int_out_kernel, dim3(grid_of(n, 256)), dim3(256), 0, h->stream, src, h->perm, c0, n, stage);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(flips, stage, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    if (reset)
+        hipLaunchKernelGGL(fill_i_kernel, dim3(grid_of(n, 256)), dim3(256), 0, h->stream, src + c0, (size_t)n);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return CF_OK;
+}
+
 extern "C" int cf_tlm_run(cf_handle *h) {
     if (!h || !h->adj || !h->adj->recorded || !h->adj->dman || h->pending >= 0) return CF_ERR_ARG;
     AdjState *A = h->adj;
@@ -1069,6 +1211,8 @@ extern "C" int cf_tlm_run(cf_handle *h) {
     HIPCHK(hipStreamSynchronize(h->stream));
     A->tlm_cur = A->nsteps & 1;
     A->tlm_ready = true;
+    if (A->mask_mode == 1 || A->mask_mode == 3) A->mask_rec = true;
+    if (A->mask_mode == 3) A->sig_rec = true;
     return CF_OK;
 }
 

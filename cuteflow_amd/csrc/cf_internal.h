@@ -232,6 +232,7 @@ struct cf_handle {
     std::vector<Graph> graphs;
     int use_graphs = 1;
     cf::AdjState *adj = nullptr;
+    unsigned params_ver = 0;           // bumped by every inflow or hydrograph change
     int ad_flags = 0;                  // CF_ADJ_* bits of the derivative runs
 };
 

@@ -1239,6 +1239,7 @@ extern "C" int cf_get_discharge(cf_handle *h, double *q_in, double *q_out, int *
 extern "C" int cf_batch_set_inflow(cf_handle *h, int b, const double *debitglob, int n) {
     if (bad_inst(h, b) || !debitglob || n < 0 || n > CF_MAX_IO) return CF_ERR_ARG;
     for (int i = 0; i < n; i++) h->prm[b].debitglob[i] = debitglob[i];
+    h->params_ver++;
     h->kp[b] = make_kparams(h->prm[b], h->nelt[b], b * kHydroPerInst);
     h->kp[b].ad_flags = h->ad_flags;
     return upload_params(h, b);
@@ -1251,6 +1252,7 @@ extern "C" int cf_batch_set_hydrograph(cf_handle *h, int b, int inlet, int npts,
         return CF_ERR_ARG;
     for (int i = 0; i + 1 < npts; i++)
         if (!(t[i + 1] > t[i])) return CF_ERR_ARG;
+    h->params_ver++;
     h->prm[b].hydro_n[inlet - 1] = npts;
     for (int i = 0; i < npts; i++) {
         h->prm[b].hydro_t[(inlet - 1) * CF_MAX_HYDRO + i] = t[i];

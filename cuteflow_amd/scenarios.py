@@ -71,6 +71,34 @@ def reach(ncells: int, seed: int = 1, q_peak: float = 3000.0, t_ramp: float = 36
     return Scenario(mesh=r.mesh, params=p, state=r.state, channel=r.channel)
 
 
+def river_reach(nx: int = 3200, ny: int = 320, order: str = "rcm", seed: int = 1):
+    """The river-reach configuration: 20 km x 2 km (scaled with nx, ny)
+    meshed as nx x ny jittered quads split into triangles (2,048,000 at the
+    default size), stored in RCM, random or generation ("natural") order;
+    bed slope 1/2000 + 0.5 m bars (300 m) + a seeded smooth field (0.3 m,
+    200 m), a 3 m deep 200 m channel in a dry floodplain, n 0.03 / 0.06,
+    inflow ramped 0 -> 3000 m3/s over 1 h, fixed outlet level, dry
+    threshold 1e-3 m, WAF + superbee + fricimplic 1.  Returns (mesh,
+    params, initial state, perm) with cell k the generated cell perm[k]."""
+    r = M.river_mesh(nx=nx, ny=ny, length=20000.0 * nx / 3200, width=2000.0 * ny / 320, seed=seed)
+    m, U0 = r.mesh, r.state
+    if order == "rcm":
+        perm = M.rcm_order(m)
+    elif order == "random":
+        perm = M.random_order(m, seed)
+    elif order == "natural":
+        perm = np.arange(m.nelt)
+    else:
+        raise ValueError(order)
+    if order != "natural":
+        m, U0 = m.permuted(perm), U0[perm]
+    p = Params(iflux=3, ilimiteur=1, tolisec=1e-3, cfl=0.9, ts=1e300, inlet="inflow",
+               outlet="fixedheight", friction=1, fricimplic=1, nombre_entrees=1, nombre_sorties=1)
+    p.h_sortie[0] = r.h_out
+    p.hydrographs = {1: ([0.0, r.t_ramp], [0.0, r.q_peak])}
+    return m, p, np.ascontiguousarray(U0), perm
+
+
 def zones_of(m: M.Mesh, channel: np.ndarray, nbands: int = 4) -> np.ndarray:
     """8 zones by default: nbands reaches along the flow, channel and
     floodplain in each (zone = 2 * band + floodplain)."""
@@ -96,18 +124,38 @@ def ensemble(nmembers: int = 32, nmin: int = 50_000, nmax: int = 500_000, seed: 
     return out
 
 
-def calibration(ncells: int = 100_000, ngauges: int = 20, seed: int = 7, iflux: int = 3) -> Scenario:
-    """Zonal calibration reach: 8 zones (zones_of), hidden truth n uniform in
-    0.025-0.05 per zone (Scenario.n_zones), gauges at seeded channel cells in
-    the downstream three quarters (where the flood reaches within the run)."""
+def calibration(ncells: int = 100_000, ngauges: int = 20, seed: int = 7, iflux: int = 3,
+                flood: float = 0.5) -> Scenario:
+    """Zonal calibration reach: 8 zones (zones_of: 4 reaches along the flow x
+    channel/floodplain), hidden truth n uniform in 0.025-0.05 per zone
+    (Scenario.n_zones).  Every zone is active from the first step: the
+    reach starts flooded, its water surface `flood` m above the mean
+    floodplain level and inclined like the bed (1/2000), at rest, so that
+    the water drains down the reach over channel and floodplain alike (the
+    outlet holds that level, the inflow hydrograph adds its wave); bed highs
+    of the floodplain stay dry.  Gauges: the same number in each zone (the
+    remainder in the channel zones, upstream first), at seeded cells at
+    least 0.2 m deep at the start."""
     rng = np.random.default_rng(seed)
     s = reach(ncells, seed=seed, iflux=iflux)
-    s.zone = zones_of(s.mesh, s.channel)
+    m = s.mesh
+    s.zone = zones_of(m, s.channel)
     s.n_zones = rng.uniform(0.025, 0.05, 8)
-    s.mesh.manning = s.n_zones[s.zone].copy()
-    xc, _ = s.mesh.centroids()
-    cand = np.flatnonzero(s.channel & (xc > 0.25 * xc.max()))
-    s.gauges = np.sort(rng.choice(cand, size=ngauges, replace=False)).astype(np.int32)
+    m.manning = s.n_zones[s.zone].copy()
+    xc, _ = m.centroids()
+    length = float(m.coords[:, 0].max())
+    zc = m.coords[m.conn - 1, 2].mean(axis=1)
+    eta0 = -xc / 2000.0 + flood
+    s.state = np.zeros((m.nelt, 3))
+    s.state[:, 0] = np.maximum(eta0 - zc, 0.0)
+    s.params.h_sortie[0] = -length / 2000.0 + flood
+    per = np.full(8, ngauges // 8)
+    per[[0, 2, 4, 6][: ngauges % 8]] += 1
+    g = []
+    for z in range(8):
+        cand = np.flatnonzero((s.zone == z) & (s.state[:, 0] >= 0.2))
+        g.append(rng.choice(cand, size=min(per[z], len(cand)), replace=False))
+    s.gauges = np.sort(np.concatenate(g)).astype(np.int32)
     return s
 
 

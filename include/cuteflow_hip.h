@@ -301,7 +301,10 @@ int cf_adj_get_samples(cf_handle *h, int inst, double *eta, double *hmax);
  * hmax (nelt) and the final state (3*nelt); NULL = 0. */
 int cf_adj_set_seed(cf_handle *h, int inst, const double *g_eta, const double *g_hmax, const double *g_state);
 /* Reverse sweep over the recorded run: segments of checkpoint_every steps
- * replayed from their checkpoints, then reversed step by step. */
+ * replayed from their checkpoints, then reversed step by step.  It takes
+ * the Manning n of the recorded run, whatever n was set since (the
+ * tangent-linear run takes the current n); after an inflow or hydrograph
+ * change since the record it returns CF_ERR_ARG (record again). */
 int cf_adj_backward(cf_handle *h);
 /* Gradient of instance inst with respect to per-cell Manning n (nelt) and
  * to the initial state (3*nelt); either may be NULL. */
@@ -315,6 +318,21 @@ int cf_tlm_set_direction(cf_handle *h, int inst, const double *dn, const double 
 int cf_tlm_run(cf_handle *h);
 int cf_tlm_get(cf_handle *h, int inst, double *eta, double *d_eta, double *d_hmax, double *d_state,
                double *state);
+/* Branch decisions of tangent-linear runs, for finite-difference checks
+ * (no reference counterpart).  mode 1: the next cf_tlm_run records, per
+ * step and cell, whether the update left the cell wet (the clamp of
+ * finite_volumes_method.cuf:297-303 not taken); mode 2: the next runs count
+ * per cell the steps whose decision differs from the recorded run (a run at
+ * perturbed n that wets or dries a cell at another step).  Modes 3 and 4 do
+ * the same and also record / compare a signature of every branch of the
+ * cell's update (dry-face thresholds, wave-speed cases, solver regions,
+ * max/min selections, the clamp): 4 bytes per step and cell.  0: off.
+ * Needs a recorded forward; modes 2 and 4 need masks recorded for its step
+ * count; a new cf_adj_forward turns them off.  cf_tlm_get_flips returns the
+ * counts of instance inst (kind 0: wet/dry clamp, 1: any branch) in the
+ * caller's cell order and zeroes them if reset. */
+int cf_tlm_set_mask_mode(cf_handle *h, int mode);
+int cf_tlm_get_flips(cf_handle *h, int inst, int kind, int *flips, int reset);
 
 /* Elementwise pow used by the friction kernel, evaluated on the device for
  * n values (x > 0): lets tests compare the device pow bit-for-bit. */

@@ -259,3 +259,38 @@ def test_gpu_reverse_tile_skipping_is_exact():
         assert _same(out[k][1], out[3][1])
         assert np.array_equal(out[k][2], out[3][2])
         assert np.array_equal(out[k][3], out[3][3])
+
+
+def test_gpu_branch_masks():
+    """Diagnostic masks of tangent-linear runs: recording them changes no
+    value or tangent (the recording kernel's arithmetic is the plain one's);
+    a run at the recorded n takes every branch as recorded (no flips of
+    either kind); a run at perturbed n on a reach that wets and dries takes
+    some differently, and every changed wet/dry decision is also a changed
+    branch.  A new recorded forward turns the masks off."""
+    m, p, U0 = river_case(30, 8, seed=25, iflux=2, fricimplic=1)
+    nsteps = 80
+    rng = np.random.default_rng(3)
+    dn = rng.normal(size=m.nelt) * 0.01
+    with Solver(m, p) as s:
+        run = AdjointRun(s, _gauges(m, 6, 4), sample_every=8, checkpoint_every=20)
+        run.forward(nsteps, U0=[U0])
+        plain = run.tlm([dn])[0]
+        run.set_mask_mode(3)
+        rec = run.tlm([dn])[0]
+        for k in ("eta", "d_eta", "state", "d_state"):
+            assert _same(plain[k], rec[k]), k
+        run.set_mask_mode(4)
+        run.tlm([dn])
+        f0, f1 = run.flips(0)[0], run.flips(1)[0]
+        assert f0.sum() == 0 and f1.sum() == 0
+        run.set_manning([m.manning * 1.5])
+        run.tlm([dn])
+        f0, f1 = run.flips(0)[0], run.flips(1)[0]
+        assert f1.sum() > 0 and f0.sum() <= f1.sum()
+        assert np.all(f1[f0 > 0] > 0)
+        run.set_manning([m.manning])
+        run.forward(nsteps, U0=[U0])
+        with pytest.raises(Exception):
+            run.set_mask_mode(4)            # masks of the former record are gone
+        run.set_mask_mode(0)
