@@ -393,6 +393,18 @@ def bench_c1(args):
 
 
 def bench_calibration(args):
+    """Zonal calibration reach, with each first-order scheme: HLLC (iflux 1)
+    and the reference's calibration scheme (iflux 2)."""
+    out = {"workload": "100k-cell reach starting flooded (surface 0.5 m above the mean floodplain level, "
+                       "inclined like the bed), 8 Manning zones (4 reaches x channel/floodplain), 20 gauges "
+                       "(2-3 per zone), levels every 10 steps over 2000 steps, checkpoints every 100 steps, "
+                       "semi-implicit friction; observations from the hidden truth, gradient at +20 %"}
+    for iflux in (1, 2):
+        out[f"iflux{iflux}"] = _calibration_one(args, iflux)
+    return out
+
+
+def _calibration_one(args, iflux):
     """Zonal calibration reach: gradient cost, checks, L-BFGS."""
     import numpy as np
 
@@ -402,15 +414,12 @@ def bench_calibration(args):
     from cuteflow_amd.solver import Solver
 
     nsteps, every, K = 2000, 10, 100
-    sc = SC.calibration(100_000, ngauges=20, iflux=1)
+    sc = SC.calibration(100_000, ngauges=20, iflux=iflux)
     N = sc.mesh.nelt
     Z = np.zeros((N, 8))
     Z[np.arange(N), sc.zone] = 1.0
-    out = {"workload": "100k-cell reach, 8 Manning zones (4 reaches x channel/floodplain), 20 channel gauges, "
-                       "levels every 10 steps over 2000 steps, checkpoints every 100 steps, HLLC + "
-                       "semi-implicit friction; observations from the hidden truth, gradient at +20 %",
-           "nelt": N, "scheme": "iflux 1 (HLLC), fricimplic 1"}
-    options, out["schedule"] = section_options(args, [sc.mesh], [sc.params], [sc.state], "calibration")
+    out = {"nelt": N, "scheme": f"iflux {iflux} ({'HLLC' if iflux == 1 else 'Riadh state HLLC'}), fricimplic 1"}
+    options, out["schedule"] = section_options(args, [sc.mesh], [sc.params], [sc.state], f"calibration iflux {iflux}")
     with Solver(sc.mesh, sc.params, options=options) as s:
         s.set_state(sc.state)
         s.step(nsteps)
@@ -438,13 +447,14 @@ def bench_calibration(args):
         t_grad = min(gtimes[1:])
         g = grads[-1]
         reproducible = all(np.array_equal(grads[0], x) for x in grads)
-        log(f"bench: calibration forward {t_fwd*1e3:.1f} ms, gradient {t_grad*1e3:.1f} ms "
+        log(f"bench: calibration iflux {iflux} forward {t_fwd*1e3:.1f} ms, gradient {t_grad*1e3:.1f} ms "
             f"(first {gtimes[0]*1e3:.1f} ms), ratio {t_grad / t_fwd:.2f}")
         g_tlm = CAL.zonal_tlm_gradient(run, Z, obs)
-        g_fd = CAL.zonal_fd_gradient(run, Z, n0, obs, eps=1e-6)
-        g_fd7 = CAL.zonal_fd_gradient(run, Z, n0, obs, eps=1e-7)
+        g_fd, cf6, bf6 = CAL.zonal_fd_gradient(run, Z, n0, obs, eps=1e-6, count_flips=True)
+        g_fd7, cf7, bf7 = CAL.zonal_fd_gradient(run, Z, n0, obs, eps=1e-7, count_flips=True)
         rel = lambda a, b: (np.abs(a - b) / np.maximum(np.abs(b), 1e-300)).tolist()
-        log(f"bench: calibration gradient vs TLM {max(rel(g, g_tlm)):.1e}, vs FD(1e-6) {rel(g, g_fd)}")
+        log(f"bench: calibration iflux {iflux} gradient vs TLM {max(rel(g, g_tlm)):.1e}, vs FD(1e-7) "
+            f"{rel(g, g_fd7)} with branch flips {bf7.tolist()}")
         out.update({
             "forward_ms": t_fwd * 1e3,
             "gradient_ms": t_grad * 1e3,
@@ -460,6 +470,12 @@ def bench_calibration(args):
             "rel_err_vs_tlm": rel(g, g_tlm),
             "rel_err_vs_fd_1e-6": rel(g, g_fd),
             "rel_err_vs_fd_1e-7": rel(g, g_fd7),
+            "fd_clamp_flips_1e-6": cf6.tolist(), "fd_branch_flips_1e-6": bf6.tolist(),
+            "fd_clamp_flips_1e-7": cf7.tolist(), "fd_branch_flips_1e-7": bf7.tolist(),
+            "fd_flips_definition": "per zone: (step, cell) updates whose wet/dry clamp (clamp) or any branch "
+                                   "(branch) the two perturbed runs take differently from the run at n0",
+            "fd_within_1e-6_on_branch_flip_free_zones": bool(all(
+                e <= 1e-6 for e, f in zip(rel(g, g_fd7) + rel(g, g_fd), bf7.tolist() + bf6.tolist()) if f == 0)),
             "reproducible_bits": bool(reproducible),
         })
         if args.lbfgs_iters > 0:
@@ -559,6 +575,7 @@ def bench_ensemble(args):
 def bench_field(args):
     """16 x 1M per-cell Manning fields: batched gradients."""
     import numpy as np
+    import torch
 
     from cuteflow_amd import calibration as CAL
     from cuteflow_amd import scenarios as SC
@@ -578,6 +595,8 @@ def bench_field(args):
     with BatchSolver([e.mesh for e in mem], [e.params for e in mem], options=options) as bs:
         U0 = [e.state for e in mem]
         surf = [bs.geometry(b)["surf"] for b in range(B)]
+        # the loss and its seeds stay on the GPU (torch), as in a calibration loop
+        surf_t = [torch.as_tensor(x, device="cuda") for x in surf]
         ts = []
         for _ in range(2):
             for b in range(B):
@@ -595,14 +614,14 @@ def bench_field(args):
         scale = [max(1.0, 0.01 * a) for a in a_obs]
 
         def gradient():
-            etas, hm = run.forward(nsteps, U0=U0, manning=n0)
+            etas, hm = run.forward(nsteps, U0=U0, manning=n0, device=True)
             ge, gh, loss = [], [], 0.0
             for b in range(B):
                 ge.append(2.0 * (etas[b] - obs[b]))
-                le, dh = CAL.extent_loss(hm[b], surf[b], a_obs[b], scale[b])
+                le, dh = CAL.extent_loss(hm[b], surf_t[b], a_obs[b], scale[b])
                 gh.append(dh)
                 loss += CAL.gauge_misfit(etas[b], obs[b]) + le
-            gn, _ = run.backward(ge, gh)
+            gn, _ = run.backward(ge, gh, want_state=False, device=True)
             return gn, etas, hm, loss
 
         gt = []
@@ -613,6 +632,8 @@ def bench_field(args):
             sync()
             gt.append(time.perf_counter() - t)
         t_grad = gt[-1]
+        gn = [x.cpu().numpy() for x in gn]
+        hm = [x.cpu().numpy() for x in hm]
         log(f"bench: field ensemble forward {t_fwd*1e3:.0f} ms, gradient {t_grad*1e3:.0f} ms "
             f"(first {gt[0]*1e3:.0f} ms), ratio {t_grad / t_fwd:.2f}, {B / t_grad:.2f} grads/s")
         # tangent-linear checks along 4 random directions (per-cell smooth
@@ -657,10 +678,11 @@ def bench_scenarios(args):
     meshes, params, U0 = [x.mesh for x in sc], [x.params for x in sc], [x.state for x in sc]
     n0 = [1.2 * x.n_zones for x in sc]
     mann0 = [n0[b][sc[b].zone] for b in range(B)]
-    out = {"workload": f"{B} calibration scenarios (flood events) on one 100k-cell reach: own peak inflow "
-                       "500-4000 m3/s, own hidden n in 8 zones 0.025-0.05, 20 channel gauges each, levels every "
-                       "10 steps over 2000 steps, checkpoints every 100 steps, HLLC + semi-implicit friction, "
-                       "zonal gradient at +20 % through the autograd Function",
+    out = {"workload": f"{B} calibration scenarios (flood events) on one 100k-cell reach, water in the channel and "
+                       "the floodplain dry at the start: own peak inflow 500-4000 m3/s, own hidden n in 8 zones "
+                       "0.025-0.05, 20 channel gauges each, levels every 10 steps over 2000 steps, checkpoints "
+                       "every 100 steps, HLLC + semi-implicit friction, zonal gradient at +20 % through the "
+                       "autograd Function",
            "scenarios": B, "cells": sum(m.nelt for m in meshes), "scheme": "iflux 1 (HLLC), fricimplic 1"}
     for b in range(B):
         meshes[b].manning = mann0[b]
@@ -723,6 +745,17 @@ def bench_scenarios(args):
         checks = {}
         for eps in (1e-6, 1e-7):
             vals = np.zeros((2, B, nz))
+            # branch masks of the run at n0, then per (scenario, zone) the
+            # (step, cell) updates that the two perturbed runs take
+            # differently (wet/dry clamp; any branch)
+            run.set_manning(mann0)
+            run.set_mask_mode(3)
+            run.tlm()
+            run.set_mask_mode(4)
+            run.flips(0, reset=True)
+            run.flips(1, reset=True)
+            cflips = np.zeros((B, nz), np.int64)
+            bflips = np.zeros((B, nz), np.int64)
             for z in range(nz):
                 for si, sgn in enumerate((1.0, -1.0)):
                     nzp = [x.copy() for x in n0]
@@ -732,13 +765,23 @@ def bench_scenarios(args):
                     tl = run.tlm()
                     vals[si, :, z] = [CAL.gauge_misfit(tl[b]["eta"].reshape(-1), obs[b].reshape(-1))
                                       for b in range(B)]
+                cflips[:, z] = [int(f.sum()) for f in run.flips(0, reset=True)]
+                bflips[:, z] = [int(f.sum()) for f in run.flips(1, reset=True)]
+            run.set_mask_mode(0)
             run.set_manning(mann0)
             gfd = (vals[0] - vals[1]) / (2.0 * eps * np.array(n0))
             scale = np.max(np.abs(gfd), axis=1, keepdims=True)
             err = np.abs(g - gfd) / np.maximum(scale, 1e-300)
+            rel = np.abs(g - gfd) / np.maximum(np.abs(gfd), 1e-300)
+            free = bflips == 0
             checks[f"fd_{eps:g}"] = {"max_rel_err_normwise": float(err.max()),
                                      "median_rel_err_normwise": float(np.median(err)),
-                                     "fraction_within_1e-6": float(np.mean(err <= 1e-6))}
+                                     "fraction_within_1e-6": float(np.mean(err <= 1e-6)),
+                                     "gradients_with_clamp_flips": int(np.sum(cflips > 0)),
+                                     "gradients_with_branch_flips": int(np.sum(bflips > 0)),
+                                     "branch_flip_free_gradients": int(np.sum(free)),
+                                     "max_rel_err_branch_flip_free": float(rel[free].max()) if free.any() else None,
+                                     "misses_without_branch_flips": int(np.sum((rel > 1e-6) & free))}
         rng = np.random.default_rng(5)
         dz = [rng.normal(size=nz) * x for x in n0]
         tl = run.tlm([dz[b][sc[b].zone] for b in range(B)])

@@ -41,6 +41,15 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p), a
 
 
+def _empty(n, device: bool):
+    """A float64 array of n values: a CUDA tensor with device, else numpy."""
+    if device:
+        import torch
+
+        return torch.empty(n, dtype=torch.float64, device="cuda")
+    return np.zeros(n)
+
+
 class AdjointRun:
     """Recorded forward runs and reverse sweeps on a Solver or BatchSolver.
 
@@ -87,10 +96,10 @@ class AdjointRun:
             U = np.ascontiguousarray(U, np.float64)
             _lib.check(self.lib.cf_batch_set_state(self.h, b, _lib.dptr(U)), "cf_batch_set_state")
 
-    def forward(self, nsteps: int, U0=None, manning=None):
+    def forward(self, nsteps: int, U0=None, manning=None, device: bool = False):
         """Recorded run of nsteps steps (from U0 when given, else from the
         current state).  Returns (eta per instance (nsamp, ngauges), hmax per
-        instance or None)."""
+        instance or None); with device, hmax as float64 CUDA tensors."""
         if manning is not None:
             self.set_manning(manning)
         if U0 is not None:
@@ -103,18 +112,23 @@ class AdjointRun:
         etas, hmaxs = [], []
         for b in range(self.ninst):
             eta = np.zeros((self.nsamp, len(self.gauges[b])))
-            hm = np.zeros(self.nelt[b]) if self.want_hmax else None
-            _lib.check(self.lib.cf_adj_get_samples(self.h, b, _lib.dptr(eta),
-                                                   hm.ctypes.data_as(C.c_void_p) if hm is not None else None),
-                       "cf_adj_get_samples")
+            hm, hp = None, None
+            if self.want_hmax:
+                hm = _empty(self.nelt[b], device)
+                hp = C.c_void_p(hm.data_ptr()) if device else hm.ctypes.data_as(C.c_void_p)
+            _lib.check(self.lib.cf_adj_get_samples(self.h, b, _lib.dptr(eta), hp), "cf_adj_get_samples")
             etas.append(eta)
             hmaxs.append(hm)
         return etas, (hmaxs if self.want_hmax else None)
 
     # -- reverse ---------------------------------------------------------------
-    def backward(self, g_eta, g_hmax=None, g_state=None):
-        """Cotangents of the outputs (per instance; None = 0) -> gradients
-        (per instance) with respect to per-cell n and the initial state."""
+    def backward(self, g_eta, g_hmax=None, g_state=None, want_state: bool = True, device: bool = False,
+                 out=None):
+        """Cotangents of the outputs (per instance; None = 0; per-cell seeds
+        may be CUDA tensors) -> gradients (per instance) with respect to
+        per-cell n and the initial state (None unless want_state).  With
+        device the n gradients are float64 CUDA tensors (written into the
+        tensors of `out` when given)."""
         for b in range(self.ninst):
             ge = None if g_eta is None or g_eta[b] is None else np.ascontiguousarray(g_eta[b], np.float64)
             gh, kh = _ptr(None if g_hmax is None else g_hmax[b])
@@ -125,10 +139,11 @@ class AdjointRun:
         _lib.check(self.lib.cf_adj_backward(self.h), "cf_adj_backward")
         gns, gus = [], []
         for b in range(self.ninst):
-            gn = np.zeros(self.nelt[b])
-            gu = np.zeros((self.nelt[b], 3))
-            _lib.check(self.lib.cf_adj_get_gradient(self.h, b, gn.ctypes.data_as(C.c_void_p),
-                                                    gu.ctypes.data_as(C.c_void_p)), "cf_adj_get_gradient")
+            gn = out[b] if out is not None else _empty(self.nelt[b], device)
+            gu = np.zeros((self.nelt[b], 3)) if want_state else None
+            pn = C.c_void_p(gn.data_ptr()) if device else gn.ctypes.data_as(C.c_void_p)
+            _lib.check(self.lib.cf_adj_get_gradient(self.h, b, pn, gu.ctypes.data_as(C.c_void_p) if want_state
+                                                    else None), "cf_adj_get_gradient")
             gns.append(gn)
             gus.append(gu)
         return gns, gus
@@ -210,13 +225,14 @@ def _fn():
             @staticmethod
             def forward(ctx, n_cells, run, nsteps, U0):
                 ns = _split(n_cells.detach(), run.nelt)
-                etas, hmaxs = run.forward(nsteps, U0=U0, manning=ns)
+                on_dev = n_cells.is_cuda
+                etas, hmaxs = run.forward(nsteps, U0=U0, manning=ns, device=on_dev)
                 ctx.run = run
                 ctx.record = run.records
                 ctx.device = n_cells.device
                 eta = torch.as_tensor(np.concatenate([e.reshape(-1) for e in etas]), device=n_cells.device)
                 if run.want_hmax:
-                    hm = torch.as_tensor(np.concatenate(hmaxs), device=n_cells.device)
+                    hm = torch.cat(hmaxs) if on_dev else torch.as_tensor(np.concatenate(hmaxs))
                     return eta, hm
                 return eta
 
@@ -232,9 +248,14 @@ def _fn():
                       for g in _split(g_eta, sizes)]
                 gh = None
                 if run.want_hmax and g_hmax is not None:
-                    gh = [g.detach().cpu().numpy() for g in _split(g_hmax, run.nelt)]
-                gns, _ = run.backward(ge, gh)
-                g = torch.as_tensor(np.concatenate(gns), device=ctx.device)
+                    gh = [g.detach() for g in _split(g_hmax, run.nelt)]
+                if ctx.device.type == "cuda":
+                    # the per-cell gradients are written into one CUDA tensor
+                    g = torch.empty(sum(run.nelt), dtype=torch.float64, device=ctx.device)
+                    run.backward(ge, gh, want_state=False, device=True, out=list(_split(g, run.nelt)))
+                else:
+                    gns, _ = run.backward(ge, gh, want_state=False)
+                    g = torch.as_tensor(np.concatenate(gns))
                 return g, None, None, None
 
         _FN = _GaugeLevels

@@ -104,7 +104,16 @@ def zonal_lbfgs(run: AdjointRun, zone_of_cell, n_start, obs, nsteps: int, U0, it
 
 def soft_extent(hmax, surf, h0: float = 0.05, width: float = 0.02):
     """Flooded area sum(surf * sigmoid((hmax - h0) / width)) and its
-    derivative with respect to hmax (per cell)."""
+    derivative with respect to hmax (per cell).  hmax and surf: numpy
+    arrays, or torch tensors (the derivative is then a tensor on their
+    device)."""
+    if hasattr(hmax, "data_ptr"):
+        import torch
+
+        z = ((hmax - h0) / width).clamp(-60.0, 60.0)
+        sg = 1.0 / (1.0 + torch.exp(-z))
+        area = float(torch.sum(surf * sg))
+        return area, surf * sg * (1.0 - sg) / width
     z = (np.asarray(hmax) - h0) / width
     sg = 1.0 / (1.0 + np.exp(-np.clip(z, -60.0, 60.0)))
     area = float(np.sum(surf * sg))

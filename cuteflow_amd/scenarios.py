@@ -138,33 +138,47 @@ def calibration(ncells: int = 100_000, ngauges: int = 20, seed: int = 7, iflux: 
     least 0.2 m deep at the start."""
     rng = np.random.default_rng(seed)
     s = reach(ncells, seed=seed, iflux=iflux)
-    m = s.mesh
-    s.zone = zones_of(m, s.channel)
+    s.zone = zones_of(s.mesh, s.channel)
     s.n_zones = rng.uniform(0.025, 0.05, 8)
-    m.manning = s.n_zones[s.zone].copy()
+    s.mesh.manning = s.n_zones[s.zone].copy()
+    _flood(s, flood)
+    s.gauges = _stratified_gauges(rng, s.zone, s.state, ngauges)
+    return s
+
+
+def _flood(s: Scenario, flood: float) -> None:
+    """Initial state at rest with the water surface `flood` m above the mean
+    floodplain level, inclined like the bed (1/2000); the outlet holds that
+    level."""
+    m = s.mesh
     xc, _ = m.centroids()
     length = float(m.coords[:, 0].max())
     zc = m.coords[m.conn - 1, 2].mean(axis=1)
-    eta0 = -xc / 2000.0 + flood
     s.state = np.zeros((m.nelt, 3))
-    s.state[:, 0] = np.maximum(eta0 - zc, 0.0)
+    s.state[:, 0] = np.maximum(-xc / 2000.0 + flood - zc, 0.0)
     s.params.h_sortie[0] = -length / 2000.0 + flood
+
+
+def _stratified_gauges(rng, zone: np.ndarray, state: np.ndarray, ngauges: int) -> np.ndarray:
+    """ngauges cells spread evenly over the 8 zones (the remainder in the
+    channel zones, upstream first), each at least 0.2 m deep at the start."""
     per = np.full(8, ngauges // 8)
     per[[0, 2, 4, 6][: ngauges % 8]] += 1
     g = []
     for z in range(8):
-        cand = np.flatnonzero((s.zone == z) & (s.state[:, 0] >= 0.2))
+        cand = np.flatnonzero((zone == z) & (state[:, 0] >= 0.2))
         g.append(rng.choice(cand, size=min(per[z], len(cand)), replace=False))
-    s.gauges = np.sort(np.concatenate(g)).astype(np.int32)
-    return s
+    return np.sort(np.concatenate(g)).astype(np.int32)
 
 
 def calibration_batch(nscen: int = 64, ncells: int = 100_000, ngauges: int = 20, seed: int = 64,
                       iflux: int = 1):
-    """nscen calibration scenarios on the zonal calibration reach (one mesh,
-    8 zones): scenario k has its own peak inflow (uniform in 500-4000 m3/s
-    over the 1 h ramp), hidden true n per zone (uniform in 0.025-0.05) and
-    ngauges seeded channel gauges in the downstream three quarters."""
+    """nscen flood events on the 100k-cell reach (one mesh, 8 zones as in
+    calibration(), the river configuration's start: water in the channel,
+    the floodplain dry): scenario k has its own peak inflow (uniform in
+    500-4000 m3/s over the 1 h ramp), hidden true n per zone (uniform in
+    0.025-0.05) and ngauges seeded channel gauges in the downstream three
+    quarters."""
     rng = np.random.default_rng(seed)
     base = reach(ncells, seed=seed, iflux=iflux)
     zone = zones_of(base.mesh, base.channel)
