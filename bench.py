@@ -704,12 +704,13 @@ def bench_scenarios(args):
             ts.append(time.perf_counter() - t)
         assert int(done.min()) == nsteps and int(st.max()) == 0
         t_fwd = ts[-1]
+        U0d = [torch.as_tensor(u, device=dev) for u in U0]   # initial states resident on the GPU
         zidx = torch.tensor(np.concatenate([nz * b + sc[b].zone for b in range(B)]), device=dev)
         obs_t = torch.tensor(np.concatenate([o.reshape(-1) for o in obs]), dtype=torch.float64, device=dev)
 
         def gradient():
             nzt = torch.tensor(np.concatenate(n0), dtype=torch.float64, device=dev, requires_grad=True)
-            eta = GaugeLevels.apply(nzt[zidx], run, nsteps, U0)
+            eta = GaugeLevels.apply(nzt[zidx], run, nsteps, U0d)
             loss = ((eta - obs_t) ** 2).sum()
             loss.backward()
             return nzt.grad.view(B, nz).cpu().numpy(), float(loss)

@@ -92,7 +92,17 @@ class AdjointRun:
             del keep
 
     def set_state(self, U_list) -> None:
+        """Initial states (nelt, 3) per instance: numpy arrays, or float64
+        CUDA tensors (copied on the device)."""
         for b, U in enumerate(U_list):
+            if hasattr(U, "data_ptr") and U.is_cuda:
+                import torch
+
+                t = U.detach().to(torch.float64).contiguous()
+                torch.cuda.current_stream().synchronize()
+                _lib.check(self.lib.cf_batch_set_state_device(self.h, b, C.c_void_p(t.data_ptr())),
+                           "cf_batch_set_state_device")
+                continue
             U = np.ascontiguousarray(U, np.float64)
             _lib.check(self.lib.cf_batch_set_state(self.h, b, _lib.dptr(U)), "cf_batch_set_state")
 

@@ -84,6 +84,8 @@ def zonal_lbfgs(run: AdjointRun, zone_of_cell, n_start, obs, nsteps: int, U0, it
     zidx = torch.as_tensor(np.asarray(zone_of_cell, np.int64), device=dev)
     obs_t = torch.as_tensor(np.asarray(obs).reshape(-1), device=dev)
     n_z = torch.tensor(np.asarray(n_start, dtype=np.float64), device=dev, requires_grad=True)
+    if dev.type == "cuda":
+        U0 = torch.as_tensor(np.asarray(U0, np.float64), device=dev)   # resident for every evaluation
     with torch.no_grad():
         l0 = ((GaugeLevels.apply(zone_gather(n_z, zidx, Z), run, nsteps, [U0]) - obs_t) ** 2).sum().item()
     opt = torch.optim.LBFGS([n_z], lr=1.0, max_iter=iters, history_size=10, tolerance_grad=1e-14,

@@ -71,7 +71,7 @@ struct AdjState {
     double *xs = nullptr;     // K x 3 x ncell: residual x of each step of a segment
     double4 *lam[2] = {nullptr, nullptr};
     double4 *gstate = nullptr;  // final-state seed
-    double4 *R1[2] = {nullptr, nullptr};  // 4 x ncell: neighbours 0..2, own (by step parity)
+    double *R1[2] = {nullptr, nullptr};   // 12 x ncell: (h, hu, hv) for neighbours 0..2, own (by step parity)
     double4 *R2[2] = {nullptr, nullptr};  // 6 x ncell: face k, slot s at 2k + s
     int *r2_off = nullptr, *r2_idx = nullptr;
     double *gn = nullptr;
@@ -126,9 +126,9 @@ struct RevArgs {
     const int *inst_end;
     const double *xs;
     const double4 *lam;        // cotangent of the state after the last step (when ran_next is null)
-    const double4 *R1in;       // contributions of the reverse of step k + 1
+    const double *R1in;        // contributions of the reverse of step k + 1
     const double4 *R2in;
-    double4 *R1;               // contributions of this step's reverse
+    double *R1;                // contributions of this step's reverse
     double4 *R2;
     const int *r2_off;
     const int *r2_idx;
@@ -178,9 +178,19 @@ __device__ __forceinline__ void acc4(double4 &a, const double4 &b) {
 // levels of that sample in increasing gauge index).  `tiles_ran` is the
 // activity row of step k + 1; a cell whose tile did not run gets its seeds
 // only.
+// Contribution slots: per cell 12 doubles, (h, hu, hv) for the cells
+// across faces 0..2, then its own.
+constexpr int kR1 = 12;
+
+__device__ __forceinline__ void acc3(double4 &a, const double *__restrict__ s) {
+    a.x += s[0];
+    a.y += s[1];
+    a.z += s[2];
+}
+
 template <bool R2ON>
 __device__ __forceinline__ double4 gather_cell(int c, int tcells, const unsigned char *__restrict__ tiles_ran,
-                                               const int4 *__restrict__ nbr, const double4 *__restrict__ R1,
+                                               const int4 *__restrict__ nbr, const double *__restrict__ R1,
                                                const double4 *__restrict__ R2, const int *__restrict__ r2_off,
                                                const int *__restrict__ r2_idx, const int *__restrict__ argmax,
                                                const double *__restrict__ ghmax, int sidx,
@@ -188,13 +198,14 @@ __device__ __forceinline__ double4 gather_cell(int c, int tcells, const unsigned
                                                const int *__restrict__ gnext) {
     double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
     if (tiles_ran[c / tcells]) {
-        acc = R1[4 * (size_t)c + 3];
+        const double *o = R1 + kR1 * (size_t)c + 9;
+        acc = make_double4(o[0], o[1], o[2], 0.0);
         const int4 nb = nbr[c];
         const int nbv[3] = {nb.x, nb.y, nb.z};
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const int i = nbv[k];
-            if (i >= 0 && tiles_ran[i / tcells]) acc4(acc, R1[4 * (size_t)i + ((nb.w >> (2 * k)) & 3)]);
+            if (i >= 0 && tiles_ran[i / tcells]) acc3(acc, R1 + kR1 * (size_t)i + 3 * ((nb.w >> (2 * k)) & 3));
         }
         if (R2ON)
             for (int e = r2_off[c]; e < r2_off[c + 1]; e++) {
@@ -236,9 +247,16 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(RevWaves<
     const double lam[3] = {l4.x, l4.y, l4.z};
     double own[3], r1[3][3], r2[3][2][3], gn;
     cell_reverse<IFLUX, FRIC>(ld, kp[b], r.hydro, i, r.dt[hix], r.tc[hix], x, lam, own, r1, r2, gn);
+    double *slots = r.R1 + kR1 * (size_t)i;
 #pragma unroll
-    for (int k = 0; k < 3; k++) r.R1[4 * (size_t)i + k] = make_double4(r1[k][0], r1[k][1], r1[k][2], 0.0);
-    r.R1[4 * (size_t)i + 3] = make_double4(own[0], own[1], own[2], 0.0);
+    for (int k = 0; k < 3; k++) {
+        slots[3 * k] = r1[k][0];
+        slots[3 * k + 1] = r1[k][1];
+        slots[3 * k + 2] = r1[k][2];
+    }
+    slots[9] = own[0];
+    slots[10] = own[1];
+    slots[11] = own[2];
     if (IFLUX >= 3) {
         const int mask = r.nbr[i].w >> kR2Shift;
 #pragma unroll
@@ -254,7 +272,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(RevWaves<
 // Cotangent of the initial state: the gather of step 0's contributions
 // (the initial state carries no seeds), every cell.
 template <bool R2ON>
-__global__ void gather_kernel(const int4 *__restrict__ nbr, const double4 *__restrict__ R1,
+__global__ void gather_kernel(const int4 *__restrict__ nbr, const double *__restrict__ R1,
                               const double4 *__restrict__ R2, const int *__restrict__ r2_off,
                               const int *__restrict__ r2_idx, const int *__restrict__ tile_inst,
                               const int *__restrict__ inst_end, const unsigned char *__restrict__ ran,
@@ -702,12 +720,12 @@ static int adj_alloc(cf_handle *h, AdjState *A) {
     }
     if (!A->lam[0]) {
         drop_adj_graphs(A);
-        if (!dalloc(A->lam[0], N) || !dalloc(A->man_rec, N) || !dalloc(A->lam[1], N) || !dalloc(A->gstate, N) || !dalloc(A->R1[0], 4 * N) ||
-            !dalloc(A->R1[1], 4 * N) || !dalloc(A->gn, N) || !dalloc(A->hmax, N) || !dalloc(A->ghmax, N) ||
+        if (!dalloc(A->lam[0], N) || !dalloc(A->man_rec, N) || !dalloc(A->lam[1], N) || !dalloc(A->gstate, N) || !dalloc(A->R1[0], kR1 * N) ||
+            !dalloc(A->R1[1], kR1 * N) || !dalloc(A->gn, N) || !dalloc(A->hmax, N) || !dalloc(A->ghmax, N) ||
             !dalloc(A->dhmax, N) || !dalloc(A->argmax, N))
             return CF_ERR_NOMEM;
-        fill0(A->R1[0], 4 * N, h->stream);
-        fill0(A->R1[1], 4 * N, h->stream);
+        fill0(A->R1[0], kR1 * N, h->stream);
+        fill0(A->R1[1], kR1 * N, h->stream);
     }
     return CF_OK;
 }
