@@ -210,6 +210,51 @@ def test_reference_gpu_discharge_series_iflux2():
             assert q_in == q_in_sum == p.debitglob[0]
 
 
+def test_reference_gpu_run_river_piers_whole_loop():
+    """The river-with-piers reference GPU run (Calibration_manning/
+    docs_calibration/River_piers/Froude/outfile.out; 1,408 cells, iflux 2,
+    fricimplic 1, inflow 62 m3/s, outlet level 1e-4 m, plane IC at 0.05 m,
+    CFL 0.6, override n 0.01: the case's own donnees.f, whose TS of 360 s the
+    logged run had raised to 3600 s -- the log ends at t = 3593.1 s after
+    iteration 274000).  Over the whole run of 274,526 iterations:
+      - the oracle's loop ends at exactly the logged iteration (tc crosses
+        TS at the same step: the dt sequence of the whole run agrees);
+      - the outflow discharge printed every 1000 iterations is the oracle's
+        to 32 ulps (the reference sums its per-edge discharges in its own
+        order; the differences stay at the ulp level through the transient
+        and the steady state, max 24 ulps);
+      - each printed t lies in the range its racy sum allows: the reference
+        adds, at each iteration, the dt of that iteration or of the next one
+        (an unsynchronised cudaMemcpyAsync, cfl_condition.cuf:125,
+        finite_volumes_method.cuf:232)."""
+    from cuteflow_amd.params import read_donnees
+    from cuteflow_amd.solver import initial_state
+
+    m, _, exp = _fixture("river_piers")
+    log = exp["logs"]["outfile.out"]
+    p = read_donnees(os.path.join(GOLDEN, "river_piers.donnees.f"))
+    assert p.iflux == 2 and p.fricimplic == 1 and p.plan == 1 and p.is_override_manning == 1
+    assert p.override_manning == 0.01 and p.cfl == 0.6 and p.ts == 360.0
+    assert p.debitglob[0] == float(log["debitglob_1"]) and p.h_sortie[0] == float(log["h_sortie_1"])
+    assert p.tolisec == float(log["tolisec"]) and int(log["iflux"]) == 2
+    p.ts = 3600.0
+    prep = O.Prepared(m)
+    U0 = initial_state(p, m, prep.zm)
+    end = int(log["loop_end_iteration"])
+    r = O.run(prep, p, U0, end + 10, qout=True)
+    # the loop condition stopped it (not the step budget): tc reached TS
+    assert r["status"] == 0 and r["steps"] == end == 274526 and r["tc"] >= p.ts
+    d = np.asarray(r["dt_hist"])[: end + 1]
+    lo = np.concatenate([[0.0], np.cumsum(np.minimum(d[:-1], d[1:]))])
+    hi = np.concatenate([[0.0], np.cumsum(np.maximum(d[:-1], d[1:]))])
+    ser = log["series"]
+    assert len(ser) == 274
+    for it, t, q_in, q_in_sum, q_out in ser:
+        assert abs(r["qout"][it - 1] - q_out) <= 32 * math.ulp(q_out), (it, r["qout"][it - 1], q_out)
+        assert lo[it] <= t <= hi[it], (it, lo[it], t, hi[it])
+        assert q_in == q_in_sum == 62.00000000000002
+
+
 def test_steady_state_stop():
     """check_steady_state (finite_volumes_method.cuf:370-393): the run ends
     after the step that makes more than 1000 consecutive steps with
