@@ -26,7 +26,7 @@ DONNEES = """&DONNEES_NAMELIST
   eqlin_barrage=1, x1eqbar=150.0, x2eqbar=150.0001, y1eqbar=0.0, y2eqbar=200.0,
   H_AMONT={h_amont}, U_AMONT=0, V_AMONT=0, H_AVAL={h_aval}, U_AVAL=0, V_AVAL=0,
   plan=0, solinit=0, fich_sol_init='',
-  inlet='inflow', inlet_name="in", debitglob=120.0,
+  inlet='{inlet}', inlet_name="in", debitglob=120.0,
   outlet="fixedheight", outlet_name="out", H_sortie={h_out},
   IFLUX=3, ilimiteur=1, iupwind=1, tolisec=1.0E-03, friction=1, fricimplic=1,
   TS={ts}, CFL=0.9, is_dt_constant=0, constant_dt=0.0, tol_reg_perm=0.0,
@@ -35,13 +35,13 @@ DONNEES = """&DONNEES_NAMELIST
 """
 
 
-def _make_case(tmp_path, ts=20.0):
+def _make_case(tmp_path, ts=20.0, inlet="inflow"):
     r = M.river_mesh(nx=40, ny=10, length=400.0, width=200.0, channel_width=80.0, bar_len=100.0,
                      field_corr=50.0, seed=11)
     M.write_text_mesh(str(tmp_path / "reach.txt"), r.mesh)
     z = r.mesh.coords[:, 2]
     (tmp_path / "donnees.f").write_text(DONNEES.format(h_amont=float(z.max()) + 0.5, h_aval=float(z.max()) - 1.0,
-                                                       h_out=r.h_out, ts=ts))
+                                                       h_out=r.h_out, ts=ts, inlet=inlet))
     return r.mesh
 
 
@@ -100,6 +100,40 @@ def test_fortran_driver_matches_python(tmp_path):
     assert len(printed) == len(iters) > 0
     for it, q in zip(iters, printed):
         assert abs(q - ref["qout"][it - 1]) <= 1e-15 * max(abs(ref["qout"][it - 1]), 1e-300), (it, q)
+
+
+def test_drivers_phys_inflow_match_oracle(tmp_path):
+    """inlet='phys_inflow' (the inlet discharge shared over the inlet faces in
+    proportion to the depth, flux_riemann.cuf:127-137) through both hosts:
+    the Python driver's states, step count, tc and per-step discharges equal
+    the oracle's bit for bit, the Fortran driver's restart rows and printed
+    discharges equal the oracle's to the digits they print."""
+    dirs = []
+    for who in ("python", "fortran"):
+        d = tmp_path / who
+        d.mkdir()
+        mesh = _make_case(d, inlet="phys_inflow")
+        dirs.append(d)
+    p, prep, ref = _oracle(mesh, str(dirs[0]))
+    assert p.inlet == "phys_inflow" and ref["status"] == 0 and np.all(ref["qin"][1:] > 0)
+    out = run_case(str(dirs[0]), verbose=False)
+    assert out["status"] == 0 and out["steps"] == ref["steps"] and out["tc"] == ref["tc"]
+    assert np.array_equal(out["U"], ref["U"])
+    assert np.array_equal(out["q_in"], ref["qin"]) and np.array_equal(out["q_out"], ref["qout"])
+    if not os.path.exists(FORTRAN_EXE):
+        pytest.skip("Fortran driver not built")
+    proc = subprocess.run([FORTRAN_EXE], cwd=str(dirs[1]), capture_output=True, text=True, timeout=120)
+    assert proc.returncode == 0, proc.stdout + proc.stderr
+    nt = int(proc.stdout.split("Loop on time ends at iteration :")[1].split()[0])
+    assert nt == ref["steps"]
+    rows = open(dirs[1] / "solution_elements_restart.txt").read().splitlines()
+    assert rows[1:] == [cio._fmt(row) for row in ref["U"]]
+    lines = proc.stdout.splitlines()
+    q_in = [float(line.split("=")[1].split()[0]) for line in lines if "debit_entree_sum" in line]
+    iters = [int(line.split("Iteration number")[1].split(",")[0]) for line in lines if "Iteration number" in line]
+    assert len(q_in) == len(iters) > 0
+    for it, q in zip(iters, q_in):
+        assert abs(q - ref["qin"][it - 1]) <= 1e-15 * abs(ref["qin"][it - 1]), (it, q)
 
 
 def test_fortran_and_python_drivers_write_the_same_snapshots(tmp_path):

@@ -448,7 +448,7 @@ def test_tiny_meshes_at_tile_edges(nx, ny, iflux):
         _compare(U, ref["U"], np.asarray(hist), ref["dt_hist"], p.tolisec)
 
 
-@pytest.mark.parametrize("iflux", [1, 3])
+@pytest.mark.parametrize("iflux", [1, 2, 3, 4])
 @pytest.mark.parametrize("options", [0, 24])
 def test_phys_inflow_matches_oracle(iflux, options):
     """inlet = 'phys_inflow' (the inlet discharge shared in proportion to the
