@@ -188,8 +188,10 @@ __device__ __forceinline__ void acc3(double4 &a, const double *__restrict__ s) {
     a.z += s[2];
 }
 
+// tiles are 64, 128 or 256 cells: a cell's tile by a shift (tshift = log2 of
+// the tile size) rather than an integer division by a runtime value
 template <bool R2ON>
-__device__ __forceinline__ double4 gather_cell(int c, int tcells, const unsigned char *__restrict__ tiles_ran,
+__device__ __forceinline__ double4 gather_cell(int c, int tshift, const unsigned char *__restrict__ tiles_ran,
                                                const int4 *__restrict__ nbr, const double *__restrict__ R1,
                                                const double4 *__restrict__ R2, const int *__restrict__ r2_off,
                                                const int *__restrict__ r2_idx, const int *__restrict__ argmax,
@@ -197,7 +199,7 @@ __device__ __forceinline__ double4 gather_cell(int c, int tcells, const unsigned
                                                const double *__restrict__ geta_row, const int *__restrict__ gfirst,
                                                const int *__restrict__ gnext) {
     double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (tiles_ran[c / tcells]) {
+    if (tiles_ran[c >> tshift]) {
         const double *o = R1 + kR1 * (size_t)c + 9;
         acc = make_double4(o[0], o[1], o[2], 0.0);
         const int4 nb = nbr[c];
@@ -205,12 +207,12 @@ __device__ __forceinline__ double4 gather_cell(int c, int tcells, const unsigned
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const int i = nbv[k];
-            if (i >= 0 && tiles_ran[i / tcells]) acc3(acc, R1 + kR1 * (size_t)i + 3 * ((nb.w >> (2 * k)) & 3));
+            if (i >= 0 && tiles_ran[i >> tshift]) acc3(acc, R1 + kR1 * (size_t)i + 3 * ((nb.w >> (2 * k)) & 3));
         }
         if (R2ON)
             for (int e = r2_off[c]; e < r2_off[c + 1]; e++) {
                 const int src = r2_idx[e];
-                if (tiles_ran[(src / 6) / tcells]) acc4(acc, R2[src]);
+                if (tiles_ran[(src / 6) >> tshift]) acc4(acc, R2[src]);
             }
     }
     if (argmax && argmax[c] == sidx) acc.x += ghmax[c];
@@ -237,7 +239,7 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(RevWaves<
     const int b = r.tile_inst[tile];
     const int i = tile * (int)blockDim.x + threadIdx.x;
     if (i >= r.inst_end[b] || !r.ran[tile]) return;
-    const double4 l4 = r.ran_next ? gather_cell<IFLUX >= 3>(i, (int)blockDim.x, r.ran_next, r.nbr, r.R1in, r.R2in,
+    const double4 l4 = r.ran_next ? gather_cell<IFLUX >= 3>(i, __ffs((int)blockDim.x) - 1, r.ran_next, r.nbr, r.R1in, r.R2in,
                                                             r.r2_off, r.r2_idx, r.argmax, r.ghmax, r.sidx,
                                                             r.geta_row, r.gfirst, r.gnext)
                                   : r.lam[i];
@@ -281,7 +283,7 @@ __global__ void gather_kernel(const int4 *__restrict__ nbr, const double *__rest
     const int b = tile_inst[tile];
     const int c = tile * (int)blockDim.x + threadIdx.x;
     if (c >= inst_end[b]) return;
-    lam_out[c] = gather_cell<R2ON>(c, (int)blockDim.x, ran, nbr, R1, R2, r2_off, r2_idx, nullptr, nullptr, -1,
+    lam_out[c] = gather_cell<R2ON>(c, __ffs((int)blockDim.x) - 1, ran, nbr, R1, R2, r2_off, r2_idx, nullptr, nullptr, -1,
                                    nullptr, nullptr, nullptr);
 }
 
