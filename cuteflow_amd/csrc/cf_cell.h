@@ -204,6 +204,29 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
         const int cc = nb_at(nb, j);
         unb[j] = cc >= 0 ? ld.state(cc) : u0;
     }
+    // A dry cell with dry neighbours across its interior faces and no inlet
+    // face stays dry, whatever the scheme: every face is dry/dry (zero flux)
+    // or a boundary whose flux vanishes with the depth (wall, fixed-level
+    // outlet, which takes its wall branch at zero depth, transmissive), the
+    // bed source and friction are zero, and the clamp writes (+0, +0, +0):
+    // the same bits as the full update, without its face loop.  The
+    // residual is returned as zeros (a clamped cell's reverse does not
+    // read it).
+    {
+        bool still = u0.h == 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int cc = nb_at(nb, j);
+            still = still && (cc >= 0 ? unb[j].h == 0.0 : ((-cc) & 3) != 1);
+        }
+        if (still) {
+            out[0] = 0.0;
+            out[1] = 0.0;
+            out[2] = 0.0;
+            if (xout) { xout[0] = 0.0; xout[1] = 0.0; xout[2] = 0.0; }
+            return;
+        }
+    }
     D4 dfi = {0.0, 0.0, 0.0, 0.0};
     if (IFLUX >= 3) dfi = ld.dfc(i);
     T fl[3][3];
