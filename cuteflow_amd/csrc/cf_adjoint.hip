@@ -239,13 +239,20 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(RevWaves<
     const int b = r.tile_inst[tile];
     const int i = tile * (int)blockDim.x + threadIdx.x;
     if (i >= r.inst_end[b] || !r.ran[tile]) return;
-    const double4 l4 = r.ran_next ? gather_cell<IFLUX >= 3>(i, __ffs((int)blockDim.x) - 1, r.ran_next, r.nbr, r.R1in, r.R2in,
-                                                            r.r2_off, r.r2_idx, r.argmax, r.ghmax, r.sidx,
-                                                            r.geta_row, r.gfirst, r.gnext)
-                                  : r.lam[i];
-    const DevLd ld{r.st, r.nbr, r.dfc, r.geo, r.manning, r.ncell};
     const size_t hix = (size_t)b * r.hstride + r.step;
     const double x[3] = {r.xs[i], r.xs[(size_t)r.ncell + i], r.xs[2 * (size_t)r.ncell + i]};
+    // A cell whose new state the forward clamped to dry contributes nothing
+    // whatever its cotangent (cell_reverse returns zeros at the same test):
+    // its cotangent is not gathered.
+    const double r0 = (FRIC == 0 || FRIC == 1) ? x[0] : (1.0 * x[0] + 0.0 * x[1]) + 0.0 * x[2];
+    const bool clamped = r.st[i].x + r.dt[hix] * r0 <= kp[b].tolisec;
+    double4 l4 = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (!clamped)
+        l4 = r.ran_next ? gather_cell<IFLUX >= 3>(i, __ffs((int)blockDim.x) - 1, r.ran_next, r.nbr, r.R1in, r.R2in,
+                                                  r.r2_off, r.r2_idx, r.argmax, r.ghmax, r.sidx, r.geta_row,
+                                                  r.gfirst, r.gnext)
+                        : r.lam[i];
+    const DevLd ld{r.st, r.nbr, r.dfc, r.geo, r.manning, r.ncell};
     const double lam[3] = {l4.x, l4.y, l4.z};
     double own[3], r1[3][3], r2[3][2][3], gn;
     cell_reverse<IFLUX, FRIC>(ld, kp[b], r.hydro, i, r.dt[hix], r.tc[hix], x, lam, own, r1, r2, gn);
