@@ -21,6 +21,7 @@ CF_OPT_NO_TILE_SKIP = 2
 CF_OPT_TILE_128 = 4
 CF_OPT_TILE_64 = 8
 CF_OPT_XCD_CONTIG = 16
+CF_OPT_FIXED_TILE_ORDER = 32
 CF_TUNE_NCAND = 12
 
 CF_ADJ_FREEZE_LIMITERS = 1

@@ -28,6 +28,12 @@ constexpr int kSlotStride32 = 32;
 // bit pattern of +inf: no tile has folded its minimum in yet
 constexpr unsigned long long kDtSlotEmpty = 0x7ff0000000000000ull;
 
+// Tile order of the step kernel: every kReorderEvery steps of a run (and at
+// its start) the tiles are ranked by what they did at the last step that
+// ran them: held water, ran with only dry cells, skipped.
+constexpr int kReorderEvery = 32;
+enum { kTileWet = 0, kTileDry = 1, kTileSkipped = 2 };
+
 // Per-instance control block, resident on the device across runs.
 struct InstCtrl {
     double dt;        // dt of the current state (used by the next step)
@@ -90,6 +96,13 @@ struct StepArgs {
     int step;         // index of this step in the run (absolute in a replay)
     int skip;         // dry-tile skipping enabled
     int xcd;          // workgroup -> tile order: 0 identity, 1 XCD-contiguous
+    // Workgroup -> (tile, instance) of the step kernel.  With the wet-first
+    // order the step kernel also records each tile's class (kTileWet,
+    // kTileDry, kTileSkipped) in tile_class, and reorder_kernel rebuilds the
+    // map from it: wet tiles take the first workgroups, then the dry tiles
+    // that run, then the skipped ones.  Any permutation gives the same bits.
+    const int2 *__restrict__ tile_map;
+    unsigned char *tile_class;  // null: fixed order, classes not recorded
 };
 
 // Tile processed by this workgroup.  Workgroups are dispatched round-robin
@@ -231,6 +244,11 @@ struct cf_handle {
     };
     std::vector<Graph> graphs;
     int use_graphs = 1;
+    // workgroup -> (tile, instance) of the step kernel; tile classes of the
+    // wet-first order
+    int2 *tile_map = nullptr;
+    unsigned char *tile_class = nullptr;
+    int wet_first = 1;                 // the wet-first order is on
     cf::AdjState *adj = nullptr;
     unsigned params_ver = 0;           // bumped by every inflow or hydrograph change
     int ad_flags = 0;                  // CF_ADJ_* bits of the derivative runs

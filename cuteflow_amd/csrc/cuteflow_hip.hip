@@ -80,8 +80,8 @@ __device__ __forceinline__ double wave_min(double v) {
 template <int IFLUX, int FRIC>
 __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs a, const KParams *__restrict__ kp) {
     __shared__ double s_min[kWaves];
-    const int tile = tile_of_block(a.xcd);
-    const int b = a.tile_inst[tile];
+    const int2 tb = a.tile_map[blockIdx.x];  // (tile, instance)
+    const int tile = tb.x, b = tb.y;
     const InstCtrl c = a.ctrl[b];
     const bool replay = a.dt_fix != nullptr;
     unsigned char *ran_row = a.tile_ran ? a.tile_ran + (size_t)a.step * gridDim.x : nullptr;
@@ -96,7 +96,10 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
         // hold that: it is skipped.  Reading a stamp raised concurrently only
         // makes the tile run, which is always correct.
         if (a.skip && !a.always[tile] && a.stamp[tile] + 1u < c.gstep) {
-            if (threadIdx.x == 0 && ran_row) ran_row[tile] = 0;  // no dt contribution
+            if (threadIdx.x == 0) {
+                if (ran_row) ran_row[tile] = 0;  // no dt contribution
+                if (a.tile_class) a.tile_class[tile] = kTileSkipped;
+            }
             return;
         }
     } else if (ran_row && !ran_row[tile]) {
@@ -185,11 +188,96 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
         if (ran_row) ran_row[tile] = 1;
     }
     // a wet tile keeps itself and its neighbour tiles running for two steps
-    if (a.skip && __syncthreads_or(wet_local)) {
+    // and is ranked among the first tiles of the next steps
+    const int wet = (a.skip || a.tile_class) ? __syncthreads_or(wet_local) : 0;
+    if (a.skip && wet) {
         const int o = a.tile_adj_off[tile], n = a.tile_adj_off[tile + 1] - o;
         for (int k = threadIdx.x; k <= n; k += (int)blockDim.x)
             a.stamp[k == 0 ? tile : a.tile_adj[o + k - 1]] = c.gstep + 1u;
     }
+    if (a.tile_class && threadIdx.x == 0) a.tile_class[tile] = wet ? kTileWet : kTileDry;
+}
+
+// The step kernel's workgroup -> (tile, instance) map of the wet-first
+// order, one workgroup of 1024 threads.  A stable partition of the tiles by
+// class (wet, dry that ran, skipped: tile_class, written by the step
+// kernel) gives each class a run of workgroups; with xcd set the tiles of a
+// class are cut into one contiguous chunk per XCD (workgroup b runs on XCD
+// b mod 8, which keeps a chunk's neighbour reads in that XCD's L2), sized
+// by the number of the class's workgroups on each XCD.  Every tile gets
+// exactly one workgroup.
+__global__ __launch_bounds__(1024) void reorder_kernel(const unsigned char *__restrict__ tile_class,
+                                                       const int *__restrict__ tile_inst, int ntiles, int xcd,
+                                                       int2 *__restrict__ map) {
+    __shared__ unsigned long long s_w[1024 / 64];
+    const int tid = (int)threadIdx.x, nth = (int)blockDim.x, lane = tid & 63, w = tid >> 6, nw = nth >> 6;
+    const int per = (ntiles + nth - 1) / nth, t0 = min(ntiles, tid * per), t1 = min(ntiles, t0 + per);
+    // the three class counts of this thread's tiles, packed 21 bits each
+    // (ntiles < 2^21), scanned over the workgroup with wave shuffles
+    unsigned long long mine = 0;
+    for (int t = t0; t < t1; t++) mine += 1ull << (21 * min((int)tile_class[t], 2));
+    unsigned long long v = mine;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned long long o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
+    }
+    if (lane == 63) s_w[w] = v;
+    __syncthreads();
+    if (w == 0) {
+        unsigned long long x = lane < nw ? s_w[lane] : 0ull;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned long long o = __shfl_up(x, off, 64);
+            if (lane >= off) x += o;
+        }
+        if (lane < nw) s_w[lane] = x;
+    }
+    __syncthreads();
+    const unsigned long long excl = v - mine + (w > 0 ? s_w[w - 1] : 0ull), tot = s_w[nw - 1];
+    constexpr unsigned long long kM = (1ull << 21) - 1;
+    int pos[3], nc[3], base[3];
+    for (int c = 0; c < 3; c++) {
+        nc[c] = (int)((tot >> (21 * c)) & kM);
+        base[c] = (int)((excl >> (21 * c)) & kM);
+    }
+    pos[0] = 0;
+    pos[1] = nc[0];
+    pos[2] = nc[0] + nc[1];
+    for (int t = t0; t < t1; t++) {
+        const int c = min((int)tile_class[t], 2);
+        const int j = base[c]++;  // rank of tile t within its class
+        const int p = pos[c], n = nc[c];
+        int blk = p + j;
+        if (xcd) {
+            // the class's workgroups p .. p+n-1 on XCD x: n/8, one more for
+            // the first n%8 XCDs in cyclic order from p's; chunk x of the
+            // class's tiles goes to them in increasing order
+            int off = 0;
+            for (int x = 0; x < 8; x++) {
+                const int cx = n / 8 + ((((x - p) & 7) < (n & 7)) ? 1 : 0);
+                if (j < off + cx) {
+                    blk = p + ((x - p) & 7) + 8 * (j - off);
+                    break;
+                }
+                off += cx;
+            }
+        }
+        map[blk] = make_int2(t, tile_inst[t]);
+    }
+}
+
+// The fixed workgroup -> (tile, instance) map: the identity, or with xcd set
+// one contiguous run of tiles per XCD (tile_of_block's order).
+__global__ void static_map_kernel(const int *__restrict__ tile_inst, int ntiles, int xcd, int2 *__restrict__ map) {
+    const int bid = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (bid >= ntiles) return;
+    int t = bid;
+    if (xcd) {
+        const int x = bid & 7, q = ntiles >> 3, r = ntiles & 7;
+        t = x * q + (x < r ? x : r) + (bid >> 3);
+    }
+    map[bid] = make_int2(t, tile_inst[t]);
 }
 
 // Reduces per-tile minima (NaN marks a NaN state, +inf a skipped tile)
@@ -649,6 +737,7 @@ static void destroy_buffers(cf_handle *h) {
     dfree(h->d_kp); dfree(h->ctrl); dfree(h->partial); dfree(h->stamp);
     dfree(h->tile_adj_off); dfree(h->tile_adj); dfree(h->always);
     dfree(h->tile_inst); dfree(h->inst_end); dfree(h->inst_tile);
+    dfree(h->tile_map); dfree(h->tile_class);
     free_record(h);
     dfree(h->dtslot); dfree(h->cntslot); dfree(h->nanflag);
     dfree(h->io_cell); dfree(h->io_off); dfree(h->q_io);
@@ -672,7 +761,7 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
             if (m->conn[i] < 1 || m->conn[i] > m->nnt) return CF_ERR_MESH;
     }
     if ((params[0].options & ~(CF_OPT_KEEP_ORDER | CF_OPT_NO_TILE_SKIP | CF_OPT_TILE_128 | CF_OPT_TILE_64 |
-                               CF_OPT_XCD_CONTIG)) ||
+                               CF_OPT_XCD_CONTIG | CF_OPT_FIXED_TILE_ORDER)) ||
         ((params[0].options & CF_OPT_TILE_128) && (params[0].options & CF_OPT_TILE_64)))
         return CF_ERR_ARG;
     const int tile = (params[0].options & CF_OPT_TILE_64) ? 64 : (params[0].options & CF_OPT_TILE_128) ? 128 : kTile;
@@ -719,6 +808,7 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
     for (int b = 0; b < ninst; b++)
         if (params[b].is_dt_constant == 1) h->skip = 0;
     h->xcd = (params[0].options & CF_OPT_XCD_CONTIG) ? 1 : 0;
+    h->wet_first = (params[0].options & CF_OPT_FIXED_TILE_ORDER) ? 0 : 1;
     for (int b = 0; b < ninst; b++)
         if (params[b].inlet == 3) h->phys_in = 1;
     int cells = 0, tiles = 0;
@@ -737,6 +827,7 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
     h->tile_off[ninst] = tiles;
     h->ncell = cells;
     h->ntiles = tiles;
+    if (tiles >= (1 << 21)) h->wet_first = 0;  // reorder_kernel packs tile counts in 21 bits
     const size_t N = (size_t)cells;
 
     // global (batched) host tables
@@ -787,6 +878,7 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
             !dalloc(h->stamp, (size_t)tiles) || !dalloc(h->tile_adj_off, adj_off.size()) ||
             !dalloc(h->tile_adj, adj.size()) || !dalloc(h->always, (size_t)tiles) ||
             !dalloc(h->tile_inst, (size_t)tiles) || !dalloc(h->inst_end, (size_t)ninst) ||
+            !dalloc(h->tile_map, (size_t)tiles) || !dalloc(h->tile_class, (size_t)tiles) ||
             !dalloc(h->inst_tile, (size_t)ninst + 1) || !dalloc(h->io_cell, io_cell.size()) || !dalloc(h->io_off, (size_t)ninst + 1) ||
             !dalloc(h->q_io, 2 * std::max<size_t>((size_t)h->nio, 1)) ||
             !dalloc(h->surf_in, (size_t)ninst * CF_MAX_IO) ||
@@ -823,6 +915,14 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
             const size_t n64 = (size_t)ninst * kDtSlots * kSlotStride64, n32 = (size_t)ninst * kDtSlots * kSlotStride32;
             hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_of((int)n64, 256)), dim3(256), 0, h->stream, h->dtslot, n64,
                                kDtSlotEmpty);
+            // every tile starts in the first class: the first ranking keeps
+            // the fixed order
+            hipLaunchKernelGGL(static_map_kernel, dim3(grid_of(tiles, 256)), dim3(256), 0, h->stream, h->tile_inst,
+                               tiles, h->xcd, h->tile_map);
+            if (hipMemsetAsync(h->tile_class, 0, (size_t)tiles, h->stream) != hipSuccess) {
+                rc = CF_ERR_HIP;
+                break;
+            }
             hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_of((int)n32, 256)), dim3(256), 0, h->stream, h->cntslot,
                                (int)n32, 0u);
             hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_of(ninst * kSlotStride32, 256)), dim3(256), 0, h->stream,
@@ -1017,6 +1117,8 @@ StepArgs cf::base_args(cf_handle *h) {
     a.hist_stride = h->cap + 1;
     a.skip = h->skip;
     a.xcd = h->xcd;
+    a.tile_map = h->tile_map;
+    a.tile_class = h->wet_first ? h->tile_class : nullptr;
     return a;
 }
 
@@ -1027,6 +1129,9 @@ static void enqueue_run(cf_handle *h, int nsteps, hipStream_t s) {
     step_launcher L = pick_launcher(h->iflux, h->fric);
     for (int k = 0; k < nsteps; k++) {
         a.step = k;
+        if (h->wet_first && k % kReorderEvery == 0)
+            hipLaunchKernelGGL(reorder_kernel, dim3(1), dim3(1024), 0, s, h->tile_class, h->tile_inst, h->ntiles, h->xcd,
+                               h->tile_map);
         if (h->phys_in)
             hipLaunchKernelGGL(surf_in_kernel, dim3(h->ninst), dim3(32), 0, s, a, h->io_cell, h->io_off, h->surf_in);
         L(a, h->ntiles, s);
@@ -1189,6 +1294,9 @@ int cf::set_schedule(cf_handle *h, int skip, int xcd) {
             hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_of(nt, 256)), dim3(256), 0, h->stream, h->stamp + t0, nt,
                                h->hctrl[b].gstep);
         }
+    if (xcd != h->xcd)
+        hipLaunchKernelGGL(static_map_kernel, dim3(grid_of(h->ntiles, 256)), dim3(256), 0, h->stream, h->tile_inst,
+                           h->ntiles, xcd, h->tile_map);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
     drop_graphs(h);

@@ -25,14 +25,17 @@ extern "C" {
 
 /* cf_params.options bits.  By default the library stores cells in its own
  * order (a Hilbert curve of the centroids, so that each 256-cell tile is a
- * compact patch) and skips tiles that are dry with dry neighbours; both are
- * exact: per-cell results are identical bits in any order, with or without
- * skipping. */
+ * compact patch), skips tiles that are dry with dry neighbours and hands
+ * the tiles holding water to the first workgroups of each step; all of this
+ * is exact: per-cell results are identical bits in any order, with or
+ * without skipping. */
 #define CF_OPT_KEEP_ORDER 1      /* keep the caller's cell order on the device */
 #define CF_OPT_NO_TILE_SKIP 2    /* run every tile every step */
 #define CF_OPT_TILE_128 4        /* 128-cell tiles (workgroups) instead of 256 */
 #define CF_OPT_TILE_64 8         /* 64-cell tiles (one wavefront) */
-#define CF_OPT_XCD_CONTIG 16     /* each XCD (L2) runs one contiguous run of tiles */
+#define CF_OPT_XCD_CONTIG 16     /* each XCD (L2) runs contiguous runs of tiles */
+#define CF_OPT_FIXED_TILE_ORDER 32 /* workgroups take the tiles in a fixed order instead of tiles
+                                      holding water first (re-ranked every 32 steps) */
 
 enum {
     CF_OK = 0,
