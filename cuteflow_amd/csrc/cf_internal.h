@@ -248,6 +248,7 @@ struct cf_handle {
     // wet-first order
     int2 *tile_map = nullptr;
     unsigned char *tile_class = nullptr;
+    unsigned long long *rank_counts = nullptr;  // per 256 tiles: packed class counts of the ranking
     int wet_first = 1;                 // the wet-first order is on
     cf::AdjState *adj = nullptr;
     unsigned params_ver = 0;           // bumped by every inflow or hydrograph change

@@ -199,24 +199,25 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
 }
 
 // The step kernel's workgroup -> (tile, instance) map of the wet-first
-// order, one workgroup of 1024 threads.  A stable partition of the tiles by
-// class (wet, dry that ran, skipped: tile_class, written by the step
-// kernel) gives each class a run of workgroups; with xcd set the tiles of a
-// class are cut into one contiguous chunk per XCD (workgroup b runs on XCD
-// b mod 8, which keeps a chunk's neighbour reads in that XCD's L2), sized
-// by the number of the class's workgroups on each XCD.  Every tile gets
-// exactly one workgroup.
-__global__ __launch_bounds__(1024) void reorder_kernel(const unsigned char *__restrict__ tile_class,
-                                                       const int *__restrict__ tile_inst, int ntiles, int xcd,
-                                                       int2 *__restrict__ map) {
-    __shared__ unsigned long long s_w[1024 / 64];
-    const int tid = (int)threadIdx.x, nth = (int)blockDim.x, lane = tid & 63, w = tid >> 6, nw = nth >> 6;
-    const int per = (ntiles + nth - 1) / nth, t0 = min(ntiles, tid * per), t1 = min(ntiles, t0 + per);
-    // the three class counts of this thread's tiles, packed 21 bits each
-    // (ntiles < 2^21), scanned over the workgroup with wave shuffles
-    unsigned long long mine = 0;
-    for (int t = t0; t < t1; t++) mine += 1ull << (21 * min((int)tile_class[t], 2));
-    unsigned long long v = mine;
+// order, in two launches of 256-tile blocks.  A stable partition of the
+// tiles by class (wet, dry that ran, skipped: tile_class, written by the
+// step kernel) gives each class a run of workgroups; with xcd set the tiles
+// of a class are cut into one contiguous chunk per XCD (workgroup b runs on
+// XCD b mod 8, which keeps a chunk's neighbour reads in that XCD's L2),
+// sized by the number of the class's workgroups on each XCD.  Every tile
+// gets exactly one workgroup.  Class counts travel packed, 21 bits each
+// (ntiles < 2^21).
+constexpr int kRankBlock = 256;
+
+__device__ __forceinline__ unsigned long long class_bit(const unsigned char *__restrict__ tile_class, int t,
+                                                       int ntiles) {
+    return t < ntiles ? 1ull << (21 * min((int)tile_class[t], 2)) : 0ull;
+}
+
+// inclusive scan of packed counts over the workgroup (kRankBlock threads)
+__device__ __forceinline__ unsigned long long block_scan_packed(unsigned long long v, unsigned long long &total) {
+    __shared__ unsigned long long s_w[kRankBlock / 64];
+    const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
 #pragma unroll
     for (int off = 1; off < 64; off <<= 1) {
         const unsigned long long o = __shfl_up(v, off, 64);
@@ -224,47 +225,80 @@ __global__ __launch_bounds__(1024) void reorder_kernel(const unsigned char *__re
     }
     if (lane == 63) s_w[w] = v;
     __syncthreads();
-    if (w == 0) {
-        unsigned long long x = lane < nw ? s_w[lane] : 0ull;
+    unsigned long long before = 0, all = 0;
 #pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const unsigned long long o = __shfl_up(x, off, 64);
-            if (lane >= off) x += o;
-        }
-        if (lane < nw) s_w[lane] = x;
+    for (int k = 0; k < kRankBlock / 64; k++) {
+        before += k < w ? s_w[k] : 0ull;
+        all += s_w[k];
+    }
+    total = all;
+    return v + before;
+}
+
+// counts[k]: the packed class counts of tiles 256k .. 256k + 255
+__global__ __launch_bounds__(kRankBlock) void rank_count_kernel(const unsigned char *__restrict__ tile_class,
+                                                                int ntiles, unsigned long long *__restrict__ counts) {
+    unsigned long long total;
+    (void)block_scan_packed(class_bit(tile_class, (int)(blockIdx.x * kRankBlock + threadIdx.x), ntiles), total);
+    if (threadIdx.x == 0) counts[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kRankBlock) void rank_map_kernel(const unsigned char *__restrict__ tile_class,
+                                                              const int *__restrict__ tile_inst, int ntiles, int xcd,
+                                                              const unsigned long long *__restrict__ counts,
+                                                              int2 *__restrict__ map) {
+    __shared__ unsigned long long s_sum[2][kRankBlock / 64];
+    const int nb = (int)gridDim.x, k0 = (int)blockIdx.x, lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+    // the counts of the blocks before this one, and of all blocks
+    unsigned long long before = 0, all = 0;
+    for (int k = (int)threadIdx.x; k < nb; k += kRankBlock) {
+        const unsigned long long c = counts[k];
+        all += c;
+        if (k < k0) before += c;
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        before += __shfl_xor(before, off, 64);
+        all += __shfl_xor(all, off, 64);
+    }
+    if (lane == 0) {
+        s_sum[0][w] = before;
+        s_sum[1][w] = all;
     }
     __syncthreads();
-    const unsigned long long excl = v - mine + (w > 0 ? s_w[w - 1] : 0ull), tot = s_w[nw - 1];
+    before = 0;
+    all = 0;
+#pragma unroll
+    for (int q = 0; q < kRankBlock / 64; q++) {
+        before += s_sum[0][q];
+        all += s_sum[1][q];
+    }
+    const int t = k0 * kRankBlock + (int)threadIdx.x;
+    const unsigned long long mine = class_bit(tile_class, t, ntiles);
+    unsigned long long total;
+    const unsigned long long incl = block_scan_packed(mine, total);
+    if (t >= ntiles) return;
     constexpr unsigned long long kM = (1ull << 21) - 1;
-    int pos[3], nc[3], base[3];
-    for (int c = 0; c < 3; c++) {
-        nc[c] = (int)((tot >> (21 * c)) & kM);
-        base[c] = (int)((excl >> (21 * c)) & kM);
-    }
-    pos[0] = 0;
-    pos[1] = nc[0];
-    pos[2] = nc[0] + nc[1];
-    for (int t = t0; t < t1; t++) {
-        const int c = min((int)tile_class[t], 2);
-        const int j = base[c]++;  // rank of tile t within its class
-        const int p = pos[c], n = nc[c];
-        int blk = p + j;
-        if (xcd) {
-            // the class's workgroups p .. p+n-1 on XCD x: n/8, one more for
-            // the first n%8 XCDs in cyclic order from p's; chunk x of the
-            // class's tiles goes to them in increasing order
-            int off = 0;
-            for (int x = 0; x < 8; x++) {
-                const int cx = n / 8 + ((((x - p) & 7) < (n & 7)) ? 1 : 0);
-                if (j < off + cx) {
-                    blk = p + ((x - p) & 7) + 8 * (j - off);
-                    break;
-                }
-                off += cx;
+    const int c = min((int)tile_class[t], 2);
+    const int j = (int)(((before + incl - mine) >> (21 * c)) & kM);  // rank of tile t within its class
+    const int n0 = (int)(all & kM), n1 = (int)((all >> 21) & kM), n2 = (int)((all >> 42) & kM);
+    const int p = c == 0 ? 0 : (c == 1 ? n0 : n0 + n1), n = c == 0 ? n0 : (c == 1 ? n1 : n2);
+    int blk = p + j;
+    if (xcd) {
+        // the class's workgroups p .. p+n-1 on XCD x: n/8, one more for the
+        // first n%8 XCDs in cyclic order from p's; chunk x of the class's
+        // tiles goes to them in increasing order
+        int off = 0;
+        for (int x = 0; x < 8; x++) {
+            const int cx = n / 8 + ((((x - p) & 7) < (n & 7)) ? 1 : 0);
+            if (j < off + cx) {
+                blk = p + ((x - p) & 7) + 8 * (j - off);
+                break;
             }
+            off += cx;
         }
-        map[blk] = make_int2(t, tile_inst[t]);
     }
+    map[blk] = make_int2(t, tile_inst[t]);
 }
 
 // The fixed workgroup -> (tile, instance) map: the identity, or with xcd set
@@ -737,7 +771,7 @@ static void destroy_buffers(cf_handle *h) {
     dfree(h->d_kp); dfree(h->ctrl); dfree(h->partial); dfree(h->stamp);
     dfree(h->tile_adj_off); dfree(h->tile_adj); dfree(h->always);
     dfree(h->tile_inst); dfree(h->inst_end); dfree(h->inst_tile);
-    dfree(h->tile_map); dfree(h->tile_class);
+    dfree(h->tile_map); dfree(h->tile_class); dfree(h->rank_counts);
     free_record(h);
     dfree(h->dtslot); dfree(h->cntslot); dfree(h->nanflag);
     dfree(h->io_cell); dfree(h->io_off); dfree(h->q_io);
@@ -879,6 +913,7 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
             !dalloc(h->tile_adj, adj.size()) || !dalloc(h->always, (size_t)tiles) ||
             !dalloc(h->tile_inst, (size_t)tiles) || !dalloc(h->inst_end, (size_t)ninst) ||
             !dalloc(h->tile_map, (size_t)tiles) || !dalloc(h->tile_class, (size_t)tiles) ||
+            !dalloc(h->rank_counts, (size_t)grid_of(tiles, kRankBlock)) ||
             !dalloc(h->inst_tile, (size_t)ninst + 1) || !dalloc(h->io_cell, io_cell.size()) || !dalloc(h->io_off, (size_t)ninst + 1) ||
             !dalloc(h->q_io, 2 * std::max<size_t>((size_t)h->nio, 1)) ||
             !dalloc(h->surf_in, (size_t)ninst * CF_MAX_IO) ||
@@ -1129,9 +1164,13 @@ static void enqueue_run(cf_handle *h, int nsteps, hipStream_t s) {
     step_launcher L = pick_launcher(h->iflux, h->fric);
     for (int k = 0; k < nsteps; k++) {
         a.step = k;
-        if (h->wet_first && k % kReorderEvery == 0)
-            hipLaunchKernelGGL(reorder_kernel, dim3(1), dim3(1024), 0, s, h->tile_class, h->tile_inst, h->ntiles, h->xcd,
-                               h->tile_map);
+        if (h->wet_first && k % kReorderEvery == 0) {
+            const int nb = grid_of(h->ntiles, kRankBlock);
+            hipLaunchKernelGGL(rank_count_kernel, dim3(nb), dim3(kRankBlock), 0, s, h->tile_class, h->ntiles,
+                               h->rank_counts);
+            hipLaunchKernelGGL(rank_map_kernel, dim3(nb), dim3(kRankBlock), 0, s, h->tile_class, h->tile_inst,
+                               h->ntiles, h->xcd, h->rank_counts, h->tile_map);
+        }
         if (h->phys_in)
             hipLaunchKernelGGL(surf_in_kernel, dim3(h->ninst), dim3(32), 0, s, a, h->io_cell, h->io_off, h->surf_in);
         L(a, h->ntiles, s);
