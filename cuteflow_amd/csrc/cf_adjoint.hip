@@ -784,6 +784,7 @@ static void enqueue_forward(cf_handle *h, AdjState *A, hipStream_t s) {
     a.hist_stride = S;
     for (int k = 0; k < A->nsteps; k++) {
         a.step = k;
+        if (k % kReorderEvery == 0) launch_rank(h, s);
         launch_step(a, h->iflux, h->fric, h->ntiles, s);
         launch_finalize(h, A->dt, A->tc, A->ran, S, k, s);
         if ((k + 1) % A->every == 0) {

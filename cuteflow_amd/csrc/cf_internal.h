@@ -294,6 +294,8 @@ void launch_finalize(cf_handle *h, double *dt_hist, double *tc_hist, int *ran, i
 void launch_run_init(cf_handle *h, double *dt_hist, double *tc_hist, int stride, hipStream_t s,
                      InstCtrl *mirror = nullptr);
 int set_schedule(cf_handle *h, int skip, int xcd);
+// the wet-first ranking of the tiles (no-op with the fixed order)
+void launch_rank(cf_handle *h, hipStream_t s);
 // defined in cf_adjoint.hip
 void adj_free(cf_handle *h);
 void adj_drop_graphs(cf_handle *h);

@@ -1110,6 +1110,14 @@ void cf::launch_finalize(cf_handle *h, double *dt_hist, double *tc_hist, int *ra
                        qin_hist, qout_hist, mirror);
 }
 
+void cf::launch_rank(cf_handle *h, hipStream_t s) {
+    if (!h->wet_first) return;
+    const int nb = grid_of(h->ntiles, kRankBlock);
+    hipLaunchKernelGGL(rank_count_kernel, dim3(nb), dim3(kRankBlock), 0, s, h->tile_class, h->ntiles, h->rank_counts);
+    hipLaunchKernelGGL(rank_map_kernel, dim3(nb), dim3(kRankBlock), 0, s, h->tile_class, h->tile_inst, h->ntiles,
+                       h->xcd, h->rank_counts, h->tile_map);
+}
+
 void cf::launch_run_init(cf_handle *h, double *dt_hist, double *tc_hist, int stride, hipStream_t s,
                          InstCtrl *mirror) {
     hipLaunchKernelGGL(run_init_kernel, dim3(grid_of(h->ninst, 256)), dim3(256), 0, s, h->ctrl, h->d_kp, dt_hist,
@@ -1164,13 +1172,7 @@ static void enqueue_run(cf_handle *h, int nsteps, hipStream_t s) {
     step_launcher L = pick_launcher(h->iflux, h->fric);
     for (int k = 0; k < nsteps; k++) {
         a.step = k;
-        if (h->wet_first && k % kReorderEvery == 0) {
-            const int nb = grid_of(h->ntiles, kRankBlock);
-            hipLaunchKernelGGL(rank_count_kernel, dim3(nb), dim3(kRankBlock), 0, s, h->tile_class, h->ntiles,
-                               h->rank_counts);
-            hipLaunchKernelGGL(rank_map_kernel, dim3(nb), dim3(kRankBlock), 0, s, h->tile_class, h->tile_inst,
-                               h->ntiles, h->xcd, h->rank_counts, h->tile_map);
-        }
+        if (k % kReorderEvery == 0) launch_rank(h, s);
         if (h->phys_in)
             hipLaunchKernelGGL(surf_in_kernel, dim3(h->ninst), dim3(32), 0, s, a, h->io_cell, h->io_off, h->surf_in);
         L(a, h->ntiles, s);
