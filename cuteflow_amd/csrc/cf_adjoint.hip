@@ -150,7 +150,10 @@ struct RevArgs {
     int hstride;
     int step;
     int ncell;
-    int xcd;  // workgroup -> tile order (tile_of_block)
+    int xcd;  // workgroup -> tile order (tile_of_block) when tile_map is null
+    // workgroup -> (tile, instance): the forward's wet-first map, so that the
+    // tiles that ran (wet ones first) take the first workgroups
+    const int2 *tile_map;
 };
 
 // Occupancy target of the reverse kernel (waves per SIMD): the first-order
@@ -235,8 +238,15 @@ __device__ __forceinline__ double4 gather_cell(int c, int tshift, const unsigned
 template <int IFLUX, int FRIC>
 __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(RevWaves<IFLUX>::value, 8))) void revg_kernel(
     RevArgs r, const KParams *__restrict__ kp) {
-    const int tile = tile_of_block(r.xcd);
-    const int b = r.tile_inst[tile];
+    int tile, b;
+    if (r.tile_map) {
+        const int2 tb = r.tile_map[blockIdx.x];
+        tile = tb.x;
+        b = tb.y;
+    } else {
+        tile = tile_of_block(r.xcd);
+        b = r.tile_inst[tile];
+    }
     const int i = tile * (int)blockDim.x + threadIdx.x;
     if (i >= r.inst_end[b] || !r.ran[tile]) return;
     const size_t hix = (size_t)b * r.hstride + r.step;
@@ -398,6 +408,7 @@ struct TlmArgs {
     const int *tile_inst, *inst_end;
     const double *dt, *tc;
     int hstride, step, ncell, tile_cells, xcd;
+    const int2 *tile_map;  // workgroup -> (tile, instance), as in the forward (null: tile_of_block)
     // Diagnostic masks (0 none): 1 / 3 record, 2 / 4 count the differences
     // from the recorded run.  Every mode > 0 keeps the wet bit of each (step,
     // cell); modes 3 and 4 also a signature of every branch the cell's update
@@ -412,8 +423,15 @@ struct TlmArgs {
 template <int IFLUX, int FRIC, bool REC>
 __global__ __launch_bounds__(kTile) void tlm_kernel(TlmArgs a) {
     using D = typename std::conditional<REC, DualR, Dual>::type;
-    const int tile = tile_of_block(a.xcd);
-    const int b = a.tile_inst[tile];
+    int tile, b;
+    if (a.tile_map) {
+        const int2 tb = a.tile_map[blockIdx.x];
+        tile = tb.x;
+        b = tb.y;
+    } else {
+        tile = tile_of_block(a.xcd);
+        b = a.tile_inst[tile];
+    }
     const int i = tile * (int)blockDim.x + threadIdx.x;
     const bool valid = i < a.inst_end[b];
     double h_new = 0.0;
@@ -1020,6 +1038,7 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
     r.ncell = h->ncell;
     r.tile_cells = h->tile;
     r.xcd = h->xcd;
+    r.tile_map = h->tile_map;
     for (int sg = A->nseg - 1; sg >= 0; sg--) {
         const int k0 = sg * A->K, k1 = std::min(k0 + A->K, A->nsteps);
         const double4 *ck = A->ckpt + (size_t)sg * N;
@@ -1152,6 +1171,7 @@ static void enqueue_tlm(cf_handle *h, AdjState *A, hipStream_t s) {
     t.ncell = h->ncell;
     t.tile_cells = h->tile;
     t.xcd = h->xcd;
+    t.tile_map = h->tile_map;
     t.mask_mode = mode;
     t.wmask = A->wmask;
     t.flips = A->flips;
