@@ -612,9 +612,13 @@ def bench_field(args):
         a_obs = [CAL.soft_extent(hobs[b], surf[b])[0] for b in range(B)]
         n0 = [1.2 * e.mesh.manning for e in mem]
         scale = [max(1.0, 0.01 * a) for a in a_obs]
+        # the initial states and the fields being calibrated are resident in
+        # HBM, like the forward's initial states
+        U0_d = [torch.as_tensor(u, device="cuda") for u in U0]
+        n0_d = [torch.as_tensor(n, device="cuda") for n in n0]
 
         def gradient():
-            etas, hm = run.forward(nsteps, U0=U0, manning=n0, device=True)
+            etas, hm = run.forward(nsteps, U0=U0_d, manning=n0_d, device=True)
             ge, gh, loss = [], [], 0.0
             for b in range(B):
                 ge.append(2.0 * (etas[b] - obs[b]))

@@ -76,6 +76,9 @@ struct AdjState {
     int *r2_off = nullptr, *r2_idx = nullptr;
     double *gn = nullptr;
     unsigned char *tile_ran = nullptr;  // nsteps x ntiles: tile ran in the recorded run
+    // seg and xs hold the last segment's steps as the recorded forward wrote
+    // them (the reverse sweep then does not replay that segment)
+    bool tape_last = false;
     int lam_cur = 0;
     // tangent-linear run
     double4 *tst[2] = {nullptr, nullptr}, *tdst[2] = {nullptr, nullptr};
@@ -102,7 +105,9 @@ struct AdjState {
     bool r2_built = false;
     bool gauges_dirty = true;
     int alloc_steps = 0, alloc_K = 0, alloc_nsamp = 0, alloc_gtotal = 0;
-    // captured launch sequences (recorded forward, reverse sweep, TLM run)
+    // captured launch sequences, by kind: 0 recorded forward, 1 reverse
+    // sweep, 2 TLM run, 3 reverse sweep that takes the last segment's tape
+    // from the recorded forward
     struct Graph {
         int kind, nsteps, every, K, hmax;
         hipGraphExec_t exec;
@@ -348,12 +353,30 @@ __global__ void sample_gauges_kernel(StPair st, const InstCtrl *ctrl, const int 
     eta[(size_t)sidx * gtotal + g] = u.x + u.w;
 }
 
+// Maximum depth over the samples (sample sidx, after step k).  `ran`
+// holds the activity rows of the `every` steps since the previous sample:
+// a tile that ran at none of them holds that sample's state bit for bit (a
+// skipped tile writes nothing and both state buffers hold its dry state), so
+// its maxima stand and the tile is not read.  Several tiles per workgroup.
 __global__ void sample_hmax_kernel(StPair st, const InstCtrl *ctrl, const int *__restrict__ tile_inst,
-                                   const int *__restrict__ inst_end, int sidx, double *__restrict__ hmax,
+                                   const int *__restrict__ inst_end, const unsigned char *__restrict__ ran,
+                                   int ntiles, int every, int tshift, int sidx, double *__restrict__ hmax,
                                    int *__restrict__ argmax) {
-    const int tile = blockIdx.x;
+    const int c = blockIdx.x * (int)blockDim.x + threadIdx.x;
+    // tiles are 64 cells or more: one tile per wave
+    const int tile = __builtin_amdgcn_readfirstlane(c >> tshift);
+    if (tile >= ntiles) return;
+    if (sidx > 0) {
+        // lane j reads the row of step j: one round of loads, not a chain
+        const int lane = (int)(threadIdx.x & 63);
+        bool any = false;
+        for (int j0 = 0; j0 < every && !any; j0 += 64) {
+            const int j = j0 + lane;
+            any = __any(j < every && ran[(size_t)j * ntiles + tile] != 0);
+        }
+        if (!any) return;
+    }
     const int b = tile_inst[tile];
-    const int c = tile * (int)blockDim.x + threadIdx.x;
     if (c >= inst_end[b]) return;
     const double h = cur_state(st.s, ctrl, b)[c].x;
     if (sidx == 0 || h > hmax[c]) {
@@ -800,8 +823,16 @@ static void enqueue_forward(cf_handle *h, AdjState *A, hipStream_t s) {
     a.ran = A->ran;
     a.tile_ran = A->tile_ran;
     a.hist_stride = S;
+    // the steps of the last segment also write the tape the reverse sweep
+    // reads (states to seg, residuals to xs), so that it replays one segment
+    // less
+    const int klast = (A->nseg - 1) * A->K;
     for (int k = 0; k < A->nsteps; k++) {
         a.step = k;
+        if (k >= klast) {
+            a.out_x = A->seg + (size_t)(k - klast) * h->ncell;
+            a.xs = A->xs + (size_t)(k - klast) * 3 * h->ncell;
+        }
         if (k % kReorderEvery == 0) launch_rank(h, s);
         launch_step(a, h->iflux, h->fric, h->ntiles, s);
         launch_finalize(h, A->dt, A->tc, A->ran, S, k, s);
@@ -811,8 +842,10 @@ static void enqueue_forward(cf_handle *h, AdjState *A, hipStream_t s) {
                 hipLaunchKernelGGL(sample_gauges_kernel, dim3(grid_of(A->gtotal, 256)), dim3(256), 0, s, sp, h->ctrl,
                                    A->d_gcell, A->d_ginst, A->gtotal, sidx, A->eta);
             if (A->want_hmax)
-                hipLaunchKernelGGL(sample_hmax_kernel, dim3(h->ntiles), dim3(h->tile), 0, s, sp, h->ctrl, h->tile_inst,
-                                   h->inst_end, sidx, A->hmax, A->argmax);
+                hipLaunchKernelGGL(sample_hmax_kernel, dim3(grid_of(h->ntiles * h->tile, 256)), dim3(256), 0, s, sp,
+                                   h->ctrl, h->tile_inst, h->inst_end,
+                                   A->tile_ran + (size_t)(k + 1 - A->every) * h->ntiles, h->ntiles, A->every,
+                                   __builtin_ctz((unsigned)h->tile), sidx, A->hmax, A->argmax);
         }
         if ((k + 1) % A->K == 0 && k + 1 < A->nsteps)
             hipLaunchKernelGGL(ckpt_kernel, dim3(h->ntiles), dim3(h->tile), 0, s, sp, h->ctrl, h->tile_inst,
@@ -830,6 +863,7 @@ extern "C" int cf_adj_forward(cf_handle *h, int nsteps, int sample_every, int ch
     if (!A) return CF_ERR_NOMEM;
     A->recorded = false;
     A->tlm_ready = false;
+    A->tape_last = false;
     if (A->mask_mode) drop_adj_graphs(A);
     A->mask_mode = 0;
     A->mask_rec = A->sig_rec = false;
@@ -868,6 +902,7 @@ extern "C" int cf_adj_forward(cf_handle *h, int nsteps, int sample_every, int ch
     HIPCHK(hipStreamSynchronize(s));
     A->params_ver = h->params_ver;
     A->recorded = true;
+    A->tape_last = true;
     return CF_OK;
 }
 
@@ -1044,9 +1079,14 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
         const double4 *ck = A->ckpt + (size_t)sg * N;
         // replay: state k0+m+1 -> seg[m], residual of step k0+m -> xs[m].
         // seg[m] last held step k0+K+m of the segment after this one (none
-        // for the first segment reversed: its content is unknown).
-        for (int k = k0; k < k1; k++) {
+        // for the first segment reversed: its content is unknown).  After a
+        // recorded forward, seg and xs hold the last segment's tape.
+        const bool taped = sg == A->nseg - 1 && A->tape_last;
+        for (int k = k0; k < k1 && !taped; k++) {
             a.step = k;
+            // workgroups to the tiles that ran at one of the next steps first
+            if ((k - k0) % kReorderEvery == 0)
+                launch_rank(h, s, A->tile_ran + (size_t)k * h->ntiles, std::min(kReorderEvery, k1 - k));
             a.in_x = (k == k0) ? ck : A->seg + (size_t)(k - k0 - 1) * N;
             a.out_x = A->seg + (size_t)(k - k0) * N;
             a.xs = A->xs + (size_t)(k - k0) * 3 * N;
@@ -1056,6 +1096,10 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
         }
         for (int k = k1 - 1; k >= k0; k--) {
             r.step = k;
+            if ((k1 - 1 - k) % kReorderEvery == 0) {
+                const int lo = std::max(k0, k - kReorderEvery + 1);
+                launch_rank(h, s, A->tile_ran + (size_t)lo * h->ntiles, k - lo + 1);
+            }
             r.st = (k == k0) ? ck : A->seg + (size_t)(k - k0 - 1) * N;
             r.xs = A->xs + (size_t)(k - k0) * 3 * N;
             r.ran = A->tile_ran + (size_t)k * h->ntiles;
@@ -1082,8 +1126,10 @@ extern "C" int cf_adj_backward(cf_handle *h) {
     if (!h || !h->adj || !h->adj->recorded || h->pending >= 0) return CF_ERR_ARG;
     AdjState *A = h->adj;
     if (A->params_ver != h->params_ver) return CF_ERR_ARG;  // inflow or hydrograph changed since the record
-    int rc = run_captured(h, A, 1, enqueue_backward);
+    int rc = run_captured(h, A, A->tape_last ? 3 : 1, enqueue_backward);
     if (rc) return rc;
+    // the replays of the other segments overwrote the tape
+    if (A->nseg > 1) A->tape_last = false;
     HIPCHK(hipStreamSynchronize(h->stream));
     A->lam_cur = 0;  // the buffer holding the initial-state cotangent
     return CF_OK;

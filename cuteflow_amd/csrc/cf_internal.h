@@ -73,11 +73,15 @@ struct StepArgs {
     // Replay of a recorded run (reverse sweep): when dt_fix is set the step
     // takes dt and tc of step `step` from dt_fix/tc_fix, reads in_x, writes
     // out_x and the residual x to xs (3 x ncell), and skips the dt reduction.
+    // A recorded forward that also keeps the tape of its steps (out_x set,
+    // dt_fix null) writes the states after the step to out_x and the
+    // residuals to xs, as the replay would.
     const double *dt_fix;
     const double *tc_fix;
     const double4 *in_x;
     double4 *out_x;
     double *xs;
+    const double *zm;  // bed elevation per cell (the w of every state)
     // Tile activity of a recorded run, row per step, one byte per tile: the
     // recorded forward writes whether each tile ran.  In a replay a tile
     // that did not run holds dry cells with dry neighbours, whose state
@@ -294,8 +298,10 @@ void launch_finalize(cf_handle *h, double *dt_hist, double *tc_hist, int *ran, i
 void launch_run_init(cf_handle *h, double *dt_hist, double *tc_hist, int stride, hipStream_t s,
                      InstCtrl *mirror = nullptr);
 int set_schedule(cf_handle *h, int skip, int xcd);
-// the wet-first ranking of the tiles (no-op with the fixed order)
-void launch_rank(cf_handle *h, hipStream_t s);
+// the wet-first ranking of the tiles (no-op with the fixed order): by the
+// classes the last step wrote, or with `ran` by `nrows` consecutive activity
+// rows of a recorded run (the tiles that ran at one of those steps first)
+void launch_rank(cf_handle *h, hipStream_t s, const unsigned char *ran = nullptr, int nrows = 0);
 // defined in cf_adjoint.hip
 void adj_free(cf_handle *h);
 void adj_drop_graphs(cf_handle *h);

@@ -77,13 +77,20 @@ __device__ __forceinline__ double wave_min(double v) {
 // effect on parameters, and with it the compiler keeps the per-instance
 // constants in scalar registers (scalar loads) instead of reloading them per
 // lane after the first store; that also frees vector registers (fewer spills).
-template <int IFLUX, int FRIC>
+// MODE kStepRun: a step of a run.  kStepReplay: the replay of a recorded
+// run (a.dt_fix set): the recorded dt and tile activity, states from a.in_x
+// to a.out_x and the residuals to a.xs.  kStepRunTape: a step of a run that
+// also writes what the replay would (a.out_x, a.xs).  One kernel per mode, so
+// that no mode carries another's code.
+enum { kStepRun = 0, kStepReplay = 1, kStepRunTape = 2 };
+
+template <int IFLUX, int FRIC, int MODE>
 __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs a, const KParams *__restrict__ kp) {
     __shared__ double s_min[kWaves];
     const int2 tb = a.tile_map[blockIdx.x];  // (tile, instance)
     const int tile = tb.x, b = tb.y;
     const InstCtrl c = a.ctrl[b];
-    const bool replay = a.dt_fix != nullptr;
+    constexpr bool replay = MODE == kStepReplay, tape = MODE == kStepRunTape;
     unsigned char *ran_row = a.tile_ran ? a.tile_ran + (size_t)a.step * gridDim.x : nullptr;
     if (!replay) {
         if (c.status != 0) {  // this instance has stopped (ts reached) or failed
@@ -100,6 +107,10 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
                 if (ran_row) ran_row[tile] = 0;  // no dt contribution
                 if (a.tile_class) a.tile_class[tile] = kTileSkipped;
             }
+            if (tape) {
+                const int ic = tile * (int)blockDim.x + threadIdx.x;
+                a.out_x[ic] = make_double4(0.0, 0.0, 0.0, a.zm[ic]);
+            }
             return;
         }
     } else if (ran_row && !ran_row[tile]) {
@@ -108,7 +119,7 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
         // it last held (or that step is unknown)
         if (!a.prev_use || a.prev_use[tile]) {
             const int ic = tile * (int)blockDim.x + threadIdx.x;
-            a.out_x[ic] = make_double4(0.0, 0.0, 0.0, a.in_x[ic].w);
+            a.out_x[ic] = make_double4(0.0, 0.0, 0.0, a.zm[ic]);
         }
         return;
     }
@@ -142,17 +153,18 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
         const double zml = st_in[i].w;
 #endif
         double o[3], xo[3];
-        cell_update<double, IFLUX, FRIC>(ld, p, a.hydro, i, dt, tc, o, a.xs ? xo : nullptr,
+        cell_update<double, IFLUX, FRIC>(ld, p, a.hydro, i, dt, tc, o, (MODE != kStepRun && a.xs) ? xo : nullptr,
                                          a.surf_in ? a.surf_in + (size_t)b * CF_MAX_IO : nullptr);
         const double h = o[0], hu = o[1], hv = o[2];
         st_out[i] = make_double4(h, hu, hv, zml);
-        if (replay) {
+        if (replay || tape) {
+            if (tape) a.out_x[i] = make_double4(h, hu, hv, zml);
             if (a.xs) {
                 a.xs[i] = xo[0];
                 a.xs[(size_t)a.ncell + i] = xo[1];
                 a.xs[2 * (size_t)a.ncell + i] = xo[2];
             }
-            return;
+            if (replay) return;
         }
         isnan_local = (h != h) || (hu != hu) || (hv != hv);
         wet_local = (h != 0.0) || isnan_local;
@@ -209,9 +221,23 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
 // (ntiles < 2^21).
 constexpr int kRankBlock = 256;
 
-__device__ __forceinline__ unsigned long long class_bit(const unsigned char *__restrict__ tile_class, int t,
+// A tile's class: its tile_class entry, or with `nrows` activity rows of a
+// recorded run (ran[j * ntiles + t] = 1: tile t ran at that step), wet for
+// the tiles that ran at one of those steps and skipped for the others.
+__device__ __forceinline__ int class_of(const unsigned char *__restrict__ tile_class,
+                                        const unsigned char *__restrict__ ran, int nrows, int t, int ntiles) {
+    if (!ran) return min((int)tile_class[t], 2);
+    // independent loads (no early exit: a chain of dependent loads is slower)
+    int any = 0;
+#pragma unroll 8
+    for (int j = 0; j < nrows; j++) any |= ran[(size_t)j * ntiles + t];
+    return any ? kTileWet : kTileSkipped;
+}
+
+__device__ __forceinline__ unsigned long long class_bit(const unsigned char *__restrict__ tile_class,
+                                                       const unsigned char *__restrict__ ran, int nrows, int t,
                                                        int ntiles) {
-    return t < ntiles ? 1ull << (21 * min((int)tile_class[t], 2)) : 0ull;
+    return t < ntiles ? 1ull << (21 * class_of(tile_class, ran, nrows, t, ntiles)) : 0ull;
 }
 
 // inclusive scan of packed counts over the workgroup (kRankBlock threads)
@@ -237,13 +263,16 @@ __device__ __forceinline__ unsigned long long block_scan_packed(unsigned long lo
 
 // counts[k]: the packed class counts of tiles 256k .. 256k + 255
 __global__ __launch_bounds__(kRankBlock) void rank_count_kernel(const unsigned char *__restrict__ tile_class,
+                                                                const unsigned char *__restrict__ ran, int nrows,
                                                                 int ntiles, unsigned long long *__restrict__ counts) {
     unsigned long long total;
-    (void)block_scan_packed(class_bit(tile_class, (int)(blockIdx.x * kRankBlock + threadIdx.x), ntiles), total);
+    (void)block_scan_packed(class_bit(tile_class, ran, nrows, (int)(blockIdx.x * kRankBlock + threadIdx.x), ntiles),
+                            total);
     if (threadIdx.x == 0) counts[blockIdx.x] = total;
 }
 
 __global__ __launch_bounds__(kRankBlock) void rank_map_kernel(const unsigned char *__restrict__ tile_class,
+                                                              const unsigned char *__restrict__ ran, int nrows,
                                                               const int *__restrict__ tile_inst, int ntiles, int xcd,
                                                               const unsigned long long *__restrict__ counts,
                                                               int2 *__restrict__ map) {
@@ -274,12 +303,12 @@ __global__ __launch_bounds__(kRankBlock) void rank_map_kernel(const unsigned cha
         all += s_sum[1][q];
     }
     const int t = k0 * kRankBlock + (int)threadIdx.x;
-    const unsigned long long mine = class_bit(tile_class, t, ntiles);
+    const unsigned long long mine = class_bit(tile_class, ran, nrows, t, ntiles);
     unsigned long long total;
     const unsigned long long incl = block_scan_packed(mine, total);
     if (t >= ntiles) return;
     constexpr unsigned long long kM = (1ull << 21) - 1;
-    const int c = min((int)tile_class[t], 2);
+    const int c = class_of(tile_class, ran, nrows, t, ntiles);
     const int j = (int)(((before + incl - mine) >> (21 * c)) & kM);  // rank of tile t within its class
     const int n0 = (int)(all & kM), n1 = (int)((all >> 21) & kM), n2 = (int)((all >> 42) & kM);
     const int p = c == 0 ? 0 : (c == 1 ? n0 : n0 + n1), n = c == 0 ? n0 : (c == 1 ? n1 : n2);
@@ -1081,8 +1110,13 @@ extern "C" int cf_get_time(cf_handle *h, double *tc, double *dt) { return cf_bat
 
 template <int IFLUX, int FRIC>
 static void launch_step_t(const StepArgs &a, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((step_kernel<IFLUX, FRIC>), dim3(grid), dim3(a.tile_cells),
-                       CF_STEP_LDS ? (size_t)a.tile_cells * (sizeof(double4) + sizeof(int4)) : 0, s, a, a.kp);
+    const size_t lds = CF_STEP_LDS ? (size_t)a.tile_cells * (sizeof(double4) + sizeof(int4)) : 0;
+    if (a.dt_fix)
+        hipLaunchKernelGGL((step_kernel<IFLUX, FRIC, kStepReplay>), dim3(grid), dim3(a.tile_cells), lds, s, a, a.kp);
+    else if (a.out_x)
+        hipLaunchKernelGGL((step_kernel<IFLUX, FRIC, kStepRunTape>), dim3(grid), dim3(a.tile_cells), lds, s, a, a.kp);
+    else
+        hipLaunchKernelGGL((step_kernel<IFLUX, FRIC, kStepRun>), dim3(grid), dim3(a.tile_cells), lds, s, a, a.kp);
 }
 
 typedef void (*step_launcher)(const StepArgs &, int, hipStream_t);
@@ -1110,12 +1144,13 @@ void cf::launch_finalize(cf_handle *h, double *dt_hist, double *tc_hist, int *ra
                        qin_hist, qout_hist, mirror);
 }
 
-void cf::launch_rank(cf_handle *h, hipStream_t s) {
+void cf::launch_rank(cf_handle *h, hipStream_t s, const unsigned char *ran, int nrows) {
     if (!h->wet_first) return;
     const int nb = grid_of(h->ntiles, kRankBlock);
-    hipLaunchKernelGGL(rank_count_kernel, dim3(nb), dim3(kRankBlock), 0, s, h->tile_class, h->ntiles, h->rank_counts);
-    hipLaunchKernelGGL(rank_map_kernel, dim3(nb), dim3(kRankBlock), 0, s, h->tile_class, h->tile_inst, h->ntiles,
-                       h->xcd, h->rank_counts, h->tile_map);
+    hipLaunchKernelGGL(rank_count_kernel, dim3(nb), dim3(kRankBlock), 0, s, h->tile_class, ran, nrows, h->ntiles,
+                       h->rank_counts);
+    hipLaunchKernelGGL(rank_map_kernel, dim3(nb), dim3(kRankBlock), 0, s, h->tile_class, ran, nrows, h->tile_inst,
+                       h->ntiles, h->xcd, h->rank_counts, h->tile_map);
 }
 
 void cf::launch_run_init(cf_handle *h, double *dt_hist, double *tc_hist, int stride, hipStream_t s,
@@ -1128,6 +1163,7 @@ StepArgs cf::base_args(cf_handle *h) {
     StepArgs a{};
     a.st[0] = h->st[0];
     a.st[1] = h->st[1];
+    a.zm = h->zm;
     a.nbr = h->nbr;
     a.geo = h->geo;
     a.dfc = h->dfc;

@@ -13,7 +13,9 @@
   derivatives and central differences of the forward with the recorded dt;
 - a few L-BFGS iterations on zonal n reduce the misfit;
 - each instance of a batch gets the gradient of the same problem run alone,
-  bit for bit.
+  bit for bit;
+- the reverse sweep that takes the last segment from the recorded forward's
+  tape equals one that replays it.
 """
 import numpy as np
 import pytest
@@ -232,33 +234,71 @@ def test_gpu_reverse_tile_skipping_is_exact():
     skipped by the replay, reverse and gather kernels too.  Gradients with
     256- and 64-cell tiles (and 128-cell tiles in XCD-contiguous order)
     equal those of a run that processes every tile (CF_OPT_NO_TILE_SKIP),
-    value for value."""
+    value for value.  The maximum depths (whose sampling passes over the
+    tiles that did not run since the previous sample) equal the maxima of
+    the states a plain run reaches at the sample steps, bit for bit."""
     from cuteflow_amd import _lib
     from cuteflow_amd import scenarios as SC
 
     sc = SC.reach(40_000, seed=81, iflux=1, q_peak=800.0, t_ramp=600.0)
     m, p, U0 = sc.mesh, sc.params, sc.state
-    nsteps = 120
+    nsteps, every = 120, 10
     g = _gauges(m, 20, 4)
     rng = np.random.default_rng(12)
     ge = rng.normal(size=(12, 20))
+    gh = rng.normal(size=m.nelt)
     gs = rng.normal(size=(m.nelt, 3))
     out = []
     for opt in (0, _lib.CF_OPT_TILE_64, _lib.CF_OPT_TILE_128 | _lib.CF_OPT_XCD_CONTIG, _lib.CF_OPT_NO_TILE_SKIP):
         with Solver(m, p, options=opt) as s:
             s.set_state(U0)
-            s.step(nsteps)
+            hmax_ref = None
+            for _ in range(nsteps // every):
+                s.step(every)
+                h = s.get_state()[:, 0]
+                hmax_ref = h.copy() if hmax_ref is None else np.where(h > hmax_ref, h, hmax_ref)
             frac = s.last_run_stats()["active_fraction"]
-            run = AdjointRun(s, g, sample_every=10, checkpoint_every=50)
-            eta, _ = run.forward(nsteps, U0=[U0])
-            gn, gU = run.backward([ge], None, [gs])
-            out.append((frac, eta[0], gn[0], gU[0]))
+            run = AdjointRun(s, g, sample_every=every, checkpoint_every=50, want_hmax=True)
+            eta, hm = run.forward(nsteps, U0=[U0])
+            assert _same(hm[0], hmax_ref), opt
+            gn, gU = run.backward([ge], [gh], [gs])
+            out.append((frac, eta[0], gn[0], gU[0], hm[0]))
     print("active fractions", [o[0] for o in out], flush=True)
     assert all(o[0] < 1.0 for o in out[:3]) and out[3][0] == 1.0  # some tiles were skipped
     for k in (0, 1, 2):
         assert _same(out[k][1], out[3][1])
+        assert _same(out[k][4], out[3][4])
         assert np.array_equal(out[k][2], out[3][2])
         assert np.array_equal(out[k][3], out[3][3])
+
+
+def test_gpu_reverse_from_forward_tape_equals_replay():
+    """The reverse sweep after a recorded forward reads the last segment's
+    states and residuals as the forward wrote them; a second sweep over the
+    same record replays every segment, and a record in one segment is all
+    tape.  The three give the same gradients, value for value."""
+    from cuteflow_amd import scenarios as SC
+
+    sc = SC.reach(40_000, seed=81, iflux=1, q_peak=800.0, t_ramp=600.0)
+    m, p, U0 = sc.mesh, sc.params, sc.state
+    nsteps = 110
+    g = _gauges(m, 20, 4)
+    rng = np.random.default_rng(13)
+    ge = rng.normal(size=(11, 20))
+    gh = rng.normal(size=m.nelt)
+    gs = rng.normal(size=(m.nelt, 3))
+    out = []
+    for K, sweeps in ((50, 2), (200, 1)):  # segments of 50, 50 and 10 steps; one segment
+        with Solver(m, p) as s:
+            run = AdjointRun(s, g, sample_every=10, checkpoint_every=K, want_hmax=True)
+            run.forward(nsteps, U0=[U0])
+            for _ in range(sweeps):
+                gn, gU = run.backward([ge], [gh], [gs])
+                out.append((gn[0], gU[0]))
+    assert np.any(out[0][0] != 0.0)
+    for k in (1, 2):
+        assert np.array_equal(out[k][0], out[0][0])
+        assert np.array_equal(out[k][1], out[0][1])
 
 
 def test_gpu_branch_masks():
