@@ -72,6 +72,10 @@ struct AdjState {
     double4 *lam[2] = {nullptr, nullptr};
     double4 *gstate = nullptr;  // final-state seed
     double *R1[2] = {nullptr, nullptr};   // 12 x ncell: (h, hu, hv) for neighbours 0..2, own (by step parity)
+    // by step parity, one bit per cell (64 cells a word): the cell's R1/R2
+    // contributions are nonzero (it was not clamped); the slots of a clamped
+    // cell are neither written nor read
+    unsigned long long *cmask[2] = {nullptr, nullptr};
     double4 *R2[2] = {nullptr, nullptr};  // 6 x ncell: face k, slot s at 2k + s
     int *r2_off = nullptr, *r2_idx = nullptr;
     double *gn = nullptr;
@@ -133,6 +137,8 @@ struct RevArgs {
     const double4 *lam;        // cotangent of the state after the last step (when ran_next is null)
     const double *R1in;        // contributions of the reverse of step k + 1
     const double4 *R2in;
+    const unsigned long long *cmask_in;  // cells of step k + 1 with nonzero contributions
+    unsigned long long *cmask;           // the same for step k
     double *R1;                // contributions of this step's reverse
     double4 *R2;
     const int *r2_off;
@@ -198,8 +204,16 @@ __device__ __forceinline__ void acc3(double4 &a, const double *__restrict__ s) {
 
 // tiles are 64, 128 or 256 cells: a cell's tile by a shift (tshift = log2 of
 // the tile size) rather than an integer division by a runtime value
+__device__ __forceinline__ bool live_cell(const unsigned long long *__restrict__ cm, int c) {
+    return ((cm[c >> 6] >> (c & 63)) & 1ull) != 0ull;
+}
+
+// `cm`: the cells of step k + 1 whose contributions are nonzero; the slots
+// of the others are zero and not read (skipping the addition of +0 changes
+// at most the sign of a zero sum)
 template <bool R2ON>
 __device__ __forceinline__ double4 gather_cell(int c, int tshift, const unsigned char *__restrict__ tiles_ran,
+                                               const unsigned long long *__restrict__ cm,
                                                const int4 *__restrict__ nbr, const double *__restrict__ R1,
                                                const double4 *__restrict__ R2, const int *__restrict__ r2_off,
                                                const int *__restrict__ r2_idx, const int *__restrict__ argmax,
@@ -208,19 +222,22 @@ __device__ __forceinline__ double4 gather_cell(int c, int tshift, const unsigned
                                                const int *__restrict__ gnext) {
     double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
     if (tiles_ran[c >> tshift]) {
-        const double *o = R1 + kR1 * (size_t)c + 9;
-        acc = make_double4(o[0], o[1], o[2], 0.0);
+        if (live_cell(cm, c)) {
+            const double *o = R1 + kR1 * (size_t)c + 9;
+            acc = make_double4(o[0], o[1], o[2], 0.0);
+        }
         const int4 nb = nbr[c];
         const int nbv[3] = {nb.x, nb.y, nb.z};
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const int i = nbv[k];
-            if (i >= 0 && tiles_ran[i >> tshift]) acc3(acc, R1 + kR1 * (size_t)i + 3 * ((nb.w >> (2 * k)) & 3));
+            if (i >= 0 && tiles_ran[i >> tshift] && live_cell(cm, i))
+                acc3(acc, R1 + kR1 * (size_t)i + 3 * ((nb.w >> (2 * k)) & 3));
         }
         if (R2ON)
             for (int e = r2_off[c]; e < r2_off[c + 1]; e++) {
-                const int src = r2_idx[e];
-                if (tiles_ran[(src / 6) >> tshift]) acc4(acc, R2[src]);
+                const int src = r2_idx[e], sc = src / 6;
+                if (tiles_ran[sc >> tshift] && live_cell(cm, sc)) acc4(acc, R2[src]);
             }
     }
     if (argmax && argmax[c] == sidx) acc.x += ghmax[c];
@@ -258,15 +275,19 @@ __global__ __launch_bounds__(kTile) __attribute__((amdgpu_waves_per_eu(RevWaves<
     const double x[3] = {r.xs[i], r.xs[(size_t)r.ncell + i], r.xs[2 * (size_t)r.ncell + i]};
     // A cell whose new state the forward clamped to dry contributes nothing
     // whatever its cotangent (cell_reverse returns zeros at the same test):
-    // its cotangent is not gathered.
+    // its cotangent is not gathered, its slots are not written (its bit in
+    // r.cmask tells the gather of step k - 1 not to read them) and its
+    // Manning cotangent is +0 (gn, a sum from +0, is never -0: unchanged).
     const double r0 = (FRIC == 0 || FRIC == 1) ? x[0] : (1.0 * x[0] + 0.0 * x[1]) + 0.0 * x[2];
     const bool clamped = r.st[i].x + r.dt[hix] * r0 <= kp[b].tolisec;
-    double4 l4 = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (!clamped)
-        l4 = r.ran_next ? gather_cell<IFLUX >= 3>(i, __ffs((int)blockDim.x) - 1, r.ran_next, r.nbr, r.R1in, r.R2in,
-                                                  r.r2_off, r.r2_idx, r.argmax, r.ghmax, r.sidx, r.geta_row,
-                                                  r.gfirst, r.gnext)
-                        : r.lam[i];
+    // the real cells of a tile come first: lane 0 of a wave with any is one
+    const unsigned long long live = __ballot(!clamped);
+    if ((threadIdx.x & 63) == 0) r.cmask[i >> 6] = live;
+    if (clamped) return;
+    const double4 l4 = r.ran_next ? gather_cell<IFLUX >= 3>(i, __ffs((int)blockDim.x) - 1, r.ran_next, r.cmask_in,
+                                                            r.nbr, r.R1in, r.R2in, r.r2_off, r.r2_idx, r.argmax,
+                                                            r.ghmax, r.sidx, r.geta_row, r.gfirst, r.gnext)
+                                  : r.lam[i];
     const DevLd ld{r.st, r.nbr, r.dfc, r.geo, r.manning, r.ncell};
     const double lam[3] = {l4.x, l4.y, l4.z};
     double own[3], r1[3][3], r2[3][2][3], gn;
@@ -300,13 +321,13 @@ __global__ void gather_kernel(const int4 *__restrict__ nbr, const double *__rest
                               const double4 *__restrict__ R2, const int *__restrict__ r2_off,
                               const int *__restrict__ r2_idx, const int *__restrict__ tile_inst,
                               const int *__restrict__ inst_end, const unsigned char *__restrict__ ran,
-                              double4 *__restrict__ lam_out) {
+                              const unsigned long long *__restrict__ cm, double4 *__restrict__ lam_out) {
     const int tile = blockIdx.x;
     const int b = tile_inst[tile];
     const int c = tile * (int)blockDim.x + threadIdx.x;
     if (c >= inst_end[b]) return;
-    lam_out[c] = gather_cell<R2ON>(c, __ffs((int)blockDim.x) - 1, ran, nbr, R1, R2, r2_off, r2_idx, nullptr, nullptr, -1,
-                                   nullptr, nullptr, nullptr);
+    lam_out[c] = gather_cell<R2ON>(c, __ffs((int)blockDim.x) - 1, ran, cm, nbr, R1, R2, r2_off, r2_idx, nullptr, nullptr,
+                                   -1, nullptr, nullptr, nullptr);
 }
 
 // lam = final-state seed + hmax seed of the last sample (cells not seeded: 0)
@@ -606,7 +627,7 @@ void cf::adj_free(cf_handle *h) {
     dfree(A->dt); dfree(A->tc); dfree(A->ran);
     dfree(A->ckpt); dfree(A->seg); dfree(A->xs);
     dfree(A->lam[0]); dfree(A->lam[1]); dfree(A->gstate);
-    dfree(A->R1[0]); dfree(A->R1[1]); dfree(A->R2[0]); dfree(A->R2[1]); dfree(A->r2_off); dfree(A->r2_idx); dfree(A->gn); dfree(A->tile_ran);
+    dfree(A->R1[0]); dfree(A->R1[1]); dfree(A->cmask[0]); dfree(A->cmask[1]); dfree(A->R2[0]); dfree(A->R2[1]); dfree(A->r2_off); dfree(A->r2_idx); dfree(A->gn); dfree(A->tile_ran);
     dfree(A->tst[0]); dfree(A->tst[1]); dfree(A->tdst[0]); dfree(A->tdst[1]); dfree(A->tdir); dfree(A->dman);
     dfree(A->wmask); dfree(A->flips); dfree(A->man_rec); dfree(A->bsig); dfree(A->bflips);
     delete A;
@@ -772,7 +793,8 @@ static int adj_alloc(cf_handle *h, AdjState *A) {
         drop_adj_graphs(A);
         if (!dalloc(A->lam[0], N) || !dalloc(A->man_rec, N) || !dalloc(A->lam[1], N) || !dalloc(A->gstate, N) || !dalloc(A->R1[0], kR1 * N) ||
             !dalloc(A->R1[1], kR1 * N) || !dalloc(A->gn, N) || !dalloc(A->hmax, N) || !dalloc(A->ghmax, N) ||
-            !dalloc(A->dhmax, N) || !dalloc(A->argmax, N))
+            !dalloc(A->dhmax, N) || !dalloc(A->argmax, N) || !dalloc(A->cmask[0], N / 64 + 1) ||
+            !dalloc(A->cmask[1], N / 64 + 1))
             return CF_ERR_NOMEM;
         fill0(A->R1[0], kR1 * N, h->stream);
         fill0(A->R1[1], kR1 * N, h->stream);
@@ -1106,6 +1128,8 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
             r.ran_next = k + 1 < A->nsteps ? A->tile_ran + (size_t)(k + 1) * h->ntiles : nullptr;
             r.R1in = A->R1[(k + 1) & 1];
             r.R2in = A->R2[(k + 1) & 1];
+            r.cmask_in = A->cmask[(k + 1) & 1];
+            r.cmask = A->cmask[k & 1];
             r.R1 = A->R1[k & 1];
             r.R2 = A->R2[k & 1];
             // seeds of the state after step k (sample sidx)
@@ -1116,10 +1140,10 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
     }
     if (h->iflux >= 3)
         hipLaunchKernelGGL(gather_kernel<true>, dim3(h->ntiles), dim3(h->tile), 0, s, h->nbr, A->R1[0], A->R2[0],
-                           A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, A->tile_ran, A->lam[0]);
+                           A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, A->tile_ran, A->cmask[0], A->lam[0]);
     else
         hipLaunchKernelGGL(gather_kernel<false>, dim3(h->ntiles), dim3(h->tile), 0, s, h->nbr, A->R1[0], A->R2[0],
-                           A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, A->tile_ran, A->lam[0]);
+                           A->r2_off, A->r2_idx, h->tile_inst, h->inst_end, A->tile_ran, A->cmask[0], A->lam[0]);
 }
 
 extern "C" int cf_adj_backward(cf_handle *h) {
