@@ -224,7 +224,7 @@ def bench_headline(args):
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
             "traffic": traffic,
-            "kernel": "step_kernel<3,2>",
+            "kernel": "step_kernel<3,2,0>",
             "bytes_per_cell_update": BYTES_WAF,
             "cells_per_launch": active_cells,
             "active_tile_fraction": stats["active_fraction"],
