@@ -304,7 +304,9 @@ int cf_adj_get_samples(cf_handle *h, int inst, double *eta, double *hmax);
  * hmax (nelt) and the final state (3*nelt); NULL = 0. */
 int cf_adj_set_seed(cf_handle *h, int inst, const double *g_eta, const double *g_hmax, const double *g_state);
 /* Reverse sweep over the recorded run: segments of checkpoint_every steps
- * replayed from their checkpoints, then reversed step by step.  It takes
+ * replayed from their checkpoints, then reversed step by step (the first
+ * sweep after a record takes the last segment as the recorded forward wrote
+ * it; a later sweep over the same record replays it too).  It takes
  * the Manning n of the recorded run, whatever n was set since (the
  * tangent-linear run takes the current n); after an inflow or hydrograph
  * change since the record it returns CF_ERR_ARG (record again). */
