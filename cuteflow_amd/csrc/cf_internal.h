@@ -66,7 +66,7 @@ struct StepArgs {
     unsigned *stamp;                     // per tile
     const int *__restrict__ tile_adj_off;
     const int *__restrict__ tile_adj;
-    const unsigned char *__restrict__ always;
+    const unsigned *__restrict__ always;   // per tile: 1 = always runs (inflow faces); 32-bit for a scalar load
     const int *__restrict__ tile_inst;   // instance of each tile
     const int *__restrict__ inst_end;    // per instance: one past its last real cell
     const int *__restrict__ inst_tile;   // per instance: first tile; [ninst] = total
@@ -212,7 +212,7 @@ struct cf_handle {
     unsigned *nanflag = nullptr;
     unsigned *stamp = nullptr;
     int *tile_adj_off = nullptr, *tile_adj = nullptr;
-    unsigned char *always = nullptr;
+    unsigned *always = nullptr;
     int *tile_inst = nullptr, *inst_end = nullptr, *inst_tile = nullptr;
     double *surf_in = nullptr;         // ninst x CF_MAX_IO inlet areas (phys_inflow)
     int phys_in = 0;                   // an instance uses inlet = phys_inflow

@@ -91,29 +91,35 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
     const int tile = tb.x, b = tb.y;
     const InstCtrl c = a.ctrl[b];
     constexpr bool replay = MODE == kStepReplay, tape = MODE == kStepRunTape;
-    unsigned char *ran_row = a.tile_ran ? a.tile_ran + (size_t)a.step * gridDim.x : nullptr;
+    // the tile's byte in this step's activity row of a recorded run (none
+    // otherwise), addressed where it is used: off the prologue's load chain
+    const auto ran_at = [&]() { return a.tile_ran + (size_t)a.step * gridDim.x + tile; };
     if (!replay) {
-        if (c.status != 0) {  // this instance has stopped (ts reached) or failed
-            if (threadIdx.x == 0 && ran_row) ran_row[tile] = 0;
-            return;
-        }
         // A tile whose cells and neighbour tiles' cells were all dry after
         // the last two steps would recompute (0, 0, 0) for every cell (all
         // faces dry/dry, no source, clamp), and both state buffers already
         // hold that: it is skipped.  Reading a stamp raised concurrently only
-        // makes the tile run, which is always correct.
-        if (a.skip && !a.always[tile] && a.stamp[tile] + 1u < c.gstep) {
+        // makes the tile run, which is always correct.  The stamp and the
+        // always flag are read beside the control block and one branch takes
+        // both exits (this instance stopped or failed; the tile skipped), so
+        // that the loads form one round after the map rather than a chain:
+        // most workgroups end here.
+        const unsigned stamp = a.stamp[tile];
+        const bool always = a.always[tile] != 0;
+        const bool stopped = c.status != 0;
+        // bitwise, not short-circuit: every load is used on every path
+        if (stopped | ((a.skip != 0) & !always & (stamp + 1u < c.gstep))) {
             if (threadIdx.x == 0) {
-                if (ran_row) ran_row[tile] = 0;  // no dt contribution
-                if (a.tile_class) a.tile_class[tile] = kTileSkipped;
+                if (a.tile_ran) *ran_at() = 0;  // no dt contribution
+                if (a.tile_class && !stopped) a.tile_class[tile] = kTileSkipped;
             }
-            if (tape) {
+            if (tape && !stopped) {
                 const int ic = tile * (int)blockDim.x + threadIdx.x;
                 a.out_x[ic] = make_double4(0.0, 0.0, 0.0, a.zm[ic]);
             }
             return;
         }
-    } else if (ran_row && !ran_row[tile]) {
+    } else if (a.tile_ran && !*ran_at()) {
         // skipped in the recorded run: dry cells stay (+0, +0, +0, zm); the
         // output buffer already holds that unless the tile ran at the step
         // it last held (or that step is unknown)
@@ -197,7 +203,7 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
         atomicAdd(a.cntslot + so * kSlotStride32, 1u);
         // stored last: an early store would keep the uniform loads above
         // from being scalar loads (they could alias it)
-        if (ran_row) ran_row[tile] = 1;
+        if (a.tile_ran) *ran_at() = 1;
     }
     // a wet tile keeps itself and its neighbour tiles running for two steps
     // and is ranked among the first tiles of the next steps
@@ -897,7 +903,7 @@ extern "C" int cf_batch_create(int ninst, const cf_mesh *meshes, const cf_params
     std::vector<int4> nbr(N, make_int4(-3, -3, -3, 0));
     std::vector<double> mann(N, 0.0);
     std::vector<int> perm(N, 0), tile_inst(tiles), inst_end(ninst), inst_tile(ninst + 1), adj_off(tiles + 1, 0), adj;
-    std::vector<unsigned char> always(tiles, 0);
+    std::vector<unsigned> always(tiles, 0u);
     // discharge slots: the instances' inflow/outflow cells, caller order
     std::vector<int> io_cell, io_off(ninst + 1, 0);
     for (int b = 0; b < ninst; b++) {

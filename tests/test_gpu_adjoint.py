@@ -272,6 +272,28 @@ def test_gpu_reverse_tile_skipping_is_exact():
         assert np.array_equal(out[k][3], out[3][3])
 
 
+def test_gpu_hmax_sampling_long_interval():
+    """Samples 70 steps apart (the activity check of a tile reads 70 rows,
+    more than one wave's lanes): the maximum depths equal the maxima of the
+    states a plain run reaches at the sample steps, bit for bit."""
+    from cuteflow_amd import scenarios as SC
+
+    sc = SC.reach(40_000, seed=82, iflux=1, q_peak=800.0, t_ramp=600.0)
+    m, p, U0 = sc.mesh, sc.params, sc.state
+    nsteps, every = 210, 70
+    with Solver(m, p) as s:
+        s.set_state(U0)
+        hmax_ref = None
+        for _ in range(nsteps // every):
+            s.step(every)
+            h = s.get_state()[:, 0]
+            hmax_ref = h.copy() if hmax_ref is None else np.where(h > hmax_ref, h, hmax_ref)
+        assert s.last_run_stats()["active_fraction"] < 1.0
+        run = AdjointRun(s, _gauges(m, 5, 2), sample_every=every, checkpoint_every=100, want_hmax=True)
+        _, hm = run.forward(nsteps, U0=[U0])
+    assert _same(hm[0], hmax_ref)
+
+
 def test_gpu_reverse_from_forward_tape_equals_replay():
     """The reverse sweep after a recorded forward reads the last segment's
     states and residuals as the forward wrote them; a second sweep over the
