@@ -1106,9 +1106,12 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
         const bool taped = sg == A->nseg - 1 && A->tape_last;
         for (int k = k0; k < k1 && !taped; k++) {
             a.step = k;
-            // workgroups to the tiles that ran at one of the next steps first
+            // workgroups to the tiles that ran at one of the next steps first,
+            // in XCD-contiguous chunks whatever the forward's schedule: the
+            // replay and the reverse gain from the L2 locality (C5 gradient
+            // 2364 -> 2307 ms, the forward equal either way)
             if ((k - k0) % kReorderEvery == 0)
-                launch_rank(h, s, A->tile_ran + (size_t)k * h->ntiles, std::min(kReorderEvery, k1 - k));
+                launch_rank(h, s, A->tile_ran + (size_t)k * h->ntiles, std::min(kReorderEvery, k1 - k), 1);
             a.in_x = (k == k0) ? ck : A->seg + (size_t)(k - k0 - 1) * N;
             a.out_x = A->seg + (size_t)(k - k0) * N;
             a.xs = A->xs + (size_t)(k - k0) * 3 * N;
@@ -1120,7 +1123,7 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
             r.step = k;
             if ((k1 - 1 - k) % kReorderEvery == 0) {
                 const int lo = std::max(k0, k - kReorderEvery + 1);
-                launch_rank(h, s, A->tile_ran + (size_t)lo * h->ntiles, k - lo + 1);
+                launch_rank(h, s, A->tile_ran + (size_t)lo * h->ntiles, k - lo + 1, 1);
             }
             r.st = (k == k0) ? ck : A->seg + (size_t)(k - k0 - 1) * N;
             r.xs = A->xs + (size_t)(k - k0) * 3 * N;

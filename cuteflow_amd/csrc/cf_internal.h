@@ -300,8 +300,10 @@ void launch_run_init(cf_handle *h, double *dt_hist, double *tc_hist, int stride,
 int set_schedule(cf_handle *h, int skip, int xcd);
 // the wet-first ranking of the tiles (no-op with the fixed order): by the
 // classes the last step wrote, or with `ran` by `nrows` consecutive activity
-// rows of a recorded run (the tiles that ran at one of those steps first)
-void launch_rank(cf_handle *h, hipStream_t s, const unsigned char *ran = nullptr, int nrows = 0);
+// rows of a recorded run (the tiles that ran at one of those steps first);
+// each class in XCD-contiguous chunks when xcd is 1 (-1: as the handle's
+// schedule says)
+void launch_rank(cf_handle *h, hipStream_t s, const unsigned char *ran = nullptr, int nrows = 0, int xcd = -1);
 // defined in cf_adjoint.hip
 void adj_free(cf_handle *h);
 void adj_drop_graphs(cf_handle *h);

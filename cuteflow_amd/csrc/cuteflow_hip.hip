@@ -1150,13 +1150,13 @@ void cf::launch_finalize(cf_handle *h, double *dt_hist, double *tc_hist, int *ra
                        qin_hist, qout_hist, mirror);
 }
 
-void cf::launch_rank(cf_handle *h, hipStream_t s, const unsigned char *ran, int nrows) {
+void cf::launch_rank(cf_handle *h, hipStream_t s, const unsigned char *ran, int nrows, int xcd) {
     if (!h->wet_first) return;
     const int nb = grid_of(h->ntiles, kRankBlock);
     hipLaunchKernelGGL(rank_count_kernel, dim3(nb), dim3(kRankBlock), 0, s, h->tile_class, ran, nrows, h->ntiles,
                        h->rank_counts);
     hipLaunchKernelGGL(rank_map_kernel, dim3(nb), dim3(kRankBlock), 0, s, h->tile_class, ran, nrows, h->tile_inst,
-                       h->ntiles, h->xcd, h->rank_counts, h->tile_map);
+                       h->ntiles, xcd < 0 ? h->xcd : xcd, h->rank_counts, h->tile_map);
 }
 
 void cf::launch_run_init(cf_handle *h, double *dt_hist, double *tc_hist, int stride, hipStream_t s,
