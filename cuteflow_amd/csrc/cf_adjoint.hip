@@ -80,6 +80,7 @@ struct AdjState {
     int *r2_off = nullptr, *r2_idx = nullptr;
     double *gn = nullptr;
     unsigned char *tile_ran = nullptr;  // nsteps x ntiles: tile ran in the recorded run
+    unsigned char *act = nullptr;       // K x ntiles: the replay's action per step of a segment and tile
     // seg and xs hold the last segment's steps as the recorded forward wrote
     // them (the reverse sweep then does not replay that segment)
     bool tape_last = false;
@@ -406,6 +407,22 @@ __global__ void sample_hmax_kernel(StPair st, const InstCtrl *ctrl, const int *_
     }
 }
 
+// The replay's action for each tile at steps k0 .. k0 + n - 1 of a segment
+// (row m): run the tile if it ran in the recorded run; else rewrite its
+// output slot to the dry state unless the slot is known to hold it already
+// (the slot last held step k0 + m + K, rows `next`, for m < nnext, and the
+// tile was skipped then too).
+__global__ void replay_action_kernel(const unsigned char *__restrict__ ran, int n, int ntiles,
+                                     const unsigned char *__restrict__ next, int nnext,
+                                     unsigned char *__restrict__ act) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    for (int m = 0; m < n; m++) {
+        const size_t o = (size_t)m * ntiles + t;
+        act[o] = ran[o] ? kReplayRun : ((next && m < nnext && !next[o]) ? kReplayKeep : kReplayRewrite);
+    }
+}
+
 __global__ void ckpt_kernel(StPair st, const InstCtrl *ctrl, const int *__restrict__ tile_inst,
                             const int *__restrict__ inst_end, double4 *__restrict__ dst) {
     const int tile = blockIdx.x;
@@ -627,7 +644,7 @@ void cf::adj_free(cf_handle *h) {
     dfree(A->dt); dfree(A->tc); dfree(A->ran);
     dfree(A->ckpt); dfree(A->seg); dfree(A->xs);
     dfree(A->lam[0]); dfree(A->lam[1]); dfree(A->gstate);
-    dfree(A->R1[0]); dfree(A->R1[1]); dfree(A->cmask[0]); dfree(A->cmask[1]); dfree(A->R2[0]); dfree(A->R2[1]); dfree(A->r2_off); dfree(A->r2_idx); dfree(A->gn); dfree(A->tile_ran);
+    dfree(A->R1[0]); dfree(A->R1[1]); dfree(A->cmask[0]); dfree(A->cmask[1]); dfree(A->R2[0]); dfree(A->R2[1]); dfree(A->r2_off); dfree(A->r2_idx); dfree(A->gn); dfree(A->tile_ran); dfree(A->act);
     dfree(A->tst[0]); dfree(A->tst[1]); dfree(A->tdst[0]); dfree(A->tdst[1]); dfree(A->tdir); dfree(A->dman);
     dfree(A->wmask); dfree(A->flips); dfree(A->man_rec); dfree(A->bsig); dfree(A->bflips);
     delete A;
@@ -773,9 +790,11 @@ static int adj_alloc(cf_handle *h, AdjState *A) {
     if (A->alloc_steps != A->nsteps || A->alloc_K != A->K) {
         drop_adj_graphs(A);
         dfree(A->dt); dfree(A->tc); dfree(A->ran); dfree(A->ckpt); dfree(A->seg); dfree(A->xs); dfree(A->tile_ran);
+        dfree(A->act);
         if (!dalloc(A->dt, B * S) || !dalloc(A->tc, B * S) || !dalloc(A->ran, B * S) ||
             !dalloc(A->ckpt, (size_t)A->nseg * N) || !dalloc(A->seg, (size_t)A->K * N) ||
-            !dalloc(A->xs, (size_t)A->K * 3 * N) || !dalloc(A->tile_ran, (size_t)A->nsteps * h->ntiles))
+            !dalloc(A->xs, (size_t)A->K * 3 * N) || !dalloc(A->tile_ran, (size_t)A->nsteps * h->ntiles) ||
+            !dalloc(A->act, (size_t)A->K * h->ntiles))
             return CF_ERR_NOMEM;
         A->alloc_steps = A->nsteps;
         A->alloc_K = A->K;
@@ -1104,6 +1123,13 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
         // for the first segment reversed: its content is unknown).  After a
         // recorded forward, seg and xs hold the last segment's tape.
         const bool taped = sg == A->nseg - 1 && A->tape_last;
+        if (!taped) {
+            const bool has_next = sg + 1 < A->nseg;
+            hipLaunchKernelGGL(replay_action_kernel, dim3(grid_of(h->ntiles, 256)), dim3(256), 0, s,
+                               A->tile_ran + (size_t)k0 * h->ntiles, k1 - k0, h->ntiles,
+                               has_next ? A->tile_ran + (size_t)(k0 + A->K) * h->ntiles : nullptr,
+                               has_next ? std::min(k1 - k0, A->nsteps - k0 - A->K) : 0, A->act);
+        }
         for (int k = k0; k < k1 && !taped; k++) {
             a.step = k;
             // workgroups to the tiles that ran at one of the next steps first,
@@ -1115,8 +1141,7 @@ static void enqueue_backward(cf_handle *h, AdjState *A, hipStream_t s) {
             a.in_x = (k == k0) ? ck : A->seg + (size_t)(k - k0 - 1) * N;
             a.out_x = A->seg + (size_t)(k - k0) * N;
             a.xs = A->xs + (size_t)(k - k0) * 3 * N;
-            a.prev_use = (sg + 1 < A->nseg && k + A->K < A->nsteps)
-                             ? A->tile_ran + (size_t)(k + A->K) * h->ntiles : nullptr;
+            a.replay_act = A->act + (size_t)(k - k0) * h->ntiles;
             launch_step(a, h->iflux, h->fric, h->ntiles, s);
         }
         for (int k = k1 - 1; k >= k0; k--) {

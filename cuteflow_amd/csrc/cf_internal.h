@@ -33,6 +33,8 @@ constexpr unsigned long long kDtSlotEmpty = 0x7ff0000000000000ull;
 // ran them: held water, ran with only dry cells, skipped.
 constexpr int kReorderEvery = 32;
 enum { kTileWet = 0, kTileDry = 1, kTileSkipped = 2 };
+// replay actions per tile and step
+enum { kReplayKeep = 0, kReplayRun = 1, kReplayRewrite = 2 };
 
 // Per-instance control block, resident on the device across runs.
 struct InstCtrl {
@@ -86,11 +88,11 @@ struct StepArgs {
     // recorded forward writes whether each tile ran.  In a replay a tile
     // that did not run holds dry cells with dry neighbours, whose state
     // after the step is (+0, +0, +0, zm), as before it.  out_x is a reused
-    // buffer: prev_use is the activity row of the step it last held (null
-    // if unknown), and only tiles that ran then (or all, if unknown) are
-    // rewritten to (+0, +0, +0, zm).
+    // buffer, so the replay reads one action byte per tile (replay_act,
+    // kReplay*): run the tile, rewrite its slot to (+0, +0, +0, zm) (the slot
+    // last held a step at which the tile ran, or an unknown one), or nothing.
     unsigned char *tile_ran;
-    const unsigned char *prev_use;
+    const unsigned char *replay_act;
     // phys_inflow: per instance, CF_MAX_IO wetted inlet areas of the step's
     // input state (null when no instance uses phys_inflow)
     const double *surf_in;

@@ -119,15 +119,18 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
             }
             return;
         }
-    } else if (a.tile_ran && !*ran_at()) {
+    } else {
         // skipped in the recorded run: dry cells stay (+0, +0, +0, zm); the
         // output buffer already holds that unless the tile ran at the step
-        // it last held (or that step is unknown)
-        if (!a.prev_use || a.prev_use[tile]) {
-            const int ic = tile * (int)blockDim.x + threadIdx.x;
-            a.out_x[ic] = make_double4(0.0, 0.0, 0.0, a.zm[ic]);
+        // it last held (or that step is unknown): one byte says which
+        const int act = a.replay_act[tile];
+        if (act != kReplayRun) {
+            if (act == kReplayRewrite) {
+                const int ic = tile * (int)blockDim.x + threadIdx.x;
+                a.out_x[ic] = make_double4(0.0, 0.0, 0.0, a.zm[ic]);
+            }
+            return;
         }
-        return;
     }
     const double4 *__restrict__ st_in = replay ? a.in_x : a.st[(c.par + a.step) & 1];
     double4 *__restrict__ st_out = replay ? a.out_x : a.st[(c.par + a.step + 1) & 1];
