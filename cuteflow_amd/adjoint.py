@@ -24,9 +24,10 @@ import numpy as np
 from . import _lib
 
 
-def _ptr(a):
+def _ptr(a, sync: bool = True):
     """(pointer, keep-alive) for a per-cell array: numpy (host) or a
-    contiguous float64 CUDA tensor (device)."""
+    contiguous float64 CUDA tensor (device; with sync, after the work queued
+    on torch's current stream, which may still be writing it)."""
     if a is None:
         return None, None
     if hasattr(a, "data_ptr"):
@@ -34,7 +35,8 @@ def _ptr(a):
 
         t = a.detach().to(torch.float64).contiguous()
         if t.is_cuda:
-            torch.cuda.current_stream().synchronize()
+            if sync:
+                torch.cuda.current_stream().synchronize()
             return C.c_void_p(t.data_ptr()), t
         a = t.numpy()
     a = np.ascontiguousarray(a, np.float64)
@@ -139,10 +141,17 @@ class AdjointRun:
         per-cell n and the initial state (None unless want_state).  With
         device the n gradients are float64 CUDA tensors (written into the
         tensors of `out` when given)."""
+        # per-cell seeds as pointers (CUDA tensors converted first, then one
+        # synchronisation with torch's stream for all of them)
+        gh_all = [_ptr(None if g_hmax is None else g_hmax[b], sync=False) for b in range(self.ninst)]
+        gs_all = [_ptr(None if g_state is None else g_state[b], sync=False) for b in range(self.ninst)]
+        if any(hasattr(k, "is_cuda") and k.is_cuda for _, k in gh_all + gs_all):
+            import torch
+
+            torch.cuda.current_stream().synchronize()
         for b in range(self.ninst):
             ge = None if g_eta is None or g_eta[b] is None else np.ascontiguousarray(g_eta[b], np.float64)
-            gh, kh = _ptr(None if g_hmax is None else g_hmax[b])
-            gs, ks = _ptr(None if g_state is None else g_state[b])
+            (gh, kh), (gs, ks) = gh_all[b], gs_all[b]
             _lib.check(self.lib.cf_adj_set_seed(self.h, b, _lib.dptr(ge) if ge is not None else None, gh, gs),
                        "cf_adj_set_seed")
             del kh, ks

@@ -52,6 +52,11 @@ struct AdjState {
     int nsteps = 0, every = 1, K = 1, nseg = 0, nsamp = 0, want_hmax = 0;
     bool recorded = false;
     bool tlm_ready = false;
+    // host copies: the recorded gauge levels (read once per record) and the
+    // gauge seeds (cf_adj_set_seed writes columns of it; uploaded once, by
+    // the next reverse sweep)
+    std::vector<double> eta_host, geta_host;
+    bool eta_host_ok = false, geta_dirty = false;
     int gtotal = 0;
     std::vector<std::vector<int>> gcells;  // per instance: caller-local gauge cells
     std::vector<int> goff;                 // per instance: first gauge (ninst + 1)
@@ -905,6 +910,7 @@ extern "C" int cf_adj_forward(cf_handle *h, int nsteps, int sample_every, int ch
     A->recorded = false;
     A->tlm_ready = false;
     A->tape_last = false;
+    A->eta_host_ok = false;
     if (A->mask_mode) drop_adj_graphs(A);
     A->mask_mode = 0;
     A->mask_rec = A->sig_rec = false;
@@ -944,6 +950,9 @@ extern "C" int cf_adj_forward(cf_handle *h, int nsteps, int sample_every, int ch
     A->params_ver = h->params_ver;
     A->recorded = true;
     A->tape_last = true;
+    // the seeds were zeroed on the device above
+    A->geta_host.assign((size_t)std::max(A->nsamp, 1) * (size_t)std::max(A->gtotal, 1), 0.0);
+    A->geta_dirty = false;
     return CF_OK;
 }
 
@@ -992,13 +1001,22 @@ static int in_state3(cf_handle *h, int b, const double *src, double4 *dst) {
 }
 
 // The (nsamp x ngauges) block of instance inst of a sample-major array.
-static int get_samples(cf_handle *h, int inst, const double *src, double *dst) {
+// Columns of instance inst from a sample-major device array (nsamp x
+// gtotal).  `cache`, when given, holds the host copy of src across calls
+// (`ok`: already copied).
+static int get_samples(cf_handle *h, int inst, const double *src, double *dst, std::vector<double> *cache = nullptr,
+                       bool *ok = nullptr) {
     AdjState *A = h->adj;
     if (A->nsamp <= 0 || A->gtotal <= 0) return CF_OK;
     const int ng = A->goff[inst + 1] - A->goff[inst];
-    std::vector<double> all((size_t)A->nsamp * A->gtotal);
-    HIPCHK(hipMemcpyAsync(all.data(), src, all.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    std::vector<double> local;
+    std::vector<double> &all = cache ? *cache : local;
+    if (!cache || !*ok) {
+        all.resize((size_t)A->nsamp * A->gtotal);
+        HIPCHK(hipMemcpyAsync(all.data(), src, all.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (ok) *ok = true;
+    }
     for (int s = 0; s < A->nsamp; s++)
         for (int g = 0; g < ng; g++) dst[(size_t)s * ng + g] = all[(size_t)s * A->gtotal + A->goff[inst] + g];
     return CF_OK;
@@ -1008,7 +1026,7 @@ extern "C" int cf_adj_get_samples(cf_handle *h, int inst, double *eta, double *h
     if (bad_adj(h, inst)) return CF_ERR_ARG;
     AdjState *A = h->adj;
     if (eta) {
-        int rc = get_samples(h, inst, A->eta, eta);
+        int rc = get_samples(h, inst, A->eta, eta, &A->eta_host, &A->eta_host_ok);
         if (rc) return rc;
     }
     if (hmax) {
@@ -1026,14 +1044,13 @@ extern "C" int cf_adj_set_seed(cf_handle *h, int inst, const double *g_eta, cons
     const int c0 = h->cell_off[inst], n = h->nelt[inst];
     const int ng = A->goff[inst + 1] - A->goff[inst];
     if (ng > 0 && A->nsamp > 0) {
-        // seeds of this instance's gauges: strided columns of the sample-major array
-        std::vector<double> all((size_t)A->nsamp * A->gtotal);
-        HIPCHK(hipMemcpyAsync(all.data(), A->geta, all.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
+        // seeds of this instance's gauges: strided columns of the sample-major
+        // host copy, uploaded by the next reverse sweep
+        std::vector<double> &all = A->geta_host;
         for (int s = 0; s < A->nsamp; s++)
             for (int g = 0; g < ng; g++)
                 all[(size_t)s * A->gtotal + A->goff[inst] + g] = g_eta ? g_eta[(size_t)s * ng + g] : 0.0;
-        HIPCHK(hipMemcpyAsync(A->geta, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        A->geta_dirty = true;
     }
     if (A->want_hmax) {
         if (g_hmax) {
@@ -1178,6 +1195,11 @@ extern "C" int cf_adj_backward(cf_handle *h) {
     if (!h || !h->adj || !h->adj->recorded || h->pending >= 0) return CF_ERR_ARG;
     AdjState *A = h->adj;
     if (A->params_ver != h->params_ver) return CF_ERR_ARG;  // inflow or hydrograph changed since the record
+    if (A->geta_dirty) {
+        HIPCHK(hipMemcpyAsync(A->geta, A->geta_host.data(), A->geta_host.size() * sizeof(double),
+                              hipMemcpyHostToDevice, h->stream));
+        A->geta_dirty = false;
+    }
     int rc = run_captured(h, A, A->tape_last ? 3 : 1, enqueue_backward);
     if (rc) return rc;
     // the replays of the other segments overwrote the tape
