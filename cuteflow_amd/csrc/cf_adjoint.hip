@@ -202,12 +202,6 @@ __device__ __forceinline__ void acc4(double4 &a, const double4 &b) {
 // across faces 0..2, then its own.
 constexpr int kR1 = 12;
 
-__device__ __forceinline__ void acc3(double4 &a, const double *__restrict__ s) {
-    a.x += s[0];
-    a.y += s[1];
-    a.z += s[2];
-}
-
 // tiles are 64, 128 or 256 cells: a cell's tile by a shift (tshift = log2 of
 // the tile size) rather than an integer division by a runtime value
 __device__ __forceinline__ bool live_cell(const unsigned long long *__restrict__ cm, int c) {
@@ -215,8 +209,12 @@ __device__ __forceinline__ bool live_cell(const unsigned long long *__restrict__
 }
 
 // `cm`: the cells of step k + 1 whose contributions are nonzero; the slots
-// of the others are zero and not read (skipping the addition of +0 changes
-// at most the sign of a zero sum)
+// of the others are zero and not added (skipping the addition of +0 changes
+// at most the sign of a zero sum).  Every load whose address is known is
+// issued before any test of a loaded value: the codes, the activity bytes and
+// mask words of the cell and its neighbours and all four slots (a missing
+// neighbour reads a slot of the cell's own row, unused), so that the gather
+// takes two rounds of loads instead of a chain of a dozen.
 template <bool R2ON>
 __device__ __forceinline__ double4 gather_cell(int c, int tshift, const unsigned char *__restrict__ tiles_ran,
                                                const unsigned long long *__restrict__ cm,
@@ -226,27 +224,44 @@ __device__ __forceinline__ double4 gather_cell(int c, int tshift, const unsigned
                                                const double *__restrict__ ghmax, int sidx,
                                                const double *__restrict__ geta_row, const int *__restrict__ gfirst,
                                                const int *__restrict__ gnext) {
-    double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (tiles_ran[c >> tshift]) {
-        if (live_cell(cm, c)) {
-            const double *o = R1 + kR1 * (size_t)c + 9;
-            acc = make_double4(o[0], o[1], o[2], 0.0);
-        }
-        const int4 nb = nbr[c];
-        const int nbv[3] = {nb.x, nb.y, nb.z};
+    const int4 nb = nbr[c];
+    const int nbv[3] = {nb.x, nb.y, nb.z};
+    const int am = argmax ? argmax[c] : -1;
+    const double gh = argmax ? ghmax[c] : 0.0;
+    const int ran_c = tiles_ran[c >> tshift];
+    const unsigned long long wc = cm[c >> 6];
+    const double *o = R1 + kR1 * (size_t)c + 9;
+    const double o0 = o[0], o1 = o[1], o2 = o[2];
+    int ii[3], ran_k[3];
+    unsigned long long w_k[3];
+    double v[3][3];
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const int i = nbv[k];
-            if (i >= 0 && tiles_ran[i >> tshift] && live_cell(cm, i))
-                acc3(acc, R1 + kR1 * (size_t)i + 3 * ((nb.w >> (2 * k)) & 3));
-        }
+    for (int k = 0; k < 3; k++) {
+        ii[k] = nbv[k] >= 0 ? nbv[k] : c;
+        ran_k[k] = tiles_ran[ii[k] >> tshift];
+        w_k[k] = cm[ii[k] >> 6];
+        const double *sk = R1 + kR1 * (size_t)ii[k] + 3 * ((nb.w >> (2 * k)) & 3);
+        v[k][0] = sk[0];
+        v[k][1] = sk[1];
+        v[k][2] = sk[2];
+    }
+    double4 acc = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (ran_c) {
+        if ((wc >> (c & 63)) & 1ull) acc = make_double4(o0, o1, o2, 0.0);
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            if ((nbv[k] >= 0) & (ran_k[k] != 0) & (((w_k[k] >> (ii[k] & 63)) & 1ull) != 0ull)) {
+                acc.x += v[k][0];
+                acc.y += v[k][1];
+                acc.z += v[k][2];
+            }
         if (R2ON)
             for (int e = r2_off[c]; e < r2_off[c + 1]; e++) {
                 const int src = r2_idx[e], sc = src / 6;
                 if (tiles_ran[sc >> tshift] && live_cell(cm, sc)) acc4(acc, R2[src]);
             }
     }
-    if (argmax && argmax[c] == sidx) acc.x += ghmax[c];
+    if (argmax && am == sidx) acc.x += gh;
     if (geta_row)
         for (int g = gfirst[c]; g >= 0; g = gnext[g]) acc.x += geta_row[g];
     acc.w = 0.0;
