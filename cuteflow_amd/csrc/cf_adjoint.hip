@@ -228,6 +228,7 @@ __device__ __forceinline__ double4 gather_cell(int c, int tshift, const unsigned
     const int nbv[3] = {nb.x, nb.y, nb.z};
     const int am = argmax ? argmax[c] : -1;
     const double gh = argmax ? ghmax[c] : 0.0;
+    const int g0 = geta_row ? gfirst[c] : -1;
     const int ran_c = tiles_ran[c >> tshift];
     const unsigned long long wc = cm[c >> 6];
     const double *o = R1 + kR1 * (size_t)c + 9;
@@ -262,8 +263,7 @@ __device__ __forceinline__ double4 gather_cell(int c, int tshift, const unsigned
             }
     }
     if (argmax && am == sidx) acc.x += gh;
-    if (geta_row)
-        for (int g = gfirst[c]; g >= 0; g = gnext[g]) acc.x += geta_row[g];
+    for (int g = g0; g >= 0; g = gnext[g]) acc.x += geta_row[g];
     acc.w = 0.0;
     return acc;
 }

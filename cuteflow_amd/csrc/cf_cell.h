@@ -513,6 +513,14 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
     const double q1 = u0.h, q2 = u0.hu, q3 = u0.hv;
     const double surf = ld.geo(G_SURF, i);
     const double x0 = x[0], x1 = x[1], x2 = x[2];
+    // the neighbour codes and states, read before the friction part so that
+    // their loads overlap it
+    const I4 nb = ld.nbr(i);
+    St<double> unb[3];
+    for (int j = 0; j < 3; j++) {
+        const int cc = nb_at(nb, j);
+        unb[j] = cc >= 0 ? ld.state(cc) : u0;
+    }
 
     // friction values and the implicit matrix (as in cell_update)
     double mann = 0.0, sf1 = 0.0, sf2 = 0.0, vitu = 0.0, vitv = 0.0, normvit = 0.0, p13 = 0.0;
@@ -644,13 +652,7 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
     }
     gn = bm;
 
-    const I4 nb = ld.nbr(i);
     const double zml = u0.zm;
-    St<double> unb[3];
-    for (int j = 0; j < 3; j++) {
-        const int cc = nb_at(nb, j);
-        unb[j] = cc >= 0 ? ld.state(cc) : u0;
-    }
     // bed slope source: sb += (g/2 (hij^2 - q1^2) ln) n over interior faces
     for (int j = 0; j < 3; j++) {
         if (nb_at(nb, j) >= 0) {
