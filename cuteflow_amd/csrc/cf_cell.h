@@ -670,6 +670,9 @@ CF_HD inline void cell_reverse(const L &ld, const KParams &p, const double *hydr
     // faces: resc = sum of flux * ln
     D4 dfi = {0.0, 0.0, 0.0, 0.0};
     if (IFLUX >= 3) dfi = ld.dfc(i);
+    // unrolled: with a runtime face index the neighbour states and the
+    // contribution rows live in private (scratch) memory
+#pragma unroll
     for (int j = 0; j < 3; j++) {
         const double n1 = ld.geo(G_N1 + j, i);
         const double n2 = ld.geo(G_N2 + j, i);
