@@ -707,9 +707,16 @@ CF_HD inline void riadh_flux_rev(double gp, double n1, double n2, double hl, dou
             bset += bP * (P * inv);
         }
     }
-    g[3 * side] += bh;
-    g[3 * side + 1] += bu;
-    g[3 * side + 2] += bv;
+    // constant indices (a runtime index would keep g in private memory)
+    if (side) {
+        g[3] += bh;
+        g[4] += bu;
+        g[5] += bv;
+    } else {
+        g[0] += bh;
+        g[1] += bu;
+        g[2] += bv;
+    }
     const double z3[3] = {0.0, 0.0, 0.0}, z2[2] = {0.0, 0.0};
     hllc_core_rev(gp, eps, hl, ul, vl, hr, ur, vr, z3, z2, z2, side ? 0.0 : bsk, side ? bsk : 0.0, bset, g);
 }

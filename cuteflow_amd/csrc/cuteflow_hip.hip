@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kTile) CF_STEP_OCCUPANCY void step_kernel(StepArgs 
         const double zml = st_in[i].w;
 #endif
         double o[3], xo[3];
-        cell_update<double, IFLUX, FRIC>(ld, p, a.hydro, i, dt, tc, o, (MODE != kStepRun && a.xs) ? xo : nullptr,
+        cell_update<double, IFLUX, FRIC>(ld, p, a.hydro, i, dt, tc, o, MODE != kStepRun ? xo : nullptr,
                                          a.surf_in ? a.surf_in + (size_t)b * CF_MAX_IO : nullptr);
         const double h = o[0], hu = o[1], hv = o[2];
         st_out[i] = make_double4(h, hu, hv, zml);
