@@ -227,8 +227,6 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
             return;
         }
     }
-    D4 dfi = {0.0, 0.0, 0.0, 0.0};
-    if (IFLUX >= 3) dfi = ld.dfc(i);
     T fl[3][3];
     double fn1[3], fn2[3], fln[3];
 #pragma unroll
@@ -302,6 +300,11 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
                 if (dry) {
                     zero_flux(n1, n2, flux);
                 } else {
+                    // read here, per wet face: read ahead of the face loop,
+                    // the centroid distances took registers across the whole
+                    // loop and one of them was spilled, the spill waiting for
+                    // its load before the geometry loads could start
+                    const D4 dfi = ld.dfc(i);
                     const double d_mrl = d4_at(dfi, j);
                     const int ja = (j + 1) % 3, jb = (j + 2) % 3;
                     const bool own1 = nb_at(nb, ja) < 0, own2 = nb_at(nb, jb) < 0;
