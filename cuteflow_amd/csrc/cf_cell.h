@@ -303,8 +303,10 @@ CF_HD inline void cell_update(const L &ld, const KParams &p, const double *hydro
                     // read here, per wet face: read ahead of the face loop,
                     // the centroid distances took registers across the whole
                     // loop and one of them was spilled, the spill waiting for
-                    // its load before the geometry loads could start
-                    const D4 dfi = ld.dfc(i);
+                    // its load before the geometry loads could start.  The
+                    // opaque index keeps the compiler from sharing (and then
+                    // spilling) one address across the three faces.
+                    const D4 dfi = ld.dfc(cf_opaque(i));
                     const double d_mrl = d4_at(dfi, j);
                     const int ja = (j + 1) % 3, jb = (j + 2) % 3;
                     const bool own1 = nb_at(nb, ja) < 0, own2 = nb_at(nb, jb) < 0;

@@ -8,3 +8,15 @@
 #include <math.h>
 #define CF_HD
 #endif
+
+// v, hidden from the GPU compiler's common-subexpression elimination (an
+// empty asm that may have changed the register; it emits no instruction):
+// an address formed from it is formed again where it is used instead of
+// being kept, or spilled to private memory, across a loop.  Identity on the
+// host.
+CF_HD inline int cf_opaque(int v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm volatile("" : "+v"(v));
+#endif
+    return v;
+}
