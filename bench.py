@@ -66,12 +66,18 @@ def make_workload(nx, ny, order, seed=1):
     return SC.river_reach(nx, ny, order, seed)
 
 
-def traffic_from_profile(workload_key):
+def traffic_from_profile(workload_key, options):
+    """HBM bytes per step-kernel launch from the committed counter profile of
+    this workload, if it was profiled with the same schedule options (the
+    traffic depends on the tile size, order and skipping), else None."""
     path = os.path.join(ROOT, "profiles", f"traffic_{workload_key}.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
-        return json.load(fh).get("hbm_bytes_per_launch")
+        prof = json.load(fh)
+    if prof.get("schedule_options") != options:
+        return None
+    return prof.get("hbm_bytes_per_launch")
 
 
 def sync():
@@ -189,7 +195,7 @@ def bench_headline(args):
         f"{achieved:.0f} GB/s algorithmic over processed cells, wet cells {wet}")
 
     workload_key = f"river_waf_{m.nelt}_{args.order}"
-    traffic = traffic_from_profile(workload_key)
+    traffic = traffic_from_profile(workload_key, options)
     result = {
         "metric": METRIC,
         "value": value,
